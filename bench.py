@@ -1,0 +1,194 @@
+"""Benchmark of the POREMAPS Stokes sweep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
+                    [--mode jacobi|sor3] [--precision f32dc|f64] [--omega w]
+                    [--tte] [--no-cpu]
+
+A step is one it_eval interval of the solver (100 sweeps of the bulk sweep,
+with the defect-correction fold and residual of that interval) over the whole
+synthetic field, fields resident in HBM.  value = GLUP/s = nx*ny*nz*sweeps/s.
+
+Workloads (synthetic, generated from a seed):
+  config 1: overlapping-sphere pack 256^3, phi 0.35, r 6-12, bc 2, dx 1e-5
+  config 2: thresholded Gaussian random field 512^3, corr. length 8, phi 0.20,
+            bc 1, dx 5e-6 (default: the largest single-GPU configuration)
+
+roofline: the dominant kernel group ("sweep") timed with HIP events on the
+library's own stream over the timed region; achieved = algorithmic bytes per
+launch (8*S+1 B per voxel for the state, + 16 B of FP32 residual in the
+defect-correction mode) / average launch duration; peak = 8 TB/s HBM.
+cpu_baseline: the CPU oracle's reference-order sweep (FP64, one thread) timed
+on this host on a bounded sample of the same field.
+--tte: time-to-eps (eps 1e-6) of the plain FP64 Jacobi solver and of the
+selected mode, reported beside the line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GLUP/s + time-to-eps (FP32-refined vs FP64 plain); solves/hr, p99 latency"
+PEAK_HBM_GBS = 8000.0
+
+
+def workload(cfg: int):
+    from poremaps_amd import geometry as geo
+
+    cache = f"/tmp/poremaps_bench_cfg{cfg}.npy"
+    if cfg == 1:
+        spec = dict(name="sphere_pack_256", n=256, bc=2, dx=1e-5,
+                    desc="overlapping-sphere pack 256^3, phi 0.35, r U[6,12], seed 1, bc 2 (no-slip x/y)")
+    elif cfg == 2:
+        spec = dict(name="grf_512", n=512, bc=1, dx=5e-6,
+                    desc="Gaussian random field 512^3, corr. length 8, phi 0.20, seed 1, bc 1 (slip x/y)")
+    else:
+        raise SystemExit(f"unknown config {cfg}")
+    if os.path.exists(cache):
+        g = np.load(cache)
+    else:
+        t0 = time.time()
+        if cfg == 1:
+            g = geo.sphere_pack(256, 0.35, 6, 12, seed=1)
+        else:
+            g = geo.gaussian_field(512, 8.0, 0.20, seed=1)
+        np.save(cache, g)
+        print(f"[bench] generated {spec['name']} in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    return g, spec
+
+
+def cpu_baseline(g, bc, budget_s=20.0):
+    """Reference-order FP64 sweeps of the CPU oracle on this host (1 thread)."""
+    import oracle as O
+
+    G = O.geom_halo(g, bc)
+    code = O.eval_geometry(G)
+    st = [a.copy() for a in O.init_fields(G, bc)]
+    O.pins(G, st[0])
+    n = g.size
+    done, t0 = 0, time.perf_counter()
+    while True:
+        O.sweep_ref(2, G, code, *st)
+        for f in st:
+            O.exchange(f, bc)
+        O.pins(G, st[0])
+        done += 1
+        el = time.perf_counter() - t0
+        if el > budget_s * 0.5 or done >= 50:
+            break
+    return {"value": round(n * done / el / 1e9, 5), "unit": "GLUP/s", "cores": 1, "kind": "port",
+            "sample": f"{done} reference-order (solving_algorithm 2) FP64 sweeps of the same "
+                      f"{g.shape[2]}x{g.shape[1]}x{g.shape[0]} field, CPU oracle, 1 thread, {el:.1f} s"}
+
+
+def time_kernels(s, n):
+    import ctypes as C
+    import poremaps_amd as pm
+
+    L = pm.lib()
+
+    class KT(C.Structure):
+        _fields_ = [("name", C.c_char * 48), ("launches", C.c_int), ("avg_ms", C.c_double),
+                    ("bytes_per_launch", C.c_double)]
+
+    out = (KT * 16)()
+    cnt = C.c_int()
+    L.pm_time_kernels.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    rc = L.pm_time_kernels(s._h, n, out, 16, C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError(L.pm_last_error().decode())
+    return {out[i].name.decode(): dict(launches=out[i].launches, avg_ms=out[i].avg_ms,
+                                       bytes=out[i].bytes_per_launch) for i in range(cnt.value)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--mode", default="jacobi")
+    ap.add_argument("--precision", default="f32dc")
+    ap.add_argument("--omega", type=float, default=2.0)
+    ap.add_argument("--it-eval", type=int, default=100)
+    ap.add_argument("--tte", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build solves one geometry per GPU and has no "
+              f"multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import poremaps_amd as pm
+
+    g, spec = workload(a.config)
+    n = g.size
+    kw = dict(boundary_method=spec["bc"], voxel_size=spec["dx"], mode=a.mode, precision=a.precision,
+              omega=a.omega, it_eval=a.it_eval, eps=1e-6)
+    print(f"[bench] {spec['desc']}: loading", file=sys.stderr, flush=True)
+    s = pm.Solver(g, **kw)
+    for _ in range(a.warmup):
+        s.step(a.it_eval)
+        s.permeability()
+    t0 = time.perf_counter()
+    kt = time_kernels(s, a.steps * a.it_eval)
+    s.permeability()
+    el = time.perf_counter() - t0
+    sweeps = a.steps * a.it_eval
+    glups = n * sweeps / el / 1e9
+    k = s.permeability()
+    s.close()
+    sw = kt["sweep"]
+    achieved = sw["bytes"] / (sw["avg_ms"] * 1e-3) / 1e9
+    print(f"[bench] {sweeps} sweeps in {el:.3f}s: {glups:.2f} GLUP/s; kernels {json.dumps(kt)}",
+          file=sys.stderr, flush=True)
+    line = {
+        "metric": METRIC,
+        "value": round(glups, 3),
+        "unit": "GLUP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(el / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if a.precision == "f32dc" else "f64",
+        "data": "synthetic",
+        "config": {"workload": spec["desc"], "nx": g.shape[2], "ny": g.shape[1], "nz": g.shape[0],
+                   "mode": a.mode, "precision": a.precision, "omega": a.omega,
+                   "sweeps_per_step": a.it_eval, "porosity": float((g == 0).mean())},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "sweep", "avg_ms": round(sw["avg_ms"], 4),
+                     "bytes_per_launch": sw["bytes"]},
+        "kernels": {k2: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"],
+                         "GBps": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k2, v in kt.items()},
+    }
+    if a.tte:
+        tte = {}
+        for label, mk in (("fp64_plain_jacobi", dict(mode="jacobi", precision="f64", omega=1.0)),
+                          (f"{a.precision}_{a.mode}_w{a.omega}", dict(mode=a.mode, precision=a.precision,
+                                                                     omega=a.omega))):
+            s = pm.Solver(g, **dict(kw, **mk))
+            res, _ = s.solve()
+            s.close()
+            tte[label] = {"seconds": round(res.seconds, 3), "sweeps": res.sweeps,
+                          "wk33": res.last.wk33, "glups": round(res.glups, 2)}
+            print(f"[bench] tte {label}: {tte[label]}", file=sys.stderr, flush=True)
+        line["time_to_eps"] = tte
+    if not a.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(g, spec["bc"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

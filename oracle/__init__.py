@@ -93,6 +93,12 @@ def _declare(L):
     L.orc_outlet_plane.argtypes = [_u8p, _I, _I, _I, _I] + [_f64p] * 4 + [_D, _f64p]
     L.orc_residual.argtypes = [_u8p, _I, _I, _I] + [_f64p] * 5 + [_f32p] * 4
     L.orc_fold.argtypes = [_u8p, _I, _I, _I, _I] + [_f64p] * 4 + [_f32p] * 4
+    L.orc_d2_ref.argtypes = [_I] + [_D] * 5
+    L.orc_d2_ref.restype = _D
+    L.orc_d2_f64.argtypes = [_I] + [_D] * 5
+    L.orc_d2_f64.restype = _D
+    L.orc_d2_f32.argtypes = [_I] + [_F] * 5
+    L.orc_d2_f32.restype = _F
 
 
 def _dims(a):
@@ -173,6 +179,11 @@ def calc_flow(G, code, bc: int, state, *, solver=2, eps=1e-6, max_iter=10**8,
 # ----------------------------------------------------------------------------
 # build-side fast modes
 # ----------------------------------------------------------------------------
+
+def d2(case, ll, l, c, r, rr, kind="ref"):
+    """get_2nd_derivation (src/solver.cc:57-91); kind ref | f64 | f32."""
+    return getattr(lib(), f"orc_d2_{kind}")(case, ll, l, c, r, rr)
+
 
 def pack_code(code: np.ndarray) -> np.ndarray:
     pc = np.empty(code.shape, np.uint8)

@@ -483,3 +483,17 @@ void orc_fold(const uint8_t *G, int nx, int ny, int nz, int bc,
             }
     orc_ghost_f64(G, nx, ny, nz, bc, Xp, Xu, Xv, Xw, NULL, (double)(nz + 2), 2.0, 3.0);
 }
+
+/* Exported for the known-answer tests of get_2nd_derivation (src/solver.cc:57-91). */
+double orc_d2_ref(int cs, double ll, double l, double c, double r, double rr)
+{
+    return d2_ref(cs, ll, l, c, r, rr);
+}
+double orc_d2_f64(int cs, double ll, double l, double c, double r, double rr)
+{
+    return d2_f64(cs, ll, l, c, r, rr);
+}
+float orc_d2_f32(int cs, float ll, float l, float c, float r, float rr)
+{
+    return d2_f32(cs, ll, l, c, r, rr);
+}

@@ -112,6 +112,11 @@ void orc_fold(const uint8_t *G, int nx, int ny, int nz, int bc,
               double *Xp, double *Xu, double *Xv, double *Xw,
               float *dp, float *du, float *dv, float *dw);
 
+/* get_2nd_derivation (reference form) and the fast-mode forms. */
+double orc_d2_ref(int cs, double ll, double l, double c, double r, double rr);
+double orc_d2_f64(int cs, double ll, double l, double c, double r, double rr);
+float orc_d2_f32(int cs, float ll, float l, float c, float r, float rr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,220 @@
+"""poremaps_amd -- MI355X-native POREMAPS Stokes permeability solver.
+
+Python view of the C ABI in ``include/poremaps_amd.h`` (ctypes over the
+in-tree ``libporemaps_amd.so``).  The solver itself is C++/HIP; this module
+only moves host buffers across the boundary, for tests, benchmarks and
+scripting.  There is no CPU fallback: every call goes to the GPU library, and
+importing the binding fails loudly when the library has not been built.
+
+The call sequence mirrors the reference's ``POREMAPS input.inp`` run
+(src/main_stokes_solver.cc:195-411): parameters, geometry, ``calc_flow``,
+field output.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libporemaps_amd.so")
+CLI_PATH = os.path.join(_HERE, "bin", "POREMAPS")
+
+PM_OK = 0
+MODES = {"reference": 0, "jacobi": 1, "sor3": 2}
+PRECISIONS = {"f64": 0, "f32dc": 1}
+FIELDS = {"velx": 0, "vely": 1, "velz": 2, "press": 3, "nhood": 4}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("boundary_method", C.c_int), ("voxel_size", C.c_double),
+        ("max_iter", C.c_int), ("it_eval", C.c_int), ("it_write", C.c_int),
+        ("solving_algorithm", C.c_int), ("eps", C.c_double), ("porosity", C.c_double),
+        ("dom_interest", C.c_int * 6),
+        ("mode", C.c_int), ("precision", C.c_int), ("omega", C.c_double), ("device", C.c_int),
+    ]
+
+
+class LogRow(C.Structure):
+    _fields_ = [("iteration", C.c_int)] + [
+        (n, C.c_double) for n in ("conv", "tps", "wmax_velz", "wmean_velz", "k13", "k23", "k33",
+                                  "wk13", "wk23", "wk33")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Result(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("conv", C.c_double), ("porosity", C.c_double),
+                ("seconds", C.c_double), ("glups", C.c_double), ("n_log_rows", C.c_int),
+                ("last", LogRow)]
+
+
+class InpExtra(C.Structure):
+    _fields_ = [("dom_decomposition", C.c_int * 3), ("geometry_file", C.c_char * 400),
+                ("log_file", C.c_char * 400), ("write_output", C.c_int * 4)]
+
+
+LOG_FN = C.CFUNCTYPE(C.c_int, C.POINTER(LogRow), C.c_void_p)
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p)
+
+_lib = None
+
+
+class PoremapsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded C library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m poremaps_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.pm_last_error.restype = C.c_char_p
+        L.pm_io_last_error.restype = C.c_char_p
+        L.pm_params_default.argtypes = [P(Params)]
+        L.pm_ctx_create.argtypes = [P(Params), P(C.c_void_p)]
+        L.pm_load_geometry.argtypes = [C.c_void_p, C.c_void_p]
+        L.pm_solve.argtypes = [C.c_void_p, LOG_FN, PROGRESS_FN, C.c_void_p, P(Result)]
+        L.pm_get_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.pm_get_field_halo.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.pm_porosity.argtypes = [C.c_void_p, P(C.c_double)]
+        L.pm_permeability.argtypes = [C.c_void_p, C.c_void_p]
+        L.pm_step.argtypes = [C.c_void_p, C.c_int]
+        L.pm_device_bytes.argtypes = [P(Params)]
+        L.pm_device_bytes.restype = C.c_int64
+        L.pm_destroy.argtypes = [C.c_void_p]
+        L.pm_read_inp.argtypes = [C.c_char_p, P(Params), P(InpExtra)]
+        L.pm_write_raw_f64.argtypes = [C.c_char_p, C.c_void_p, C.c_int64, C.c_double, C.c_int]
+        L.pm_write_raw_i32.argtypes = [C.c_char_p, C.c_void_p, C.c_int64]
+        L.pm_write_log_row.argtypes = [C.c_char_p, P(LogRow), C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc, io=False):
+    if rc != PM_OK:
+        L = lib()
+        msg = (L.pm_io_last_error() if io else L.pm_last_error()) or b""
+        raise PoremapsError(rc, msg.decode(errors="replace"))
+
+
+def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_000_000,
+                it_eval=100, it_write=100, solving_algorithm=2, eps=1e-6, porosity=-1.0,
+                dom_interest=(0, 0, 0, 0, 0, 0), mode="reference", precision="f64", omega=1.0,
+                device=0) -> Params:
+    p = Params()
+    lib().pm_params_default(C.byref(p))
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.boundary_method = boundary_method
+    p.voxel_size = voxel_size
+    p.max_iter, p.it_eval, p.it_write = max_iter, it_eval, it_write
+    p.solving_algorithm = solving_algorithm
+    p.eps, p.porosity = eps, porosity
+    p.dom_interest = (C.c_int * 6)(*dom_interest)
+    p.mode = MODES[mode] if isinstance(mode, str) else mode
+    p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+    p.omega = omega
+    p.device = device
+    return p
+
+
+def device_bytes(params: Params) -> int:
+    return int(lib().pm_device_bytes(C.byref(params)))
+
+
+class Solver:
+    """One geometry on one GPU (a pm_ctx)."""
+
+    def __init__(self, raw: np.ndarray, **kw):
+        raw = np.ascontiguousarray(raw, dtype=np.uint8)
+        nz, ny, nx = raw.shape
+        self.shape = (nz, ny, nx)
+        self.params = make_params(nx, ny, nz, **kw)
+        h = C.c_void_p()
+        _check(lib().pm_ctx_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        _check(lib().pm_load_geometry(self._h, raw.ctypes.data))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def solve(self, log=None, progress=None):
+        """Iterate to eps; returns (Result, list of log-row dicts)."""
+        rows = []
+
+        def _log(r, _u):
+            rows.append(r.contents.as_dict())
+            return 0 if log is None else int(bool(log(rows[-1])))
+
+        def _prog(it, conv, secs, _u):
+            return 0 if progress is None else int(bool(progress(it, conv, secs)))
+
+        lf, pf = LOG_FN(_log), PROGRESS_FN(_prog)
+        res = Result()
+        _check(lib().pm_solve(self._h, lf, pf if progress else PROGRESS_FN(0), None, C.byref(res)))
+        return res, rows
+
+    def step(self, n: int):
+        _check(lib().pm_step(self._h, int(n)))
+
+    def field(self, name: str) -> np.ndarray:
+        nz, ny, nx = self.shape
+        dt = np.int32 if name == "nhood" else np.float64
+        out = np.empty((nz, ny, nx), dt)
+        _check(lib().pm_get_field(self._h, FIELDS[name], out.ctypes.data))
+        return out
+
+    def field_halo(self, name: str) -> np.ndarray:
+        nz, ny, nx = self.shape
+        out = np.empty((nz + 4, ny + 4, nx + 4), np.float64)
+        _check(lib().pm_get_field_halo(self._h, FIELDS[name], out.ctypes.data))
+        return out
+
+    def state_halo(self):
+        """(p, u, v, w) with halos, the reference's array layout."""
+        return tuple(self.field_halo(n) for n in ("press", "velx", "vely", "velz"))
+
+    @property
+    def porosity(self) -> float:
+        v = C.c_double()
+        _check(lib().pm_porosity(self._h, C.byref(v)))
+        return v.value
+
+    def permeability(self) -> dict:
+        out = (C.c_double * 8)()
+        _check(lib().pm_permeability(self._h, out))
+        keys = ("k13", "k23", "k33", "wk13", "wk23", "wk33", "wmax_velz", "wmean_velz")
+        return dict(zip(keys, list(out)))
+
+
+def read_inp(path: str):
+    """Parse an ordered .inp (src/read_problem_params.cc:75-90)."""
+    p, x = Params(), InpExtra()
+    _check(lib().pm_read_inp(path.encode(), C.byref(p), C.byref(x)), io=True)
+    return p, x

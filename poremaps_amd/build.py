@@ -1,0 +1,89 @@
+"""Build recipe: compiles the HIP library, the POREMAPS command line and the
+CPU oracle in-tree.
+
+    python -m poremaps_amd.build
+
+Outputs (git-ignored, shipped with the tree):
+    poremaps_amd/libporemaps_amd.so   HIP kernels + host solver + C ABI
+    poremaps_amd/bin/POREMAPS         the drop-in command line
+    oracle/libpm_oracle.so            CPU checker (tests only)
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libporemaps_amd.so")
+CLI = os.path.join(HERE, "bin", "POREMAPS")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# -ffp-contract=off: every floating-point operation is rounded on its own, so
+# the FP64 kernels match the CPU oracle bit for bit.  Packed FP32 instructions
+# are disabled for all HIP code.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+SOURCES = ["pm_kernels.hip", "pm_solver.hip", "pm_batch.hip", "pm_io.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    noise = "is not a recognized feature for this target"
+    err = "\n".join(l for l in r.stderr.splitlines() if noise not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}\n{err}")
+    return err
+
+
+def _compile(src):
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
+    _run(cmd)
+    return obj
+
+
+def build_library() -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_cli() -> str:
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    tmp = CLI + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", "-o", tmp, os.path.join(CSRC, "poremaps_main.cpp"),
+          "-L" + HERE, "-lporemaps_amd", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, CLI)
+    return CLI
+
+
+def build_oracle() -> str:
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    return os.path.join(ROOT, "oracle", "libpm_oracle.so")
+
+
+def build_all() -> None:
+    build_library()
+    build_cli()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB, CLI, file=sys.stderr)

@@ -1,0 +1,62 @@
+// pm_kernels.h -- launchers of the HIP kernels (pm_kernels.hip).
+#pragma once
+
+#include "pm_grid.h"
+
+namespace pm {
+
+// dom_interest box in 0-based interior coordinates (x min/max, y, z).
+struct DomBox {
+    int on;
+    int d[6];
+};
+
+// Results of a state reduction: sum[0..2] fluid u, v, w; sum[3..5] box fluid
+// u, v, w; sum[6], sum[7] p on the box's zmin / zmax planes; count[0..3] the
+// matching fluid counts; wmax = max(0, max fluid w).
+struct RedSums {
+    double sum[8];
+    long long count[4];
+    double wmax;
+};
+
+void launch_geom_fill(const Grid &g, const uint8_t *raw, uint8_t *G, hipStream_t s);
+template <typename T>
+void launch_exchange_full(const Grid &g, T *f0, T *f1, T *f2, T *f3, hipStream_t s);
+void launch_eval_geometry(const Grid &g, const uint8_t *G, int32_t *code, uint8_t *pc, hipStream_t s);
+void launch_init_fields(const Grid &g, const uint8_t *G, double *p, double *u, double *v, double *w,
+                        hipStream_t s);
+template <typename T>
+void launch_pins(const Grid &g, const uint8_t *G, T *p, T pin_in, T pin_hi, T pin_out, hipStream_t s);
+
+// Reference order: `nsweeps` complete sweeps (hyperplanes, exchange, pins).
+void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
+                       double *v, double *w, int solver, int nsweeps, bool one_wg, hipStream_t s);
+
+template <typename T>
+void launch_jacobi(const Grid &g, const uint8_t *pc, const T *p0, const T *u0, const T *v0,
+                   const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu, T cp,
+                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+                   hipStream_t s);
+template <typename T>
+void launch_colour(const Grid &g, const uint8_t *pc, T *p, T *u, T *v, T *w, T *q, T cu, T cp,
+                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+                   hipStream_t s);
+template <typename T>
+void launch_ghost_fast(const Grid &g, const uint8_t *G, T *p, T *u, T *v, T *w, T *q, T pin_in, T pin_hi,
+                       T pin_out, hipStream_t s);
+
+void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const double *u, const double *v,
+                   const double *w, const DomBox &box, bool full, void *scratch, RedSums *out,
+                   hipStream_t s);
+size_t reduce_scratch_bytes();
+
+void launch_outlet_plane(const Grid &g, const uint8_t *pc, const double *p, const double *u,
+                         const double *v, const double *w, double pin_out, double *q, hipStream_t s);
+void launch_residual(const Grid &g, const uint8_t *pc, const double *p, const double *u, const double *v,
+                     const double *w, const double *q, float *Rp, float *Ru, float *Rv, float *Rw,
+                     hipStream_t s);
+void launch_fold(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw,
+                 float *dp, float *du, float *dv, float *dw, hipStream_t s);
+
+}  // namespace pm

@@ -1,0 +1,767 @@
+// pm_kernels.hip -- HIP kernels of the POREMAPS Stokes solver for gfx950.
+//
+// Reference mode: the in-place lexicographic sweep of src/solver.cc:112-223,
+// reproduced exactly by updating the hyperplanes i+j+k = s in increasing order
+// (every voxel reads new values only from its -1/-2 neighbours, whose i+j+k is
+// smaller, and old values only from +1/+2 neighbours; halos are frozen during
+// a sweep, src/calc_flow.cc:109-117).  FP64, no contraction, divisions kept.
+//
+// Fast modes: symplectic Jacobi (out of place) and 3-colour SOR (in place) in
+// FP64 or FP32, with a one-pass ghost fill, device reductions and the FP64
+// defect-correction passes.
+#include "pm_kernels.h"
+
+#include <algorithm>
+
+#pragma clang fp contract(off)
+
+namespace pm {
+
+// ---------------------------------------------------------------------------
+// geometry and initial state
+// ---------------------------------------------------------------------------
+
+// src/geometry.cc:98-115: interior bytes shifted by the halo, everything else 0.
+__global__ void k_geom_fill(Grid g, const uint8_t *__restrict__ raw, uint8_t *__restrict__ G) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.NX) return;
+    const bool in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
+    G[g.at(i, j, k)] =
+        in ? raw[((int64_t)(i - 2) * g.ny + (j - 2)) * g.nx + (k - 2)] : (uint8_t)0;
+}
+
+// Two-layer periodic copy along one axis over the full extent of the other
+// two (src/parallelization.cc:67-190, :193-318 at one rank).  Up to four
+// fields of one type; null pointers are skipped.
+template <typename T>
+__global__ void k_exchange_axis(Grid g, int axis, T *f0, T *f1, T *f2, T *f3) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    T *fs[4] = {f0, f1, f2, f3};
+    if (axis == 0) {
+        if (t >= (int64_t)g.NZ * g.NY) return;
+        const int i = t / g.NY, j = t % g.NY;
+        const int64_t r = g.at(i, j, 0);
+        for (int q = 0; q < 4; q++) {
+            T *f = fs[q];
+            if (!f) continue;
+            const T a0 = f[r + 2], a1 = f[r + 3], b0 = f[r + g.NX - 4], b1 = f[r + g.NX - 3];
+            f[r + g.NX - 2] = a0; f[r + g.NX - 1] = a1; f[r] = b0; f[r + 1] = b1;
+        }
+    } else if (axis == 1) {
+        if (t >= (int64_t)g.NZ * g.NX) return;
+        const int i = t / g.NX, k = t % g.NX;
+        for (int q = 0; q < 4; q++) {
+            T *f = fs[q];
+            if (!f) continue;
+            const T a0 = f[g.at(i, 2, k)], a1 = f[g.at(i, 3, k)];
+            const T b0 = f[g.at(i, g.NY - 4, k)], b1 = f[g.at(i, g.NY - 3, k)];
+            f[g.at(i, g.NY - 2, k)] = a0; f[g.at(i, g.NY - 1, k)] = a1;
+            f[g.at(i, 0, k)] = b0; f[g.at(i, 1, k)] = b1;
+        }
+    } else {
+        if (t >= (int64_t)g.NY * g.NX) return;
+        const int j = t / g.NX, k = t % g.NX;
+        for (int q = 0; q < 4; q++) {
+            T *f = fs[q];
+            if (!f) continue;
+            const T a0 = f[g.at(2, j, k)], a1 = f[g.at(3, j, k)];
+            const T b0 = f[g.at(g.NZ - 4, j, k)], b1 = f[g.at(g.NZ - 3, j, k)];
+            f[g.at(g.NZ - 2, j, k)] = a0; f[g.at(g.NZ - 1, j, k)] = a1;
+            f[g.at(0, j, k)] = b0; f[g.at(1, j, k)] = b1;
+        }
+    }
+}
+
+// src/geometry.cc:174-302: per-axis wall case of every interior fluid cell,
+// code = 100*cx + 10*cy + cz; solid -1, other cells 0.  Also the packed
+// uint8 code the fast kernels read ((cx-1)*36 + (cy-1)*6 + (cz-1), 255 solid).
+__device__ __forceinline__ int axis_case(const uint8_t *G, int64_t c, int64_t s) {
+    const int l = G[c - s] == 0 ? 0 : 1;
+    const int r = G[c + s] == 0 ? 0 : 1;
+    int ll = -1, rr = -1;
+    if (l == 0) ll = G[c - 2 * s] == 0 ? 0 : 1;
+    if (r == 0) rr = G[c + 2 * s] == 0 ? 0 : 1;
+    if (l == 1 && r == 1) return 1;
+    if (l == 1 && r == 0 && rr == 1) return 2;
+    if (ll == 1 && l == 0 && r == 1) return 3;
+    if (l == 1 && r == 0 && rr == 0) return 4;
+    if (ll == 0 && l == 0 && r == 1) return 5;
+    if (l == 0 && r == 0) return 6;
+    return 0;
+}
+
+__global__ void k_eval_geometry(Grid g, const uint8_t *__restrict__ G, int32_t *__restrict__ code,
+                                uint8_t *__restrict__ pc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.NX) return;
+    const int64_t c = g.at(i, j, k);
+    const bool in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
+    if (G[c] != 0) { code[c] = -1; pc[c] = kSolid; return; }
+    if (!in) { code[c] = 0; pc[c] = kSolid; return; }
+    const int cx = axis_case(G, c, 1), cy = axis_case(G, c, g.sx), cz = axis_case(G, c, g.sy);
+    code[c] = 100 * cx + 10 * cy + cz;
+    pc[c] = (uint8_t)((cx - 1) * 36 + (cy - 1) * 6 + (cz - 1));
+}
+
+// src/initialize.cc:69-112: u = v = w = 0, p = -1; interior fluid p is linear
+// in z ((nz - i) + 4 at one rank), interior solid p = 0.
+__global__ void k_init_fields(Grid g, const uint8_t *__restrict__ G, double *p, double *u,
+                              double *v, double *w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.NX) return;
+    const int64_t c = g.at(i, j, k);
+    const bool in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
+    u[c] = 0.0; v[c] = 0.0; w[c] = 0.0;
+    p[c] = in ? (G[c] == 0 ? (double)((g.nz - 0 - i) + 2 + 2) : 0.0) : -1.0;
+}
+
+// src/boundary_conditions.cc:46-78 at one rank.
+template <typename T>
+__global__ void k_pins(Grid g, const uint8_t *__restrict__ G, T *p, T pin_in, T pin_hi,
+                       T pin_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y + 2;
+    if (k >= g.nx + 2) return;
+    const int64_t c1 = g.at(1, j, k), chi = g.at(g.NZ - 2, j, k), cout = g.at(g.NZ - 3, j, k);
+    if (G[c1] == 0) p[c1] = pin_in;
+    if (G[chi] == 0) p[chi] = pin_hi;
+    if (G[cout] == 0) p[cout] = pin_out;
+}
+
+// ---------------------------------------------------------------------------
+// reference mode (exact lexicographic order)
+// ---------------------------------------------------------------------------
+
+// src/solver.cc:136-219 for one voxel, expression for expression.
+__device__ __forceinline__ void ref_voxel(const Grid &g, const uint8_t *__restrict__ G,
+                                          const int32_t *__restrict__ code, double *p, double *u,
+                                          double *v, double *w, int solver, int i, int j, int k) {
+    const int64_t c = g.at(i, j, k), sj = g.sx, si = g.sy;
+    if (G[c] != 0) return;
+    double dp_dz, dp_dy, dp_dx;
+    if (G[c - si] == 0) dp_dz = -p[c - si] + p[c]; else dp_dz = 0.;
+    if (G[c - sj] == 0) dp_dy = -p[c - sj] + p[c]; else dp_dy = 0.;
+    if (G[c - 1] == 0) dp_dx = -p[c - 1] + p[c]; else dp_dx = 0.;
+    int num = code[c];
+    const int cz = num % 10; num /= 10;
+    const int cy = num % 10; num /= 10;
+    const int cx = num % 10;
+    const double d2vxdx2 = (u[c - 1] - 2 * u[c] + u[c + 1]);
+    const double d2vydy2 = (v[c - sj] - 2 * v[c] + v[c + sj]);
+    const double d2vzdz2 = (w[c - si] - 2 * w[c] + w[c + si]);
+    const double d2vydx2 = d2<double>(cx, v[c - 2], v[c - 1], v[c], v[c + 1], v[c + 2]);
+    const double d2vzdx2 = d2<double>(cx, w[c - 2], w[c - 1], w[c], w[c + 1], w[c + 2]);
+    const double d2vxdy2 = d2<double>(cy, u[c - 2 * sj], u[c - sj], u[c], u[c + sj], u[c + 2 * sj]);
+    const double d2vzdy2 = d2<double>(cy, w[c - 2 * sj], w[c - sj], w[c], w[c + sj], w[c + 2 * sj]);
+    const double d2vxdz2 = d2<double>(cz, u[c - 2 * si], u[c - si], u[c], u[c + si], u[c + 2 * si]);
+    const double d2vydz2 = d2<double>(cz, v[c - 2 * si], v[c - si], v[c], v[c + si], v[c + 2 * si]);
+    const double velx_sum = d2vxdx2 + d2vxdy2 + d2vxdz2;
+    const double vely_sum = d2vydy2 + d2vydz2 + d2vydx2;
+    const double velz_sum = d2vzdz2 + d2vzdx2 + d2vzdy2;
+    double v_sum = 0.0;
+    if (solver == 1)
+        v_sum = (w[c + si] - w[c] + v[c + sj] - v[c] + u[c + 1] - u[c]);
+    if (solver == 1 || solver == 2) {
+        if (G[c - 1] == 1) u[c] = 0.0;
+        else u[c] = u[c] + (-dp_dx + velx_sum / kRe) * kDt;
+        if (G[c - sj] == 1) v[c] = 0.0;
+        else v[c] = v[c] + (-dp_dy + vely_sum / kRe) * kDt;
+        if (G[c - si] == 1) w[c] = 0.0;
+        else w[c] = w[c] + (-dp_dz + velz_sum / kRe) * kDt;
+    } else if (solver == 3) {
+        u[c] = (1.0 - kOmegaRef) * u[c] + kOmegaRef * (-dp_dx + velx_sum / kRe) * kDt;
+        v[c] = (1.0 - kOmegaRef) * v[c] + kOmegaRef * (-dp_dy + vely_sum / kRe) * kDt;
+        w[c] = (1.0 - kOmegaRef) * w[c] + kOmegaRef * (-dp_dz + velz_sum / kRe) * kDt;
+    }
+    if (solver == 2 || solver == 3)
+        v_sum = (w[c + si] - w[c] + v[c + sj] - v[c] + u[c + 1] - u[c]);
+    p[c] = p[c] - kC2 * v_sum * kDt;
+}
+
+// Hyperplane s of the interior (0-based coordinates a=z, b=y, c=x, a+b+c=s).
+__device__ __forceinline__ void hyperplane_range(const Grid &g, int s, int &a_lo, int &a_hi) {
+    a_lo = max(0, s - (g.ny - 1) - (g.nx - 1));
+    a_hi = min(g.nz - 1, s);
+}
+
+// One hyperplane per launch (large domains).
+__global__ void k_ref_hyperplane(Grid g, const uint8_t *__restrict__ G,
+                                 const int32_t *__restrict__ code, double *p, double *u,
+                                 double *v, double *w, int solver, int s) {
+    int a_lo, a_hi;
+    hyperplane_range(g, s, a_lo, a_hi);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)(a_hi - a_lo + 1) * g.ny) return;
+    const int a = a_lo + (int)(t / g.ny), b = (int)(t % g.ny), c = s - a - b;
+    if (c < 0 || c >= g.nx) return;
+    ref_voxel(g, G, code, p, u, v, w, solver, a + 2, b + 2, c + 2);
+}
+
+// Whole sweeps in one workgroup (small domains): hyperplanes, then the
+// reference's exchange X, Y, Z of p, u, v, w, then the pins -- the exact
+// order of src/calc_flow.cc:109-117.
+__global__ __launch_bounds__(1024) void k_ref_sweeps_1wg(Grid g, const uint8_t *__restrict__ G,
+                                                         const int32_t *__restrict__ code,
+                                                         double *p, double *u, double *v,
+                                                         double *w, int solver, int nsweeps) {
+    const int smax = (g.nx - 1) + (g.ny - 1) + (g.nz - 1);
+    double *fs[4] = {p, u, v, w};
+    for (int it = 0; it < nsweeps; it++) {
+        for (int s = 0; s <= smax; s++) {
+            int a_lo, a_hi;
+            hyperplane_range(g, s, a_lo, a_hi);
+            const int n = (a_hi - a_lo + 1) * g.ny;
+            for (int t = threadIdx.x; t < n; t += blockDim.x) {
+                const int a = a_lo + t / g.ny, b = t % g.ny, c = s - a - b;
+                if (c >= 0 && c < g.nx) ref_voxel(g, G, code, p, u, v, w, solver, a + 2, b + 2, c + 2);
+            }
+            __syncthreads();
+        }
+        if (g.px) {
+            for (int t = threadIdx.x; t < g.NZ * g.NY; t += blockDim.x) {
+                const int64_t r = g.at(t / g.NY, t % g.NY, 0);
+                for (int q = 0; q < 4; q++) {
+                    double *f = fs[q];
+                    const double a0 = f[r + 2], a1 = f[r + 3], b0 = f[r + g.NX - 4], b1 = f[r + g.NX - 3];
+                    f[r + g.NX - 2] = a0; f[r + g.NX - 1] = a1; f[r] = b0; f[r + 1] = b1;
+                }
+            }
+            __syncthreads();
+        }
+        if (g.py) {
+            for (int t = threadIdx.x; t < g.NZ * g.NX; t += blockDim.x) {
+                const int i = t / g.NX, k = t % g.NX;
+                for (int q = 0; q < 4; q++) {
+                    double *f = fs[q];
+                    const double a0 = f[g.at(i, 2, k)], a1 = f[g.at(i, 3, k)];
+                    const double b0 = f[g.at(i, g.NY - 4, k)], b1 = f[g.at(i, g.NY - 3, k)];
+                    f[g.at(i, g.NY - 2, k)] = a0; f[g.at(i, g.NY - 1, k)] = a1;
+                    f[g.at(i, 0, k)] = b0; f[g.at(i, 1, k)] = b1;
+                }
+            }
+            __syncthreads();
+        }
+        if (g.pz) {
+            for (int t = threadIdx.x; t < g.NY * g.NX; t += blockDim.x) {
+                const int j = t / g.NX, k = t % g.NX;
+                for (int q = 0; q < 4; q++) {
+                    double *f = fs[q];
+                    const double a0 = f[g.at(2, j, k)], a1 = f[g.at(3, j, k)];
+                    const double b0 = f[g.at(g.NZ - 4, j, k)], b1 = f[g.at(g.NZ - 3, j, k)];
+                    f[g.at(g.NZ - 2, j, k)] = a0; f[g.at(g.NZ - 1, j, k)] = a1;
+                    f[g.at(0, j, k)] = b0; f[g.at(1, j, k)] = b1;
+                }
+            }
+            __syncthreads();
+        }
+        const double pin_in = (double)(g.nz + 2);
+        for (int t = threadIdx.x; t < g.ny * g.nx; t += blockDim.x) {
+            const int j = t / g.nx + 2, k = t % g.nx + 2;
+            const int64_t c1 = g.at(1, j, k);
+            if (G[c1] == 0) p[c1] = pin_in;
+        }
+        for (int t = threadIdx.x; t < g.ny * g.nx; t += blockDim.x) {
+            const int j = t / g.nx + 2, k = t % g.nx + 2;
+            const int64_t chi = g.at(g.NZ - 2, j, k), cout = g.at(g.NZ - 3, j, k);
+            if (G[chi] == 0) p[chi] = 2.0;
+            if (G[cout] == 0) p[cout] = 3.0;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fast modes
+// ---------------------------------------------------------------------------
+
+// Symplectic Jacobi, velocity pass: (p0,u0,v0,w0) -> (u1,v1,w1).
+template <typename T, bool HAS_R>
+__global__ __launch_bounds__(256) void k_jac_vel(Grid g, const uint8_t *__restrict__ pc,
+                                                 const T *__restrict__ p0, const T *__restrict__ u0,
+                                                 const T *__restrict__ v0, const T *__restrict__ w0,
+                                                 const T *__restrict__ q0, T *__restrict__ u1,
+                                                 T *__restrict__ v1, T *__restrict__ w1,
+                                                 const float *__restrict__ Ru,
+                                                 const float *__restrict__ Rv,
+                                                 const float *__restrict__ Rw, T cu) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int i = blockIdx.z + 2;
+    if (k >= g.nx + 2 || j >= g.ny + 2) return;
+    const int64_t c = g.at(i, j, k);
+    const uint8_t code = pc[c];
+    if (code == kSolid) return;
+    const Cases cs = decode(code);
+    const bool out = (i == g.nz + 1);
+    T fu, fv, fw;
+    fast_force<T>(cs, c, g.sx, g.sy, p0, u0, v0, w0, out ? q0 : nullptr, g.qat(j, k), fu, fv, fw);
+    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
+    u1[c] = cs.bx ? (T)0 : u0[c] + fu * cu;
+    v1[c] = cs.by ? (T)0 : v0[c] + fv * cu;
+    w1[c] = cs.bz ? (T)0 : w0[c] + fw * cu;
+}
+
+// Symplectic Jacobi, pressure pass: div of the new velocities (periodic +1
+// neighbours wrapped, wall halos frozen) and the old pressure.
+template <typename T, bool HAS_R>
+__global__ __launch_bounds__(256) void k_jac_pres(Grid g, const uint8_t *__restrict__ pc,
+                                                  const T *__restrict__ p0,
+                                                  const T *__restrict__ u1,
+                                                  const T *__restrict__ v1,
+                                                  const T *__restrict__ w1, T *__restrict__ p1,
+                                                  T *__restrict__ q1, const float *__restrict__ Rp,
+                                                  T cp, T pin_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int i = blockIdx.z + 2;
+    if (k >= g.nx + 2 || j >= g.ny + 2) return;
+    const int64_t c = g.at(i, j, k);
+    if (pc[c] == kSolid) return;
+    const int64_t cxp = (k == g.nx + 1 && g.px) ? g.at(i, j, 2) : c + 1;
+    const int64_t cyp = (j == g.ny + 1 && g.py) ? g.at(i, 2, k) : c + g.sx;
+    const int64_t czp = (i == g.nz + 1 && g.pz) ? g.at(2, j, k) : c + g.sy;
+    const T div0 = ((((w1[czp] - w1[c]) + v1[cyp]) - v1[c]) + u1[cxp]) - u1[c];
+    T div = div0;
+    if (HAS_R) div = div + (T)Rp[c];
+    p1[c] = p0[c] - div * cp;
+    if (i == g.nz + 1) q1[g.qat(j, k)] = pin_out - div0 * (T)(kC2 * kDt);
+}
+
+// 3-colour SOR, one colour stage in place.
+template <typename T, bool HAS_R>
+__global__ __launch_bounds__(256) void k_colour(Grid g, const uint8_t *__restrict__ pc, T *p,
+                                                T *u, T *v, T *w, T *q, int colour, T cu, T cp,
+                                                T pin_out, const float *__restrict__ Rp,
+                                                const float *__restrict__ Ru,
+                                                const float *__restrict__ Rv,
+                                                const float *__restrict__ Rw) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int i = blockIdx.z + 2;
+    if (k >= g.nx + 2 || j >= g.ny + 2) return;
+    if ((i + j + k) % 3 != colour) return;
+    const int64_t c = g.at(i, j, k);
+    const uint8_t code = pc[c];
+    if (code == kSolid) return;
+    const Cases cs = decode(code);
+    const bool out = (i == g.nz + 1);
+    T fu, fv, fw;
+    fast_force<T>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, g.qat(j, k), fu, fv, fw);
+    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
+    const T un = cs.bx ? (T)0 : u[c] + fu * cu;
+    const T vn = cs.by ? (T)0 : v[c] + fv * cu;
+    const T wn = cs.bz ? (T)0 : w[c] + fw * cu;
+    u[c] = un; v[c] = vn; w[c] = wn;
+    const T div0 = ((((w[c + g.sy] - wn) + v[c + g.sx]) - vn) + u[c + 1]) - un;
+    T div = div0;
+    if (HAS_R) div = div + (T)Rp[c];
+    p[c] = p[c] - div * cp;
+    if (out) q[g.qat(j, k)] = pin_out - div0 * (T)(kC2 * kDt);
+}
+
+// Fast-mode ghost fill in one pass over the face-halo lines: the outlet
+// plane's interior is pinned, periodic faces are copied (the outlet plane's
+// copies taken from the pinned values), then the inlet/top halo planes are
+// pinned.  Corner halo cells are never read by the axis-aligned stencil and
+// are left alone.
+template <typename T>
+__global__ void k_ghost_fast(Grid g, const uint8_t *__restrict__ G, T *p, T *u, T *v, T *w,
+                             T *q, T pin_in, T pin_hi, T pin_out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nX = (int64_t)g.nz * g.ny, nY = (int64_t)g.nz * g.nx, nZ = (int64_t)g.ny * g.nx;
+    const int iout = g.nz + 1;
+    if (t < nX) {
+        if (!g.px) return;
+        const int i = (int)(t / g.ny) + 2, j = (int)(t % g.ny) + 2;
+        const int64_t r = g.at(i, j, 0);
+        const int src[4] = {g.nx, g.nx + 1, 2, 3};
+        const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
+        for (int e = 0; e < 4; e++) {
+            const int64_t s = r + src[e], d = r + dst[e];
+            u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
+            p[d] = (i == iout && G[s] == 0) ? pin_out : p[s];
+            if (q && i == iout) q[g.qat(j, dst[e])] = q[g.qat(j, src[e])];
+        }
+    } else if (t < nX + nY) {
+        if (!g.py) return;
+        const int64_t tt = t - nX;
+        const int i = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
+        const int src[4] = {g.ny, g.ny + 1, 2, 3};
+        const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
+        for (int e = 0; e < 4; e++) {
+            const int64_t s = g.at(i, src[e], k), d = g.at(i, dst[e], k);
+            u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
+            p[d] = (i == iout && G[s] == 0) ? pin_out : p[s];
+            if (q && i == iout) q[g.qat(dst[e], k)] = q[g.qat(src[e], k)];
+        }
+    } else if (t < nX + nY + nZ) {
+        const int64_t tt = t - nX - nY;
+        const int j = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
+        const int64_t cout = g.at(iout, j, k);
+        const bool out_fluid = G[cout] == 0;
+        if (g.pz) {
+            const int src[4] = {g.nz, g.nz + 1, 2, 3};
+            const int dst[4] = {0, 1, g.nz + 2, g.nz + 3};
+            for (int e = 0; e < 4; e++) {
+                const int64_t s = g.at(src[e], j, k), d = g.at(dst[e], j, k);
+                u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
+                p[d] = (src[e] == iout && out_fluid) ? pin_out : p[s];
+            }
+        }
+        const int64_t c1 = g.at(1, j, k), chi = g.at(g.nz + 2, j, k);
+        if (G[c1] == 0) p[c1] = pin_in;
+        if (G[chi] == 0) p[chi] = pin_hi;
+        if (out_fluid) p[cout] = pin_out;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reductions (deterministic, compensated FP64)
+// ---------------------------------------------------------------------------
+
+struct NSum {
+    double s, c;
+};
+
+__device__ __forceinline__ void nadd(NSum &a, double x) {
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+    else a.c += (x - t) + a.s;
+    a.s = t;
+}
+__device__ __forceinline__ void nmerge(NSum &a, const NSum &b) {
+    nadd(a, b.s);
+    a.c += b.c;
+}
+
+struct RedAcc {
+    NSum s[8];         // u, v, w (fluid); u, v, w (box fluid); p(z=zmin), p(z=zmax)
+    long long n[4];    // fluid, box fluid, zmin fluid, zmax fluid
+    double wmax;
+};
+
+__device__ __forceinline__ void acc_zero(RedAcc &a) {
+    for (int q = 0; q < 8; q++) { a.s[q].s = 0.0; a.s[q].c = 0.0; }
+    for (int q = 0; q < 4; q++) a.n[q] = 0;
+    a.wmax = 0.0;
+}
+__device__ __forceinline__ void acc_merge(RedAcc &a, const RedAcc &b) {
+    for (int q = 0; q < 8; q++) nmerge(a.s[q], b.s[q]);
+    for (int q = 0; q < 4; q++) a.n[q] += b.n[q];
+    a.wmax = fmax(a.wmax, b.wmax);
+}
+
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 240;
+
+__device__ void block_reduce_store(RedAcc &a, RedAcc *out) {
+    __shared__ RedAcc sh[kRedThreads];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) acc_merge(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// FULL = false: only the sum of w and the fluid count (compute_convergence).
+template <bool FULL>
+__global__ __launch_bounds__(kRedThreads) void k_reduce_state(
+    Grid g, const uint8_t *__restrict__ G, const double *__restrict__ p,
+    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+    DomBox box, RedAcc *__restrict__ partial) {
+    RedAcc a;
+    acc_zero(a);
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nx);
+        const int64_t r = t / g.nx;
+        const int j = (int)(r % g.ny), i = (int)(r / g.ny);
+        const int64_t c = g.at(i + 2, j + 2, k + 2);
+        if (G[c] != 0) continue;
+        const double wc = w[c];
+        a.n[0] += 1;
+        nadd(a.s[2], wc);
+        if (FULL) {
+            const double uc = u[c], vc = v[c];
+            nadd(a.s[0], uc);
+            nadd(a.s[1], vc);
+            if (wc > a.wmax) a.wmax = wc;
+            if (box.on && k >= box.d[0] && k <= box.d[1] && j >= box.d[2] && j <= box.d[3] &&
+                i >= box.d[4] && i <= box.d[5]) {
+                a.n[1] += 1;
+                nadd(a.s[3], uc);
+                nadd(a.s[4], vc);
+                nadd(a.s[5], wc);
+                if (i == box.d[4]) { a.n[2] += 1; nadd(a.s[6], p[c]); }
+                if (i == box.d[5]) { a.n[3] += 1; nadd(a.s[7], p[c]); }
+            }
+        }
+    }
+    block_reduce_store(a, partial + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_reduce_final(const RedAcc *__restrict__ partial,
+                                                              int nparts, RedSums *__restrict__ out) {
+    RedAcc a;
+    acc_zero(a);
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) acc_merge(a, partial[b]);
+    __shared__ RedAcc res;
+    block_reduce_store(a, &res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 8; q++) out->sum[q] = res.s[q].s + res.s[q].c;
+        for (int q = 0; q < 4; q++) out->count[q] = res.n[q];
+        out->wmax = res.wmax;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// defect correction
+// ---------------------------------------------------------------------------
+
+// Outlet-plane buffer of a FP64 state: pin_out - c2*dt*div at interior fluid
+// cells of z = nz+1; elsewhere p at the (periodic image of the) cell.
+__global__ void k_outlet_plane(Grid g, const uint8_t *__restrict__ pc,
+                               const double *__restrict__ p, const double *__restrict__ u,
+                               const double *__restrict__ v, const double *__restrict__ w,
+                               double pin_out, double *__restrict__ q) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (k >= g.NX) return;
+    const int i = g.nz + 1;
+    int jj = j, kk = k;
+    if (g.px) { if (kk < 2) kk += g.nx; else if (kk >= g.nx + 2) kk -= g.nx; }
+    if (g.py) { if (jj < 2) jj += g.ny; else if (jj >= g.ny + 2) jj -= g.ny; }
+    const bool in = jj >= 2 && jj < g.ny + 2 && kk >= 2 && kk < g.nx + 2;
+    const int64_t c = g.at(i, jj, kk);
+    double val;
+    if (in && pc[c] != kSolid) {
+        const double div = ((((w[c + g.sy] - w[c]) + v[c + g.sx]) - v[c]) + u[c + 1]) - u[c];
+        val = pin_out - div * (kC2 * kDt);
+    } else {
+        val = p[c];
+    }
+    q[g.qat(j, k)] = val;
+}
+
+__global__ __launch_bounds__(256) void k_residual(Grid g, const uint8_t *__restrict__ pc,
+                                                  const double *__restrict__ p,
+                                                  const double *__restrict__ u,
+                                                  const double *__restrict__ v,
+                                                  const double *__restrict__ w,
+                                                  const double *__restrict__ q,
+                                                  float *__restrict__ Rp, float *__restrict__ Ru,
+                                                  float *__restrict__ Rv,
+                                                  float *__restrict__ Rw) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int i = blockIdx.z + 2;
+    if (k >= g.nx + 2 || j >= g.ny + 2) return;
+    const int64_t c = g.at(i, j, k);
+    const uint8_t code = pc[c];
+    if (code == kSolid) return;
+    const Cases cs = decode(code);
+    double fu, fv, fw;
+    fast_force<double>(cs, c, g.sx, g.sy, p, u, v, w, (i == g.nz + 1) ? q : nullptr, g.qat(j, k),
+                       fu, fv, fw);
+    Ru[c] = (float)fu; Rv[c] = (float)fv; Rw[c] = (float)fw;
+    const double div = ((((w[c + g.sy] - w[c]) + v[c + g.sx]) - v[c]) + u[c + 1]) - u[c];
+    Rp[c] = (float)div;
+}
+
+// X += d on the fluid interior, d = 0 everywhere.
+__global__ void k_fold(Grid g, const uint8_t *__restrict__ pc, double *Xp, double *Xu,
+                       double *Xv, double *Xw, float *dp, float *du, float *dv, float *dw) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.total) return;
+    const int64_t e = c - g.base;
+    bool in = false;
+    if (e >= 0) {
+        const int i = (int)(e / g.sy);
+        const int64_t r = e % g.sy;
+        const int j = (int)(r / g.sx), k = (int)(r % g.sx);
+        in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
+    }
+    if (in && pc[c] != kSolid) {
+        Xp[c] = Xp[c] + (double)dp[c];
+        Xu[c] = Xu[c] + (double)du[c];
+        Xv[c] = Xv[c] + (double)dv[c];
+        Xw[c] = Xw[c] + (double)dw[c];
+    }
+    dp[c] = 0.f; du[c] = 0.f; dv[c] = 0.f; dw[c] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static inline dim3 row_grid(const Grid &g) {
+    return dim3((unsigned)((g.NX + 63) / 64), (unsigned)g.NY, (unsigned)g.NZ);
+}
+static inline dim3 int_block() { return dim3(64, 4, 1); }
+static inline dim3 int_grid(const Grid &g) {
+    return dim3((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nz);
+}
+static inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_geom_fill(const Grid &g, const uint8_t *raw, uint8_t *G, hipStream_t s) {
+    hipLaunchKernelGGL(k_geom_fill, row_grid(g), dim3(64), 0, s, g, raw, G);
+}
+
+template <typename T>
+void launch_exchange_full(const Grid &g, T *f0, T *f1, T *f2, T *f3, hipStream_t s) {
+    if (g.px)
+        hipLaunchKernelGGL(k_exchange_axis<T>, dim3(nblocks((int64_t)g.NZ * g.NY, 256)), dim3(256), 0, s,
+                           g, 0, f0, f1, f2, f3);
+    if (g.py)
+        hipLaunchKernelGGL(k_exchange_axis<T>, dim3(nblocks((int64_t)g.NZ * g.NX, 256)), dim3(256), 0, s,
+                           g, 1, f0, f1, f2, f3);
+    if (g.pz)
+        hipLaunchKernelGGL(k_exchange_axis<T>, dim3(nblocks((int64_t)g.NY * g.NX, 256)), dim3(256), 0, s,
+                           g, 2, f0, f1, f2, f3);
+}
+template void launch_exchange_full<uint8_t>(const Grid &, uint8_t *, uint8_t *, uint8_t *, uint8_t *, hipStream_t);
+template void launch_exchange_full<double>(const Grid &, double *, double *, double *, double *, hipStream_t);
+template void launch_exchange_full<float>(const Grid &, float *, float *, float *, float *, hipStream_t);
+
+void launch_eval_geometry(const Grid &g, const uint8_t *G, int32_t *code, uint8_t *pc, hipStream_t s) {
+    hipLaunchKernelGGL(k_eval_geometry, row_grid(g), dim3(64), 0, s, g, G, code, pc);
+}
+
+void launch_init_fields(const Grid &g, const uint8_t *G, double *p, double *u, double *v, double *w,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_init_fields, row_grid(g), dim3(64), 0, s, g, G, p, u, v, w);
+    launch_exchange_full<double>(g, p, nullptr, nullptr, nullptr, s);
+}
+
+template <typename T>
+void launch_pins(const Grid &g, const uint8_t *G, T *p, T pin_in, T pin_hi, T pin_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_pins<T>, dim3(nblocks(g.nx, 64), (unsigned)g.ny), dim3(64), 0, s, g, G, p, pin_in,
+                       pin_hi, pin_out);
+}
+template void launch_pins<double>(const Grid &, const uint8_t *, double *, double, double, double, hipStream_t);
+template void launch_pins<float>(const Grid &, const uint8_t *, float *, float, float, float, hipStream_t);
+
+void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
+                       double *v, double *w, int solver, int nsweeps, bool one_wg, hipStream_t s) {
+    if (nsweeps <= 0) return;
+    if (one_wg) {
+        hipLaunchKernelGGL(k_ref_sweeps_1wg, dim3(1), dim3(1024), 0, s, g, G, code, p, u, v, w, solver,
+                           nsweeps);
+        return;
+    }
+    const int smax = (g.nx - 1) + (g.ny - 1) + (g.nz - 1);
+    for (int it = 0; it < nsweeps; it++) {
+        for (int sp = 0; sp <= smax; sp++) {
+            const int a_lo = std::max(0, sp - (g.ny - 1) - (g.nx - 1));
+            const int a_hi = std::min(g.nz - 1, sp);
+            const int64_t n = (int64_t)(a_hi - a_lo + 1) * g.ny;
+            hipLaunchKernelGGL(k_ref_hyperplane, dim3(nblocks(n, 256)), dim3(256), 0, s, g, G, code, p, u, v,
+                               w, solver, sp);
+        }
+        launch_exchange_full<double>(g, p, u, v, w, s);
+        launch_pins<double>(g, G, p, (double)(g.nz + 2), 2.0, 3.0, s);
+    }
+}
+
+template <typename T>
+void launch_jacobi(const Grid &g, const uint8_t *pc, const T *p0, const T *u0, const T *v0,
+                   const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu, T cp,
+                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+                   hipStream_t s) {
+    if (Rp) {
+        hipLaunchKernelGGL((k_jac_vel<T, true>), int_grid(g), int_block(), 0, s, g, pc, p0, u0, v0, w0, q0,
+                           u1, v1, w1, Ru, Rv, Rw, cu);
+        hipLaunchKernelGGL((k_jac_pres<T, true>), int_grid(g), int_block(), 0, s, g, pc, p0, u1, v1, w1,
+                           p1, q1, Rp, cp, pin_out);
+    } else {
+        hipLaunchKernelGGL((k_jac_vel<T, false>), int_grid(g), int_block(), 0, s, g, pc, p0, u0, v0, w0,
+                           q0, u1, v1, w1, Ru, Rv, Rw, cu);
+        hipLaunchKernelGGL((k_jac_pres<T, false>), int_grid(g), int_block(), 0, s, g, pc, p0, u1, v1, w1,
+                           p1, q1, Rp, cp, pin_out);
+    }
+}
+template void launch_jacobi<double>(const Grid &, const uint8_t *, const double *, const double *,
+                                    const double *, const double *, const double *, double *, double *,
+                                    double *, double *, double *, double, double, double, const float *,
+                                    const float *, const float *, const float *, hipStream_t);
+template void launch_jacobi<float>(const Grid &, const uint8_t *, const float *, const float *,
+                                   const float *, const float *, const float *, float *, float *, float *,
+                                   float *, float *, float, float, float, const float *, const float *,
+                                   const float *, const float *, hipStream_t);
+
+template <typename T>
+void launch_colour(const Grid &g, const uint8_t *pc, T *p, T *u, T *v, T *w, T *q, T cu, T cp,
+                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+                   hipStream_t s) {
+    for (int col = 0; col < 3; col++) {
+        if (Rp)
+            hipLaunchKernelGGL((k_colour<T, true>), int_grid(g), int_block(), 0, s, g, pc, p, u, v, w, q, col,
+                               cu, cp, pin_out, Rp, Ru, Rv, Rw);
+        else
+            hipLaunchKernelGGL((k_colour<T, false>), int_grid(g), int_block(), 0, s, g, pc, p, u, v, w, q,
+                               col, cu, cp, pin_out, Rp, Ru, Rv, Rw);
+    }
+}
+template void launch_colour<double>(const Grid &, const uint8_t *, double *, double *, double *, double *,
+                                    double *, double, double, double, const float *, const float *,
+                                    const float *, const float *, hipStream_t);
+template void launch_colour<float>(const Grid &, const uint8_t *, float *, float *, float *, float *, float *,
+                                   float, float, float, const float *, const float *, const float *,
+                                   const float *, hipStream_t);
+
+template <typename T>
+void launch_ghost_fast(const Grid &g, const uint8_t *G, T *p, T *u, T *v, T *w, T *q, T pin_in, T pin_hi,
+                       T pin_out, hipStream_t s) {
+    const int64_t n = (int64_t)g.nz * g.ny + (int64_t)g.nz * g.nx + (int64_t)g.ny * g.nx;
+    hipLaunchKernelGGL(k_ghost_fast<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, g, G, p, u, v, w, q, pin_in,
+                       pin_hi, pin_out);
+}
+template void launch_ghost_fast<double>(const Grid &, const uint8_t *, double *, double *, double *, double *,
+                                        double *, double, double, double, hipStream_t);
+template void launch_ghost_fast<float>(const Grid &, const uint8_t *, float *, float *, float *, float *,
+                                       float *, float, float, float, hipStream_t);
+
+void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const double *u, const double *v,
+                   const double *w, const DomBox &box, bool full, void *scratch, RedSums *out,
+                   hipStream_t s) {
+    RedAcc *part = (RedAcc *)scratch;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
+    if (full)
+        hipLaunchKernelGGL(k_reduce_state<true>, dim3(nb), dim3(kRedThreads), 0, s, g, G, p, u, v, w, box, part);
+    else
+        hipLaunchKernelGGL(k_reduce_state<false>, dim3(nb), dim3(kRedThreads), 0, s, g, G, p, u, v, w, box, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, s, part, nb, out);
+}
+
+size_t reduce_scratch_bytes() { return sizeof(RedAcc) * kRedBlocks; }
+
+void launch_outlet_plane(const Grid &g, const uint8_t *pc, const double *p, const double *u,
+                         const double *v, const double *w, double pin_out, double *q, hipStream_t s) {
+    hipLaunchKernelGGL(k_outlet_plane, dim3(nblocks(g.NX, 64), (unsigned)g.NY), dim3(64), 0, s, g, pc, p, u,
+                       v, w, pin_out, q);
+}
+
+void launch_residual(const Grid &g, const uint8_t *pc, const double *p, const double *u, const double *v,
+                     const double *w, const double *q, float *Rp, float *Ru, float *Rv, float *Rw,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_residual, int_grid(g), int_block(), 0, s, g, pc, p, u, v, w, q, Rp, Ru, Rv, Rw);
+}
+
+void launch_fold(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw,
+                 float *dp, float *du, float *dv, float *dw, hipStream_t s) {
+    hipLaunchKernelGGL(k_fold, dim3(nblocks(g.total, 256)), dim3(256), 0, s, g, pc, Xp, Xu, Xv, Xw, dp, du,
+                       dv, dw);
+}
+
+}  // namespace pm

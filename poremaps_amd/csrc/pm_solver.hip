@@ -1,0 +1,685 @@
+// pm_solver.hip -- host side of the solver: context, device memory, the
+// iteration loop of src/calc_flow.cc:53-169 and the C ABI of poremaps_amd.h.
+#include "../../include/poremaps_amd.h"
+#include "pm_kernels.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(PM_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),      \
+                        __FILE__, __LINE__);                                                  \
+    } while (0)
+
+using clk = std::chrono::steady_clock;
+
+__global__ void k_fill_f32(float *a, int64_t n, float v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+}  // namespace
+
+// Event pairs around kernel groups (pm_time_kernels); inactive when null.
+struct Timer {
+    struct Rec { int cat; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    int open_cat = -1;
+    hipEvent_t open_ev = nullptr;
+};
+
+enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_N };
+static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep"};
+
+struct pm_ctx {
+    Timer *timer = nullptr;
+    pm_params prm;
+    pm::Grid g;
+    hipStream_t stream = nullptr;
+    bool loaded = false;
+    // geometry
+    uint8_t *G = nullptr, *pc = nullptr;
+    int32_t *code = nullptr;
+    // FP64 state (reference mode, FP64 fast modes, defect-correction master)
+    double *X[4] = {nullptr, nullptr, nullptr, nullptr};   // p, u, v, w
+    double *X2[4] = {nullptr, nullptr, nullptr, nullptr};  // Jacobi second buffer
+    double *qX[2] = {nullptr, nullptr};
+    // FP32 correction state
+    float *D[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *D2[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *qD[2] = {nullptr, nullptr};
+    float *R[4] = {nullptr, nullptr, nullptr, nullptr};
+    // reductions
+    void *red_scratch = nullptr;
+    pm::RedSums *red_dev = nullptr;
+    pm::RedSums *red_host = nullptr;
+    // iteration state
+    int t = 0;            // sweeps done
+    double kprev = 0.0;   // permeability of the previous convergence test
+    double phi = 0.0;
+    int64_t fluid = 0;
+
+    bool fast() const { return prm.mode != PM_MODE_REFERENCE; }
+    bool dc() const { return fast() && prm.precision == PM_PREC_F32DC; }
+    bool jacobi() const { return prm.mode == PM_MODE_JACOBI; }
+};
+
+extern "C" const char *pm_last_error(void) { return g_err.c_str(); }
+
+extern "C" void pm_params_default(pm_params *p) {
+    std::memset(p, 0, sizeof *p);
+    p->boundary_method = 0;
+    p->voxel_size = 1.0;
+    p->max_iter = 100000000;
+    p->it_eval = 100;
+    p->it_write = 100;
+    p->solving_algorithm = 2;
+    p->eps = 1e-6;
+    p->porosity = -1.0;
+    p->mode = PM_MODE_REFERENCE;
+    p->precision = PM_PREC_F64;
+    p->omega = 1.0;
+    p->device = 0;
+}
+
+static int check_params(const pm_params *p) {
+    if (!p) return fail(PM_E_ARG, "null parameters");
+    if (p->nx < 2 || p->ny < 2 || p->nz < 2)
+        return fail(PM_E_ARG, "size_x_y_z must be >= 2 on every axis (got %d %d %d)", p->nx, p->ny, p->nz);
+    if ((int64_t)p->nx * p->ny * p->nz > (int64_t)1 << 31)
+        return fail(PM_E_ARG, "domain of %d x %d x %d exceeds 2^31 voxels", p->nx, p->ny, p->nz);
+    if (p->boundary_method < 0 || p->boundary_method > 4)
+        return fail(PM_E_ARG, "boundary_method must be 0..4 (got %d)", p->boundary_method);
+    if (p->it_eval <= 0 || p->it_write <= 0)
+        return fail(PM_E_ARG, "it_eval and it_write must be positive");
+    if (p->mode < PM_MODE_REFERENCE || p->mode > PM_MODE_SOR3)
+        return fail(PM_E_ARG, "unknown mode %d", p->mode);
+    if (p->precision != PM_PREC_F64 && p->precision != PM_PREC_F32DC)
+        return fail(PM_E_ARG, "unknown precision %d", p->precision);
+    if (p->mode == PM_MODE_REFERENCE && p->precision != PM_PREC_F64)
+        return fail(PM_E_ARG, "the reference ordering runs in FP64 only");
+    if (p->mode != PM_MODE_REFERENCE && !(p->omega > 0.0 && p->omega < 100.0))
+        return fail(PM_E_ARG, "omega must be in (0, 100) (got %g)", p->omega);
+    return PM_OK;
+}
+
+extern "C" int64_t pm_device_bytes(const pm_params *p) {
+    if (check_params(p) != PM_OK) return -1;
+    pm::Grid g = pm::make_grid(p->nx, p->ny, p->nz, p->boundary_method);
+    int64_t per = 1 + 1 + 4 + 32;  // G, pc, code, X
+    const bool fast = p->mode != PM_MODE_REFERENCE;
+    const bool dc = fast && p->precision == PM_PREC_F32DC;
+    const bool jac = p->mode == PM_MODE_JACOBI;
+    if (fast && !dc && jac) per += 32;
+    if (dc) per += 16 + 16 + (jac ? 16 : 0);
+    return per * g.total + 4 * g.qtotal() * 8 + (int64_t)pm::reduce_scratch_bytes() + (1 << 20);
+}
+
+static void free_all(pm_ctx *c) {
+    auto f = [](void *p) { if (p) (void)hipFree(p); };
+    f(c->G); f(c->pc); f(c->code);
+    for (int q = 0; q < 4; q++) { f(c->X[q]); f(c->X2[q]); f(c->D[q]); f(c->D2[q]); f(c->R[q]); }
+    for (int q = 0; q < 2; q++) { f(c->qX[q]); f(c->qD[q]); }
+    f(c->red_scratch); f(c->red_dev);
+    if (c->red_host) (void)hipHostFree(c->red_host);
+    c->G = c->pc = nullptr; c->code = nullptr;
+    for (int q = 0; q < 4; q++) c->X[q] = c->X2[q] = nullptr, c->D[q] = c->D2[q] = c->R[q] = nullptr;
+    c->qX[0] = c->qX[1] = nullptr; c->qD[0] = c->qD[1] = nullptr;
+    c->red_scratch = nullptr; c->red_dev = nullptr; c->red_host = nullptr;
+}
+
+extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
+    if (!out) return fail(PM_E_ARG, "null output pointer");
+    *out = nullptr;
+    int rc = check_params(p);
+    if (rc != PM_OK) return rc;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (p->device < 0 || p->device >= ndev)
+        return fail(PM_E_ARG, "device %d not present (%d devices)", p->device, ndev);
+    HIPCHK(hipSetDevice(p->device));
+    pm_ctx *c = new (std::nothrow) pm_ctx;
+    if (!c) return fail(PM_E_MEMORY, "host allocation failed");
+    c->prm = *p;
+    c->g = pm::make_grid(p->nx, p->ny, p->nz, p->boundary_method);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PM_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return PM_OK;
+}
+
+extern "C" void pm_destroy(pm_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->prm.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+template <typename T>
+static int dalloc(T **p, int64_t n) {
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(PM_E_MEMORY, "hipMalloc of %lld bytes: %s", (long long)(sizeof(T) * n),
+                    hipGetErrorString(e));
+    }
+    return PM_OK;
+}
+
+#define RC(expr)                      \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != PM_OK) return rc_; \
+    } while (0)
+
+static int fill_f32(pm_ctx *c, float *a, int64_t n, float v) {
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a, n, v);
+    HIPCHK(hipGetLastError());
+    return PM_OK;
+}
+
+static void tb(pm_ctx *c, int cat) {
+    if (!c->timer) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, c->stream);
+    c->timer->open_cat = cat;
+    c->timer->open_ev = e;
+}
+static void te(pm_ctx *c) {
+    if (!c->timer || c->timer->open_cat < 0) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, c->stream);
+    c->timer->recs.push_back({c->timer->open_cat, c->timer->open_ev, e});
+    c->timer->open_cat = -1;
+}
+
+// Outlet-plane buffer, residual: the start of every defect-correction cycle.
+static int dc_prepare(pm_ctx *c) {
+    const pm::Grid &g = c->g;
+    tb(c, TC_RESIDUAL);
+    pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
+    pm::launch_residual(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R[0], c->R[1], c->R[2],
+                        c->R[3], c->stream);
+    te(c);
+    RC(fill_f32(c, c->qD[0], g.qtotal(), 0.f));
+    RC(fill_f32(c, c->qD[1], g.qtotal(), 0.f));
+    HIPCHK(hipGetLastError());
+    return PM_OK;
+}
+
+extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
+    if (!c || !raw) return fail(PM_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->prm.device));
+    const pm::Grid &g = c->g;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    int64_t fluid = 0;
+    for (int64_t q = 0; q < n; q++) {
+        if (raw[q] > 1)
+            return fail(PM_E_GEOMETRY, "geometry byte %lld is %u; only 0 (fluid) and 1 (solid) are valid",
+                        (long long)q, (unsigned)raw[q]);
+        fluid += raw[q] == 0;
+    }
+    free_all(c);
+    c->loaded = false;
+    RC(dalloc(&c->G, g.total));
+    RC(dalloc(&c->pc, g.total));
+    RC(dalloc(&c->code, g.total));
+    for (int q = 0; q < 4; q++) RC(dalloc(&c->X[q], g.total));
+    for (int q = 0; q < 2; q++) RC(dalloc(&c->qX[q], g.qtotal()));
+    if (c->fast() && !c->dc() && c->jacobi())
+        for (int q = 0; q < 4; q++) RC(dalloc(&c->X2[q], g.total));
+    if (c->dc()) {
+        for (int q = 0; q < 4; q++) RC(dalloc(&c->D[q], g.total));
+        for (int q = 0; q < 4; q++) RC(dalloc(&c->R[q], g.total));
+        if (c->jacobi())
+            for (int q = 0; q < 4; q++) RC(dalloc(&c->D2[q], g.total));
+        for (int q = 0; q < 2; q++) RC(dalloc(&c->qD[q], g.qtotal()));
+    }
+    RC(dalloc((uint8_t **)&c->red_scratch, (int64_t)pm::reduce_scratch_bytes()));
+    RC(dalloc(&c->red_dev, 1));
+    HIPCHK(hipHostMalloc((void **)&c->red_host, sizeof(pm::RedSums), hipHostMallocDefault));
+
+    uint8_t *raw_dev = nullptr;
+    RC(dalloc(&raw_dev, n));
+    hipError_t e = hipMemcpyAsync(raw_dev, raw, (size_t)n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->G, 0, (size_t)g.total, c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(raw_dev);
+        return fail(PM_E_HIP, "geometry upload: %s", hipGetErrorString(e));
+    }
+    pm::launch_geom_fill(g, raw_dev, c->G, c->stream);
+    pm::launch_exchange_full<uint8_t>(g, c->G, nullptr, nullptr, nullptr, c->stream);
+    HIPCHK(hipMemsetAsync(c->pc, 0xff, (size_t)g.total, c->stream));
+    HIPCHK(hipMemsetAsync(c->code, 0, sizeof(int32_t) * (size_t)g.total, c->stream));
+    pm::launch_eval_geometry(g, c->G, c->code, c->pc, c->stream);
+    for (int q = 0; q < 4; q++) HIPCHK(hipMemsetAsync(c->X[q], 0, sizeof(double) * (size_t)g.total, c->stream));
+    pm::launch_init_fields(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->stream);
+    // src/calc_flow.cc:105-108: the pins are applied before sweep 0
+    pm::launch_pins<double>(g, c->G, c->X[0], (double)(g.nz + 2), 2.0, 3.0, c->stream);
+    if (c->fast()) {
+        pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
+        HIPCHK(hipMemcpyAsync(c->qX[1], c->qX[0], sizeof(double) * (size_t)g.qtotal(),
+                              hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (c->X2[0])
+        for (int q = 0; q < 4; q++)
+            HIPCHK(hipMemcpyAsync(c->X2[q], c->X[q], sizeof(double) * (size_t)g.total, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    if (c->dc()) {
+        for (int q = 0; q < 4; q++) {
+            RC(fill_f32(c, c->D[q], g.total, 0.f));
+            RC(fill_f32(c, c->R[q], g.total, 0.f));
+            if (c->D2[q]) RC(fill_f32(c, c->D2[q], g.total, 0.f));
+        }
+        RC(dc_prepare(c));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(raw_dev);
+    c->fluid = fluid;
+    // src/geometry.cc:335-364: fluid_count / vox_count in double
+    c->phi = (double)fluid / (double)(g.nx * g.ny * g.nz);
+    c->t = 0;
+    c->kprev = 0.0;
+    c->loaded = true;
+    return PM_OK;
+}
+
+// One sweep of the context's ordering, ghost fill and pins included; for the
+// defect-correction mode also the fold when the sweep index hits it_eval.
+static int one_sweep(pm_ctx *c) {
+    const pm::Grid &g = c->g;
+    hipStream_t s = c->stream;
+    const double om = c->prm.omega;
+    const double cu = om * pm::kDt, cp = om * pm::kC2 * pm::kDt;
+    if (!c->fast()) {
+        const bool one_wg = (int64_t)g.nx * g.ny * g.nz <= 400000;
+        tb(c, TC_REF);
+        pm::launch_ref_sweeps(g, c->G, c->code, c->X[0], c->X[1], c->X[2], c->X[3], c->prm.solving_algorithm,
+                              1, one_wg, s);
+        te(c);
+    } else if (!c->dc()) {
+        const double pin_in = (double)(g.nz + 2);
+        if (c->jacobi()) {
+            tb(c, TC_SWEEP);
+            pm::launch_jacobi<double>(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->X2[0], c->X2[1],
+                                      c->X2[2], c->X2[3], c->qX[1], cu, cp, 3.0, nullptr, nullptr, nullptr,
+                                      nullptr, s);
+            te(c);
+            tb(c, TC_GHOST);
+            pm::launch_ghost_fast<double>(g, c->G, c->X2[0], c->X2[1], c->X2[2], c->X2[3], c->qX[1], pin_in, 2.0,
+                                          3.0, s);
+            te(c);
+            for (int q = 0; q < 4; q++) std::swap(c->X[q], c->X2[q]);
+            std::swap(c->qX[0], c->qX[1]);
+        } else {
+            tb(c, TC_SWEEP);
+            pm::launch_colour<double>(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0,
+                                      nullptr, nullptr, nullptr, nullptr, s);
+            te(c);
+            tb(c, TC_GHOST);
+            pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], pin_in, 2.0,
+                                          3.0, s);
+            te(c);
+        }
+    } else {
+        const float fcu = (float)cu, fcp = (float)cp;
+        if (c->jacobi()) {
+            tb(c, TC_SWEEP);
+            pm::launch_jacobi<float>(g, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], c->D2[0], c->D2[1],
+                                     c->D2[2], c->D2[3], c->qD[1], fcu, fcp, 0.f, c->R[0], c->R[1], c->R[2],
+                                     c->R[3], s);
+            te(c);
+            tb(c, TC_GHOST);
+            pm::launch_ghost_fast<float>(g, c->G, c->D2[0], c->D2[1], c->D2[2], c->D2[3], c->qD[1], 0.f, 0.f, 0.f,
+                                         s);
+            te(c);
+            for (int q = 0; q < 4; q++) std::swap(c->D[q], c->D2[q]);
+            std::swap(c->qD[0], c->qD[1]);
+        } else {
+            tb(c, TC_SWEEP);
+            pm::launch_colour<float>(g, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], fcu, fcp, 0.f,
+                                     c->R[0], c->R[1], c->R[2], c->R[3], s);
+            te(c);
+            tb(c, TC_GHOST);
+            pm::launch_ghost_fast<float>(g, c->G, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], 0.f, 0.f, 0.f, s);
+            te(c);
+        }
+        if (c->t % c->prm.it_eval == 0) {
+            tb(c, TC_FOLD);
+            pm::launch_fold(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->D[0], c->D[1], c->D[2], c->D[3], s);
+            pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr,
+                                          (double)(g.nz + 2), 2.0, 3.0, s);
+            te(c);
+            RC(dc_prepare(c));
+        }
+    }
+    HIPCHK(hipGetLastError());
+    c->t++;
+    return PM_OK;
+}
+
+// Fold the outstanding correction into the FP64 state (defect-correction
+// mode), so that X is the solution state.
+static int dc_flush(pm_ctx *c) {
+    if (!c->dc()) return PM_OK;
+    const pm::Grid &g = c->g;
+    pm::launch_fold(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->D[0], c->D[1], c->D[2], c->D[3], c->stream);
+    pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr, (double)(g.nz + 2), 2.0,
+                                  3.0, c->stream);
+    RC(dc_prepare(c));
+    HIPCHK(hipGetLastError());
+    return PM_OK;
+}
+
+static int reduce(pm_ctx *c, bool full, pm::RedSums *out) {
+    pm::DomBox box;
+    const int *d = c->prm.dom_interest;
+    box.on = !(d[0] == 0 && d[1] == 0 && d[2] == 0 && d[3] == 0 && d[4] == 0 && d[5] == 0);
+    for (int q = 0; q < 6; q++) box.d[q] = d[q];
+    pm::launch_reduce(c->g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], box, full, c->red_scratch, c->red_dev,
+                      c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->red_host, c->red_dev, sizeof(pm::RedSums), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = *c->red_host;
+    return PM_OK;
+}
+
+// src/evaluation.cc:216-262 (one rank).
+static double convergence_from(const pm::RedSums &r, double *kprev) {
+    double velz_sum = r.sum[2];
+    int flcount = (int)r.count[0];
+    if (flcount == 0) velz_sum = 0.0;
+    const double mean_velz = velz_sum / flcount;
+    const double kcur = mean_velz / pm::kRe;
+    const double conv = std::fabs(kcur - *kprev) / kcur;
+    *kprev = kcur;
+    return conv;
+}
+
+// src/evaluation.cc:54-211 (one rank); out = k13 k23 k33 wk13 wk23 wk33 wmax wmean.
+static void permeability_from(const pm::RedSums &r, const int *dom, double dx, double phi, double *out) {
+    const bool dofi = !(dom[0] == 0 && dom[1] == 0 && dom[2] == 0 && dom[3] == 0 && dom[4] == 0 && dom[5] == 0);
+    double wsx = r.sum[0], wsy = r.sum[1], wsz = r.sum[2], wmax = r.wmax;
+    int wfl = (int)r.count[0];
+    double sx = r.sum[3], sy = r.sum[4], sz = r.sum[5], p0 = r.sum[6], pend = r.sum[7];
+    int fl = dofi ? (int)r.count[1] : 0, p0n = (int)r.count[2], pendn = (int)r.count[3];
+    if (wfl == 0) { wsx = 0.0; wsy = 0.0; wsz = 0.0; wmax = 0.0; }
+    if (fl == 0) { sx = 0.0; sy = 0.0; sz = 0.0; }
+    if (p0n == 0) p0 = 0.0;
+    if (pendn == 0) pend = 0.0;
+    const double dx2 = dx * dx;
+    if (dofi) {
+        const double pgrad = ((pend / pendn) - (p0 / p0n)) / (dom[5] - dom[4]);
+        out[0] = (sx / fl) * (1.0 / pm::kRe) * phi * (1.0 / pgrad) * dx2;
+        out[1] = (sy / fl) * (1.0 / pm::kRe) * phi * (1.0 / pgrad) * dx2;
+        out[2] = (sz / fl) * (1.0 / pm::kRe) * phi * (1.0 / pgrad) * dx2;
+    } else {
+        out[0] = out[1] = out[2] = 0.0;
+    }
+    out[3] = (wsx / wfl) * (1.0 / pm::kRe) * phi * dx2;
+    out[4] = (wsy / wfl) * (1.0 / pm::kRe) * phi * dx2;
+    out[5] = (wsz / wfl) * (1.0 / pm::kRe) * phi * dx2;
+    out[6] = wmax * dx;
+    out[7] = wsz / wfl * dx;
+}
+
+static double used_porosity(const pm_ctx *c) {
+    // src/calc_flow.cc:92-97
+    return c->prm.porosity == -1.0 ? c->phi : c->prm.porosity;
+}
+
+static void fill_row(pm_log_row *row, int t, double conv, double tps, const double *k) {
+    row->iteration = t;
+    row->conv = conv;
+    row->tps = tps;
+    row->wmax_velz = k[6];
+    row->wmean_velz = k[7];
+    row->k13 = k[0]; row->k23 = k[1]; row->k33 = k[2];
+    row->wk13 = k[3]; row->wk23 = k[4]; row->wk33 = k[5];
+}
+
+extern "C" int pm_solve(pm_ctx *c, pm_log_fn log_fn, pm_progress_fn progress_fn, void *user, pm_result *res) {
+    if (!c) return fail(PM_E_ARG, "null context");
+    if (!c->loaded) return fail(PM_E_STATE, "pm_solve before pm_load_geometry");
+    HIPCHK(hipSetDevice(c->prm.device));
+    const pm_params &P = c->prm;
+    const double phi = used_porosity(c);
+    double conv = 1.0 + P.eps;
+    int nrows = 0;
+    pm_log_row last;
+    std::memset(&last, 0, sizeof last);
+    int status = PM_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const auto t_start = clk::now();
+    auto t_mark = t_start;
+    int t_mark_it = c->t;
+    // src/calc_flow.cc:101: while (iteration < max_iterations && perm_conv >= eps)
+    while (c->t < P.max_iter && conv >= P.eps) {
+        // run sweeps up to the next iteration that needs the host
+        int stop = c->t;
+        while (stop + 1 < P.max_iter) {
+            const bool eval = (stop % P.it_eval == 0 && stop > 1000);
+            const bool prog = progress_fn && stop % 100 == 0;
+            const bool fold = c->dc() && stop % P.it_eval == 0;
+            if (eval || prog || fold) break;
+            stop++;
+        }
+        while (c->t <= stop) RC(one_sweep(c));
+        const int it = stop;  // the iteration index whose end we are at
+        if (it % P.it_eval == 0 && it > 1000) {
+            pm::RedSums r;
+            RC(reduce(c, false, &r));
+            conv = convergence_from(r, &c->kprev);
+            if (it % P.it_write == 0) {
+                RC(reduce(c, true, &r));
+                double k[8];
+                permeability_from(r, P.dom_interest, P.voxel_size, phi, k);
+                const auto now = clk::now();
+                const double dt = std::chrono::duration<double>(now - t_mark).count();
+                const int nit = std::max(1, c->t - t_mark_it);
+                fill_row(&last, it, conv, nit / std::max(dt, 1e-12), k);
+                nrows++;
+                if (log_fn && log_fn(&last, user) != 0) { status = fail(PM_E_ABORTED, "log callback stopped the solve"); break; }
+            }
+        }
+        if (progress_fn && it % 100 == 0) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            const auto now = clk::now();
+            const double dt = std::chrono::duration<double>(now - t_mark).count();
+            const int nit = std::max(1, c->t - t_mark_it);
+            t_mark = now;
+            t_mark_it = c->t;
+            if (progress_fn(it, conv, dt / nit, user) != 0) { status = fail(PM_E_ABORTED, "progress callback stopped the solve"); break; }
+        }
+        if (!std::isfinite(conv)) break;
+    }
+    RC(dc_flush(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double secs = std::chrono::duration<double>(clk::now() - t_start).count();
+    if (res) {
+        pm::RedSums r;
+        RC(reduce(c, true, &r));
+        double k[8];
+        permeability_from(r, P.dom_interest, P.voxel_size, phi, k);
+        std::memset(res, 0, sizeof *res);
+        res->sweeps = c->t;
+        res->conv = conv;
+        res->porosity = phi;
+        res->seconds = secs;
+        res->glups = (double)c->g.nx * c->g.ny * c->g.nz * c->t / std::max(secs, 1e-12) / 1e9;
+        res->n_log_rows = nrows;
+        fill_row(&res->last, c->t - 1, conv, 0.0, k);
+    }
+    if (status != PM_OK) return status;
+    if (!std::isfinite(conv)) return fail(PM_E_DIVERGED, "convergence value is %g at iteration %d", conv, c->t - 1);
+    return PM_OK;
+}
+
+extern "C" int pm_step(pm_ctx *c, int n) {
+    if (!c) return fail(PM_E_ARG, "null context");
+    if (!c->loaded) return fail(PM_E_STATE, "pm_step before pm_load_geometry");
+    if (n < 0) return fail(PM_E_ARG, "negative sweep count");
+    HIPCHK(hipSetDevice(c->prm.device));
+    for (int q = 0; q < n; q++) RC(one_sweep(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
+static int copy_field(pm_ctx *c, int field, std::vector<double> &host) {
+    const pm::Grid &g = c->g;
+    host.resize((size_t)g.total);
+    double *src = nullptr;
+    if (field >= PM_FIELD_VELX && field <= PM_FIELD_VELZ) src = c->X[1 + field - PM_FIELD_VELX];
+    else if (field == PM_FIELD_PRESS) src = c->X[0];
+    else return fail(PM_E_ARG, "unknown field %d", field);
+    HIPCHK(hipMemcpyAsync(host.data(), src, sizeof(double) * (size_t)g.total, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
+extern "C" int pm_get_field(pm_ctx *c, int field, void *out) {
+    if (!c || !out) return fail(PM_E_ARG, "null argument");
+    if (!c->loaded) return fail(PM_E_STATE, "no geometry loaded");
+    HIPCHK(hipSetDevice(c->prm.device));
+    const pm::Grid &g = c->g;
+    if (field == PM_FIELD_NHOOD) {
+        std::vector<int32_t> h((size_t)g.total);
+        HIPCHK(hipMemcpyAsync(h.data(), c->code, sizeof(int32_t) * (size_t)g.total, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int32_t *o = (int32_t *)out;
+        for (int i = 0; i < g.nz; i++)
+            for (int j = 0; j < g.ny; j++)
+                std::memcpy(o + ((int64_t)i * g.ny + j) * g.nx, h.data() + g.at(i + 2, j + 2, 2),
+                            sizeof(int32_t) * g.nx);
+        return PM_OK;
+    }
+    std::vector<double> h;
+    RC(copy_field(c, field, h));
+    double *o = (double *)out;
+    for (int i = 0; i < g.nz; i++)
+        for (int j = 0; j < g.ny; j++)
+            std::memcpy(o + ((int64_t)i * g.ny + j) * g.nx, h.data() + g.at(i + 2, j + 2, 2), sizeof(double) * g.nx);
+    return PM_OK;
+}
+
+extern "C" int pm_get_field_halo(pm_ctx *c, int field, double *out) {
+    if (!c || !out) return fail(PM_E_ARG, "null argument");
+    if (!c->loaded) return fail(PM_E_STATE, "no geometry loaded");
+    HIPCHK(hipSetDevice(c->prm.device));
+    const pm::Grid &g = c->g;
+    std::vector<double> h;
+    RC(copy_field(c, field, h));
+    for (int i = 0; i < g.NZ; i++)
+        for (int j = 0; j < g.NY; j++)
+            std::memcpy(out + ((int64_t)i * g.NY + j) * g.NX, h.data() + g.at(i, j, 0), sizeof(double) * g.NX);
+    return PM_OK;
+}
+
+extern "C" int pm_porosity(pm_ctx *c, double *phi) {
+    if (!c || !phi) return fail(PM_E_ARG, "null argument");
+    if (!c->loaded) return fail(PM_E_STATE, "no geometry loaded");
+    *phi = c->phi;
+    return PM_OK;
+}
+
+extern "C" int pm_permeability(pm_ctx *c, double out[8]) {
+    if (!c || !out) return fail(PM_E_ARG, "null argument");
+    if (!c->loaded) return fail(PM_E_STATE, "no geometry loaded");
+    HIPCHK(hipSetDevice(c->prm.device));
+    pm::RedSums r;
+    RC(reduce(c, true, &r));
+    permeability_from(r, c->prm.dom_interest, c->prm.voxel_size, used_porosity(c), out);
+    return PM_OK;
+}
+
+// Algorithmic HBM bytes of one launch group per voxel of the interior
+// (S = bytes per value of the swept state): every operand read once, every
+// result written once.
+static double algo_bytes(const pm_ctx *c, int cat) {
+    const double n = (double)c->g.nx * c->g.ny * c->g.nz;
+    const double S = c->dc() ? 4.0 : 8.0;
+    const double R = c->dc() ? 16.0 : 0.0;
+    switch (cat) {
+    case TC_SWEEP:  // read p,u,v,w + code (+ R), write p,u,v,w
+        return n * (8.0 * S + 1.0 + R);
+    case TC_REF:    // same minimum for the exact-order sweep (FP64, int32 code + bool geometry)
+        return n * (8.0 * 8.0 + 5.0);
+    case TC_GHOST: {
+        const pm::Grid &g = c->g;
+        const double faces = 2.0 * ((double)g.ny * g.nz + (double)g.nx * g.nz + (double)g.nx * g.ny);
+        return faces * 2.0 * 4.0 * S * 2.0;  // 2 layers, 4 fields, read + write
+    }
+    case TC_FOLD:      // read X (32) + d (16) + code, write X (32) + d (16)
+        return n * 97.0;
+    case TC_RESIDUAL:  // read X (32) + code, write R (16)
+        return n * 49.0;
+    default:
+        return 0.0;
+    }
+}
+
+extern "C" int pm_time_kernels(pm_ctx *c, int n, pm_kernel_timing *out, int max_out, int *n_out) {
+    if (!c || !out || !n_out) return fail(PM_E_ARG, "null argument");
+    if (!c->loaded) return fail(PM_E_STATE, "no geometry loaded");
+    HIPCHK(hipSetDevice(c->prm.device));
+    Timer tm;
+    c->timer = &tm;
+    int rc = PM_OK;
+    for (int q = 0; q < n && rc == PM_OK; q++) rc = one_sweep(c);
+    c->timer = nullptr;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double ms[TC_N] = {0};
+    int cnt[TC_N] = {0};
+    for (auto &r : tm.recs) {
+        float e = 0.f;
+        (void)hipEventElapsedTime(&e, r.a, r.b);
+        ms[r.cat] += e;
+        cnt[r.cat]++;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    if (rc != PM_OK) return rc;
+    int k = 0;
+    for (int cat = 0; cat < TC_N && k < max_out; cat++) {
+        if (!cnt[cat]) continue;
+        std::snprintf(out[k].name, sizeof out[k].name, "%s", kTimerNames[cat]);
+        out[k].launches = cnt[cat];
+        out[k].avg_ms = ms[cat] / cnt[cat];
+        out[k].bytes_per_launch = algo_bytes(c, cat);
+        k++;
+    }
+    *n_out = k;
+    return PM_OK;
+}

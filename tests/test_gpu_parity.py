@@ -1,0 +1,283 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bit-exact (np.array_equal on every compared cell) for the integer geometry
+codes, the initial state, the reference-order sweeps and the FP64/FP32 fast
+sweeps; the reductions are compensated parallel sums, compared to 1e-12
+relative and on the log's printed digits.  Full-size runs are checked through
+sweep counts and k against the reference binary's known answers.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import poremaps_amd as pm
+from poremaps_amd import geometry as geo
+from conftest import GOLDEN, random_geometry
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def nonc(a):
+    """Interior and face-halo cells (corner halo cells are never read)."""
+    m = np.zeros(a.shape, bool)
+    h = [np.zeros(n, bool) for n in a.shape]
+    for ax, n in enumerate(a.shape):
+        h[ax][:2] = True
+        h[ax][-2:] = True
+    Z, Y, X = np.meshgrid(h[0], h[1], h[2], indexing="ij")
+    m = (Z.astype(int) + Y.astype(int) + X.astype(int)) <= 1
+    return m
+
+
+def oracle_state(raw, bc):
+    G = O.geom_halo(raw, bc)
+    code = O.eval_geometry(G)
+    st = [a.copy() for a in O.init_fields(G, bc)]
+    O.pins(G, st[0])
+    return G, code, st
+
+
+# ---------------------------------------------------------------------------
+# geometry, porosity, initial state
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("bc", range(5))
+def test_geometry_codes_porosity_and_initial_state(bc):
+    raw = random_geometry((9, 11, 13), 0.45, 100 + bc)
+    say("geometry", bc, raw.shape)
+    G, code, st = oracle_state(raw, bc)
+    with pm.Solver(raw, boundary_method=bc) as s:
+        assert np.array_equal(s.field("nhood"), code[2:-2, 2:-2, 2:-2])
+        assert s.porosity == O.porosity(G)
+        for a, b in zip(s.state_halo(), st):
+            assert np.array_equal(a, b)
+
+
+def test_geometry_rejects_non_binary_bytes():
+    raw = np.zeros((4, 4, 4), np.uint8)
+    raw[1, 2, 3] = 2
+    with pytest.raises(pm.PoremapsError) as e:
+        pm.Solver(raw)
+    assert e.value.code == -5
+
+
+# ---------------------------------------------------------------------------
+# reference order, per iteration
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("solver", [1, 2, 3])
+@pytest.mark.parametrize("bc", range(5))
+def test_reference_sweeps_bit_exact(solver, bc):
+    raw = random_geometry((13, 9, 11), 0.4, 7 * bc + solver)
+    say("reference sweeps", solver, bc)
+    G, code, st = oracle_state(raw, bc)
+    with pm.Solver(raw, boundary_method=bc, solving_algorithm=solver) as s:
+        done = 0
+        for n in (1, 6, 33):
+            for _ in range(n):
+                O.sweep_ref(solver, G, code, *st)
+                for f in st:
+                    O.exchange(f, bc)
+                O.pins(G, st[0])
+            s.step(n)
+            done += n
+            for name, a, b in zip("puvw", s.state_halo(), st):
+                assert np.array_equal(a, b), f"{name} differs after {done} sweeps"
+
+
+def test_reference_sweeps_multi_launch_path_bit_exact():
+    # above the single-workgroup size limit: one launch per hyperplane
+    raw = random_geometry((60, 70, 98), 0.35, 5)
+    say("reference multi-launch", raw.shape)
+    G, code, st = oracle_state(raw, 0)
+    with pm.Solver(raw, boundary_method=0, solving_algorithm=2) as s:
+        for _ in range(2):
+            O.sweep_ref(2, G, code, *st)
+            for f in st:
+                O.exchange(f, 0)
+            O.pins(G, st[0])
+        s.step(2)
+        for a, b in zip(s.state_halo(), st):
+            assert np.array_equal(a, b)
+
+
+def test_reference_solve_channel_matches_binary_and_oracle(channel_raw, known):
+    kn = known["channel_gs"]
+    say("reference solve channel")
+    G = O.geom_halo(channel_raw, 0)
+    code = O.eval_geometry(G)
+    t, rows = O.calc_flow(G, code, 0, O.init_fields(G, 0), solver=2, eps=1e-6, dx=5e-5)
+    with pm.Solver(channel_raw, boundary_method=0, solving_algorithm=2, eps=1e-6, voxel_size=5e-5) as s:
+        res, grows = s.solve()
+    assert res.sweeps == kn["sweeps"] == t
+    assert len(grows) == len(rows)
+    for g, r in zip(grows, rows):
+        assert g["iteration"] == r[0]
+        for key, col in (("conv", 1), ("wmax_velz", 3), ("wmean_velz", 4), ("wk13", 8),
+                         ("wk23", 9), ("wk33", 10)):
+            assert g[key] == pytest.approx(r[col], rel=1e-12, abs=1e-300)
+    assert "%e" % grows[-1]["wk33"] == kn["wk33"]
+
+
+def test_reference_solve_channel_dom_interest(channel_raw, known):
+    kn = known["channel_dom_interest_t1500"]
+    say("reference dom_interest")
+    with pm.Solver(channel_raw, boundary_method=0, solving_algorithm=2, eps=1e-6, voxel_size=5e-5,
+                   dom_interest=tuple(kn["dom_interest"]), max_iter=1501) as s:
+        _, rows = s.solve()
+    r = [x for x in rows if x["iteration"] == kn["iteration"]][0]
+    assert "%e" % r["k33"] == kn["k33"] and "%e" % r["wk33"] == kn["wk33"]
+
+
+def test_reference_solve_tube_matches_binary(known):
+    """Config 0 (54x54x50 tube) to eps, exact order: the reference binary's
+    sweep count and k, and Hagen-Poiseuille within discretisation error."""
+    raw = geo.config0()
+    for case in ("tube_sor_ref", "tube_gs"):
+        kn = known[case]
+        say("reference tube", case)
+        with pm.Solver(raw, boundary_method=0, solving_algorithm=kn["solver"], eps=1e-6,
+                       voxel_size=2e-5) as s:
+            res, rows = s.solve()
+        assert res.sweeps == kn["sweeps"]
+        assert "%e" % rows[-1]["wk33"] == kn["wk33"]
+    assert "%e" % rows[-1]["wmean_velz"] == known["tube_gs"]["wmean_velz"]
+    k_hp = geo.hagen_poiseuille_k(raw, 20, 2e-5, 50)
+    assert abs(rows[-1]["wk33"] / k_hp - 1) < 0.03
+
+
+# ---------------------------------------------------------------------------
+# fast modes, per iteration
+# ---------------------------------------------------------------------------
+def oracle_fast(raw, bc, mode, omega, n):
+    G, code, A = oracle_state(raw, bc)
+    pc = O.pack_code(code)
+    B = [a.copy() for a in A]
+    qa = O.outlet_plane(pc, bc, A)
+    qb = qa.copy()
+    pins = O.solution_pins(G)
+    for _ in range(n):
+        if mode == "jacobi":
+            O.sweep_jacobi(pc, bc, omega, A, B, qa, qb, 3.0)
+            O.ghost(G, bc, B, qb, *pins)
+            A, B, qa, qb = B, A, qb, qa
+        else:
+            O.sweep_colour(pc, omega, A, qa, 3.0)
+            O.ghost(G, bc, A, qa, *pins)
+    return A
+
+
+def oracle_dc(raw, bc, mode, omega, n, it_eval):
+    G, code, X = oracle_state(raw, bc)
+    pc = O.pack_code(code)
+    d = [np.zeros(G.shape, np.float32) for _ in range(4)]
+    d2 = [np.zeros(G.shape, np.float32) for _ in range(4)]
+    qd = np.zeros(G.shape[1:], np.float32)
+    qd2 = qd.copy()
+    R = O.residual(pc, X, O.outlet_plane(pc, bc, X))
+    z = np.float32(0)
+    for t in range(n):
+        if mode == "jacobi":
+            O.sweep_jacobi(pc, bc, omega, d, d2, qd, qd2, z, R=R)
+            O.ghost(G, bc, d2, qd2, z, z, z)
+            d, d2, qd, qd2 = d2, d, qd2, qd
+        else:
+            O.sweep_colour(pc, omega, d, qd, z, R=R)
+            O.ghost(G, bc, d, qd, z, z, z)
+        if t % it_eval == 0:
+            O.fold(G, bc, X, d)
+            R = O.residual(pc, X, O.outlet_plane(pc, bc, X))
+            qd[:] = 0
+            qd2[:] = 0
+    return X
+
+
+@pytest.mark.parametrize("mode", ["jacobi", "sor3"])
+@pytest.mark.parametrize("bc", [0, 2, 4])
+def test_fast_fp64_sweeps_bit_exact(mode, bc):
+    raw = random_geometry((12, 10, 14), 0.35, 31 + bc)
+    say("fast f64", mode, bc)
+    ref = oracle_fast(raw, bc, mode, 1.7, 25)
+    with pm.Solver(raw, boundary_method=bc, mode=mode, omega=1.7) as s:
+        s.step(25)
+        m = nonc(ref[0])
+        for name, a, b in zip("puvw", s.state_halo(), ref):
+            assert np.array_equal(a[m], b[m]), f"{name} differs"
+
+
+@pytest.mark.parametrize("mode", ["jacobi", "sor3"])
+@pytest.mark.parametrize("bc", [0, 1, 3])
+def test_fast_f32dc_sweeps_bit_exact(mode, bc):
+    raw = random_geometry((12, 10, 14), 0.35, 41 + bc)
+    say("fast f32dc", mode, bc)
+    ref = oracle_dc(raw, bc, mode, 1.6, 31, 10)
+    with pm.Solver(raw, boundary_method=bc, mode=mode, omega=1.6, precision="f32dc", it_eval=10) as s:
+        s.step(31)
+        m = nonc(ref[0])
+        for name, a, b in zip("puvw", s.state_halo(), ref):
+            assert np.array_equal(a[m], b[m]), f"{name} differs"
+
+
+# ---------------------------------------------------------------------------
+# fast modes, to convergence: the reference's fixed point
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mode,omega,prec", [("jacobi", 1.0, "f64"), ("jacobi", 2.5, "f32dc"),
+                                             ("sor3", 1.8, "f64"), ("sor3", 1.8, "f32dc")])
+def test_fast_modes_reach_reference_fixed_point(channel_raw, mode, omega, prec):
+    say("fast fixed point", mode, omega, prec)
+    G = O.geom_halo(channel_raw, 0)
+    code = O.eval_geometry(G)
+    _, tight = O.calc_flow(G, code, 0, O.init_fields(G, 0), solver=2, eps=1e-11, dx=5e-5)
+    _, at_eps = O.calc_flow(G, code, 0, O.init_fields(G, 0), solver=2, eps=1e-6, dx=5e-5)
+    with pm.Solver(channel_raw, boundary_method=0, mode=mode, omega=omega, precision=prec,
+                   eps=1e-11, voxel_size=5e-5) as s:
+        res, _ = s.solve()
+    tol = 1e-9 if prec == "f64" else 1e-6
+    assert res.last.wk33 == pytest.approx(tight[-1][10], rel=tol)
+    with pm.Solver(channel_raw, boundary_method=0, mode=mode, omega=omega, precision=prec,
+                   eps=1e-6, voxel_size=5e-5) as s:
+        res, _ = s.solve()
+    assert res.last.wk33 == pytest.approx(at_eps[-1][10], rel=1e-4)
+
+
+# ---------------------------------------------------------------------------
+# the POREMAPS command line
+# ---------------------------------------------------------------------------
+def test_cli_drop_in_channel(tmp_path, channel_raw):
+    say("cli channel")
+    (tmp_path / "rect_channel_24_14_50_vs_5e-05.raw").write_bytes(channel_raw.tobytes())
+    inp = open(os.path.join(GOLDEN, "input_rect_channel_24_14_50_vs_5e-05.inp")).read()
+    inp = inp.replace("eps 1e-07", "eps 1e-06").replace("porosity 1.0", "porosity -1.0")
+    (tmp_path / "input.inp").write_text(inp)
+    r = subprocess.run([pm.CLI_PATH, "input.inp"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Iteration: 9300\t Convergence:" in r.stdout
+    G = O.geom_halo(channel_raw, 0)
+    code = O.eval_geometry(G)
+    st = O.init_fields(G, 0)
+    t, rows = O.calc_flow(G, code, 0, st, solver=2, eps=1e-6, dx=5e-5)
+    geom = "rect_channel_24_14_50_vs_5e-05.raw"
+    log = (tmp_path / "permeability_rect_channel_24_14_50_vs_5e-05.log").read_text().splitlines()
+    assert log[0].startswith("# iteration, conv, TPS")
+    assert len(log) == 1 + len(rows)
+    for line, row in zip(log[1:], rows):
+        f = line.split(", ")
+        exp = ["%i" % row[0]] + ["%e" % x for x in row[1:]]
+        assert [f[0], f[1]] + f[3:] == [exp[0], exp[1]] + exp[3:]
+    nh = np.fromfile(tmp_path / ("voxel_neighborhood_" + geom), np.int32).reshape(50, 14, 24)
+    assert np.array_equal(nh, code[2:-2, 2:-2, 2:-2])
+    p = np.fromfile(tmp_path / ("press_" + geom), np.float64).reshape(50, 14, 24)
+    assert np.array_equal(p, st[0][2:-2, 2:-2, 2:-2])
+    for name, f in zip(("velx_", "vely_", "velz_"), st[1:]):
+        v = np.fromfile(tmp_path / (name + geom), np.float64).reshape(50, 14, 24)
+        assert np.array_equal(v, f[2:-2, 2:-2, 2:-2] * 5e-5)
+    dd = np.fromfile(tmp_path / ("domain_decomp_" + geom), np.int32)
+    assert dd.size == 24 * 14 * 50 and (dd == 0).all()
