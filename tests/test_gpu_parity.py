@@ -118,11 +118,15 @@ def test_reference_solve_channel_matches_binary_and_oracle(channel_raw, known):
         res, grows = s.solve()
     assert res.sweeps == kn["sweeps"] == t
     assert len(grows) == len(rows)
+    # the oracle sums serially like the reference (error up to n*eps relative,
+    # n = 10^4 fluid voxels); the GPU sums are compensated.  conv is a
+    # relative difference of two such sums, so it is compared absolutely.
     for g, r in zip(grows, rows):
         assert g["iteration"] == r[0]
-        for key, col in (("conv", 1), ("wmax_velz", 3), ("wmean_velz", 4), ("wk13", 8),
-                         ("wk23", 9), ("wk33", 10)):
-            assert g[key] == pytest.approx(r[col], rel=1e-12, abs=1e-300)
+        assert g["conv"] == pytest.approx(r[1], rel=0, abs=1e-11)
+        for key, col in (("wmax_velz", 3), ("wmean_velz", 4), ("wk13", 8), ("wk23", 9), ("wk33", 10)):
+            assert g[key] == pytest.approx(r[col], rel=1e-11, abs=1e-300)
+        assert "%e" % g["wk33"] == "%e" % r[10]
     assert "%e" % grows[-1]["wk33"] == kn["wk33"]
 
 
