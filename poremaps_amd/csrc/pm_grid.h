@@ -87,6 +87,121 @@ __device__ __forceinline__ Cases decode(uint8_t pc) {
     return c;
 }
 
+// A case in {1, 3, 5} means the +1 neighbour on that axis is solid.
+__device__ __forceinline__ bool fwd_solid(int cs) { return cs == 1 || cs == 3 || cs == 5; }
+
+// get_2nd_derivation reading only the neighbours the case uses.  Velocities
+// of solid cells are exactly +0 (never written, copied as 0 into halos), so
+// the arithmetic is the same as loading every operand.
+template <typename T>
+__device__ __forceinline__ T d2_load(int cs, const T *__restrict__ f, int64_t c, int64_t s, T vc) {
+    const T K83 = (T)(8.0 / 3.0), K163 = (T)(16.0 / 3.0), K15 = (T)(1.0 / 5.0);
+    switch (cs) {
+    case 1: return (T)-8.0 * vc;
+    case 2: return K83 * f[c + s] - K163 * vc;
+    case 3: return K83 * f[c - s] - K163 * vc;
+    case 4: return ((T)2.0 * f[c + s] - (T)5.0 * vc) - K15 * f[c + 2 * s];
+    case 5: return ((T)2.0 * f[c - s] - K15 * f[c - 2 * s]) - (T)5.0 * vc;
+    default: return (f[c - s] - (T)2.0 * vc) + f[c + s];
+    }
+}
+
+// Own-axis second difference (l - 2c) + r with solid neighbours read as 0.
+template <typename T>
+__device__ __forceinline__ T lap_load(int cs, const T *__restrict__ f, int64_t c, int64_t s, T vc) {
+    const T l = back_solid(cs) ? (T)0 : f[c - s];
+    const T r = fwd_solid(cs) ? (T)0 : f[c + s];
+    return (l - (T)2.0 * vc) + r;
+}
+
+// get_2nd_derivation with all four neighbours loaded up front and the case's
+// expression picked by select: no divergent branches around the loads, the
+// same arithmetic as d2_load.
+template <typename T>
+__device__ __forceinline__ T d2_sel(int cs, const T *__restrict__ f, int64_t c, int64_t s, T vc) {
+    const T K83 = (T)(8.0 / 3.0), K163 = (T)(16.0 / 3.0), K15 = (T)(1.0 / 5.0);
+    const T l = f[c - s], r = f[c + s], ll = f[c - 2 * s], rr = f[c + 2 * s];
+    const T r1 = (T)-8.0 * vc;
+    const T r2 = K83 * r - K163 * vc;
+    const T r3 = K83 * l - K163 * vc;
+    const T r4 = ((T)2.0 * r - (T)5.0 * vc) - K15 * rr;
+    const T r5 = ((T)2.0 * l - K15 * ll) - (T)5.0 * vc;
+    const T r6 = (l - (T)2.0 * vc) + r;
+    T o = r6;
+    o = cs == 1 ? r1 : o;
+    o = cs == 2 ? r2 : o;
+    o = cs == 3 ? r3 : o;
+    o = cs == 4 ? r4 : o;
+    o = cs == 5 ? r5 : o;
+    return o;
+}
+
+// Momentum forces, every neighbour loaded (branch-free), same arithmetic as
+// fast_force_p.
+template <typename T>
+__device__ __forceinline__ void fast_force_s(const Cases &cs, int64_t c, int64_t sj, int64_t si,
+                                             const T *__restrict__ p, const T *__restrict__ u,
+                                             const T *__restrict__ v, const T *__restrict__ w,
+                                             const T *__restrict__ q, int64_t qc,
+                                             T &fu, T &fv, T &fw) {
+    const T INV_RE = (T)(1.0 / kRe);
+    const T pc = p[c];
+    const T pxm = q ? q[qc - 1] : p[c - 1];
+    const T pym = q ? q[qc - sj] : p[c - sj];
+    const T pzm = p[c - si];
+    const T dpx = cs.bx ? (T)0 : pc - pxm;
+    const T dpy = cs.by ? (T)0 : pc - pym;
+    const T dpz = cs.bz ? (T)0 : pc - pzm;
+    const T uc = u[c], vc = v[c], wc = w[c];
+    const T ul = u[c - 1], ur = u[c + 1], vl = v[c - sj], vr = v[c + sj], wl = w[c - si], wr = w[c + si];
+    const T lxx = ((cs.bx ? (T)0 : ul) - (T)2.0 * uc) + (fwd_solid(cs.cx) ? (T)0 : ur);
+    const T lyy = ((cs.by ? (T)0 : vl) - (T)2.0 * vc) + (fwd_solid(cs.cy) ? (T)0 : vr);
+    const T lzz = ((cs.bz ? (T)0 : wl) - (T)2.0 * wc) + (fwd_solid(cs.cz) ? (T)0 : wr);
+    const T dyx = d2_sel<T>(cs.cx, v, c, 1, vc);
+    const T dzx = d2_sel<T>(cs.cx, w, c, 1, wc);
+    const T dxy = d2_sel<T>(cs.cy, u, c, sj, uc);
+    const T dzy = d2_sel<T>(cs.cy, w, c, sj, wc);
+    const T dxz = d2_sel<T>(cs.cz, u, c, si, uc);
+    const T dyz = d2_sel<T>(cs.cz, v, c, si, vc);
+    const T su = (lxx + dxy) + dxz;
+    const T sv = (lyy + dyz) + dyx;
+    const T sw = (lzz + dzx) + dzy;
+    fu = cs.bx ? (T)0 : su * INV_RE - dpx;
+    fv = cs.by ? (T)0 : sv * INV_RE - dpy;
+    fw = cs.bz ? (T)0 : sw * INV_RE - dpz;
+}
+
+// Momentum forces with case-predicated neighbour loads; same operation order
+// as fast_force below and the CPU oracle's stencil_force.
+template <typename T>
+__device__ __forceinline__ void fast_force_p(const Cases &cs, int64_t c, int64_t sj, int64_t si,
+                                             const T *__restrict__ p, const T *__restrict__ u,
+                                             const T *__restrict__ v, const T *__restrict__ w,
+                                             const T *__restrict__ q, int64_t qc,
+                                             T &fu, T &fv, T &fw) {
+    const T INV_RE = (T)(1.0 / kRe);
+    const T pc = p[c];
+    const T dpx = cs.bx ? (T)0 : pc - (q ? q[qc - 1] : p[c - 1]);
+    const T dpy = cs.by ? (T)0 : pc - (q ? q[qc - sj] : p[c - sj]);
+    const T dpz = cs.bz ? (T)0 : pc - p[c - si];
+    const T uc = u[c], vc = v[c], wc = w[c];
+    const T lxx = lap_load<T>(cs.cx, u, c, 1, uc);
+    const T lyy = lap_load<T>(cs.cy, v, c, sj, vc);
+    const T lzz = lap_load<T>(cs.cz, w, c, si, wc);
+    const T dyx = d2_load<T>(cs.cx, v, c, 1, vc);
+    const T dzx = d2_load<T>(cs.cx, w, c, 1, wc);
+    const T dxy = d2_load<T>(cs.cy, u, c, sj, uc);
+    const T dzy = d2_load<T>(cs.cy, w, c, sj, wc);
+    const T dxz = d2_load<T>(cs.cz, u, c, si, uc);
+    const T dyz = d2_load<T>(cs.cz, v, c, si, vc);
+    const T su = (lxx + dxy) + dxz;
+    const T sv = (lyy + dyz) + dyx;
+    const T sw = (lzz + dzx) + dzy;
+    fu = cs.bx ? (T)0 : su * INV_RE - dpx;
+    fv = cs.by ? (T)0 : sv * INV_RE - dpy;
+    fw = cs.bz ? (T)0 : sw * INV_RE - dpz;
+}
+
 // Momentum forces of the fast modes at cell c (element index), reading the
 // state through raw pointers with the given strides; q (outlet plane only)
 // replaces the -x/-y pressure neighbours.  Same operation order as the CPU

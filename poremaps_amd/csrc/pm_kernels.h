@@ -33,15 +33,27 @@ void launch_pins(const Grid &g, const uint8_t *G, T *p, T pin_in, T pin_hi, T pi
 void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
                        double *v, double *w, int solver, int nsweeps, bool one_wg, hipStream_t s);
 
+// Device lists of the interior fluid cells' element offsets, ascending
+// (z, y, x): all of them, and split by colour (i+j+k) mod 3.
+struct FluidLists {
+    int32_t *all = nullptr;
+    int n_all = 0;
+    int32_t *col[3] = {nullptr, nullptr, nullptr};
+    int n_col[3] = {0, 0, 0};
+    // stencil loads: every neighbour up front + select (true) or only the
+    // neighbours the case needs, behind branches (false)
+    bool select_loads = true;
+};
+
 template <typename T>
-void launch_jacobi(const Grid &g, const uint8_t *pc, const T *p0, const T *u0, const T *v0,
-                   const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu, T cp,
-                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0, const T *u0,
+                   const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu,
+                   T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
                    hipStream_t s);
 template <typename T>
-void launch_colour(const Grid &g, const uint8_t *pc, T *p, T *u, T *v, T *w, T *q, T cu, T cp,
-                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
-                   hipStream_t s);
+void launch_colour(const Grid &g, const FluidLists &L, const uint8_t *pc, T *p, T *u, T *v, T *w, T *q,
+                   T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv,
+                   const float *Rw, hipStream_t s);
 template <typename T>
 void launch_ghost_fast(const Grid &g, const uint8_t *G, T *p, T *u, T *v, T *w, T *q, T pin_in, T pin_hi,
                        T pin_out, hipStream_t s);
@@ -53,9 +65,9 @@ size_t reduce_scratch_bytes();
 
 void launch_outlet_plane(const Grid &g, const uint8_t *pc, const double *p, const double *u,
                          const double *v, const double *w, double pin_out, double *q, hipStream_t s);
-void launch_residual(const Grid &g, const uint8_t *pc, const double *p, const double *u, const double *v,
-                     const double *w, const double *q, float *Rp, float *Ru, float *Rv, float *Rw,
-                     hipStream_t s);
+void launch_residual(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
+                     const double *v, const double *w, const double *q, float *Rp, float *Ru, float *Rv,
+                     float *Rw, hipStream_t s);
 void launch_fold(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw,
                  float *dp, float *du, float *dv, float *dw, hipStream_t s);
 

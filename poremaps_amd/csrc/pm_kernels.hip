@@ -280,91 +280,6 @@ __global__ __launch_bounds__(1024) void k_ref_sweeps_1wg(Grid g, const uint8_t *
 // fast modes
 // ---------------------------------------------------------------------------
 
-// Symplectic Jacobi, velocity pass: (p0,u0,v0,w0) -> (u1,v1,w1).
-template <typename T, bool HAS_R>
-__global__ __launch_bounds__(256) void k_jac_vel(Grid g, const uint8_t *__restrict__ pc,
-                                                 const T *__restrict__ p0, const T *__restrict__ u0,
-                                                 const T *__restrict__ v0, const T *__restrict__ w0,
-                                                 const T *__restrict__ q0, T *__restrict__ u1,
-                                                 T *__restrict__ v1, T *__restrict__ w1,
-                                                 const float *__restrict__ Ru,
-                                                 const float *__restrict__ Rv,
-                                                 const float *__restrict__ Rw, T cu) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int i = blockIdx.z + 2;
-    if (k >= g.nx + 2 || j >= g.ny + 2) return;
-    const int64_t c = g.at(i, j, k);
-    const uint8_t code = pc[c];
-    if (code == kSolid) return;
-    const Cases cs = decode(code);
-    const bool out = (i == g.nz + 1);
-    T fu, fv, fw;
-    fast_force<T>(cs, c, g.sx, g.sy, p0, u0, v0, w0, out ? q0 : nullptr, g.qat(j, k), fu, fv, fw);
-    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
-    u1[c] = cs.bx ? (T)0 : u0[c] + fu * cu;
-    v1[c] = cs.by ? (T)0 : v0[c] + fv * cu;
-    w1[c] = cs.bz ? (T)0 : w0[c] + fw * cu;
-}
-
-// Symplectic Jacobi, pressure pass: div of the new velocities (periodic +1
-// neighbours wrapped, wall halos frozen) and the old pressure.
-template <typename T, bool HAS_R>
-__global__ __launch_bounds__(256) void k_jac_pres(Grid g, const uint8_t *__restrict__ pc,
-                                                  const T *__restrict__ p0,
-                                                  const T *__restrict__ u1,
-                                                  const T *__restrict__ v1,
-                                                  const T *__restrict__ w1, T *__restrict__ p1,
-                                                  T *__restrict__ q1, const float *__restrict__ Rp,
-                                                  T cp, T pin_out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int i = blockIdx.z + 2;
-    if (k >= g.nx + 2 || j >= g.ny + 2) return;
-    const int64_t c = g.at(i, j, k);
-    if (pc[c] == kSolid) return;
-    const int64_t cxp = (k == g.nx + 1 && g.px) ? g.at(i, j, 2) : c + 1;
-    const int64_t cyp = (j == g.ny + 1 && g.py) ? g.at(i, 2, k) : c + g.sx;
-    const int64_t czp = (i == g.nz + 1 && g.pz) ? g.at(2, j, k) : c + g.sy;
-    const T div0 = ((((w1[czp] - w1[c]) + v1[cyp]) - v1[c]) + u1[cxp]) - u1[c];
-    T div = div0;
-    if (HAS_R) div = div + (T)Rp[c];
-    p1[c] = p0[c] - div * cp;
-    if (i == g.nz + 1) q1[g.qat(j, k)] = pin_out - div0 * (T)(kC2 * kDt);
-}
-
-// 3-colour SOR, one colour stage in place.
-template <typename T, bool HAS_R>
-__global__ __launch_bounds__(256) void k_colour(Grid g, const uint8_t *__restrict__ pc, T *p,
-                                                T *u, T *v, T *w, T *q, int colour, T cu, T cp,
-                                                T pin_out, const float *__restrict__ Rp,
-                                                const float *__restrict__ Ru,
-                                                const float *__restrict__ Rv,
-                                                const float *__restrict__ Rw) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int i = blockIdx.z + 2;
-    if (k >= g.nx + 2 || j >= g.ny + 2) return;
-    if ((i + j + k) % 3 != colour) return;
-    const int64_t c = g.at(i, j, k);
-    const uint8_t code = pc[c];
-    if (code == kSolid) return;
-    const Cases cs = decode(code);
-    const bool out = (i == g.nz + 1);
-    T fu, fv, fw;
-    fast_force<T>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, g.qat(j, k), fu, fv, fw);
-    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
-    const T un = cs.bx ? (T)0 : u[c] + fu * cu;
-    const T vn = cs.by ? (T)0 : v[c] + fv * cu;
-    const T wn = cs.bz ? (T)0 : w[c] + fw * cu;
-    u[c] = un; v[c] = vn; w[c] = wn;
-    const T div0 = ((((w[c + g.sy] - wn) + v[c + g.sx]) - vn) + u[c + 1]) - un;
-    T div = div0;
-    if (HAS_R) div = div + (T)Rp[c];
-    p[c] = p[c] - div * cp;
-    if (out) q[g.qat(j, k)] = pin_out - div0 * (T)(kC2 * kDt);
-}
-
 // Fast-mode ghost fill in one pass over the face-halo lines: the outlet
 // plane's interior is pinned, periodic faces are copied (the outlet plane's
 // copies taken from the pinned values), then the inlet/top halo planes are
@@ -458,7 +373,7 @@ __device__ __forceinline__ void acc_merge(RedAcc &a, const RedAcc &b) {
 }
 
 constexpr int kRedThreads = 256;
-constexpr int kRedBlocks = 240;
+constexpr int kRedBlocks = 1024;
 
 __device__ void block_reduce_store(RedAcc &a, RedAcc *out) {
     __shared__ RedAcc sh[kRedThreads];
@@ -479,11 +394,11 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_state(
     DomBox box, RedAcc *__restrict__ partial) {
     RedAcc a;
     acc_zero(a);
-    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int k = (int)(t % g.nx);
-        const int64_t r = t / g.nx;
+    // 64 lanes along x, 4 rows per block step: coalesced, no per-cell division
+    const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    const int64_t rows = (int64_t)g.ny * g.nz;
+    for (int64_t r = (int64_t)blockIdx.x * 4 + sub; r < rows; r += (int64_t)gridDim.x * 4)
+    for (int k = lane; k < g.nx; k += 64) {
         const int j = (int)(r % g.ny), i = (int)(r / g.ny);
         const int64_t c = g.at(i + 2, j + 2, k + 2);
         if (G[c] != 0) continue;
@@ -551,31 +466,6 @@ __global__ void k_outlet_plane(Grid g, const uint8_t *__restrict__ pc,
         val = p[c];
     }
     q[g.qat(j, k)] = val;
-}
-
-__global__ __launch_bounds__(256) void k_residual(Grid g, const uint8_t *__restrict__ pc,
-                                                  const double *__restrict__ p,
-                                                  const double *__restrict__ u,
-                                                  const double *__restrict__ v,
-                                                  const double *__restrict__ w,
-                                                  const double *__restrict__ q,
-                                                  float *__restrict__ Rp, float *__restrict__ Ru,
-                                                  float *__restrict__ Rv,
-                                                  float *__restrict__ Rw) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int i = blockIdx.z + 2;
-    if (k >= g.nx + 2 || j >= g.ny + 2) return;
-    const int64_t c = g.at(i, j, k);
-    const uint8_t code = pc[c];
-    if (code == kSolid) return;
-    const Cases cs = decode(code);
-    double fu, fv, fw;
-    fast_force<double>(cs, c, g.sx, g.sy, p, u, v, w, (i == g.nz + 1) ? q : nullptr, g.qat(j, k),
-                       fu, fv, fw);
-    Ru[c] = (float)fu; Rv[c] = (float)fv; Rw[c] = (float)fw;
-    const double div = ((((w[c + g.sy] - w[c]) + v[c + g.sx]) - v[c]) + u[c + 1]) - u[c];
-    Rp[c] = (float)div;
 }
 
 // X += d on the fluid interior, d = 0 everywhere.
@@ -674,52 +564,6 @@ void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, dou
 }
 
 template <typename T>
-void launch_jacobi(const Grid &g, const uint8_t *pc, const T *p0, const T *u0, const T *v0,
-                   const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu, T cp,
-                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
-                   hipStream_t s) {
-    if (Rp) {
-        hipLaunchKernelGGL((k_jac_vel<T, true>), int_grid(g), int_block(), 0, s, g, pc, p0, u0, v0, w0, q0,
-                           u1, v1, w1, Ru, Rv, Rw, cu);
-        hipLaunchKernelGGL((k_jac_pres<T, true>), int_grid(g), int_block(), 0, s, g, pc, p0, u1, v1, w1,
-                           p1, q1, Rp, cp, pin_out);
-    } else {
-        hipLaunchKernelGGL((k_jac_vel<T, false>), int_grid(g), int_block(), 0, s, g, pc, p0, u0, v0, w0,
-                           q0, u1, v1, w1, Ru, Rv, Rw, cu);
-        hipLaunchKernelGGL((k_jac_pres<T, false>), int_grid(g), int_block(), 0, s, g, pc, p0, u1, v1, w1,
-                           p1, q1, Rp, cp, pin_out);
-    }
-}
-template void launch_jacobi<double>(const Grid &, const uint8_t *, const double *, const double *,
-                                    const double *, const double *, const double *, double *, double *,
-                                    double *, double *, double *, double, double, double, const float *,
-                                    const float *, const float *, const float *, hipStream_t);
-template void launch_jacobi<float>(const Grid &, const uint8_t *, const float *, const float *,
-                                   const float *, const float *, const float *, float *, float *, float *,
-                                   float *, float *, float, float, float, const float *, const float *,
-                                   const float *, const float *, hipStream_t);
-
-template <typename T>
-void launch_colour(const Grid &g, const uint8_t *pc, T *p, T *u, T *v, T *w, T *q, T cu, T cp,
-                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
-                   hipStream_t s) {
-    for (int col = 0; col < 3; col++) {
-        if (Rp)
-            hipLaunchKernelGGL((k_colour<T, true>), int_grid(g), int_block(), 0, s, g, pc, p, u, v, w, q, col,
-                               cu, cp, pin_out, Rp, Ru, Rv, Rw);
-        else
-            hipLaunchKernelGGL((k_colour<T, false>), int_grid(g), int_block(), 0, s, g, pc, p, u, v, w, q,
-                               col, cu, cp, pin_out, Rp, Ru, Rv, Rw);
-    }
-}
-template void launch_colour<double>(const Grid &, const uint8_t *, double *, double *, double *, double *,
-                                    double *, double, double, double, const float *, const float *,
-                                    const float *, const float *, hipStream_t);
-template void launch_colour<float>(const Grid &, const uint8_t *, float *, float *, float *, float *, float *,
-                                   float, float, float, const float *, const float *, const float *,
-                                   const float *, hipStream_t);
-
-template <typename T>
 void launch_ghost_fast(const Grid &g, const uint8_t *G, T *p, T *u, T *v, T *w, T *q, T pin_in, T pin_hi,
                        T pin_out, hipStream_t s) {
     const int64_t n = (int64_t)g.nz * g.ny + (int64_t)g.nz * g.nx + (int64_t)g.ny * g.nx;
@@ -736,7 +580,8 @@ void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const doubl
                    hipStream_t s) {
     RedAcc *part = (RedAcc *)scratch;
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads));
+    const int64_t rows = n / g.nx;
+    const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (rows + 3) / 4));
     if (full)
         hipLaunchKernelGGL(k_reduce_state<true>, dim3(nb), dim3(kRedThreads), 0, s, g, G, p, u, v, w, box, part);
     else
@@ -750,12 +595,6 @@ void launch_outlet_plane(const Grid &g, const uint8_t *pc, const double *p, cons
                          const double *v, const double *w, double pin_out, double *q, hipStream_t s) {
     hipLaunchKernelGGL(k_outlet_plane, dim3(nblocks(g.NX, 64), (unsigned)g.NY), dim3(64), 0, s, g, pc, p, u,
                        v, w, pin_out, q);
-}
-
-void launch_residual(const Grid &g, const uint8_t *pc, const double *p, const double *u, const double *v,
-                     const double *w, const double *q, float *Rp, float *Ru, float *Rv, float *Rw,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_residual, int_grid(g), int_block(), 0, s, g, pc, p, u, v, w, q, Rp, Ru, Rv, Rw);
 }
 
 void launch_fold(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw,

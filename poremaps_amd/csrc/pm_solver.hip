@@ -3,10 +3,12 @@
 #include "../../include/poremaps_amd.h"
 #include "pm_kernels.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -73,6 +75,7 @@ struct pm_ctx {
     float *qD[2] = {nullptr, nullptr};
     float *R[4] = {nullptr, nullptr, nullptr, nullptr};
     // reductions
+    pm::FluidLists lists;   // fluid cells of the interior (fast modes)
     void *red_scratch = nullptr;
     pm::RedSums *red_dev = nullptr;
     pm::RedSums *red_host = nullptr;
@@ -111,6 +114,8 @@ static int check_params(const pm_params *p) {
         return fail(PM_E_ARG, "size_x_y_z must be >= 2 on every axis (got %d %d %d)", p->nx, p->ny, p->nz);
     if ((int64_t)p->nx * p->ny * p->nz > (int64_t)1 << 31)
         return fail(PM_E_ARG, "domain of %d x %d x %d exceeds 2^31 voxels", p->nx, p->ny, p->nz);
+    if (pm::make_grid(p->nx, p->ny, p->nz, 0).total >= ((int64_t)1 << 31))
+        return fail(PM_E_ARG, "padded domain of %d x %d x %d exceeds 32-bit cell offsets", p->nx, p->ny, p->nz);
     if (p->boundary_method < 0 || p->boundary_method > 4)
         return fail(PM_E_ARG, "boundary_method must be 0..4 (got %d)", p->boundary_method);
     if (p->it_eval <= 0 || p->it_write <= 0)
@@ -135,6 +140,7 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     const bool jac = p->mode == PM_MODE_JACOBI;
     if (fast && !dc && jac) per += 32;
     if (dc) per += 16 + 16 + (jac ? 16 : 0);
+    if (fast) per += jac ? 4 : 8;  // fluid lists (upper bound: every cell fluid)
     return per * g.total + 4 * g.qtotal() * 8 + (int64_t)pm::reduce_scratch_bytes() + (1 << 20);
 }
 
@@ -144,6 +150,9 @@ static void free_all(pm_ctx *c) {
     for (int q = 0; q < 4; q++) { f(c->X[q]); f(c->X2[q]); f(c->D[q]); f(c->D2[q]); f(c->R[q]); }
     for (int q = 0; q < 2; q++) { f(c->qX[q]); f(c->qD[q]); }
     f(c->red_scratch); f(c->red_dev);
+    f(c->lists.all);
+    for (int q = 0; q < 3; q++) f(c->lists.col[q]);
+    c->lists = pm::FluidLists();
     if (c->red_host) (void)hipHostFree(c->red_host);
     c->G = c->pc = nullptr; c->code = nullptr;
     for (int q = 0; q < 4; q++) c->X[q] = c->X2[q] = nullptr, c->D[q] = c->D2[q] = c->R[q] = nullptr;
@@ -228,7 +237,7 @@ static int dc_prepare(pm_ctx *c) {
     const pm::Grid &g = c->g;
     tb(c, TC_RESIDUAL);
     pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
-    pm::launch_residual(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R[0], c->R[1], c->R[2],
+    pm::launch_residual(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R[0], c->R[1], c->R[2],
                         c->R[3], c->stream);
     te(c);
     RC(fill_f32(c, c->qD[0], g.qtotal(), 0.f));
@@ -268,6 +277,35 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     RC(dalloc((uint8_t **)&c->red_scratch, (int64_t)pm::reduce_scratch_bytes()));
     RC(dalloc(&c->red_dev, 1));
     HIPCHK(hipHostMalloc((void **)&c->red_host, sizeof(pm::RedSums), hipHostMallocDefault));
+    if (c->fast()) {
+        // fluid lists in (z, y, x) order, all and per colour (i+j+k) mod 3 of
+        // the halo-including indices
+        std::vector<int32_t> all;
+        std::vector<int32_t> col[3];
+        all.reserve((size_t)fluid);
+        for (int q = 0; q < 3; q++) col[q].reserve((size_t)fluid / 3 + 1);
+        int64_t idx = 0;
+        for (int i = 0; i < g.nz; i++)
+            for (int j = 0; j < g.ny; j++)
+                for (int k = 0; k < g.nx; k++, idx++) {
+                    if (raw[idx] != 0) continue;
+                    const int32_t e = (int32_t)g.at(i + 2, j + 2, k + 2);
+                    all.push_back(e);
+                    col[(i + j + k + 6) % 3].push_back(e);
+                }
+        const char *ld = std::getenv("POREMAPS_SWEEP_LOADS");
+        c->lists.select_loads = !(ld && std::strcmp(ld, "branch") == 0);
+        c->lists.n_all = (int)all.size();
+        RC(dalloc(&c->lists.all, std::max<int64_t>(1, c->lists.n_all)));
+        HIPCHK(hipMemcpy(c->lists.all, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
+        if (c->prm.mode == PM_MODE_SOR3)
+            for (int q = 0; q < 3; q++) {
+                c->lists.n_col[q] = (int)col[q].size();
+                RC(dalloc(&c->lists.col[q], std::max<int64_t>(1, c->lists.n_col[q])));
+                HIPCHK(hipMemcpy(c->lists.col[q], col[q].data(), sizeof(int32_t) * col[q].size(),
+                                 hipMemcpyHostToDevice));
+            }
+    }
 
     uint8_t *raw_dev = nullptr;
     RC(dalloc(&raw_dev, n));
@@ -332,7 +370,7 @@ static int one_sweep(pm_ctx *c) {
         const double pin_in = (double)(g.nz + 2);
         if (c->jacobi()) {
             tb(c, TC_SWEEP);
-            pm::launch_jacobi<double>(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->X2[0], c->X2[1],
+            pm::launch_jacobi<double>(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->X2[0], c->X2[1],
                                       c->X2[2], c->X2[3], c->qX[1], cu, cp, 3.0, nullptr, nullptr, nullptr,
                                       nullptr, s);
             te(c);
@@ -344,7 +382,7 @@ static int one_sweep(pm_ctx *c) {
             std::swap(c->qX[0], c->qX[1]);
         } else {
             tb(c, TC_SWEEP);
-            pm::launch_colour<double>(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0,
+            pm::launch_colour<double>(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0,
                                       nullptr, nullptr, nullptr, nullptr, s);
             te(c);
             tb(c, TC_GHOST);
@@ -356,7 +394,7 @@ static int one_sweep(pm_ctx *c) {
         const float fcu = (float)cu, fcp = (float)cp;
         if (c->jacobi()) {
             tb(c, TC_SWEEP);
-            pm::launch_jacobi<float>(g, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], c->D2[0], c->D2[1],
+            pm::launch_jacobi<float>(g, c->lists, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], c->D2[0], c->D2[1],
                                      c->D2[2], c->D2[3], c->qD[1], fcu, fcp, 0.f, c->R[0], c->R[1], c->R[2],
                                      c->R[3], s);
             te(c);
@@ -368,7 +406,7 @@ static int one_sweep(pm_ctx *c) {
             std::swap(c->qD[0], c->qD[1]);
         } else {
             tb(c, TC_SWEEP);
-            pm::launch_colour<float>(g, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], fcu, fcp, 0.f,
+            pm::launch_colour<float>(g, c->lists, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], fcu, fcp, 0.f,
                                      c->R[0], c->R[1], c->R[2], c->R[3], s);
             te(c);
             tb(c, TC_GHOST);
