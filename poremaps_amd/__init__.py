@@ -213,6 +213,108 @@ class Solver:
         return dict(zip(keys, list(out)))
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("precision", C.c_int), ("omega", C.c_double), ("it_eval", C.c_int), ("it_write", C.c_int),
+                ("max_iter", C.c_int), ("eps", C.c_double), ("device", C.c_int), ("use_graph", C.c_int),
+                ("budget_bytes", C.c_int64)]
+
+
+SAMPLE_STATES = {0: "running", 1: "converged", 2: "max_iter", 3: "diverged"}
+
+
+def _batch_lib():
+    L = lib()
+    if not getattr(L, "_batch_declared", False):
+        P = C.POINTER
+        L.pm_batch_params_default.argtypes = [P(BatchParams)]
+        L.pm_batch_create.argtypes = [P(BatchParams), P(C.c_void_p)]
+        L.pm_batch_sample_bytes.argtypes = [P(BatchParams), C.c_int, C.c_int, C.c_int]
+        L.pm_batch_sample_bytes.restype = C.c_int64
+        L.pm_batch_add.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, P(C.c_int)]
+        L.pm_batch_step.argtypes = [C.c_void_p, P(C.c_int)]
+        L.pm_batch_status.argtypes = [C.c_void_p, C.c_int, P(C.c_int), P(Result)]
+        L.pm_batch_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.pm_batch_remove.argtypes = [C.c_void_p, C.c_int]
+        L.pm_batch_memory.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64)]
+        L.pm_batch_destroy.argtypes = [C.c_void_p]
+        L._batch_declared = True
+    return L
+
+
+class Batch:
+    """Many geometries on one GPU (include/poremaps_amd_batch.h)."""
+
+    def __init__(self, *, precision="f32dc", omega=2.0, it_eval=100, it_write=100, max_iter=100_000_000,
+                 eps=1e-6, device=0, use_graph=True, budget_bytes=0):
+        L = _batch_lib()
+        p = BatchParams()
+        L.pm_batch_params_default(C.byref(p))
+        p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+        p.omega, p.it_eval, p.it_write, p.max_iter = omega, it_eval, it_write, max_iter
+        p.eps, p.device, p.use_graph, p.budget_bytes = eps, device, int(use_graph), budget_bytes
+        self.params = p
+        h = C.c_void_p()
+        _check(L.pm_batch_create(C.byref(p), C.byref(h)))
+        self._h = h
+        self.shapes = {}
+
+    def sample_bytes(self, shape) -> int:
+        nz, ny, nx = shape
+        return int(_batch_lib().pm_batch_sample_bytes(C.byref(self.params), nx, ny, nz))
+
+    def add(self, raw: np.ndarray, *, boundary_method=0, voxel_size=1.0, porosity=-1.0) -> int:
+        raw = np.ascontiguousarray(raw, dtype=np.uint8)
+        nz, ny, nx = raw.shape
+        slot = C.c_int()
+        _check(_batch_lib().pm_batch_add(self._h, raw.ctypes.data, nx, ny, nz, boundary_method, voxel_size,
+                                         porosity, C.byref(slot)))
+        self.shapes[slot.value] = raw.shape
+        return slot.value
+
+    def step(self) -> int:
+        n = C.c_int()
+        _check(_batch_lib().pm_batch_step(self._h, C.byref(n)))
+        return n.value
+
+    def status(self, slot: int):
+        st, res = C.c_int(), Result()
+        _check(_batch_lib().pm_batch_status(self._h, slot, C.byref(st), C.byref(res)))
+        return SAMPLE_STATES[st.value], res
+
+    def field(self, slot: int, name: str) -> np.ndarray:
+        dt = np.int32 if name == "nhood" else np.float64
+        out = np.empty(self.shapes[slot], dt)
+        _check(_batch_lib().pm_batch_get_field(self._h, slot, FIELDS[name], out.ctypes.data))
+        return out
+
+    def remove(self, slot: int):
+        _check(_batch_lib().pm_batch_remove(self._h, slot))
+        self.shapes.pop(slot, None)
+
+    def memory(self):
+        u, b = C.c_int64(), C.c_int64()
+        _check(_batch_lib().pm_batch_memory(self._h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _batch_lib().pm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 def read_inp(path: str):
     """Parse an ordered .inp (src/read_problem_params.cc:75-90)."""
     p, x = Params(), InpExtra()

@@ -1,0 +1,577 @@
+// pm_batch.hip -- many geometries solved together on one GPU (no reference
+// counterpart; the reference runs one geometry per process).
+//
+// Every admitted geometry is a full solver context (pm_ctx: its own arrays,
+// fluid list, convergence state).  One interval = it_eval sweeps of every
+// running sample + the defect-correction fold/residual + the per-sample
+// reductions, launched as batched kernels: each phase is one launch over a
+// table of (sample, first fluid voxel) chunks of 256, so 64 geometries cost
+// the launches of one.  The interval is captured once as a HIP graph and
+// replayed while the set of running samples is unchanged; finished samples
+// drop out of the tables (eviction) and new ones join between intervals.
+#include "../../include/poremaps_amd_batch.h"
+#include "pm_ctx.h"
+#include "pm_reduce.h"
+#include "pm_voxel.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace pm {
+
+constexpr int kB = 256;  // fluid voxels / halo lines / plane cells per chunk
+
+// Device descriptor of one running sample.  A/B: the swept state (FP32
+// correction in the defect-correction mode, FP64 solution otherwise), A the
+// current one at the start of an interval.
+struct BSample {
+    Grid g;
+    const int32_t *list;
+    int n;
+    const uint8_t *G, *pc;
+    void *A[4], *B[4];
+    void *qA, *qB;
+    float *R[4];
+    double *X[4];
+    double *qX;
+    int chunk_begin, chunk_end;
+};
+
+struct BChunk {
+    int s, start;
+};
+
+template <typename T>
+__device__ __forceinline__ T *fld(void *const *set, int q) { return (T *)set[q]; }
+
+template <typename T, bool DC>
+__global__ __launch_bounds__(kB) void k_b_vel(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
+                                              T cu) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    void *const *src = parity ? s.B : s.A;
+    void *const *dst = parity ? s.A : s.B;
+    const T *q0 = (const T *)(parity ? s.qB : s.qA);
+    vel_voxel<T, DC, true>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(src, 1), fld<T>(src, 2), fld<T>(src, 3), q0,
+                           fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3), DC ? s.R[1] : nullptr,
+                           DC ? s.R[2] : nullptr, DC ? s.R[3] : nullptr, cu);
+}
+
+template <typename T, bool DC>
+__global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
+                                               T cp) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    void *const *src = parity ? s.B : s.A;
+    void *const *dst = parity ? s.A : s.B;
+    T *q1 = (T *)(parity ? s.qA : s.qB);
+    pres_voxel<T, DC>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3),
+                      fld<T>(dst, 0), q1, DC ? s.R[0] : nullptr, cp, DC ? (T)0 : (T)3.0);
+}
+
+// which: 0 = set A, 1 = set B (the swept state, correction pins 0 in the
+// defect-correction mode), 2 = the FP64 solution X (solution pins).
+template <typename T, bool DC>
+__global__ __launch_bounds__(kB) void k_b_ghost(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                                int which) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int64_t t = (int64_t)ch.start + threadIdx.x;
+    if (t >= ghost_lines(s.g)) return;
+    const bool solution = which == 2 || !DC;
+    const T pin_in = solution ? (T)(s.g.nz + 2) : (T)0, pin_hi = solution ? (T)2.0 : (T)0,
+            pin_out = solution ? (T)3.0 : (T)0;
+    if (which == 2) {
+        ghost_line<T>(s.g, t, s.G, (T *)s.X[0], (T *)s.X[1], (T *)s.X[2], (T *)s.X[3], nullptr, pin_in, pin_hi,
+                      pin_out);
+    } else {
+        void *const *set = which ? s.B : s.A;
+        ghost_line<T>(s.g, t, s.G, fld<T>(set, 0), fld<T>(set, 1), fld<T>(set, 2), fld<T>(set, 3),
+                      (T *)(which ? s.qB : s.qA), pin_in, pin_hi, pin_out);
+    }
+}
+
+// Defect-correction fold over the fluid list: X += d, d = 0 (the halo copies of
+// d are refilled with zeros by the following ghost fill).
+__global__ __launch_bounds__(kB) void k_b_fold(const BSample *__restrict__ S, const BChunk *__restrict__ C, int which) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    void *const *d = which ? s.B : s.A;
+    for (int q = 0; q < 4; q++) {
+        float *dq = fld<float>(d, q);
+        s.X[q][c] = s.X[q][c] + (double)dq[c];
+        dq[c] = 0.f;
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_b_outlet(const BSample *__restrict__ S, const BChunk *__restrict__ C) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.g.NY * s.g.NX) return;
+    outlet_cell(s.g, t / s.g.NX, t % s.g.NX, s.pc, s.X[0], s.X[1], s.X[2], s.X[3], 3.0, s.qX);
+}
+
+__global__ __launch_bounds__(kB) void k_b_zero_q(const BSample *__restrict__ S, const BChunk *__restrict__ C) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.g.NY * s.g.NX) return;
+    const int64_t e = s.g.qat(t / s.g.NX, t % s.g.NX);
+    ((float *)s.qA)[e] = 0.f;
+    ((float *)s.qB)[e] = 0.f;
+}
+
+__global__ __launch_bounds__(kB) void k_b_residual(const BSample *__restrict__ S, const BChunk *__restrict__ C) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    residual_voxel(s.g, c, s.pc[c], s.X[0], s.X[1], s.X[2], s.X[3], s.qX, s.R[0], s.R[1], s.R[2], s.R[3]);
+}
+
+// Per-chunk partial sums of the solution state (whole-domain part of
+// src/evaluation.cc:54-262).  which: 0/1 = set A/B (FP64 mode), 2 = X.
+__global__ __launch_bounds__(kB) void k_b_reduce(const BSample *__restrict__ S, const BChunk *__restrict__ C, int which,
+                                                 RedAcc *__restrict__ part) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    RedAcc a;
+    acc_zero(a);
+    if (t < s.n) {
+        const int64_t c = s.list[t];
+        const double *u = which == 2 ? s.X[1] : (const double *)(which ? s.B[1] : s.A[1]);
+        const double *v = which == 2 ? s.X[2] : (const double *)(which ? s.B[2] : s.A[2]);
+        const double *w = which == 2 ? s.X[3] : (const double *)(which ? s.B[3] : s.A[3]);
+        const double wc = w[c];
+        a.n[0] = 1;
+        nadd(a.s[0], u[c]);
+        nadd(a.s[1], v[c]);
+        nadd(a.s[2], wc);
+        a.wmax = wc > 0.0 ? wc : 0.0;
+    }
+    block_reduce_store(a, part + blockIdx.x);
+}
+
+// One block per sample: merge its chunks' partials in chunk order.
+__global__ __launch_bounds__(kB) void k_b_reduce_final(const BSample *__restrict__ S, const RedAcc *__restrict__ part,
+                                                       RedSums *__restrict__ out) {
+    const BSample &s = S[blockIdx.x];
+    RedAcc a;
+    acc_zero(a);
+    for (int b = s.chunk_begin + threadIdx.x; b < s.chunk_end; b += kB) acc_merge(a, part[b]);
+    __shared__ RedAcc res;
+    block_reduce_store(a, &res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        RedSums &o = out[blockIdx.x];
+        for (int q = 0; q < 8; q++) o.sum[q] = res.s[q].s + res.s[q].c;
+        for (int q = 0; q < 4; q++) o.count[q] = res.n[q];
+        o.wmax = res.wmax;
+    }
+}
+
+}  // namespace pm
+
+using pm::BChunk;
+using pm::BSample;
+
+struct pm_batch {
+    pm_batch_params prm;
+    hipStream_t stream = nullptr;
+    std::vector<pm_ctx *> ctx;       // slot -> sample (nullptr: free)
+    std::vector<int> state;          // PM_SAMPLE_*
+    std::vector<double> conv;
+    std::vector<int64_t> bytes;
+    std::vector<pm_log_row> last;
+    std::vector<int> n_rows;
+    int64_t used = 0, budget = 0;
+    // tables of the running samples
+    std::vector<int> run;            // table index -> slot
+    BSample *dS = nullptr;
+    BChunk *dC = nullptr, *dG = nullptr, *dP = nullptr;
+    pm::RedAcc *dPart = nullptr;
+    pm::RedSums *dOut = nullptr, *hOut = nullptr;
+    int capS = 0, capC = 0, capG = 0, capP = 0;
+    int nS = 0, nC = 0, nG = 0, nP = 0;
+    bool shape_dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dc() const { return prm.precision == PM_PREC_F32DC; }
+};
+
+extern "C" void pm_batch_params_default(pm_batch_params *p) {
+    std::memset(p, 0, sizeof *p);
+    p->precision = PM_PREC_F32DC;
+    p->omega = 2.0;
+    p->it_eval = 100;
+    p->it_write = 100;
+    p->max_iter = 100000000;
+    p->eps = 1e-6;
+    p->device = 0;
+    p->use_graph = 1;
+    p->budget_bytes = 0;
+}
+
+static pm_params sample_params(const pm_batch_params *p, int nx, int ny, int nz, int bc, double dx, double phi) {
+    pm_params q;
+    pm_params_default(&q);
+    q.nx = nx; q.ny = ny; q.nz = nz;
+    q.boundary_method = bc;
+    q.voxel_size = dx;
+    q.porosity = phi;
+    q.max_iter = p->max_iter;
+    q.it_eval = p->it_eval;
+    q.it_write = p->it_write;
+    q.eps = p->eps;
+    q.mode = PM_MODE_JACOBI;
+    q.precision = p->precision;
+    q.omega = p->omega;
+    q.device = p->device;
+    return q;
+}
+
+#define BCHK(expr)                                                                                 \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return pm_fail(PM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+extern "C" int pm_batch_create(const pm_batch_params *p, pm_batch **out) {
+    if (!p || !out) return pm_fail(PM_E_ARG, "null argument");
+    *out = nullptr;
+    if (p->precision != PM_PREC_F64 && p->precision != PM_PREC_F32DC)
+        return pm_fail(PM_E_ARG, "unknown precision %d", p->precision);
+    if (p->it_eval <= 0 || p->it_write <= 0) return pm_fail(PM_E_ARG, "it_eval and it_write must be positive");
+    if (!(p->omega > 0.0 && p->omega < 100.0)) return pm_fail(PM_E_ARG, "omega must be in (0, 100)");
+    BCHK(hipSetDevice(p->device));
+    pm_batch *b = new (std::nothrow) pm_batch;
+    if (!b) return pm_fail(PM_E_MEMORY, "host allocation failed");
+    b->prm = *p;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return pm_fail(PM_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    if (p->budget_bytes > 0) {
+        b->budget = p->budget_bytes;
+    } else {
+        size_t fr = 0, tot = 0;
+        (void)hipMemGetInfo(&fr, &tot);
+        b->budget = (int64_t)(0.9 * (double)fr);
+    }
+    *out = b;
+    return PM_OK;
+}
+
+extern "C" int64_t pm_batch_sample_bytes(const pm_batch_params *p, int nx, int ny, int nz) {
+    if (!p) return -1;
+    pm_params q = sample_params(p, nx, ny, nz, 0, 1.0, -1.0);
+    return pm_device_bytes(&q);
+}
+
+extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int nz, int bc, double dx, double phi,
+                            int *slot) {
+    if (!b || !raw || !slot) return pm_fail(PM_E_ARG, "null argument");
+    pm_params q = sample_params(&b->prm, nx, ny, nz, bc, dx, phi);
+    const int64_t need = pm_device_bytes(&q);
+    if (need < 0) return PM_E_ARG;
+    if (b->used + need > b->budget)
+        return pm_fail(PM_E_MEMORY, "admission refused: %lld + %lld bytes exceed the budget of %lld",
+                       (long long)b->used, (long long)need, (long long)b->budget);
+    pm_ctx *c = nullptr;
+    int rc = pm_ctx_create(&q, &c);
+    if (rc != PM_OK) return rc;
+    rc = pm_load_geometry(c, raw);
+    // the first sweep (and, in the defect-correction mode, the fold after
+    // sweep 0) runs on the sample's own stream; afterwards every interval
+    // ends on an iteration index that is a multiple of it_eval
+    if (rc == PM_OK) rc = pm_one_sweep(c);
+    if (rc == PM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = pm_fail(PM_E_HIP, "first sweep failed");
+    if (rc != PM_OK) {
+        pm_destroy(c);
+        return rc;
+    }
+    int s = -1;
+    for (size_t i = 0; i < b->ctx.size(); i++)
+        if (!b->ctx[i]) { s = (int)i; break; }
+    if (s < 0) {
+        s = (int)b->ctx.size();
+        b->ctx.push_back(nullptr);
+        b->state.push_back(0);
+        b->conv.push_back(0.0);
+        b->bytes.push_back(0);
+        b->last.push_back(pm_log_row());
+        b->n_rows.push_back(0);
+    }
+    b->ctx[s] = c;
+    b->state[s] = PM_SAMPLE_RUNNING;
+    b->conv[s] = 1.0 + b->prm.eps;
+    b->bytes[s] = need;
+    std::memset(&b->last[s], 0, sizeof(pm_log_row));
+    b->n_rows[s] = 0;
+    b->used += need;
+    b->shape_dirty = true;
+    *slot = s;
+    return PM_OK;
+}
+
+template <typename T>
+static int grow(T **p, int *cap, int n) {
+    if (n <= *cap) return PM_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    const int c = std::max(n, *cap * 2);
+    if (hipMalloc((void **)p, sizeof(T) * (size_t)c) != hipSuccess)
+        return pm_fail(PM_E_MEMORY, "batch table allocation of %d entries failed", c);
+    *cap = c;
+    return PM_OK;
+}
+
+// Descriptors of the running samples from their contexts' current pointers.
+static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BChunk> &C, std::vector<BChunk> &G,
+                      std::vector<BChunk> &P) {
+    for (int slot : b->run) {
+        pm_ctx *c = b->ctx[slot];
+        BSample d;
+        std::memset(&d, 0, sizeof d);
+        d.g = c->g;
+        d.list = c->lists.all;
+        d.n = c->lists.n_all;
+        d.G = c->G;
+        d.pc = c->pc;
+        for (int q = 0; q < 4; q++) {
+            d.A[q] = b->dc() ? (void *)c->D[q] : (void *)c->X[q];
+            d.B[q] = b->dc() ? (void *)c->D2[q] : (void *)c->X2[q];
+            d.R[q] = c->R[q];
+            d.X[q] = c->X[q];
+        }
+        d.qA = b->dc() ? (void *)c->qD[0] : (void *)c->qX[0];
+        d.qB = b->dc() ? (void *)c->qD[1] : (void *)c->qX[1];
+        d.qX = c->qX[0];
+        const int s = (int)S.size();
+        d.chunk_begin = (int)C.size();
+        for (int t = 0; t < d.n; t += pm::kB) C.push_back({s, t});
+        d.chunk_end = (int)C.size();
+        const int64_t lines = pm::ghost_lines(d.g);
+        for (int64_t t = 0; t < lines; t += pm::kB) G.push_back({s, (int)t});
+        const int plane = d.g.NY * d.g.NX;
+        for (int t = 0; t < plane; t += pm::kB) P.push_back({s, t});
+        S.push_back(d);
+    }
+}
+
+static void launch_interval(pm_batch *b) {
+    using namespace pm;
+    hipStream_t st = b->stream;
+    const double om = b->prm.omega;
+    const int n = b->prm.it_eval;
+    const dim3 blk(kB);
+    if (b->dc()) {
+        const float cu = (float)(om * kDt), cp = (float)(om * kC2 * kDt);
+        for (int s = 0; s < n; s++) {
+            const int par = s & 1;
+            hipLaunchKernelGGL((k_b_vel<float, true>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
+            hipLaunchKernelGGL((k_b_pres<float, true>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cp);
+            hipLaunchKernelGGL((k_b_ghost<float, true>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
+        }
+        const int cur = n & 1;
+        hipLaunchKernelGGL(k_b_fold, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur);
+        hipLaunchKernelGGL((k_b_ghost<float, true>), dim3(b->nG), blk, 0, st, b->dS, b->dG, cur);
+        hipLaunchKernelGGL((k_b_ghost<double, true>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 2);
+        hipLaunchKernelGGL(k_b_outlet, dim3(b->nP), blk, 0, st, b->dS, b->dP);
+        hipLaunchKernelGGL(k_b_residual, dim3(b->nC), blk, 0, st, b->dS, b->dC);
+        hipLaunchKernelGGL(k_b_zero_q, dim3(b->nP), blk, 0, st, b->dS, b->dP);
+        hipLaunchKernelGGL(k_b_reduce, dim3(b->nC), blk, 0, st, b->dS, b->dC, 2, b->dPart);
+    } else {
+        const double cu = om * kDt, cp = om * kC2 * kDt;
+        for (int s = 0; s < n; s++) {
+            const int par = s & 1;
+            hipLaunchKernelGGL((k_b_vel<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
+            hipLaunchKernelGGL((k_b_pres<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cp);
+            hipLaunchKernelGGL((k_b_ghost<double, false>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
+        }
+        hipLaunchKernelGGL(k_b_reduce, dim3(b->nC), blk, 0, st, b->dS, b->dC, n & 1, b->dPart);
+    }
+    hipLaunchKernelGGL(k_b_reduce_final, dim3(b->nS), blk, 0, st, b->dS, b->dPart, b->dOut);
+}
+
+static void drop_graph(pm_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// After an interval with an odd number of sweeps the other buffer is current:
+// swap the context's pointers as pm_one_sweep does.
+static void swap_current(pm_batch *b, pm_ctx *c) {
+    if (b->dc()) {
+        for (int q = 0; q < 4; q++) std::swap(c->D[q], c->D2[q]);
+        std::swap(c->qD[0], c->qD[1]);
+    } else {
+        for (int q = 0; q < 4; q++) std::swap(c->X[q], c->X2[q]);
+        std::swap(c->qX[0], c->qX[1]);
+    }
+}
+
+extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
+    if (!b) return pm_fail(PM_E_ARG, "null batch");
+    BCHK(hipSetDevice(b->prm.device));
+    if (b->shape_dirty) {
+        b->run.clear();
+        for (size_t s = 0; s < b->ctx.size(); s++)
+            if (b->ctx[s] && b->state[s] == PM_SAMPLE_RUNNING && b->ctx[s]->lists.n_all > 0)
+                b->run.push_back((int)s);
+    }
+    if (b->run.empty()) {
+        if (n_running) *n_running = 0;
+        return PM_OK;
+    }
+    std::vector<BSample> S;
+    std::vector<BChunk> C, G, P;
+    fill_desc(b, S, C, G, P);
+    int rc;
+    if ((rc = grow(&b->dS, &b->capS, (int)S.size())) || (rc = grow(&b->dC, &b->capC, (int)C.size())) ||
+        (rc = grow(&b->dG, &b->capG, (int)G.size())) || (rc = grow(&b->dP, &b->capP, (int)P.size())))
+        return rc;
+    int capPart = b->capC;
+    if (!b->dPart || b->shape_dirty) {
+        if (b->dPart) (void)hipFree(b->dPart);
+        if (b->dOut) (void)hipFree(b->dOut);
+        if (b->hOut) (void)hipHostFree(b->hOut);
+        BCHK(hipMalloc((void **)&b->dPart, sizeof(pm::RedAcc) * (size_t)capPart));
+        BCHK(hipMalloc((void **)&b->dOut, sizeof(pm::RedSums) * (size_t)b->capS));
+        BCHK(hipHostMalloc((void **)&b->hOut, sizeof(pm::RedSums) * (size_t)b->capS, hipHostMallocDefault));
+    }
+    BCHK(hipMemcpyAsync(b->dS, S.data(), sizeof(BSample) * S.size(), hipMemcpyHostToDevice, b->stream));
+    if (b->shape_dirty) {
+        BCHK(hipMemcpyAsync(b->dC, C.data(), sizeof(BChunk) * C.size(), hipMemcpyHostToDevice, b->stream));
+        BCHK(hipMemcpyAsync(b->dG, G.data(), sizeof(BChunk) * G.size(), hipMemcpyHostToDevice, b->stream));
+        BCHK(hipMemcpyAsync(b->dP, P.data(), sizeof(BChunk) * P.size(), hipMemcpyHostToDevice, b->stream));
+        b->nS = (int)S.size();
+        b->nC = (int)C.size();
+        b->nG = (int)G.size();
+        b->nP = (int)P.size();
+        drop_graph(b);
+    }
+    if (b->prm.use_graph) {
+        if (!b->exec) {
+            BCHK(hipStreamSynchronize(b->stream));
+            BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            launch_interval(b);
+            BCHK(hipStreamEndCapture(b->stream, &b->graph));
+            BCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        }
+        BCHK(hipGraphLaunch(b->exec, b->stream));
+    } else {
+        launch_interval(b);
+    }
+    BCHK(hipGetLastError());
+    BCHK(hipMemcpyAsync(b->hOut, b->dOut, sizeof(pm::RedSums) * (size_t)b->nS, hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    b->shape_dirty = false;
+    int running = 0;
+    for (size_t i = 0; i < b->run.size(); i++) {
+        const int slot = b->run[i];
+        pm_ctx *c = b->ctx[slot];
+        c->t += b->prm.it_eval;
+        if (b->prm.it_eval & 1) swap_current(b, c);
+        const int it = c->t - 1;  // iteration index of the last sweep
+        const pm::RedSums &r = b->hOut[i];
+        double k[8];
+        pm_permeability_from(r, c->prm.dom_interest, c->prm.voxel_size, pm_used_porosity(c), k);
+        pm_log_row &row = b->last[slot];
+        row.iteration = it;
+        row.wmax_velz = k[6]; row.wmean_velz = k[7];
+        row.k13 = k[0]; row.k23 = k[1]; row.k33 = k[2];
+        row.wk13 = k[3]; row.wk23 = k[4]; row.wk33 = k[5];
+        if (it % b->prm.it_eval == 0 && it > 1000) {
+            b->conv[slot] = pm_convergence_from(r, &c->kprev);
+            if (it % b->prm.it_write == 0) b->n_rows[slot]++;
+        }
+        row.conv = b->conv[slot];
+        int st = PM_SAMPLE_RUNNING;
+        if (!std::isfinite(b->conv[slot])) st = PM_SAMPLE_DIVERGED;
+        else if (b->conv[slot] < b->prm.eps) st = PM_SAMPLE_CONVERGED;
+        else if (c->t >= b->prm.max_iter) st = PM_SAMPLE_MAX_ITER;
+        b->state[slot] = st;
+        if (st != PM_SAMPLE_RUNNING) b->shape_dirty = true;
+        else running++;
+    }
+    if (n_running) *n_running = running;
+    return PM_OK;
+}
+
+extern "C" int pm_batch_status(pm_batch *b, int slot, int *state, pm_result *res) {
+    if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    pm_ctx *c = b->ctx[slot];
+    if (state) *state = b->state[slot];
+    if (res) {
+        std::memset(res, 0, sizeof *res);
+        res->sweeps = c->t;
+        res->conv = b->conv[slot];
+        res->porosity = pm_used_porosity(c);
+        res->n_log_rows = b->n_rows[slot];
+        res->last = b->last[slot];
+    }
+    return PM_OK;
+}
+
+extern "C" int pm_batch_get_field(pm_batch *b, int slot, int field, void *out) {
+    if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    BCHK(hipStreamSynchronize(b->stream));
+    return pm_get_field(b->ctx[slot], field, out);
+}
+
+extern "C" int pm_batch_remove(pm_batch *b, int slot) {
+    if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    BCHK(hipStreamSynchronize(b->stream));
+    pm_destroy(b->ctx[slot]);
+    b->ctx[slot] = nullptr;
+    b->used -= b->bytes[slot];
+    b->bytes[slot] = 0;
+    b->shape_dirty = true;
+    return PM_OK;
+}
+
+extern "C" int pm_batch_memory(pm_batch *b, int64_t *used, int64_t *budget) {
+    if (!b) return pm_fail(PM_E_ARG, "null batch");
+    if (used) *used = b->used;
+    if (budget) *budget = b->budget;
+    return PM_OK;
+}
+
+extern "C" void pm_batch_destroy(pm_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->prm.device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (pm_ctx *c : b->ctx)
+        if (c) pm_destroy(c);
+    if (b->dS) (void)hipFree(b->dS);
+    if (b->dC) (void)hipFree(b->dC);
+    if (b->dG) (void)hipFree(b->dG);
+    if (b->dP) (void)hipFree(b->dP);
+    if (b->dPart) (void)hipFree(b->dPart);
+    if (b->dOut) (void)hipFree(b->dOut);
+    if (b->hOut) (void)hipHostFree(b->hOut);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}

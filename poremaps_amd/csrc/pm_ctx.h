@@ -1,0 +1,62 @@
+// pm_ctx.h -- the solver context (one geometry on one GPU), shared by the
+// single-solve C ABI (pm_solver.hip) and the batch engine (pm_batch.hip).
+#pragma once
+
+#include "../../include/poremaps_amd.h"
+#include "pm_kernels.h"
+
+#include <vector>
+
+// Event pairs around kernel groups (pm_time_kernels); inactive when null.
+struct Timer {
+    struct Rec { int cat; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    int open_cat = -1;
+    hipEvent_t open_ev = nullptr;
+};
+
+enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_N };
+static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep"};
+
+struct pm_ctx {
+    Timer *timer = nullptr;
+    pm_params prm;
+    pm::Grid g;
+    hipStream_t stream = nullptr;
+    bool loaded = false;
+    // geometry
+    uint8_t *G = nullptr, *pc = nullptr;
+    int32_t *code = nullptr;
+    // FP64 state (reference mode, FP64 fast modes, defect-correction master)
+    double *X[4] = {nullptr, nullptr, nullptr, nullptr};   // p, u, v, w
+    double *X2[4] = {nullptr, nullptr, nullptr, nullptr};  // Jacobi second buffer
+    double *qX[2] = {nullptr, nullptr};
+    // FP32 correction state
+    float *D[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *D2[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *qD[2] = {nullptr, nullptr};
+    float *R[4] = {nullptr, nullptr, nullptr, nullptr};
+    // reductions
+    pm::FluidLists lists;   // fluid cells of the interior (fast modes)
+    void *red_scratch = nullptr;
+    pm::RedSums *red_dev = nullptr;
+    pm::RedSums *red_host = nullptr;
+    // iteration state
+    int t = 0;            // sweeps done
+    double kprev = 0.0;   // permeability of the previous convergence test
+    double phi = 0.0;
+    int64_t fluid = 0;
+
+    bool fast() const { return prm.mode != PM_MODE_REFERENCE; }
+    bool dc() const { return fast() && prm.precision == PM_PREC_F32DC; }
+    bool jacobi() const { return prm.mode == PM_MODE_JACOBI; }
+};
+
+
+// Context internals used by the batch engine (defined in pm_solver.hip).
+int pm_fail(int code, const char *fmt, ...);
+int pm_one_sweep(pm_ctx *c);
+int pm_reduce_state(pm_ctx *c, bool full, pm::RedSums *out);
+double pm_convergence_from(const pm::RedSums &r, double *kprev);
+void pm_permeability_from(const pm::RedSums &r, const int *dom, double dx, double phi, double *out);
+double pm_used_porosity(const pm_ctx *c);

@@ -11,6 +11,8 @@ struct DomBox {
     int d[6];
 };
 
+constexpr int kRedThreads = 256;
+
 // Results of a state reduction: sum[0..2] fluid u, v, w; sum[3..5] box fluid
 // u, v, w; sum[6], sum[7] p on the box's zmin / zmax planes; count[0..3] the
 // matching fluid counts; wmax = max(0, max fluid w).

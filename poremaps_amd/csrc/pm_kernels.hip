@@ -10,6 +10,7 @@
 // FP64 or FP32, with a one-pass ghost fill, device reductions and the FP64
 // defect-correction passes.
 #include "pm_kernels.h"
+#include "pm_reduce.h"
 
 #include <algorithm>
 
@@ -280,111 +281,11 @@ __global__ __launch_bounds__(1024) void k_ref_sweeps_1wg(Grid g, const uint8_t *
 // fast modes
 // ---------------------------------------------------------------------------
 
-// Fast-mode ghost fill in one pass over the face-halo lines: the outlet
-// plane's interior is pinned, periodic faces are copied (the outlet plane's
-// copies taken from the pinned values), then the inlet/top halo planes are
-// pinned.  Corner halo cells are never read by the axis-aligned stencil and
-// are left alone.
-template <typename T>
-__global__ void k_ghost_fast(Grid g, const uint8_t *__restrict__ G, T *p, T *u, T *v, T *w,
-                             T *q, T pin_in, T pin_hi, T pin_out) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nX = (int64_t)g.nz * g.ny, nY = (int64_t)g.nz * g.nx, nZ = (int64_t)g.ny * g.nx;
-    const int iout = g.nz + 1;
-    if (t < nX) {
-        if (!g.px) return;
-        const int i = (int)(t / g.ny) + 2, j = (int)(t % g.ny) + 2;
-        const int64_t r = g.at(i, j, 0);
-        const int src[4] = {g.nx, g.nx + 1, 2, 3};
-        const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
-        for (int e = 0; e < 4; e++) {
-            const int64_t s = r + src[e], d = r + dst[e];
-            u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
-            p[d] = (i == iout && G[s] == 0) ? pin_out : p[s];
-            if (q && i == iout) q[g.qat(j, dst[e])] = q[g.qat(j, src[e])];
-        }
-    } else if (t < nX + nY) {
-        if (!g.py) return;
-        const int64_t tt = t - nX;
-        const int i = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
-        const int src[4] = {g.ny, g.ny + 1, 2, 3};
-        const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
-        for (int e = 0; e < 4; e++) {
-            const int64_t s = g.at(i, src[e], k), d = g.at(i, dst[e], k);
-            u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
-            p[d] = (i == iout && G[s] == 0) ? pin_out : p[s];
-            if (q && i == iout) q[g.qat(dst[e], k)] = q[g.qat(src[e], k)];
-        }
-    } else if (t < nX + nY + nZ) {
-        const int64_t tt = t - nX - nY;
-        const int j = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
-        const int64_t cout = g.at(iout, j, k);
-        const bool out_fluid = G[cout] == 0;
-        if (g.pz) {
-            const int src[4] = {g.nz, g.nz + 1, 2, 3};
-            const int dst[4] = {0, 1, g.nz + 2, g.nz + 3};
-            for (int e = 0; e < 4; e++) {
-                const int64_t s = g.at(src[e], j, k), d = g.at(dst[e], j, k);
-                u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
-                p[d] = (src[e] == iout && out_fluid) ? pin_out : p[s];
-            }
-        }
-        const int64_t c1 = g.at(1, j, k), chi = g.at(g.nz + 2, j, k);
-        if (G[c1] == 0) p[c1] = pin_in;
-        if (G[chi] == 0) p[chi] = pin_hi;
-        if (out_fluid) p[cout] = pin_out;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // reductions (deterministic, compensated FP64)
 // ---------------------------------------------------------------------------
 
-struct NSum {
-    double s, c;
-};
-
-__device__ __forceinline__ void nadd(NSum &a, double x) {
-    const double t = a.s + x;
-    if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
-    else a.c += (x - t) + a.s;
-    a.s = t;
-}
-__device__ __forceinline__ void nmerge(NSum &a, const NSum &b) {
-    nadd(a, b.s);
-    a.c += b.c;
-}
-
-struct RedAcc {
-    NSum s[8];         // u, v, w (fluid); u, v, w (box fluid); p(z=zmin), p(z=zmax)
-    long long n[4];    // fluid, box fluid, zmin fluid, zmax fluid
-    double wmax;
-};
-
-__device__ __forceinline__ void acc_zero(RedAcc &a) {
-    for (int q = 0; q < 8; q++) { a.s[q].s = 0.0; a.s[q].c = 0.0; }
-    for (int q = 0; q < 4; q++) a.n[q] = 0;
-    a.wmax = 0.0;
-}
-__device__ __forceinline__ void acc_merge(RedAcc &a, const RedAcc &b) {
-    for (int q = 0; q < 8; q++) nmerge(a.s[q], b.s[q]);
-    for (int q = 0; q < 4; q++) a.n[q] += b.n[q];
-    a.wmax = fmax(a.wmax, b.wmax);
-}
-
-constexpr int kRedThreads = 256;
 constexpr int kRedBlocks = 1024;
-
-__device__ void block_reduce_store(RedAcc &a, RedAcc *out) {
-    __shared__ RedAcc sh[kRedThreads];
-    sh[threadIdx.x] = a;
-    __syncthreads();
-    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) acc_merge(sh[threadIdx.x], sh[threadIdx.x + off]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
-}
 
 // FULL = false: only the sum of w and the fluid count (compute_convergence).
 template <bool FULL>
@@ -440,66 +341,11 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const RedAcc *__re
 }
 
 // ---------------------------------------------------------------------------
-// defect correction
-// ---------------------------------------------------------------------------
-
-// Outlet-plane buffer of a FP64 state: pin_out - c2*dt*div at interior fluid
-// cells of z = nz+1; elsewhere p at the (periodic image of the) cell.
-__global__ void k_outlet_plane(Grid g, const uint8_t *__restrict__ pc,
-                               const double *__restrict__ p, const double *__restrict__ u,
-                               const double *__restrict__ v, const double *__restrict__ w,
-                               double pin_out, double *__restrict__ q) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (k >= g.NX) return;
-    const int i = g.nz + 1;
-    int jj = j, kk = k;
-    if (g.px) { if (kk < 2) kk += g.nx; else if (kk >= g.nx + 2) kk -= g.nx; }
-    if (g.py) { if (jj < 2) jj += g.ny; else if (jj >= g.ny + 2) jj -= g.ny; }
-    const bool in = jj >= 2 && jj < g.ny + 2 && kk >= 2 && kk < g.nx + 2;
-    const int64_t c = g.at(i, jj, kk);
-    double val;
-    if (in && pc[c] != kSolid) {
-        const double div = ((((w[c + g.sy] - w[c]) + v[c + g.sx]) - v[c]) + u[c + 1]) - u[c];
-        val = pin_out - div * (kC2 * kDt);
-    } else {
-        val = p[c];
-    }
-    q[g.qat(j, k)] = val;
-}
-
-// X += d on the fluid interior, d = 0 everywhere.
-__global__ void k_fold(Grid g, const uint8_t *__restrict__ pc, double *Xp, double *Xu,
-                       double *Xv, double *Xw, float *dp, float *du, float *dv, float *dw) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.total) return;
-    const int64_t e = c - g.base;
-    bool in = false;
-    if (e >= 0) {
-        const int i = (int)(e / g.sy);
-        const int64_t r = e % g.sy;
-        const int j = (int)(r / g.sx), k = (int)(r % g.sx);
-        in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
-    }
-    if (in && pc[c] != kSolid) {
-        Xp[c] = Xp[c] + (double)dp[c];
-        Xu[c] = Xu[c] + (double)du[c];
-        Xv[c] = Xv[c] + (double)dv[c];
-        Xw[c] = Xw[c] + (double)dw[c];
-    }
-    dp[c] = 0.f; du[c] = 0.f; dv[c] = 0.f; dw[c] = 0.f;
-}
-
-// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
 static inline dim3 row_grid(const Grid &g) {
     return dim3((unsigned)((g.NX + 63) / 64), (unsigned)g.NY, (unsigned)g.NZ);
-}
-static inline dim3 int_block() { return dim3(64, 4, 1); }
-static inline dim3 int_grid(const Grid &g) {
-    return dim3((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nz);
 }
 static inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -563,18 +409,6 @@ void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, dou
     }
 }
 
-template <typename T>
-void launch_ghost_fast(const Grid &g, const uint8_t *G, T *p, T *u, T *v, T *w, T *q, T pin_in, T pin_hi,
-                       T pin_out, hipStream_t s) {
-    const int64_t n = (int64_t)g.nz * g.ny + (int64_t)g.nz * g.nx + (int64_t)g.ny * g.nx;
-    hipLaunchKernelGGL(k_ghost_fast<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, g, G, p, u, v, w, q, pin_in,
-                       pin_hi, pin_out);
-}
-template void launch_ghost_fast<double>(const Grid &, const uint8_t *, double *, double *, double *, double *,
-                                        double *, double, double, double, hipStream_t);
-template void launch_ghost_fast<float>(const Grid &, const uint8_t *, float *, float *, float *, float *,
-                                       float *, float, float, float, hipStream_t);
-
 void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const double *u, const double *v,
                    const double *w, const DomBox &box, bool full, void *scratch, RedSums *out,
                    hipStream_t s) {
@@ -590,17 +424,5 @@ void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const doubl
 }
 
 size_t reduce_scratch_bytes() { return sizeof(RedAcc) * kRedBlocks; }
-
-void launch_outlet_plane(const Grid &g, const uint8_t *pc, const double *p, const double *u,
-                         const double *v, const double *w, double pin_out, double *q, hipStream_t s) {
-    hipLaunchKernelGGL(k_outlet_plane, dim3(nblocks(g.NX, 64), (unsigned)g.NY), dim3(64), 0, s, g, pc, p, u,
-                       v, w, pin_out, q);
-}
-
-void launch_fold(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw,
-                 float *dp, float *du, float *dv, float *dw, hipStream_t s) {
-    hipLaunchKernelGGL(k_fold, dim3(nblocks(g.total, 256)), dim3(256), 0, s, g, pc, Xp, Xu, Xv, Xw, dp, du,
-                       dv, dw);
-}
 
 }  // namespace pm

@@ -1,7 +1,7 @@
 // pm_solver.hip -- host side of the solver: context, device memory, the
 // iteration loop of src/calc_flow.cc:53-169 and the C ABI of poremaps_amd.h.
 #include "../../include/poremaps_amd.h"
-#include "pm_kernels.h"
+#include "pm_ctx.h"
 
 #include <algorithm>
 #include <chrono>
@@ -44,51 +44,6 @@ __global__ void k_fill_f32(float *a, int64_t n, float v) {
 }
 
 }  // namespace
-
-// Event pairs around kernel groups (pm_time_kernels); inactive when null.
-struct Timer {
-    struct Rec { int cat; hipEvent_t a, b; };
-    std::vector<Rec> recs;
-    int open_cat = -1;
-    hipEvent_t open_ev = nullptr;
-};
-
-enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_N };
-static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep"};
-
-struct pm_ctx {
-    Timer *timer = nullptr;
-    pm_params prm;
-    pm::Grid g;
-    hipStream_t stream = nullptr;
-    bool loaded = false;
-    // geometry
-    uint8_t *G = nullptr, *pc = nullptr;
-    int32_t *code = nullptr;
-    // FP64 state (reference mode, FP64 fast modes, defect-correction master)
-    double *X[4] = {nullptr, nullptr, nullptr, nullptr};   // p, u, v, w
-    double *X2[4] = {nullptr, nullptr, nullptr, nullptr};  // Jacobi second buffer
-    double *qX[2] = {nullptr, nullptr};
-    // FP32 correction state
-    float *D[4] = {nullptr, nullptr, nullptr, nullptr};
-    float *D2[4] = {nullptr, nullptr, nullptr, nullptr};
-    float *qD[2] = {nullptr, nullptr};
-    float *R[4] = {nullptr, nullptr, nullptr, nullptr};
-    // reductions
-    pm::FluidLists lists;   // fluid cells of the interior (fast modes)
-    void *red_scratch = nullptr;
-    pm::RedSums *red_dev = nullptr;
-    pm::RedSums *red_host = nullptr;
-    // iteration state
-    int t = 0;            // sweeps done
-    double kprev = 0.0;   // permeability of the previous convergence test
-    double phi = 0.0;
-    int64_t fluid = 0;
-
-    bool fast() const { return prm.mode != PM_MODE_REFERENCE; }
-    bool dc() const { return fast() && prm.precision == PM_PREC_F32DC; }
-    bool jacobi() const { return prm.mode == PM_MODE_JACOBI; }
-};
 
 extern "C" const char *pm_last_error(void) { return g_err.c_str(); }
 
@@ -721,3 +676,20 @@ extern "C" int pm_time_kernels(pm_ctx *c, int n, pm_kernel_timing *out, int max_
     *n_out = k;
     return PM_OK;
 }
+
+// Context internals for the batch engine (declared in pm_ctx.h).
+int pm_fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    return fail(code, "%s", buf);
+}
+int pm_one_sweep(pm_ctx *c) { return one_sweep(c); }
+int pm_reduce_state(pm_ctx *c, bool full, pm::RedSums *out) { return reduce(c, full, out); }
+double pm_convergence_from(const pm::RedSums &r, double *kprev) { return convergence_from(r, kprev); }
+void pm_permeability_from(const pm::RedSums &r, const int *dom, double dx, double phi, double *out) {
+    permeability_from(r, dom, dx, phi, out);
+}
+double pm_used_porosity(const pm_ctx *c) { return used_porosity(c); }

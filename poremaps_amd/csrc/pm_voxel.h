@@ -1,0 +1,224 @@
+// pm_voxel.h -- per-cell work of the fast modes, shared by the single-solve
+// kernels (pm_sweep.hip, pm_kernels.hip) and the batched kernels
+// (pm_batch.hip).  Each function handles one fluid cell (element offset c) or
+// one halo line of one geometry described by a Grid.
+#pragma once
+
+#include "pm_grid.h"
+
+#pragma clang fp contract(off)
+
+namespace pm {
+
+__device__ __forceinline__ void cell_ijk(const Grid &g, int64_t c, int &i, int &j, int &k) {
+    const uint32_t e = (uint32_t)(c - g.base);
+    const uint32_t sy = (uint32_t)g.sy, sx = (uint32_t)g.sx;
+    i = (int)(e / sy);
+    const uint32_t r = e - (uint32_t)i * sy;
+    j = (int)(r / sx);
+    k = (int)(r - (uint32_t)j * sx);
+}
+
+__device__ __forceinline__ bool on_outlet(const Grid &g, int64_t c) {
+    const int64_t o0 = g.at(g.nz + 1, 0, 0);
+    return c >= o0 && c < o0 + g.sy;
+}
+
+// Symplectic Jacobi, velocity part: (p0,u0,v0,w0) -> (u1,v1,w1) at c.
+template <typename T, bool HAS_R, bool SEL>
+__device__ __forceinline__ void vel_voxel(const Grid &g, int64_t c, uint8_t pcv, const T *__restrict__ p0,
+                                          const T *__restrict__ u0, const T *__restrict__ v0,
+                                          const T *__restrict__ w0, const T *__restrict__ q0,
+                                          T *__restrict__ u1, T *__restrict__ v1, T *__restrict__ w1,
+                                          const float *__restrict__ Ru, const float *__restrict__ Rv,
+                                          const float *__restrict__ Rw, T cu) {
+    const Cases cs = decode(pcv);
+    const bool out = on_outlet(g, c);
+    const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
+    T fu, fv, fw;
+    if (SEL) fast_force_s<T>(cs, c, g.sx, g.sy, p0, u0, v0, w0, out ? q0 : nullptr, qc, fu, fv, fw);
+    else fast_force_p<T>(cs, c, g.sx, g.sy, p0, u0, v0, w0, out ? q0 : nullptr, qc, fu, fv, fw);
+    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
+    u1[c] = cs.bx ? (T)0 : u0[c] + fu * cu;
+    v1[c] = cs.by ? (T)0 : v0[c] + fv * cu;
+    w1[c] = cs.bz ? (T)0 : w0[c] + fw * cu;
+}
+
+// Symplectic Jacobi, pressure part: divergence of the new velocities (+1
+// neighbours wrapped on periodic axes, 0 where solid) and the old pressure;
+// on the outlet plane also q1 = pin_out - c2*dt*div (residual excluded).
+template <typename T, bool HAS_R>
+__device__ __forceinline__ void pres_voxel(const Grid &g, int64_t c, uint8_t pcv, const T *__restrict__ p0,
+                                           const T *__restrict__ u1, const T *__restrict__ v1,
+                                           const T *__restrict__ w1, T *__restrict__ p1, T *__restrict__ q1,
+                                           const float *__restrict__ Rp, T cp, T pin_out) {
+    const Cases cs = decode(pcv);
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const int64_t cxp = (k == g.nx + 1 && g.px) ? c + 1 - g.nx : c + 1;
+    const int64_t cyp = (j == g.ny + 1 && g.py) ? c + g.sx - (int64_t)g.ny * g.sx : c + g.sx;
+    const int64_t czp = (i == g.nz + 1 && g.pz) ? c + g.sy - (int64_t)g.nz * g.sy : c + g.sy;
+    const T uc = u1[c], vc = v1[c], wc = w1[c];
+    const T wz = fwd_solid(cs.cz) ? (T)0 : w1[czp];
+    const T vy = fwd_solid(cs.cy) ? (T)0 : v1[cyp];
+    const T ux = fwd_solid(cs.cx) ? (T)0 : u1[cxp];
+    const T div0 = ((((wz - wc) + vy) - vc) + ux) - uc;
+    T div = div0;
+    if (HAS_R) div = div + (T)Rp[c];
+    p1[c] = p0[c] - div * cp;
+    if (i == g.nz + 1) q1[g.qat(j, k)] = pin_out - div0 * (T)(kC2 * kDt);
+}
+
+// 3-colour SOR: one cell of the current colour, in place.
+template <typename T, bool HAS_R, bool SEL>
+__device__ __forceinline__ void colour_voxel(const Grid &g, int64_t c, uint8_t pcv, T *p, T *u, T *v, T *w,
+                                             T *q, T cu, T cp, T pin_out, const float *__restrict__ Rp,
+                                             const float *__restrict__ Ru, const float *__restrict__ Rv,
+                                             const float *__restrict__ Rw) {
+    const Cases cs = decode(pcv);
+    const bool out = on_outlet(g, c);
+    const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
+    T fu, fv, fw;
+    if (SEL) fast_force_s<T>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, qc, fu, fv, fw);
+    else fast_force_p<T>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, qc, fu, fv, fw);
+    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
+    const T un = cs.bx ? (T)0 : u[c] + fu * cu;
+    const T vn = cs.by ? (T)0 : v[c] + fv * cu;
+    const T wn = cs.bz ? (T)0 : w[c] + fw * cu;
+    u[c] = un; v[c] = vn; w[c] = wn;
+    const T wz = fwd_solid(cs.cz) ? (T)0 : w[c + g.sy];
+    const T vy = fwd_solid(cs.cy) ? (T)0 : v[c + g.sx];
+    const T ux = fwd_solid(cs.cx) ? (T)0 : u[c + 1];
+    const T div0 = ((((wz - wn) + vy) - vn) + ux) - un;
+    T div = div0;
+    if (HAS_R) div = div + (T)Rp[c];
+    p[c] = p[c] - div * cp;
+    if (out) q[qc] = pin_out - div0 * (T)(kC2 * kDt);
+}
+
+// Momentum and continuity residual of a FP64 state at c, rounded to FP32.
+__device__ __forceinline__ void residual_voxel(const Grid &g, int64_t c, uint8_t pcv,
+                                               const double *__restrict__ p, const double *__restrict__ u,
+                                               const double *__restrict__ v, const double *__restrict__ w,
+                                               const double *__restrict__ q, float *__restrict__ Rp,
+                                               float *__restrict__ Ru, float *__restrict__ Rv,
+                                               float *__restrict__ Rw) {
+    const Cases cs = decode(pcv);
+    const bool out = on_outlet(g, c);
+    double fu, fv, fw;
+    fast_force_p<double>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, c - (int64_t)(g.nz + 1) * g.sy, fu,
+                         fv, fw);
+    Ru[c] = (float)fu; Rv[c] = (float)fv; Rw[c] = (float)fw;
+    const double wz = fwd_solid(cs.cz) ? 0.0 : w[c + g.sy];
+    const double vy = fwd_solid(cs.cy) ? 0.0 : v[c + g.sx];
+    const double ux = fwd_solid(cs.cx) ? 0.0 : u[c + 1];
+    const double div = ((((wz - w[c]) + vy) - v[c]) + ux) - u[c];
+    Rp[c] = (float)div;
+}
+
+// Number of halo lines the fast-mode ghost fill walks (x-face, y-face and
+// z-face lines of the interior).
+__host__ __device__ inline int64_t ghost_lines(const Grid &g) {
+    return (int64_t)g.nz * g.ny + (int64_t)g.nz * g.nx + (int64_t)g.ny * g.nx;
+}
+
+// Fast-mode ghost fill, line t: the outlet plane's interior is pinned,
+// periodic faces are copied (the outlet plane's copies taken from the pinned
+// values), then the inlet/top halo planes are pinned.  Corner halo cells are
+// never read by the axis-aligned stencil and are left alone.
+template <typename T>
+__device__ __forceinline__ void ghost_line(const Grid &g, int64_t t, const uint8_t *__restrict__ G, T *p, T *u,
+                                           T *v, T *w, T *q, T pin_in, T pin_hi, T pin_out) {
+    const int64_t nX = (int64_t)g.nz * g.ny, nY = (int64_t)g.nz * g.nx, nZ = (int64_t)g.ny * g.nx;
+    const int iout = g.nz + 1;
+    if (t < nX) {
+        if (!g.px) return;
+        const int i = (int)(t / g.ny) + 2, j = (int)(t % g.ny) + 2;
+        const int64_t r = g.at(i, j, 0);
+        const int src[4] = {g.nx, g.nx + 1, 2, 3};
+        const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
+        for (int e = 0; e < 4; e++) {
+            const int64_t s = r + src[e], d = r + dst[e];
+            u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
+            p[d] = (i == iout && G[s] == 0) ? pin_out : p[s];
+            if (q && i == iout) q[g.qat(j, dst[e])] = q[g.qat(j, src[e])];
+        }
+    } else if (t < nX + nY) {
+        if (!g.py) return;
+        const int64_t tt = t - nX;
+        const int i = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
+        const int src[4] = {g.ny, g.ny + 1, 2, 3};
+        const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
+        for (int e = 0; e < 4; e++) {
+            const int64_t s = g.at(i, src[e], k), d = g.at(i, dst[e], k);
+            u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
+            p[d] = (i == iout && G[s] == 0) ? pin_out : p[s];
+            if (q && i == iout) q[g.qat(dst[e], k)] = q[g.qat(src[e], k)];
+        }
+    } else if (t < nX + nY + nZ) {
+        const int64_t tt = t - nX - nY;
+        const int j = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
+        const int64_t cout = g.at(iout, j, k);
+        const bool out_fluid = G[cout] == 0;
+        if (g.pz) {
+            const int src[4] = {g.nz, g.nz + 1, 2, 3};
+            const int dst[4] = {0, 1, g.nz + 2, g.nz + 3};
+            for (int e = 0; e < 4; e++) {
+                const int64_t s = g.at(src[e], j, k), d = g.at(dst[e], j, k);
+                u[d] = u[s]; v[d] = v[s]; w[d] = w[s];
+                p[d] = (src[e] == iout && out_fluid) ? pin_out : p[s];
+            }
+        }
+        const int64_t c1 = g.at(1, j, k), chi = g.at(g.nz + 2, j, k);
+        if (G[c1] == 0) p[c1] = pin_in;
+        if (G[chi] == 0) p[chi] = pin_hi;
+        if (out_fluid) p[cout] = pin_out;
+    }
+}
+
+// Outlet-plane buffer of a FP64 state at plane position (j, k) (any j < NY,
+// k < NX): pin_out - c2*dt*div at interior fluid cells of z = nz+1, p at the
+// (periodic image of the) cell elsewhere.
+__device__ __forceinline__ void outlet_cell(const Grid &g, int j, int k, const uint8_t *__restrict__ pc,
+                                            const double *__restrict__ p, const double *__restrict__ u,
+                                            const double *__restrict__ v, const double *__restrict__ w,
+                                            double pin_out, double *__restrict__ q) {
+    const int i = g.nz + 1;
+    int jj = j, kk = k;
+    if (g.px) { if (kk < 2) kk += g.nx; else if (kk >= g.nx + 2) kk -= g.nx; }
+    if (g.py) { if (jj < 2) jj += g.ny; else if (jj >= g.ny + 2) jj -= g.ny; }
+    const bool in = jj >= 2 && jj < g.ny + 2 && kk >= 2 && kk < g.nx + 2;
+    const int64_t c = g.at(i, jj, kk);
+    double val;
+    if (in && pc[c] != kSolid) {
+        const double div = ((((w[c + g.sy] - w[c]) + v[c + g.sx]) - v[c]) + u[c + 1]) - u[c];
+        val = pin_out - div * (kC2 * kDt);
+    } else {
+        val = p[c];
+    }
+    q[g.qat(j, k)] = val;
+}
+
+// Defect-correction fold at element c of the padded array: X += d on the
+// fluid interior, d = 0 everywhere.
+__device__ __forceinline__ void fold_cell(const Grid &g, int64_t c, const uint8_t *__restrict__ pc, double *Xp,
+                                          double *Xu, double *Xv, double *Xw, float *dp, float *du, float *dv,
+                                          float *dw) {
+    const int64_t e = c - g.base;
+    bool in = false;
+    if (e >= 0) {
+        const int i = (int)(e / g.sy);
+        const int64_t r = e % g.sy;
+        const int j = (int)(r / g.sx), k = (int)(r % g.sx);
+        in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
+    }
+    if (in && pc[c] != kSolid) {
+        Xp[c] = Xp[c] + (double)dp[c];
+        Xu[c] = Xu[c] + (double)du[c];
+        Xv[c] = Xv[c] + (double)dv[c];
+        Xw[c] = Xw[c] + (double)dw[c];
+    }
+    dp[c] = 0.f; du[c] = 0.f; dv[c] = 0.f; dw[c] = 0.f;
+}
+
+}  // namespace pm

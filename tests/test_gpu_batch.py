@@ -1,0 +1,98 @@
+"""Batch engine on the GPU: every sample of a batch (any mix of sizes and
+boundary methods, samples joining and leaving between intervals) ends with the
+sweep count, k and fields of the same solve run alone -- bit for bit."""
+import numpy as np
+import pytest
+
+import poremaps_amd as pm
+from poremaps_amd import geometry as geo
+from conftest import random_geometry
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def cases(channel_raw):
+    return [
+        (random_geometry((20, 18, 22), 0.3, 3), 0, 1e-5),
+        (channel_raw, 0, 5e-5),
+        (geo.tube(30, 30, 24, 15, 15, 9), 2, 2e-5),
+        (geo.sphere_pack((22, 16, 18), 0.45, 2, 4, seed=5, require_percolation=False), 1, 1e-5),
+    ]
+
+
+def alone(raw, bc, dx, precision, omega):
+    with pm.Solver(raw, boundary_method=bc, voxel_size=dx, mode="jacobi", precision=precision, omega=omega,
+                   eps=1e-6) as s:
+        res, _ = s.solve()
+        return res, s.field("velz"), s.field("press")
+
+
+def run_batch(b, slots):
+    for _ in range(100000):
+        if b.step() == 0:
+            break
+    return {s: b.status(s) for s in slots}
+
+
+@pytest.mark.parametrize("precision,graph", [("f32dc", True), ("f32dc", False), ("f64", True)])
+def test_batch_equals_single_solves(channel_raw, precision, graph):
+    say("batch vs single", precision, graph)
+    cs = cases(channel_raw)
+    with pm.Batch(precision=precision, omega=2.0, use_graph=graph) as b:
+        slots = [b.add(raw, boundary_method=bc, voxel_size=dx) for raw, bc, dx in cs]
+        st = run_batch(b, slots)
+        for slot, (raw, bc, dx) in zip(slots, cs):
+            state, res = st[slot]
+            ref, velz, press = alone(raw, bc, dx, precision, 2.0)
+            assert state == "converged"
+            assert res.sweeps == ref.sweeps
+            assert res.last.wk33 == ref.last.wk33
+            assert np.array_equal(b.field(slot, "velz"), velz)
+            assert np.array_equal(b.field(slot, "press"), press)
+
+
+def test_batch_join_and_evict(channel_raw):
+    """Continuous batching: a sample admitted while others run, and samples
+    evicted as they finish, still reproduce their solo results."""
+    say("batch join/evict")
+    cs = cases(channel_raw)
+    with pm.Batch(precision="f32dc", omega=2.0) as b:
+        s0 = b.add(cs[0][0], boundary_method=cs[0][1], voxel_size=cs[0][2])
+        s1 = b.add(cs[1][0], boundary_method=cs[1][1], voxel_size=cs[1][2])
+        for _ in range(7):
+            b.step()
+        s2 = b.add(cs[2][0], boundary_method=cs[2][1], voxel_size=cs[2][2])
+        done = {}
+        for _ in range(100000):
+            n = b.step()
+            for s in (s0, s1, s2):
+                if s not in done and b.status(s)[0] != "running":
+                    done[s] = (b.status(s)[1], b.field(s, "velz"))
+                    b.remove(s)
+            if n == 0 and len(done) == 3:
+                break
+        for s, (raw, bc, dx) in zip((s0, s1, s2), cs[:3]):
+            res, velz = done[s]
+            ref, rvelz, _ = alone(raw, bc, dx, "f32dc", 2.0)
+            assert res.sweeps == ref.sweeps and res.last.wk33 == ref.last.wk33
+            assert np.array_equal(velz, rvelz)
+        assert b.memory()[0] == 0
+
+
+def test_batch_admission_by_memory(channel_raw):
+    say("batch admission")
+    with pm.Batch(precision="f32dc", budget_bytes=1) as b:
+        with pytest.raises(pm.PoremapsError) as e:
+            b.add(channel_raw)
+        assert e.value.code == -4
+    one = pm.Batch(precision="f32dc").sample_bytes(channel_raw.shape)
+    with pm.Batch(precision="f32dc", budget_bytes=int(one * 1.5)) as b:
+        b.add(channel_raw)
+        with pytest.raises(pm.PoremapsError):
+            b.add(channel_raw)
+        used, budget = b.memory()
+        assert used == one and budget == int(one * 1.5)
