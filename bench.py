@@ -3,6 +3,8 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
                     [--mode jacobi|sor3] [--precision f32dc|f64] [--omega w]
                     [--tte] [--no-cpu]
+    python bench.py --config 3 [--count 64] [--compare]      (batch)
+    python bench.py --config 4 [--count 24] [--rate 4] [--compare]  (serving)
 
 A step is one it_eval interval of the solver (100 sweeps of the bulk sweep,
 with the defect-correction fold and residual of that interval) over the whole
@@ -12,6 +14,8 @@ Workloads (synthetic, generated from a seed):
   config 1: overlapping-sphere pack 256^3, phi 0.35, r 6-12, bc 2, dx 1e-5
   config 2: thresholded Gaussian random field 512^3, corr. length 8, phi 0.20,
             bc 1, dx 5e-6 (default: the largest single-GPU configuration)
+  config 3: 64 media solved to eps in one batch -> solves/h (whole job timed)
+  config 4: Poisson stream of tensor requests -> solves/h, p50/p99 latency
 
 roofline: the dominant kernel group ("sweep") timed with HIP events on the
 library's own stream over the timed region; achieved = algorithmic bytes per
@@ -108,6 +112,126 @@ def time_kernels(s, n):
                                        bytes=out[i].bytes_per_launch) for i in range(cnt.value)}
 
 
+def _solve_alone(raw, bc, dx, **kw):
+    import poremaps_amd as pm
+
+    t0 = time.perf_counter()
+    with pm.Solver(raw, boundary_method=bc, voxel_size=dx, eps=1e-6, **kw) as s:
+        res, _ = s.solve()
+    return time.perf_counter() - t0, res
+
+
+def bench_batch(a):
+    """Config 3: 64 cubic media (sides log-uniform 64-192, half sphere packs,
+    half Gaussian fields, porosity 0.15-0.45), bc 0, all solved to eps 1e-6
+    in one batch.  A step is one it_eval interval of every running sample;
+    the timed region is the whole job: admission (upload, geometry
+    classification, first sweep) and every interval until the last sample
+    converges."""
+    import poremaps_amd as pm
+    from poremaps_amd import geometry as geo
+
+    t0 = time.time()
+    media = geo.random_media(a.count, 64, 192, seed=3)
+    print(f"[bench] generated {len(media)} media in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    dense = sum(m["raw"].size for m in media)
+    with pm.Batch(precision=a.precision, omega=a.omega, it_eval=a.it_eval, eps=1e-6, max_iter=a.max_iter) as b:
+        sync = time.perf_counter()
+        slots = [b.add(m["raw"], boundary_method=0, voxel_size=1e-5) for m in media]
+        t_load = time.perf_counter() - sync
+        steps = 0
+        while True:
+            n = b.step()
+            steps += 1
+            if steps % 50 == 0:
+                print(f"[bench] interval {steps}: {n} running", file=sys.stderr, flush=True)
+            if n == 0:
+                break
+        el = time.perf_counter() - sync
+        st = [b.status(s) for s in slots]
+    sweeps = [r.sweeps for _, r in st]
+    lups = sum(m["raw"].size * r.sweeps for m, (_, r) in zip(media, st))
+    line = {
+        "metric": METRIC, "value": round(len(media) / el * 3600.0, 1), "unit": "solves/h", "n_gpus": 1,
+        "steps": steps, "warmup": 0, "ms_per_step": round(el / steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if a.precision == "f32dc" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batch of {len(media)} cubic media, sides log-uniform 64-192, half sphere "
+                               f"packs (r 3-10) half Gaussian fields (corr. 4-12), phi U[0.15,0.45], bc 0, "
+                               f"eps 1e-6, seed 3", "precision": a.precision, "omega": a.omega,
+                   "it_eval": a.it_eval, "dense_voxels": dense},
+        "seconds": round(el, 3), "load_seconds": round(t_load, 3),
+        "aggregate_glups": round(lups / el / 1e9, 2),
+        "sweeps": {"min": min(sweeps), "median": int(np.median(sweeps)), "max": max(sweeps)},
+        "states": sorted({s for s, _ in st}),
+    }
+    if a.compare:
+        for label, kw in (("one_at_a_time_same_mode", dict(mode="jacobi", precision=a.precision, omega=a.omega)),
+                          ("one_at_a_time_fp64_plain", dict(mode="jacobi", precision="f64", omega=1.0))):
+            tot = 0.0
+            for i, m in enumerate(media):
+                dt, _ = _solve_alone(m["raw"], 0, 1e-5, it_eval=a.it_eval, **kw)
+                tot += dt
+                print(f"[bench] {label} {i}: {dt:.2f}s", file=sys.stderr, flush=True)
+            line[label] = {"seconds": round(tot, 3), "solves_per_hour": round(len(media) / tot * 3600.0, 1),
+                           "batch_speedup": round(tot / el, 2)}
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(a):
+    """Config 4: Poisson arrivals (4 requests/s) of permeability-tensor
+    requests, sides log-uniform 48-320, sphere packs / Gaussian fields as in
+    the batch workload, bc uniform in {0, 1, 2}, eps 1e-6, three rotated
+    solves each, continuous batching on one GPU.  Baseline: the same requests
+    solved one at a time with the plain FP64 Jacobi solver."""
+    import poremaps_amd as pm
+    from poremaps_amd import geometry as geo
+
+    rng = np.random.default_rng(4)
+    t0 = time.time()
+    media = geo.random_media(a.count, 48, 320, seed=4, axes=(0, 1, 2))
+    print(f"[bench] generated {len(media)} media in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    arrivals = np.cumsum(rng.exponential(1.0 / a.rate, len(media)))
+    arrivals -= arrivals[0]
+    bcs = rng.integers(0, 3, len(media))
+    reqs = [(m["raw"], int(bc), 1e-5, 1e-6, float(t)) for m, bc, t in zip(media, bcs, arrivals)]
+    def progress(done, running, t):
+        print(f"[bench] t={t:.1f}s completed {done}/{len(reqs)}, {running} solves running", file=sys.stderr,
+              flush=True)
+
+    resp, st = pm.serve(reqs, realtime=True, progress=progress, precision=a.precision, omega=a.omega,
+                        it_eval=a.it_eval, max_iter=a.max_iter)
+    print(f"[bench] served: {json.dumps(st.as_dict())}", file=sys.stderr, flush=True)
+    line = {
+        "metric": METRIC, "value": round(st.solves_per_hour, 1), "unit": "solves/h", "n_gpus": 1,
+        "steps": st.intervals, "warmup": 0, "ms_per_step": round(st.wall_s / max(st.intervals, 1) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if a.precision == "f32dc" else "f64", "data": "synthetic",
+        "config": {"workload": f"{len(media)} tensor requests, Poisson {a.rate}/s, sides log-uniform 48-320, "
+                               f"sphere packs / Gaussian fields, phi U[0.15,0.45], bc U{{0,1,2}}, eps 1e-6, "
+                               f"seed 4", "precision": a.precision, "omega": a.omega, "it_eval": a.it_eval},
+        "latency_p50_s": round(st.latency_p50_s, 3), "latency_p99_s": round(st.latency_p99_s, 3),
+        "latency_max_s": round(st.latency_max_s, 3), "wall_s": round(st.wall_s, 3),
+        "completed": st.completed, "max_running_solves": st.max_running,
+        "peak_device_GB": round(st.peak_bytes / 1e9, 2),
+    }
+    if a.compare:
+        n_base = min(a.baseline_requests or len(reqs), len(reqs))
+        tot = 0.0
+        for i in range(n_base):
+            raw, bc = reqs[i][0], reqs[i][1]
+            for axes in ((0, 1, 2), (2, 0, 1), (1, 2, 0)):
+                dt, _ = _solve_alone(np.ascontiguousarray(raw.transpose(axes)), bc, 1e-5, mode="jacobi",
+                                     precision="f64", omega=1.0, it_eval=a.it_eval)
+                tot += dt
+            print(f"[bench] plain one-at-a-time request {i}: {tot:.1f}s so far", file=sys.stderr, flush=True)
+        sph = 3 * n_base / tot * 3600.0
+        line["one_at_a_time_fp64_plain"] = {"requests": n_base, "seconds": round(tot, 3),
+                                            "solves_per_hour": round(sph, 1)}
+        line["speedup_vs_plain"] = round(st.solves_per_hour / sph, 2)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,11 +244,22 @@ def main():
     ap.add_argument("--it-eval", type=int, default=100)
     ap.add_argument("--tte", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--count", type=int, default=None, help="media in the batch / serving workloads")
+    ap.add_argument("--rate", type=float, default=4.0, help="serving arrival rate, requests/s")
+    ap.add_argument("--compare", action="store_true", help="also time the one-at-a-time baselines")
+    ap.add_argument("--baseline-requests", type=int, default=0)
+    ap.add_argument("--max-iter", type=int, default=400_000, help="sweep cap per solve (batch, serving)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build solves one geometry per GPU and has no "
               f"multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
         sys.exit(2)
+    if a.config == 3:
+        a.count = a.count or 64
+        return bench_batch(a)
+    if a.config == 4:
+        a.count = a.count or 24
+        return bench_serve(a)
 
     import poremaps_amd as pm
 

@@ -55,6 +55,10 @@ int64_t pm_batch_sample_bytes(const pm_batch_params *p, int nx, int ny, int nz);
 int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int nz, int boundary_method,
                  double voxel_size, double porosity, int *slot);
 
+/* Stopping tolerance of one sample (default: the batch's eps), the eps of
+ * the reference's stop rule (src/main.cc:181-185). */
+int pm_batch_set_eps(pm_batch *b, int slot, double eps);
+
 /* One it_eval interval for every running sample, then their convergence
  * tests; *n_running = samples still running afterwards. */
 int pm_batch_step(pm_batch *b, int *n_running);

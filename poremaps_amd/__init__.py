@@ -238,6 +238,7 @@ def _batch_lib():
         L.pm_batch_remove.argtypes = [C.c_void_p, C.c_int]
         L.pm_batch_memory.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64)]
         L.pm_batch_destroy.argtypes = [C.c_void_p]
+        L.pm_batch_set_eps.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L._batch_declared = True
     return L
 
@@ -248,11 +249,7 @@ class Batch:
     def __init__(self, *, precision="f32dc", omega=2.0, it_eval=100, it_write=100, max_iter=100_000_000,
                  eps=1e-6, device=0, use_graph=True, budget_bytes=0):
         L = _batch_lib()
-        p = BatchParams()
-        L.pm_batch_params_default(C.byref(p))
-        p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
-        p.omega, p.it_eval, p.it_write, p.max_iter = omega, it_eval, it_write, max_iter
-        p.eps, p.device, p.use_graph, p.budget_bytes = eps, device, int(use_graph), budget_bytes
+        p = _batch_params(precision, omega, it_eval, it_write, max_iter, eps, device, use_graph, budget_bytes)
         self.params = p
         h = C.c_void_p()
         _check(L.pm_batch_create(C.byref(p), C.byref(h)))
@@ -271,6 +268,9 @@ class Batch:
                                          porosity, C.byref(slot)))
         self.shapes[slot.value] = raw.shape
         return slot.value
+
+    def set_eps(self, slot: int, eps: float):
+        _check(_batch_lib().pm_batch_set_eps(self._h, slot, eps))
 
     def step(self) -> int:
         n = C.c_int()
@@ -313,6 +313,99 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Request(C.Structure):
+    _fields_ = [("raw", C.c_void_p), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("boundary_method", C.c_int), ("voxel_size", C.c_double), ("porosity", C.c_double),
+                ("eps", C.c_double), ("arrival_s", C.c_double)]
+
+
+class Response(C.Structure):
+    _fields_ = [("k", C.c_double * 9), ("arrival_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double), ("sweeps", C.c_int * 3), ("state", C.c_int * 3)]
+
+    @property
+    def tensor(self) -> np.ndarray:
+        return np.array(self.k[:], dtype=np.float64).reshape(3, 3)
+
+    @property
+    def latency_s(self) -> float:
+        return self.done_s - self.arrival_s
+
+
+SERVE_PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p)
+
+
+class ServeParams(C.Structure):
+    _fields_ = [("batch", BatchParams), ("realtime", C.c_int), ("progress", SERVE_PROGRESS_FN),
+                ("user", C.c_void_p)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("completed", C.c_int), ("latency_p50_s", C.c_double),
+                ("latency_p99_s", C.c_double), ("latency_mean_s", C.c_double), ("latency_max_s", C.c_double),
+                ("solves_per_hour", C.c_double), ("max_running", C.c_int), ("peak_bytes", C.c_int64),
+                ("intervals", C.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def _serve_lib():
+    L = _batch_lib()
+    if not getattr(L, "_serve_declared", False):
+        P = C.POINTER
+        L.pm_tensor.argtypes = [P(BatchParams), C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                C.c_double, P(C.c_double), P(Result)]
+        L.pm_serve.argtypes = [P(ServeParams), P(Request), C.c_int, P(Response), P(ServeStats)]
+        L._serve_declared = True
+    return L
+
+
+def _batch_params(precision="f32dc", omega=2.0, it_eval=100, it_write=100, max_iter=100_000_000, eps=1e-6,
+                  device=0, use_graph=True, budget_bytes=0) -> BatchParams:
+    p = BatchParams()
+    _batch_lib().pm_batch_params_default(C.byref(p))
+    p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+    p.omega, p.it_eval, p.it_write, p.max_iter = omega, it_eval, it_write, max_iter
+    p.eps, p.device, p.use_graph, p.budget_bytes = eps, device, int(use_graph), budget_bytes
+    return p
+
+
+def tensor(raw: np.ndarray, *, boundary_method=0, voxel_size=1.0, porosity=-1.0, **batch_kw):
+    """Full 3x3 permeability tensor k[i, j] (velocity axis i, driving axis j;
+    0 = x, 1 = y, 2 = z) from three rotated solves that share the GPU
+    (include/poremaps_amd_serve.h).  Returns (k, [Result] * 3) with the
+    results in the order flow along z, x, y."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint8)
+    nz, ny, nx = raw.shape
+    p = _batch_params(**batch_kw)
+    k = (C.c_double * 9)()
+    res = (Result * 3)()
+    _check(_serve_lib().pm_tensor(C.byref(p), raw.ctypes.data, nx, ny, nz, boundary_method, voxel_size, porosity,
+                                  k, res))
+    return np.array(k[:]).reshape(3, 3), list(res)
+
+
+def serve(requests, *, realtime=True, progress=None, **batch_kw):
+    """Serve permeability-tensor requests: `requests` is a sequence of
+    (raw, boundary_method, voxel_size, eps, arrival_s) (eps <= 0: the batch's
+    eps).  ``progress(completed, running, t_s)`` is called as requests
+    complete and about once a second; a true return stops serving.  Returns
+    (responses, stats) with one Response per request, in request order."""
+    raws = [np.ascontiguousarray(r[0], dtype=np.uint8) for r in requests]
+    n = len(raws)
+    req = (Request * max(n, 1))()
+    for i, (raw, (_, bc, dx, eps, t)) in enumerate(zip(raws, requests)):
+        nz, ny, nx = raw.shape
+        req[i] = Request(raw.ctypes.data, nx, ny, nz, bc, dx, -1.0, eps, t)
+    resp = (Response * max(n, 1))()
+    st = ServeStats()
+    cb = SERVE_PROGRESS_FN(lambda d, r, t, _u: int(bool(progress(d, r, t)))) if progress else SERVE_PROGRESS_FN(0)
+    sp = ServeParams(_batch_params(**batch_kw), int(realtime), cb, None)
+    _check(_serve_lib().pm_serve(C.byref(sp), req, n, resp, C.byref(st)))
+    return list(resp)[:n], st
 
 
 def read_inp(path: str):

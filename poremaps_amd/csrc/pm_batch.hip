@@ -321,7 +321,9 @@ extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int
         b->n_rows.push_back(0);
     }
     b->ctx[s] = c;
-    b->state[s] = PM_SAMPLE_RUNNING;
+    // a geometry without fluid voxels has nothing to iterate: its fields stay
+    // at their initial values, so it is finished as soon as it is admitted
+    b->state[s] = c->lists.n_all > 0 ? PM_SAMPLE_RUNNING : PM_SAMPLE_MAX_ITER;
     b->conv[s] = 1.0 + b->prm.eps;
     b->bytes[s] = need;
     std::memset(&b->last[s], 0, sizeof(pm_log_row));
@@ -509,13 +511,20 @@ extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
         row.conv = b->conv[slot];
         int st = PM_SAMPLE_RUNNING;
         if (!std::isfinite(b->conv[slot])) st = PM_SAMPLE_DIVERGED;
-        else if (b->conv[slot] < b->prm.eps) st = PM_SAMPLE_CONVERGED;
+        else if (b->conv[slot] < c->prm.eps) st = PM_SAMPLE_CONVERGED;
         else if (c->t >= b->prm.max_iter) st = PM_SAMPLE_MAX_ITER;
         b->state[slot] = st;
         if (st != PM_SAMPLE_RUNNING) b->shape_dirty = true;
         else running++;
     }
     if (n_running) *n_running = running;
+    return PM_OK;
+}
+
+extern "C" int pm_batch_set_eps(pm_batch *b, int slot, double eps) {
+    if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    if (!(eps > 0.0)) return pm_fail(PM_E_ARG, "eps must be positive");
+    b->ctx[slot]->prm.eps = eps;
     return PM_OK;
 }
 

@@ -1,0 +1,215 @@
+// pm_serve.cpp -- permeability tensor (three rotated solves) and the
+// continuous-batching request loop, on top of the batch engine.
+#include "../../include/poremaps_amd_serve.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+int pm_fail(int code, const char *fmt, ...);  // records the message for pm_last_error
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+// Rotation r of a (nz, ny, nx) x-fastest array so that physical axis
+// flow_axis[r] becomes the array's z: r = 0 identity (flow along z),
+// r = 1 transpose(A, (2, 0, 1)) (new z = x, y = z, x = y: flow along x),
+// r = 2 transpose(A, (1, 2, 0)) (new z = y, y = x, x = z: flow along y).
+// new_axis[r][a] = physical axis of the rotated array's axis a (0 = its x,
+// 1 = its y, 2 = its z).
+const int kNewAxis[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+
+void rotate(const uint8_t *raw, int nx, int ny, int nz, int r, std::vector<uint8_t> &out, int dims[3]) {
+    const int n[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; a++) dims[a] = n[kNewAxis[r][a]];
+    out.resize((size_t)nx * ny * nz);
+    if (r == 0) {
+        std::memcpy(out.data(), raw, out.size());
+        return;
+    }
+    const int64_t stride[3] = {1, nx, (int64_t)nx * ny};  // physical strides of x, y, z
+    const int64_t sa = stride[kNewAxis[r][0]], sb = stride[kNewAxis[r][1]], sc = stride[kNewAxis[r][2]];
+    size_t o = 0;
+    for (int c = 0; c < dims[2]; c++)
+        for (int b = 0; b < dims[1]; b++) {
+            const uint8_t *src = raw + c * sc + b * sb;
+            for (int a = 0; a < dims[0]; a++) out[o++] = src[a * sa];
+        }
+}
+
+// (wk13, wk23, wk33) of rotation r into k[3*i + j] (i velocity, j gradient).
+void scatter(int r, const pm_log_row &row, double k[9]) {
+    const int ax = kNewAxis[r][0], ay = kNewAxis[r][1], az = kNewAxis[r][2];
+    k[3 * ax + az] = row.wk13;
+    k[3 * ay + az] = row.wk23;
+    k[3 * az + az] = row.wk33;
+}
+
+}  // namespace
+
+extern "C" int pm_tensor(const pm_batch_params *p, const uint8_t *raw, int nx, int ny, int nz, int bc, double dx,
+                         double phi, double k[9], pm_result res[3]) {
+    if (!p || !raw || !k) return pm_fail(PM_E_ARG, "null argument");
+    pm_batch *b = nullptr;
+    int rc = pm_batch_create(p, &b);
+    if (rc != PM_OK) return rc;
+    int slot[3];
+    std::vector<uint8_t> rot;
+    for (int r = 0; r < 3 && rc == PM_OK; r++) {
+        int d[3];
+        rotate(raw, nx, ny, nz, r, rot, d);
+        rc = pm_batch_add(b, rot.data(), d[0], d[1], d[2], bc, dx, phi, &slot[r]);
+    }
+    int running = 1;
+    while (rc == PM_OK && running > 0) rc = pm_batch_step(b, &running);
+    std::fill(k, k + 9, 0.0);
+    for (int r = 0; r < 3 && rc == PM_OK; r++) {
+        int st;
+        pm_result rr;
+        rc = pm_batch_status(b, slot[r], &st, &rr);
+        if (rc == PM_OK) {
+            scatter(r, rr.last, k);
+            if (res) res[r] = rr;
+        }
+    }
+    pm_batch_destroy(b);
+    return rc;
+}
+
+extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, pm_response *resp,
+                        pm_serve_stats *stats) {
+    if (!p || n < 0 || (!req && n) || (!resp && n)) return pm_fail(PM_E_ARG, "null argument");
+    pm_batch *b = nullptr;
+    int rc = pm_batch_create(&p->batch, &b);
+    if (rc != PM_OK) return rc;
+    struct Live {
+        int req, rot;
+    };
+    std::vector<Live> live;                 // slot -> (request, rotation), req = -1 free
+    std::vector<int> left(n, 3);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) {
+        order[i] = i;
+        std::memset(&resp[i], 0, sizeof(pm_response));
+        resp[i].arrival_s = req[i].arrival_s;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return req[a].arrival_s < req[c].arrival_s; });
+    const auto t0 = clk::now();
+    auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    size_t next = 0;
+    int done = 0, running = 0, max_running = 0, intervals = 0;
+    double last_report = 0.0;
+    int reported = 0;
+    int64_t peak = 0;
+    std::vector<uint8_t> rot;
+    while (rc == PM_OK && done < n) {
+        // admission: first come, first served, while the three solves fit
+        while (next < order.size()) {
+            const pm_request &q = req[order[next]];
+            if (p->realtime && q.arrival_s > now()) break;
+            int64_t used, budget, need = 0;
+            pm_batch_memory(b, &used, &budget);
+            for (int r = 0; r < 3; r++) {
+                int d[3];
+                const int nn[3] = {q.nx, q.ny, q.nz};
+                for (int a = 0; a < 3; a++) d[a] = nn[kNewAxis[r][a]];
+                need += pm_batch_sample_bytes(&p->batch, d[0], d[1], d[2]);
+            }
+            if (used + need > budget && running > 0) break;  // back-pressure
+            const int id = order[next];
+            resp[id].admit_s = now();
+            for (int r = 0; r < 3 && rc == PM_OK; r++) {
+                int d[3], slot = -1;
+                rotate(q.raw, q.nx, q.ny, q.nz, r, rot, d);
+                rc = pm_batch_add(b, rot.data(), d[0], d[1], d[2], q.boundary_method, q.voxel_size, q.porosity,
+                                  &slot);
+                if (rc == PM_OK && q.eps > 0.0) rc = pm_batch_set_eps(b, slot, q.eps);
+                if (rc != PM_OK) break;
+                if ((int)live.size() <= slot) live.resize(slot + 1, Live{-1, -1});
+                live[slot] = Live{id, r};
+                running++;
+            }
+            next++;
+            int64_t u2;
+            pm_batch_memory(b, &u2, nullptr);
+            peak = std::max(peak, u2);
+        }
+        if (rc != PM_OK) break;
+        max_running = std::max(max_running, running);
+        if (running == 0) {
+            if (next < order.size()) {
+                const double wait = req[order[next]].arrival_s - now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+            }
+            continue;
+        }
+        int still = 0;
+        rc = pm_batch_step(b, &still);
+        intervals++;
+        if (rc != PM_OK) break;
+        bool progressed = still > 0;
+        for (size_t s = 0; s < live.size(); s++) {
+            if (live[s].req < 0) continue;
+            int st;
+            pm_result rr;
+            if ((rc = pm_batch_status(b, (int)s, &st, &rr)) != PM_OK) break;
+            if (st == PM_SAMPLE_RUNNING) continue;
+            const int id = live[s].req, r = live[s].rot;
+            scatter(r, rr.last, resp[id].k);
+            resp[id].sweeps[r] = rr.sweeps;
+            resp[id].state[r] = st;
+            pm_batch_remove(b, (int)s);
+            live[s] = Live{-1, -1};
+            running--;
+            progressed = true;
+            if (--left[id] == 0) {
+                resp[id].done_s = now();
+                done++;
+            }
+        }
+        if (rc == PM_OK && !progressed) rc = pm_fail(PM_E_STATE, "admitted solves cannot progress");
+        if (rc == PM_OK && p->progress && (done > reported || now() - last_report >= 1.0)) {
+            reported = done;
+            last_report = now();
+            if (p->progress(done, running, last_report, p->user)) rc = pm_fail(PM_E_ABORTED, "stopped by the caller");
+        }
+    }
+    pm_batch_destroy(b);
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        std::vector<double> lat;
+        double first = 1e300, last = 0.0;
+        for (int i = 0; i < n; i++) {
+            if (resp[i].done_s <= 0.0) continue;
+            lat.push_back(resp[i].done_s - resp[i].arrival_s);
+            first = std::min(first, resp[i].arrival_s);
+            last = std::max(last, resp[i].done_s);
+        }
+        std::sort(lat.begin(), lat.end());
+        stats->completed = (int)lat.size();
+        if (!lat.empty()) {
+            auto pct = [&](double f) {
+                const double x = f * (lat.size() - 1);
+                const size_t lo = (size_t)std::floor(x), hi = std::min(lat.size() - 1, lo + 1);
+                return lat[lo] + (lat[hi] - lat[lo]) * (x - lo);
+            };
+            stats->latency_p50_s = pct(0.50);
+            stats->latency_p99_s = pct(0.99);
+            double s = 0;
+            for (double v : lat) s += v;
+            stats->latency_mean_s = s / lat.size();
+            stats->latency_max_s = lat.back();
+            stats->wall_s = last - std::min(first, last);
+            stats->solves_per_hour = 3.0 * lat.size() / std::max(stats->wall_s, 1e-9) * 3600.0;
+        }
+        stats->max_running = max_running;
+        stats->peak_bytes = peak;
+        stats->intervals = intervals;
+    }
+    return rc;
+}

@@ -50,10 +50,10 @@ def percolates(g: np.ndarray, axis: int = 0) -> bool:
 
 
 def sphere_pack(n, phi: float, rmin: float, rmax: float, seed: int,
-                require_percolation: bool = True, max_tries: int = 50) -> np.ndarray:
+                require_percolation: bool = True, max_tries: int = 50, axes=(0,)) -> np.ndarray:
     """Overlapping solid spheres until porosity <= phi; ``n`` is an int (cube)
     or an (nz, ny, nx) tuple.  Re-seeds (seed + 1000*try) until the fluid
-    percolates along z."""
+    percolates along every array axis in ``axes`` (0 = z)."""
     shape = (n, n, n) if np.isscalar(n) else tuple(n)
     for attempt in range(max_tries):
         rng = np.random.default_rng(seed + 1000 * attempt)
@@ -74,14 +74,15 @@ def sphere_pack(n, phi: float, rmin: float, rmax: float, seed: int,
             sub = g[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
             solid += int(np.count_nonzero(ball & (sub == 0)))
             sub[ball] = 1
-        if not require_percolation or percolates(g):
+        if not require_percolation or all(percolates(g, a) for a in axes):
             return g
     raise RuntimeError("no percolating sphere pack found")
 
 
 def gaussian_field(n, corr_len: float, phi: float, seed: int,
-                   require_percolation: bool = True, max_tries: int = 20) -> np.ndarray:
-    """Thresholded Gaussian random field, fluid fraction phi."""
+                   require_percolation: bool = True, max_tries: int = 20, axes=(0,)) -> np.ndarray:
+    """Thresholded Gaussian random field, fluid fraction phi; re-seeds until
+    the fluid percolates along every array axis in ``axes`` (0 = z)."""
     from scipy import fft as sfft
 
     shape = (n, n, n) if np.isscalar(n) else tuple(n)
@@ -102,7 +103,7 @@ def gaussian_field(n, corr_len: float, phi: float, seed: int,
         thr = np.quantile(field.ravel()[:: max(1, field.size // 4_000_000)], phi)
         g = np.where(field < thr, 0, 1).astype(np.uint8)
         del field
-        if not require_percolation or percolates(g):
+        if not require_percolation or all(percolates(g, a) for a in axes):
             return g
     raise RuntimeError("no percolating Gaussian field found")
 
@@ -139,3 +140,36 @@ def duct_k(w: float, h: float, dx: float, nz: int, phi: float, terms: int = 400)
         s += np.tanh(nn * np.pi * aa / (2 * bb)) / nn ** 5
     mean = bb * bb / 12.0 * (1.0 - 192.0 * bb / (np.pi ** 5 * aa) * s) * grad
     return mean * phi * dx * dx
+
+
+def random_media(count: int, side_min: int, side_max: int, seed: int, axes=(0,)):
+    """``count`` cubic media with sides log-uniform in [side_min, side_max]:
+    even indices sphere packs (radii 3-10 voxels), odd indices Gaussian random
+    fields (correlation length uniform 4-12 voxels), porosity uniform
+    0.15-0.45, each percolating along every axis in ``axes``.  Returns a list
+    of dicts (raw, kind, phi, side, param)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        side = int(round(np.exp(rng.uniform(np.log(side_min), np.log(side_max)))))
+        phi = float(rng.uniform(0.15, 0.45))
+        s = int(rng.integers(1 << 30))
+        if i % 2 == 0:
+            raw = sphere_pack(side, phi, 3.0, 10.0, seed=s, axes=axes)
+            out.append(dict(raw=raw, kind="spheres", phi=phi, side=side, param=(3.0, 10.0)))
+        else:
+            cl = float(rng.uniform(4.0, 12.0))
+            # a level cut of a Gaussian field stops percolating near 16 % fluid:
+            # raise the fraction in small steps until a percolating draw exists
+            for bump in range(8):
+                try:
+                    raw = gaussian_field(side, cl, min(phi + 0.02 * bump, 0.45), seed=s, axes=axes,
+                                         max_tries=4)
+                    phi = min(phi + 0.02 * bump, 0.45)
+                    break
+                except RuntimeError:
+                    continue
+            else:
+                raise RuntimeError("no percolating Gaussian field found")
+            out.append(dict(raw=raw, kind="grf", phi=phi, side=side, param=cl))
+    return out

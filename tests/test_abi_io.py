@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     out = subprocess.run(["nm", "-D", "--defined-only", pm.LIB_PATH], capture_output=True,
                          text=True, check=True).stdout
     exported = set(re.findall(r" T (pm_[a-z0-9_]+)", out))
-    for h in ("poremaps_amd.h", "poremaps_amd_io.h", "poremaps_amd_batch.h"):
+    for h in ("poremaps_amd.h", "poremaps_amd_io.h", "poremaps_amd_batch.h", "poremaps_amd_serve.h"):
         if not os.path.exists(os.path.join(ROOT, "include", h)):
             continue
         missing = _declared(h) - exported
