@@ -1,7 +1,7 @@
 """Benchmark of the POREMAPS Stokes sweep on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
-                    [--mode jacobi|sor3] [--precision f32dc|f64] [--omega w]
+                    [--mode jacobi|sor3] [--precision f32dc|f64] [--omega w] [--omega-p wp]
                     [--tte] [--no-cpu]
     python bench.py --config 3 [--count 64] [--compare]      (batch)
     python bench.py --config 4 [--count 24] [--rate 4] [--compare]  (serving)
@@ -135,7 +135,8 @@ def bench_batch(a):
     media = geo.random_media(a.count, 64, 192, seed=3)
     print(f"[bench] generated {len(media)} media in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     dense = sum(m["raw"].size for m in media)
-    with pm.Batch(precision=a.precision, omega=a.omega, it_eval=a.it_eval, eps=1e-6, max_iter=a.max_iter) as b:
+    with pm.Batch(precision=a.precision, omega=a.omega, omega_p=a.omega_p, it_eval=a.it_eval, eps=1e-6,
+                  max_iter=a.max_iter) as b:
         sync = time.perf_counter()
         slots = [b.add(m["raw"], boundary_method=0, voxel_size=1e-5) for m in media]
         t_load = time.perf_counter() - sync
@@ -159,6 +160,7 @@ def bench_batch(a):
         "config": {"workload": f"batch of {len(media)} cubic media, sides log-uniform 64-192, half sphere "
                                f"packs (r 3-10) half Gaussian fields (corr. 4-12), phi U[0.15,0.45], bc 0, "
                                f"eps 1e-6, seed 3", "precision": a.precision, "omega": a.omega,
+                   "omega_p": a.omega_p,
                    "it_eval": a.it_eval, "dense_voxels": dense},
         "seconds": round(el, 3), "load_seconds": round(t_load, 3),
         "aggregate_glups": round(lups / el / 1e9, 2),
@@ -166,7 +168,8 @@ def bench_batch(a):
         "states": sorted({s for s, _ in st}),
     }
     if a.compare:
-        for label, kw in (("one_at_a_time_same_mode", dict(mode="jacobi", precision=a.precision, omega=a.omega)),
+        for label, kw in (("one_at_a_time_same_mode", dict(mode="jacobi", precision=a.precision, omega=a.omega,
+                                                           omega_p=a.omega_p)),
                           ("one_at_a_time_fp64_plain", dict(mode="jacobi", precision="f64", omega=1.0))):
             tot = 0.0
             for i, m in enumerate(media):
@@ -199,7 +202,7 @@ def bench_serve(a):
         print(f"[bench] t={t:.1f}s completed {done}/{len(reqs)}, {running} solves running", file=sys.stderr,
               flush=True)
 
-    resp, st = pm.serve(reqs, realtime=True, progress=progress, precision=a.precision, omega=a.omega,
+    resp, st = pm.serve(reqs, realtime=True, progress=progress, precision=a.precision, omega=a.omega, omega_p=a.omega_p,
                         it_eval=a.it_eval, max_iter=a.max_iter)
     print(f"[bench] served: {json.dumps(st.as_dict())}", file=sys.stderr, flush=True)
     line = {
@@ -209,7 +212,8 @@ def bench_serve(a):
         "dtype": "f32" if a.precision == "f32dc" else "f64", "data": "synthetic",
         "config": {"workload": f"{len(media)} tensor requests, Poisson {a.rate}/s, sides log-uniform 48-320, "
                                f"sphere packs / Gaussian fields, phi U[0.15,0.45], bc U{{0,1,2}}, eps 1e-6, "
-                               f"seed 4", "precision": a.precision, "omega": a.omega, "it_eval": a.it_eval},
+                               f"seed 4", "precision": a.precision, "omega": a.omega, "omega_p": a.omega_p,
+                   "it_eval": a.it_eval},
         "latency_p50_s": round(st.latency_p50_s, 3), "latency_p99_s": round(st.latency_p99_s, 3),
         "latency_max_s": round(st.latency_max_s, 3), "wall_s": round(st.wall_s, 3),
         "completed": st.completed, "max_running_solves": st.max_running,
@@ -240,7 +244,8 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--mode", default="jacobi")
     ap.add_argument("--precision", default="f32dc")
-    ap.add_argument("--omega", type=float, default=2.0)
+    ap.add_argument("--omega", type=float, default=2.5, help="velocity relaxation of the fast modes")
+    ap.add_argument("--omega-p", type=float, default=6.0, help="pressure relaxation of the fast modes")
     ap.add_argument("--it-eval", type=int, default=100)
     ap.add_argument("--tte", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -266,7 +271,7 @@ def main():
     g, spec = workload(a.config)
     n = g.size
     kw = dict(boundary_method=spec["bc"], voxel_size=spec["dx"], mode=a.mode, precision=a.precision,
-              omega=a.omega, it_eval=a.it_eval, eps=1e-6)
+              omega=a.omega, omega_p=a.omega_p, it_eval=a.it_eval, eps=1e-6)
     print(f"[bench] {spec['desc']}: loading", file=sys.stderr, flush=True)
     s = pm.Solver(g, **kw)
     for _ in range(a.warmup):
@@ -298,7 +303,7 @@ def main():
         "dtype": "f32" if a.precision == "f32dc" else "f64",
         "data": "synthetic",
         "config": {"workload": spec["desc"], "nx": g.shape[2], "ny": g.shape[1], "nz": g.shape[0],
-                   "mode": a.mode, "precision": a.precision, "omega": a.omega,
+                   "mode": a.mode, "precision": a.precision, "omega": a.omega, "omega_p": a.omega_p,
                    "sweeps_per_step": a.it_eval, "porosity": float((g == 0).mean())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
@@ -309,9 +314,9 @@ def main():
     }
     if a.tte:
         tte = {}
-        for label, mk in (("fp64_plain_jacobi", dict(mode="jacobi", precision="f64", omega=1.0)),
-                          (f"{a.precision}_{a.mode}_w{a.omega}", dict(mode=a.mode, precision=a.precision,
-                                                                     omega=a.omega))):
+        for label, mk in (("fp64_plain_jacobi", dict(mode="jacobi", precision="f64", omega=1.0, omega_p=0.0)),
+                          (f"{a.precision}_{a.mode}_w{a.omega}_wp{a.omega_p}",
+                           dict(mode=a.mode, precision=a.precision, omega=a.omega, omega_p=a.omega_p))):
             s = pm.Solver(g, **dict(kw, **mk))
             res, _ = s.solve()
             s.close()

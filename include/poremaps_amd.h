@@ -64,8 +64,11 @@ typedef struct {
     /* build knobs */
     int mode;                /* PM_MODE_* */
     int precision;           /* PM_PREC_* */
-    double omega;            /* relaxation of PM_MODE_JACOBI / PM_MODE_SOR3 */
+    double omega;            /* relaxation of PM_MODE_JACOBI / PM_MODE_SOR3 (velocities) */
     int device;              /* HIP device ordinal */
+    double omega_p;          /* pressure relaxation of the fast modes; 0: omega.  The fixed
+                                point does not depend on it; the Jacobi sweep is stable while
+                                omega * omega_p stays below about 20 */
 } pm_params;
 
 /* One log row (src/output.cc:193-230 column order). */

@@ -35,6 +35,7 @@ typedef struct {
     int device;
     int use_graph;           /* capture each interval as a HIP graph (1) or launch directly (0) */
     int64_t budget_bytes;    /* admission limit for the samples' device memory; 0: 90 % of free HBM */
+    double omega_p;          /* pressure relaxation; 0: omega (as pm_params.omega_p) */
 } pm_batch_params;
 
 /* Sample states for pm_batch_status. */

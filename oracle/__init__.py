@@ -86,9 +86,9 @@ def _declare(L):
     for t, p in (("f64", _f64p), ("f32", _f32p)):
         s = _D if t == "f64" else _F
         getattr(L, f"orc_sweep_jacobi_{t}").argtypes = (
-            [_u8p, _I, _I, _I, _I, _D] + [p] * 10 + [s] + [_VP] * 4)
+            [_u8p, _I, _I, _I, _I, _D, _D] + [p] * 10 + [s] + [_VP] * 4)
         getattr(L, f"orc_sweep_colour_{t}").argtypes = (
-            [_u8p, _I, _I, _I, _D] + [p] * 5 + [s] + [_VP] * 4)
+            [_u8p, _I, _I, _I, _D, _D] + [p] * 5 + [s] + [_VP] * 4)
         getattr(L, f"orc_ghost_{t}").argtypes = [_u8p, _I, _I, _I, _I] + [p] * 4 + [_VP] + [s] * 3
     L.orc_outlet_plane.argtypes = [_u8p, _I, _I, _I, _I] + [_f64p] * 4 + [_D, _f64p]
     L.orc_residual.argtypes = [_u8p, _I, _I, _I] + [_f64p] * 5 + [_f32p] * 4
@@ -202,15 +202,17 @@ def _tag(a):
     return "f64" if a.dtype == np.float64 else "f32"
 
 
-def sweep_jacobi(pc, bc, omega, src, dst, q0, q1, pin_out, R=None) -> None:
-    """Out-of-place symplectic Jacobi sweep from src=(p,u,v,w) into dst."""
+def sweep_jacobi(pc, bc, omega, src, dst, q0, q1, pin_out, R=None, omega_p=None) -> None:
+    """Out-of-place symplectic Jacobi sweep from src=(p,u,v,w) into dst;
+    omega relaxes the velocities, omega_p (default omega) the pressure."""
     fn = getattr(lib(), f"orc_sweep_jacobi_{_tag(src[0])}")
-    fn(pc, *_dims(pc), bc, omega, *src, *dst, q0, q1, pin_out, *_rptrs(R))
+    fn(pc, *_dims(pc), bc, omega, omega if omega_p is None else omega_p, *src, *dst, q0, q1, pin_out,
+       *_rptrs(R))
 
 
-def sweep_colour(pc, omega, state, q, pin_out, R=None) -> None:
+def sweep_colour(pc, omega, state, q, pin_out, R=None, omega_p=None) -> None:
     fn = getattr(lib(), f"orc_sweep_colour_{_tag(state[0])}")
-    fn(pc, *_dims(pc), omega, *state, q, pin_out, *_rptrs(R))
+    fn(pc, *_dims(pc), omega, omega if omega_p is None else omega_p, *state, q, pin_out, *_rptrs(R))
 
 
 def ghost(G, bc, state, q, pin_in, pin_hi, pin_out) -> None:

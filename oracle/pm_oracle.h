@@ -72,22 +72,23 @@ void orc_pack_code(const int32_t *code, int nx, int ny, int nz, uint8_t *pc);
 /* One symplectic Jacobi sweep, out of place: reads (p0,u0,v0,w0,q0), writes
  * the fluid interior of (p1,u1,v1,w1,q1) -- velocities from the old state,
  * their periodic copies, then pressure from the new velocities.  R* (all NULL
- * or all set): FP32 residual added (defect-correction correction sweep). */
-void orc_sweep_jacobi_f64(const uint8_t *pc, int nx, int ny, int nz, int bc, double omega,
+ * or all set): FP32 residual added (defect-correction correction sweep).
+ * omega relaxes the velocities, omega_p the pressure. */
+void orc_sweep_jacobi_f64(const uint8_t *pc, int nx, int ny, int nz, int bc, double omega, double omega_p,
                           const double *p0, const double *u0, const double *v0, const double *w0,
                           double *p1, double *u1, double *v1, double *w1,
                           const double *q0, double *q1, double pin_out,
                           const float *Rp, const float *Ru, const float *Rv, const float *Rw);
-void orc_sweep_jacobi_f32(const uint8_t *pc, int nx, int ny, int nz, int bc, double omega,
+void orc_sweep_jacobi_f32(const uint8_t *pc, int nx, int ny, int nz, int bc, double omega, double omega_p,
                           const float *p0, const float *u0, const float *v0, const float *w0,
                           float *p1, float *u1, float *v1, float *w1,
                           const float *q0, float *q1, float pin_out,
                           const float *Rp, const float *Ru, const float *Rv, const float *Rw);
 /* One 3-colour SOR sweep in place (colour (i+j+k) mod 3, order 0, 1, 2). */
-void orc_sweep_colour_f64(const uint8_t *pc, int nx, int ny, int nz, double omega,
+void orc_sweep_colour_f64(const uint8_t *pc, int nx, int ny, int nz, double omega, double omega_p,
                           double *p, double *u, double *v, double *w, double *q, double pin_out,
                           const float *Rp, const float *Ru, const float *Rv, const float *Rw);
-void orc_sweep_colour_f32(const uint8_t *pc, int nx, int ny, int nz, double omega,
+void orc_sweep_colour_f32(const uint8_t *pc, int nx, int ny, int nz, double omega, double omega_p,
                           float *p, float *u, float *v, float *w, float *q, float pin_out,
                           const float *Rp, const float *Ru, const float *Rv, const float *Rw);
 /* After a sweep: periodic copies of the four fields and of q (x/y), then the

@@ -26,6 +26,11 @@ PM_OK = 0
 MODES = {"reference": 0, "jacobi": 1, "sor3": 2}
 PRECISIONS = {"f64": 0, "f32dc": 1}
 FIELDS = {"velx": 0, "vely": 1, "velz": 2, "press": 3, "nhood": 4}
+# Default relaxation of the batch and serving paths (velocities, pressure):
+# the symplectic Jacobi sweep is stable while OMEGA * OMEGA_P stays below
+# about 20 (wave CFL of the pressure-velocity coupling), and the slow Darcy
+# pressure mode decays in proportion to OMEGA_P.
+OMEGA, OMEGA_P = 2.5, 6.0
 
 
 class Params(C.Structure):
@@ -36,6 +41,7 @@ class Params(C.Structure):
         ("solving_algorithm", C.c_int), ("eps", C.c_double), ("porosity", C.c_double),
         ("dom_interest", C.c_int * 6),
         ("mode", C.c_int), ("precision", C.c_int), ("omega", C.c_double), ("device", C.c_int),
+        ("omega_p", C.c_double),
     ]
 
 
@@ -113,7 +119,7 @@ def _check(rc, io=False):
 def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_000_000,
                 it_eval=100, it_write=100, solving_algorithm=2, eps=1e-6, porosity=-1.0,
                 dom_interest=(0, 0, 0, 0, 0, 0), mode="reference", precision="f64", omega=1.0,
-                device=0) -> Params:
+                device=0, omega_p=0.0) -> Params:
     p = Params()
     lib().pm_params_default(C.byref(p))
     p.nx, p.ny, p.nz = nx, ny, nz
@@ -126,6 +132,7 @@ def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_0
     p.mode = MODES[mode] if isinstance(mode, str) else mode
     p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
     p.omega = omega
+    p.omega_p = omega_p
     p.device = device
     return p
 
@@ -216,7 +223,7 @@ class Solver:
 class BatchParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("omega", C.c_double), ("it_eval", C.c_int), ("it_write", C.c_int),
                 ("max_iter", C.c_int), ("eps", C.c_double), ("device", C.c_int), ("use_graph", C.c_int),
-                ("budget_bytes", C.c_int64)]
+                ("budget_bytes", C.c_int64), ("omega_p", C.c_double)]
 
 
 SAMPLE_STATES = {0: "running", 1: "converged", 2: "max_iter", 3: "diverged"}
@@ -246,10 +253,11 @@ def _batch_lib():
 class Batch:
     """Many geometries on one GPU (include/poremaps_amd_batch.h)."""
 
-    def __init__(self, *, precision="f32dc", omega=2.0, it_eval=100, it_write=100, max_iter=100_000_000,
-                 eps=1e-6, device=0, use_graph=True, budget_bytes=0):
+    def __init__(self, *, precision="f32dc", omega=OMEGA, it_eval=100, it_write=100, max_iter=100_000_000,
+                 eps=1e-6, device=0, use_graph=True, budget_bytes=0, omega_p=OMEGA_P):
         L = _batch_lib()
-        p = _batch_params(precision, omega, it_eval, it_write, max_iter, eps, device, use_graph, budget_bytes)
+        p = _batch_params(precision, omega, it_eval, it_write, max_iter, eps, device, use_graph, budget_bytes,
+                          omega_p)
         self.params = p
         h = C.c_void_p()
         _check(L.pm_batch_create(C.byref(p), C.byref(h)))
@@ -363,13 +371,14 @@ def _serve_lib():
     return L
 
 
-def _batch_params(precision="f32dc", omega=2.0, it_eval=100, it_write=100, max_iter=100_000_000, eps=1e-6,
-                  device=0, use_graph=True, budget_bytes=0) -> BatchParams:
+def _batch_params(precision="f32dc", omega=OMEGA, it_eval=100, it_write=100, max_iter=100_000_000, eps=1e-6,
+                  device=0, use_graph=True, budget_bytes=0, omega_p=OMEGA_P) -> BatchParams:
     p = BatchParams()
     _batch_lib().pm_batch_params_default(C.byref(p))
     p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
     p.omega, p.it_eval, p.it_write, p.max_iter = omega, it_eval, it_write, max_iter
     p.eps, p.device, p.use_graph, p.budget_bytes = eps, device, int(use_graph), budget_bytes
+    p.omega_p = omega_p
     return p
 
 

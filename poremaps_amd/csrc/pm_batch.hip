@@ -219,7 +219,8 @@ struct pm_batch {
 extern "C" void pm_batch_params_default(pm_batch_params *p) {
     std::memset(p, 0, sizeof *p);
     p->precision = PM_PREC_F32DC;
-    p->omega = 2.0;
+    p->omega = 2.5;
+    p->omega_p = 6.0;
     p->it_eval = 100;
     p->it_write = 100;
     p->max_iter = 100000000;
@@ -243,6 +244,7 @@ static pm_params sample_params(const pm_batch_params *p, int nx, int ny, int nz,
     q.mode = PM_MODE_JACOBI;
     q.precision = p->precision;
     q.omega = p->omega;
+    q.omega_p = p->omega_p;
     q.device = p->device;
     return q;
 }
@@ -260,6 +262,7 @@ extern "C" int pm_batch_create(const pm_batch_params *p, pm_batch **out) {
         return pm_fail(PM_E_ARG, "unknown precision %d", p->precision);
     if (p->it_eval <= 0 || p->it_write <= 0) return pm_fail(PM_E_ARG, "it_eval and it_write must be positive");
     if (!(p->omega > 0.0 && p->omega < 100.0)) return pm_fail(PM_E_ARG, "omega must be in (0, 100)");
+    if (!(p->omega_p >= 0.0 && p->omega_p < 100.0)) return pm_fail(PM_E_ARG, "omega_p must be in [0, 100)");
     BCHK(hipSetDevice(p->device));
     pm_batch *b = new (std::nothrow) pm_batch;
     if (!b) return pm_fail(PM_E_MEMORY, "host allocation failed");
@@ -382,11 +385,11 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
 static void launch_interval(pm_batch *b) {
     using namespace pm;
     hipStream_t st = b->stream;
-    const double om = b->prm.omega;
+    const double om = b->prm.omega, wp = b->prm.omega_p > 0.0 ? b->prm.omega_p : om;
     const int n = b->prm.it_eval;
     const dim3 blk(kB);
     if (b->dc()) {
-        const float cu = (float)(om * kDt), cp = (float)(om * kC2 * kDt);
+        const float cu = (float)(om * kDt), cp = (float)(wp * kC2 * kDt);
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
             hipLaunchKernelGGL((k_b_vel<float, true>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
@@ -402,7 +405,7 @@ static void launch_interval(pm_batch *b) {
         hipLaunchKernelGGL(k_b_zero_q, dim3(b->nP), blk, 0, st, b->dS, b->dP);
         hipLaunchKernelGGL(k_b_reduce, dim3(b->nC), blk, 0, st, b->dS, b->dC, 2, b->dPart);
     } else {
-        const double cu = om * kDt, cp = om * kC2 * kDt;
+        const double cu = om * kDt, cp = wp * kC2 * kDt;
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
             hipLaunchKernelGGL((k_b_vel<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);

@@ -61,6 +61,7 @@ extern "C" void pm_params_default(pm_params *p) {
     p->precision = PM_PREC_F64;
     p->omega = 1.0;
     p->device = 0;
+    p->omega_p = 0.0;
 }
 
 static int check_params(const pm_params *p) {
@@ -83,6 +84,8 @@ static int check_params(const pm_params *p) {
         return fail(PM_E_ARG, "the reference ordering runs in FP64 only");
     if (p->mode != PM_MODE_REFERENCE && !(p->omega > 0.0 && p->omega < 100.0))
         return fail(PM_E_ARG, "omega must be in (0, 100) (got %g)", p->omega);
+    if (!(p->omega_p >= 0.0 && p->omega_p < 100.0))
+        return fail(PM_E_ARG, "omega_p must be in [0, 100) (got %g)", p->omega_p);
     return PM_OK;
 }
 
@@ -313,8 +316,8 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
 static int one_sweep(pm_ctx *c) {
     const pm::Grid &g = c->g;
     hipStream_t s = c->stream;
-    const double om = c->prm.omega;
-    const double cu = om * pm::kDt, cp = om * pm::kC2 * pm::kDt;
+    const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
+    const double cu = om * pm::kDt, cp = wp * pm::kC2 * pm::kDt;
     if (!c->fast()) {
         const bool one_wg = (int64_t)g.nx * g.ny * g.nz <= 400000;
         tb(c, TC_REF);

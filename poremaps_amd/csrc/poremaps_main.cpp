@@ -13,6 +13,7 @@
 //   --mode=reference|jacobi|sor3        POREMAPS_MODE
 //   --precision=f64|f32dc               POREMAPS_PRECISION
 //   --omega=<w>                         POREMAPS_OMEGA
+//   --omega-p=<w>                       POREMAPS_OMEGA_P (pressure relaxation; default omega)
 //   --device=<n>                        POREMAPS_DEVICE
 // Deviation: hard errors (missing file, bad .inp, bad geometry, HIP failure)
 // exit non-zero instead of continuing.
@@ -96,6 +97,7 @@ int main(int argc, char **argv) {
         else { std::fprintf(stderr, "unknown precision %s\n", v.c_str()); return 2; }
     }
     if (knob(flags, "--omega", "POREMAPS_OMEGA", &v)) prm.omega = std::atof(v.c_str());
+    if (knob(flags, "--omega-p", "POREMAPS_OMEGA_P", &v)) prm.omega_p = std::atof(v.c_str());
     if (knob(flags, "--device", "POREMAPS_DEVICE", &v)) prm.device = std::atoi(v.c_str());
 
     std::printf("Given domain decomposition nx = %i, ny = %i, nz = %i (one GPU: ignored)\n",

@@ -24,9 +24,9 @@ def cases(channel_raw):
     ]
 
 
-def alone(raw, bc, dx, precision, omega):
+def alone(raw, bc, dx, precision, omega, omega_p=pm.OMEGA_P):
     with pm.Solver(raw, boundary_method=bc, voxel_size=dx, mode="jacobi", precision=precision, omega=omega,
-                   eps=1e-6) as s:
+                   omega_p=omega_p, eps=1e-6) as s:
         res, _ = s.solve()
         return res, s.field("velz"), s.field("press")
 
@@ -42,7 +42,7 @@ def run_batch(b, slots):
 def test_batch_equals_single_solves(channel_raw, precision, graph):
     say("batch vs single", precision, graph)
     cs = cases(channel_raw)
-    with pm.Batch(precision=precision, omega=2.0, use_graph=graph) as b:
+    with pm.Batch(precision=precision, omega=2.0, omega_p=pm.OMEGA_P, use_graph=graph) as b:
         slots = [b.add(raw, boundary_method=bc, voxel_size=dx) for raw, bc, dx in cs]
         st = run_batch(b, slots)
         for slot, (raw, bc, dx) in zip(slots, cs):
@@ -60,7 +60,7 @@ def test_batch_join_and_evict(channel_raw):
     evicted as they finish, still reproduce their solo results."""
     say("batch join/evict")
     cs = cases(channel_raw)
-    with pm.Batch(precision="f32dc", omega=2.0) as b:
+    with pm.Batch(precision="f32dc", omega=2.0, omega_p=pm.OMEGA_P) as b:
         s0 = b.add(cs[0][0], boundary_method=cs[0][1], voxel_size=cs[0][2])
         s1 = b.add(cs[1][0], boundary_method=cs[1][1], voxel_size=cs[1][2])
         for _ in range(7):

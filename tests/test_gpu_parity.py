@@ -160,7 +160,7 @@ def test_reference_solve_tube_matches_binary(known):
 # ---------------------------------------------------------------------------
 # fast modes, per iteration
 # ---------------------------------------------------------------------------
-def oracle_fast(raw, bc, mode, omega, n):
+def oracle_fast(raw, bc, mode, omega, n, omega_p=None):
     G, code, A = oracle_state(raw, bc)
     pc = O.pack_code(code)
     B = [a.copy() for a in A]
@@ -169,16 +169,16 @@ def oracle_fast(raw, bc, mode, omega, n):
     pins = O.solution_pins(G)
     for _ in range(n):
         if mode == "jacobi":
-            O.sweep_jacobi(pc, bc, omega, A, B, qa, qb, 3.0)
+            O.sweep_jacobi(pc, bc, omega, A, B, qa, qb, 3.0, omega_p=omega_p)
             O.ghost(G, bc, B, qb, *pins)
             A, B, qa, qb = B, A, qb, qa
         else:
-            O.sweep_colour(pc, omega, A, qa, 3.0)
+            O.sweep_colour(pc, omega, A, qa, 3.0, omega_p=omega_p)
             O.ghost(G, bc, A, qa, *pins)
     return A
 
 
-def oracle_dc(raw, bc, mode, omega, n, it_eval):
+def oracle_dc(raw, bc, mode, omega, n, it_eval, omega_p=None):
     G, code, X = oracle_state(raw, bc)
     pc = O.pack_code(code)
     d = [np.zeros(G.shape, np.float32) for _ in range(4)]
@@ -189,11 +189,11 @@ def oracle_dc(raw, bc, mode, omega, n, it_eval):
     z = np.float32(0)
     for t in range(n):
         if mode == "jacobi":
-            O.sweep_jacobi(pc, bc, omega, d, d2, qd, qd2, z, R=R)
+            O.sweep_jacobi(pc, bc, omega, d, d2, qd, qd2, z, R=R, omega_p=omega_p)
             O.ghost(G, bc, d2, qd2, z, z, z)
             d, d2, qd, qd2 = d2, d, qd2, qd
         else:
-            O.sweep_colour(pc, omega, d, qd, z, R=R)
+            O.sweep_colour(pc, omega, d, qd, z, R=R, omega_p=omega_p)
             O.ghost(G, bc, d, qd, z, z, z)
         if t % it_eval == 0:
             O.fold(G, bc, X, d)
@@ -205,11 +205,12 @@ def oracle_dc(raw, bc, mode, omega, n, it_eval):
 
 @pytest.mark.parametrize("mode", ["jacobi", "sor3"])
 @pytest.mark.parametrize("bc", [0, 2, 4])
-def test_fast_fp64_sweeps_bit_exact(mode, bc):
+@pytest.mark.parametrize("omega,omega_p", [(1.7, 0.0), (2.0, 7.0)])
+def test_fast_fp64_sweeps_bit_exact(mode, bc, omega, omega_p):
     raw = random_geometry((12, 10, 14), 0.35, 31 + bc)
-    say("fast f64", mode, bc)
-    ref = oracle_fast(raw, bc, mode, 1.7, 25)
-    with pm.Solver(raw, boundary_method=bc, mode=mode, omega=1.7) as s:
+    say("fast f64", mode, bc, omega, omega_p)
+    ref = oracle_fast(raw, bc, mode, omega, 25, omega_p or None)
+    with pm.Solver(raw, boundary_method=bc, mode=mode, omega=omega, omega_p=omega_p) as s:
         s.step(25)
         m = nonc(ref[0])
         for name, a, b in zip("puvw", s.state_halo(), ref):
@@ -218,11 +219,13 @@ def test_fast_fp64_sweeps_bit_exact(mode, bc):
 
 @pytest.mark.parametrize("mode", ["jacobi", "sor3"])
 @pytest.mark.parametrize("bc", [0, 1, 3])
-def test_fast_f32dc_sweeps_bit_exact(mode, bc):
+@pytest.mark.parametrize("omega,omega_p", [(1.6, 0.0), (2.5, 6.0)])
+def test_fast_f32dc_sweeps_bit_exact(mode, bc, omega, omega_p):
     raw = random_geometry((12, 10, 14), 0.35, 41 + bc)
-    say("fast f32dc", mode, bc)
-    ref = oracle_dc(raw, bc, mode, 1.6, 31, 10)
-    with pm.Solver(raw, boundary_method=bc, mode=mode, omega=1.6, precision="f32dc", it_eval=10) as s:
+    say("fast f32dc", mode, bc, omega, omega_p)
+    ref = oracle_dc(raw, bc, mode, omega, 31, 10, omega_p or None)
+    with pm.Solver(raw, boundary_method=bc, mode=mode, omega=omega, omega_p=omega_p, precision="f32dc",
+                   it_eval=10) as s:
         s.step(31)
         m = nonc(ref[0])
         for name, a, b in zip("puvw", s.state_halo(), ref):
@@ -232,20 +235,21 @@ def test_fast_f32dc_sweeps_bit_exact(mode, bc):
 # ---------------------------------------------------------------------------
 # fast modes, to convergence: the reference's fixed point
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("mode,omega,prec", [("jacobi", 1.0, "f64"), ("jacobi", 2.5, "f32dc"),
-                                             ("sor3", 1.8, "f64"), ("sor3", 1.8, "f32dc")])
-def test_fast_modes_reach_reference_fixed_point(channel_raw, mode, omega, prec):
-    say("fast fixed point", mode, omega, prec)
+@pytest.mark.parametrize("mode,omega,prec,omega_p", [("jacobi", 1.0, "f64", 0.0), ("jacobi", 2.5, "f32dc", 0.0),
+                                                     ("jacobi", 2.5, "f32dc", 6.0), ("jacobi", 2.0, "f64", 8.0),
+                                                     ("sor3", 1.8, "f64", 0.0), ("sor3", 1.8, "f32dc", 0.0)])
+def test_fast_modes_reach_reference_fixed_point(channel_raw, mode, omega, prec, omega_p):
+    say("fast fixed point", mode, omega, prec, omega_p)
     G = O.geom_halo(channel_raw, 0)
     code = O.eval_geometry(G)
     _, tight = O.calc_flow(G, code, 0, O.init_fields(G, 0), solver=2, eps=1e-11, dx=5e-5)
     _, at_eps = O.calc_flow(G, code, 0, O.init_fields(G, 0), solver=2, eps=1e-6, dx=5e-5)
-    with pm.Solver(channel_raw, boundary_method=0, mode=mode, omega=omega, precision=prec,
+    with pm.Solver(channel_raw, boundary_method=0, mode=mode, omega=omega, precision=prec, omega_p=omega_p,
                    eps=1e-11, voxel_size=5e-5) as s:
         res, _ = s.solve()
     tol = 1e-9 if prec == "f64" else 1e-6
     assert res.last.wk33 == pytest.approx(tight[-1][10], rel=tol)
-    with pm.Solver(channel_raw, boundary_method=0, mode=mode, omega=omega, precision=prec,
+    with pm.Solver(channel_raw, boundary_method=0, mode=mode, omega=omega, precision=prec, omega_p=omega_p,
                    eps=1e-6, voxel_size=5e-5) as s:
         res, _ = s.solve()
     assert res.last.wk33 == pytest.approx(at_eps[-1][10], rel=1e-4)

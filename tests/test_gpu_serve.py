@@ -21,8 +21,8 @@ def say(*a):
 
 
 def solo(raw, bc, dx):
-    with pm.Solver(raw, boundary_method=bc, voxel_size=dx, mode="jacobi", precision="f32dc", omega=2.0,
-                   eps=1e-6) as s:
+    with pm.Solver(raw, boundary_method=bc, voxel_size=dx, mode="jacobi", precision="f32dc", omega=pm.OMEGA,
+                   omega_p=pm.OMEGA_P, eps=1e-6) as s:
         res, _ = s.solve()
         return res
 
@@ -34,7 +34,7 @@ def aniso():
 def test_tensor_columns_are_rotated_solves():
     raw = aniso()
     say("tensor columns", raw.shape)
-    k, res = pm.tensor(raw, boundary_method=0, voxel_size=1e-5, precision="f32dc", omega=2.0)
+    k, res = pm.tensor(raw, boundary_method=0, voxel_size=1e-5, precision="f32dc")
     # flow along z: the geometry itself
     r = solo(raw, 0, 1e-5)
     assert res[0].sweeps == r.sweeps
