@@ -284,6 +284,7 @@ def main():
     sweeps = a.steps * a.it_eval
     glups = n * sweeps / el / 1e9
     k = s.permeability()
+    choice, tuned = s.kernel_choice()
     s.close()
     sw = kt["sweep"]
     achieved = sw["bytes"] / (sw["avg_ms"] * 1e-3) / 1e9
@@ -311,6 +312,7 @@ def main():
                      "bytes_per_launch": sw["bytes"]},
         "kernels": {k2: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"],
                          "GBps": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k2, v in kt.items()},
+        "sweep_kernel": {"form": choice, "autotuner_ms_per_sweep": tuned},
     }
     if a.tte:
         tte = {}

@@ -148,6 +148,17 @@ typedef struct {
 } pm_kernel_timing;
 int pm_time_kernels(pm_ctx *ctx, int n, pm_kernel_timing *out, int max_out, int *n_out);
 
+/* Jacobi sweep kernels in use and the autotuner's measurements.  The fast
+ * Jacobi mode has several kernel forms that produce the same iterates bit
+ * for bit (velocity + pressure kernel pair over the fluid list with select or
+ * case-predicated neighbour loads, one lagged pass over the fluid list, one
+ * lagged dense 2.5-D pass); the first sweeps of a solve run each in turn and
+ * the fastest is kept (environment: POREMAPS_TUNE=0 disables,
+ * POREMAPS_JACOBI=pair|list|stream fixes the form).  name receives the form
+ * in use; tried (optional) "form=ms/sweep;..." once tuning has finished.
+ * No reference counterpart. */
+int pm_kernel_choice(pm_ctx *ctx, char *name, int len, char *tried, int tried_len);
+
 void pm_destroy(pm_ctx *ctx);
 
 #ifdef __cplusplus

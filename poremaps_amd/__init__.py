@@ -213,6 +213,19 @@ class Solver:
         _check(lib().pm_porosity(self._h, C.byref(v)))
         return v.value
 
+    def kernel_choice(self):
+        """(sweep kernel form in use, {form: ms per sweep} measured by the
+        autotuner, empty until it has finished)."""
+        L = lib()
+        L.pm_kernel_choice.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+        name, tried = C.create_string_buffer(64), C.create_string_buffer(1024)
+        _check(L.pm_kernel_choice(self._h, name, 64, tried, 1024))
+        ms = {}
+        for item in filter(None, tried.value.decode().split(";")):
+            k, v = item.split("=")
+            ms[k] = float(v)
+        return name.value.decode(), ms
+
     def permeability(self) -> dict:
         out = (C.c_double * 8)()
         _check(lib().pm_permeability(self._h, out))

@@ -306,6 +306,7 @@ extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int
     // sweep 0) runs on the sample's own stream; afterwards every interval
     // ends on an iteration index that is a multiple of it_eval
     if (rc == PM_OK) rc = pm_one_sweep(c);
+    if (rc == PM_OK) rc = pm_settle(c);
     if (rc == PM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = pm_fail(PM_E_HIP, "first sweep failed");
     if (rc != PM_OK) {
         pm_destroy(c);

@@ -16,7 +16,29 @@ struct Timer {
 };
 
 enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_N };
+enum JacobiImpl { JI_PAIR = 0, JI_LIST = 1, JI_STREAM = 2 };
 static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep"};
+
+// One Jacobi kernel configuration.  Every configuration produces the same
+// iterates bit for bit, so the choice only changes the speed.
+struct SweepVariant {
+    int impl;           // JacobiImpl
+    bool select_loads;  // fluid-list kernels: all neighbours + select, or case-predicated loads
+    int zs;             // dense kernel: planes per z segment
+};
+
+// Autotuner: the first sweeps of a solve are run with each candidate in turn
+// (windows of `window` sweeps timed with events on the solver's stream), then
+// the fastest is kept for the rest of the context's life.
+struct Tuner {
+    static constexpr int kMax = 8;
+    bool active = false, done = false;
+    int n = 0, idx = 0, in_window = 0, window = 6, first_scored = 0;
+    SweepVariant cand[kMax];
+    hipEvent_t ev0[kMax] = {}, ev1[kMax] = {};
+    float ms[kMax] = {};
+    int best = -1;
+};
 
 struct pm_ctx {
     Timer *timer = nullptr;
@@ -46,6 +68,16 @@ struct pm_ctx {
     double kprev = 0.0;   // permeability of the previous convergence test
     double phi = 0.0;
     int64_t fluid = 0;
+    // fluid-list slab height requested for the next pm_load_geometry (0: plain order)
+    int prm_tile_y = 0;
+    // Jacobi kernels: velocity + pressure pair over the fluid list, one lagged
+    // pass over the fluid list, or one lagged dense 2.5-D pass (z segments of
+    // stream_zs planes)
+    int jac_impl = JI_PAIR;
+    int stream_zs = 64;
+    Tuner tuner;
+    // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence
+    bool lagged = false;
 
     bool fast() const { return prm.mode != PM_MODE_REFERENCE; }
     bool dc() const { return fast() && prm.precision == PM_PREC_F32DC; }
@@ -56,6 +88,7 @@ struct pm_ctx {
 // Context internals used by the batch engine (defined in pm_solver.hip).
 int pm_fail(int code, const char *fmt, ...);
 int pm_one_sweep(pm_ctx *c);
+int pm_settle(pm_ctx *c);
 int pm_reduce_state(pm_ctx *c, bool full, pm::RedSums *out);
 double pm_convergence_from(const pm::RedSums &r, double *kprev);
 void pm_permeability_from(const pm::RedSums &r, const int *dom, double dx, double phi, double *out);

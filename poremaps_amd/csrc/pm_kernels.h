@@ -45,6 +45,8 @@ struct FluidLists {
     // stencil loads: every neighbour up front + select (true) or only the
     // neighbours the case needs, behind branches (false)
     bool select_loads = true;
+    // rows per slab of the list order (0: plain z, y, x order)
+    int tile_y = 0;
 };
 
 template <typename T>
@@ -52,6 +54,23 @@ void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const 
                    const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu,
                    T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
                    hipStream_t s);
+// Symplectic Jacobi in lagged form (pm_voxel.h): JAC_FIRST maps a settled
+// set 0 (u^n, p^n) to set 1 = (u^{n+1}, p^n); JAC_LAG maps set 0 =
+// (u^n, p^{n-1}) to set 1 = (u^{n+1}, p^n); JAC_SETTLE completes set 0 in
+// place: p^n and the outlet buffer from set 0's velocities (set 1 pointers:
+// p1 = set 0's p, q1 = set 0's q).
+enum { JAC_FIRST = 0, JAC_LAG = 1, JAC_SETTLE = 2 };
+template <typename T>
+void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0,
+                         const T *u0, const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1,
+                         T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+                         hipStream_t s);
+// The JAC_FIRST / JAC_LAG passes as the dense 2.5-D kernel: 64 x 8 tiles
+// marching along z in segments of zs planes (pm_sweep.hip).
+template <typename T>
+void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const T *p0, const T *u0,
+                          const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T cu, T cp, T pin_out,
+                          const float *Rp, const float *Ru, const float *Rv, const float *Rw, hipStream_t s);
 template <typename T>
 void launch_colour(const Grid &g, const FluidLists &L, const uint8_t *pc, T *p, T *u, T *v, T *w, T *q,
                    T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv,

@@ -132,6 +132,13 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     if (!c) return fail(PM_E_MEMORY, "host allocation failed");
     c->prm = *p;
     c->g = pm::make_grid(p->nx, p->ny, p->nz, p->boundary_method);
+    if (const char *ty = std::getenv("POREMAPS_LIST_TILE")) c->prm_tile_y = std::max(0, std::atoi(ty));
+    if (const char *jm = std::getenv("POREMAPS_JACOBI")) {
+        if (std::strcmp(jm, "pair") == 0) c->jac_impl = JI_PAIR;
+        else if (std::strcmp(jm, "list") == 0) c->jac_impl = JI_LIST;
+        else if (std::strcmp(jm, "stream") == 0) c->jac_impl = JI_STREAM;
+    }
+    if (const char *zs = std::getenv("POREMAPS_STREAM_ZS")) c->stream_zs = std::max(1, std::atoi(zs));
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -166,6 +173,8 @@ static int dalloc(T **p, int64_t n) {
         int rc_ = (expr);             \
         if (rc_ != PM_OK) return rc_; \
     } while (0)
+
+static void tuner_setup(pm_ctx *c);
 
 static int fill_f32(pm_ctx *c, float *a, int64_t n, float v) {
     hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a, n, v);
@@ -236,21 +245,29 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     RC(dalloc(&c->red_dev, 1));
     HIPCHK(hipHostMalloc((void **)&c->red_host, sizeof(pm::RedSums), hipHostMallocDefault));
     if (c->fast()) {
-        // fluid lists in (z, y, x) order, all and per colour (i+j+k) mod 3 of
-        // the halo-including indices
+        // fluid lists, all and per colour (i+j+k) mod 3 of the halo-including
+        // indices, in slabs of tile_y rows marching along z: (slab, z, y, x)
+        // order.  Every list kernel treats its entries independently, so the
+        // order only changes cache reuse: the +-2-plane neighbours of a cell
+        // are touched again within a few slab-planes instead of a few whole
+        // planes.  tile_y 0: plain (z, y, x).
+        const int ty = c->prm_tile_y > 0 ? std::min(c->prm_tile_y, g.ny) : g.ny;
         std::vector<int32_t> all;
         std::vector<int32_t> col[3];
         all.reserve((size_t)fluid);
         for (int q = 0; q < 3; q++) col[q].reserve((size_t)fluid / 3 + 1);
-        int64_t idx = 0;
-        for (int i = 0; i < g.nz; i++)
-            for (int j = 0; j < g.ny; j++)
-                for (int k = 0; k < g.nx; k++, idx++) {
-                    if (raw[idx] != 0) continue;
-                    const int32_t e = (int32_t)g.at(i + 2, j + 2, k + 2);
-                    all.push_back(e);
-                    col[(i + j + k + 6) % 3].push_back(e);
+        for (int j0 = 0; j0 < g.ny; j0 += ty)
+            for (int i = 0; i < g.nz; i++)
+                for (int j = j0; j < std::min(j0 + ty, g.ny); j++) {
+                    const uint8_t *row = raw + ((int64_t)i * g.ny + j) * g.nx;
+                    for (int k = 0; k < g.nx; k++) {
+                        if (row[k] != 0) continue;
+                        const int32_t e = (int32_t)g.at(i + 2, j + 2, k + 2);
+                        all.push_back(e);
+                        col[(i + j + k + 6) % 3].push_back(e);
+                    }
                 }
+        c->lists.tile_y = c->prm_tile_y > 0 ? ty : 0;
         const char *ld = std::getenv("POREMAPS_SWEEP_LOADS");
         c->lists.select_loads = !(ld && std::strcmp(ld, "branch") == 0);
         c->lists.n_all = (int)all.size();
@@ -308,11 +325,136 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     c->t = 0;
     c->kprev = 0.0;
     c->loaded = true;
+    c->lagged = false;
+    tuner_setup(c);
+    return PM_OK;
+}
+
+// Complete a lagged Jacobi state (u^n, p^{n-1}): the pressure and outlet
+// buffer of the last sweep, then the ghost fill.  Afterwards the state is the
+// one the velocity + pressure kernel pair would have left.
+static int settle(pm_ctx *c) {
+    if (!c->lagged) return PM_OK;
+    c->lagged = false;
+    const pm::Grid &g = c->g;
+    hipStream_t s = c->stream;
+    const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
+    const double cu = om * pm::kDt, cp = wp * pm::kC2 * pm::kDt;
+    if (c->dc()) {
+        pm::launch_jacobi_stage<float>(pm::JAC_SETTLE, g, c->lists, c->pc, c->D[0], c->D[1], c->D[2], c->D[3],
+                                       c->qD[0], c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], (float)cu,
+                                       (float)cp, 0.f, c->R[0], c->R[1], c->R[2], c->R[3], s);
+        pm::launch_ghost_fast<float>(g, c->G, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], 0.f, 0.f, 0.f, s);
+    } else {
+        pm::launch_jacobi_stage<double>(pm::JAC_SETTLE, g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3],
+                                        c->qX[0], c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0,
+                                        nullptr, nullptr, nullptr, nullptr, s);
+        pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], (double)(g.nz + 2),
+                                      2.0, 3.0, s);
+    }
+    HIPCHK(hipGetLastError());
+    return PM_OK;
+}
+
+// One Jacobi sweep from set A (p, u, v, w; outlet buffer qa) into set B with
+// the context's kernels: the velocity + pressure pair, or one lagged pass
+// (dense 2.5-D or fluid-list form).
+template <typename T>
+static void jacobi_pass(pm_ctx *c, T *const *A, T *qa, T *const *B, T *qb, T cu, T cp, T pin_out,
+                        float *const *R) {
+    const pm::Grid &g = c->g;
+    const float *Rp = R ? R[0] : nullptr, *Ru = R ? R[1] : nullptr, *Rv = R ? R[2] : nullptr,
+                *Rw = R ? R[3] : nullptr;
+    if (c->jac_impl == JI_PAIR) {
+        (void)settle(c);  // the pair reads a complete state
+        pm::launch_jacobi<T>(g, c->lists, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3], qb, cu, cp,
+                             pin_out, Rp, Ru, Rv, Rw, c->stream);
+        return;
+    }
+    const int st = c->lagged ? pm::JAC_LAG : pm::JAC_FIRST;
+    if (c->jac_impl == JI_STREAM)
+        pm::launch_jacobi_stream<T>(st, c->stream_zs, g, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3],
+                                    cu, cp, pin_out, Rp, Ru, Rv, Rw, c->stream);
+    else
+        pm::launch_jacobi_stage<T>(st, g, c->lists, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3], qb,
+                                   cu, cp, pin_out, Rp, Ru, Rv, Rw, c->stream);
+    c->lagged = true;
+}
+
+static void apply_variant(pm_ctx *c, const SweepVariant &v) {
+    c->jac_impl = v.impl;
+    c->lists.select_loads = v.select_loads;
+    c->stream_zs = v.zs;
+}
+
+static const char *variant_name(const SweepVariant &v, char *buf, size_t len) {
+    if (v.impl == JI_STREAM) std::snprintf(buf, len, "dense-2.5d-zs%d", v.zs);
+    else std::snprintf(buf, len, "%s-%s", v.impl == JI_PAIR ? "list-pair" : "list-lagged",
+                       v.select_loads ? "select" : "branch");
+    return buf;
+}
+
+// Candidates of the Jacobi autotuner (fast Jacobi mode), unless the kernels
+// are fixed by POREMAPS_JACOBI or tuning is off (POREMAPS_TUNE=0).
+static void tuner_setup(pm_ctx *c) {
+    Tuner &tn = c->tuner;
+    tn = Tuner();
+    const char *fixed = std::getenv("POREMAPS_JACOBI");
+    const char *off = std::getenv("POREMAPS_TUNE");
+    if (!c->fast() || !c->jacobi() || c->lists.n_all == 0 || (fixed && *fixed) || (off && std::strcmp(off, "0") == 0))
+        return;
+    // the first window (caches and clocks still cold after the load) is run
+    // with the last candidate and not scored
+    const SweepVariant cands[] = {{JI_STREAM, true, 16}, {JI_PAIR, true, 64}, {JI_PAIR, false, 64},
+                                  {JI_LIST, true, 64}, {JI_STREAM, true, 64}, {JI_STREAM, true, 16}};
+    for (const auto &v : cands) tn.cand[tn.n++] = v;
+    tn.first_scored = 1;
+    tn.active = true;
+}
+
+// Before a Jacobi sweep: the candidate of the current window.
+static int tuner_begin(pm_ctx *c) {
+    Tuner &tn = c->tuner;
+    if (!tn.active || tn.done) return PM_OK;
+    if (tn.in_window == 0) {
+        apply_variant(c, tn.cand[tn.idx]);
+        if (c->jac_impl == JI_PAIR) RC(settle(c));
+        HIPCHK(hipEventCreate(&tn.ev0[tn.idx]));
+        HIPCHK(hipEventRecord(tn.ev0[tn.idx], c->stream));
+    }
+    return PM_OK;
+}
+
+// After a Jacobi sweep (ghost fill included): close the window; after the
+// last one keep the fastest candidate.
+static int tuner_end(pm_ctx *c) {
+    Tuner &tn = c->tuner;
+    if (!tn.active || tn.done) return PM_OK;
+    if (++tn.in_window < tn.window) return PM_OK;
+    HIPCHK(hipEventCreate(&tn.ev1[tn.idx]));
+    HIPCHK(hipEventRecord(tn.ev1[tn.idx], c->stream));
+    tn.in_window = 0;
+    if (++tn.idx < tn.n) return PM_OK;
+    HIPCHK(hipEventSynchronize(tn.ev1[tn.n - 1]));
+    tn.best = tn.first_scored;
+    for (int q = 0; q < tn.n; q++) {
+        float e = 0.f;
+        (void)hipEventElapsedTime(&e, tn.ev0[q], tn.ev1[q]);
+        tn.ms[q] = e / tn.window;
+        (void)hipEventDestroy(tn.ev0[q]);
+        (void)hipEventDestroy(tn.ev1[q]);
+        tn.ev0[q] = tn.ev1[q] = nullptr;
+        if (q >= tn.first_scored && tn.ms[q] < tn.ms[tn.best]) tn.best = q;
+    }
+    tn.done = true;
+    apply_variant(c, tn.cand[tn.best]);
     return PM_OK;
 }
 
 // One sweep of the context's ordering, ghost fill and pins included; for the
 // defect-correction mode also the fold when the sweep index hits it_eval.
+// Jacobi sweeps run in lagged form (one pass each); settle() completes the
+// state before anything reads the pressure.
 static int one_sweep(pm_ctx *c) {
     const pm::Grid &g = c->g;
     hipStream_t s = c->stream;
@@ -327,10 +469,9 @@ static int one_sweep(pm_ctx *c) {
     } else if (!c->dc()) {
         const double pin_in = (double)(g.nz + 2);
         if (c->jacobi()) {
+            RC(tuner_begin(c));
             tb(c, TC_SWEEP);
-            pm::launch_jacobi<double>(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->X2[0], c->X2[1],
-                                      c->X2[2], c->X2[3], c->qX[1], cu, cp, 3.0, nullptr, nullptr, nullptr,
-                                      nullptr, s);
+            jacobi_pass<double>(c, c->X, c->qX[0], c->X2, c->qX[1], cu, cp, 3.0, nullptr);
             te(c);
             tb(c, TC_GHOST);
             pm::launch_ghost_fast<double>(g, c->G, c->X2[0], c->X2[1], c->X2[2], c->X2[3], c->qX[1], pin_in, 2.0,
@@ -338,6 +479,7 @@ static int one_sweep(pm_ctx *c) {
             te(c);
             for (int q = 0; q < 4; q++) std::swap(c->X[q], c->X2[q]);
             std::swap(c->qX[0], c->qX[1]);
+            RC(tuner_end(c));
         } else {
             tb(c, TC_SWEEP);
             pm::launch_colour<double>(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0,
@@ -351,10 +493,9 @@ static int one_sweep(pm_ctx *c) {
     } else {
         const float fcu = (float)cu, fcp = (float)cp;
         if (c->jacobi()) {
+            RC(tuner_begin(c));
             tb(c, TC_SWEEP);
-            pm::launch_jacobi<float>(g, c->lists, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], c->D2[0], c->D2[1],
-                                     c->D2[2], c->D2[3], c->qD[1], fcu, fcp, 0.f, c->R[0], c->R[1], c->R[2],
-                                     c->R[3], s);
+            jacobi_pass<float>(c, c->D, c->qD[0], c->D2, c->qD[1], fcu, fcp, 0.f, c->R);
             te(c);
             tb(c, TC_GHOST);
             pm::launch_ghost_fast<float>(g, c->G, c->D2[0], c->D2[1], c->D2[2], c->D2[3], c->qD[1], 0.f, 0.f, 0.f,
@@ -362,6 +503,7 @@ static int one_sweep(pm_ctx *c) {
             te(c);
             for (int q = 0; q < 4; q++) std::swap(c->D[q], c->D2[q]);
             std::swap(c->qD[0], c->qD[1]);
+            RC(tuner_end(c));
         } else {
             tb(c, TC_SWEEP);
             pm::launch_colour<float>(g, c->lists, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], fcu, fcp, 0.f,
@@ -373,6 +515,7 @@ static int one_sweep(pm_ctx *c) {
         }
         if (c->t % c->prm.it_eval == 0) {
             tb(c, TC_FOLD);
+            RC(settle(c));
             pm::launch_fold(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->D[0], c->D[1], c->D[2], c->D[3], s);
             pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr,
                                           (double)(g.nz + 2), 2.0, 3.0, s);
@@ -388,6 +531,7 @@ static int one_sweep(pm_ctx *c) {
 // Fold the outstanding correction into the FP64 state (defect-correction
 // mode), so that X is the solution state.
 static int dc_flush(pm_ctx *c) {
+    RC(settle(c));
     if (!c->dc()) return PM_OK;
     const pm::Grid &g = c->g;
     pm::launch_fold(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->D[0], c->D[1], c->D[2], c->D[3], c->stream);
@@ -399,6 +543,7 @@ static int dc_flush(pm_ctx *c) {
 }
 
 static int reduce(pm_ctx *c, bool full, pm::RedSums *out) {
+    RC(settle(c));
     pm::DomBox box;
     const int *d = c->prm.dom_interest;
     box.on = !(d[0] == 0 && d[1] == 0 && d[2] == 0 && d[3] == 0 && d[4] == 0 && d[5] == 0);
@@ -554,6 +699,7 @@ extern "C" int pm_step(pm_ctx *c, int n) {
 }
 
 static int copy_field(pm_ctx *c, int field, std::vector<double> &host) {
+    RC(settle(c));
     const pm::Grid &g = c->g;
     host.resize((size_t)g.total);
     double *src = nullptr;
@@ -646,6 +792,26 @@ static double algo_bytes(const pm_ctx *c, int cat) {
     }
 }
 
+extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int tried_len) {
+    if (!c || !name || len <= 0) return fail(PM_E_ARG, "null argument");
+    char buf[64];
+    const SweepVariant cur = {c->jac_impl, c->lists.select_loads, c->stream_zs};
+    std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(cur, buf, sizeof buf) : "n/a");
+    if (tried && tried_len > 0) {
+        std::string t;
+        const Tuner &tn = c->tuner;
+        if (tn.done)
+            for (int q = tn.first_scored; q < tn.n; q++) {
+                char e[96];
+                std::snprintf(e, sizeof e, "%s%s=%.4f", q > tn.first_scored ? ";" : "",
+                              variant_name(tn.cand[q], buf, sizeof buf), tn.ms[q]);
+                t += e;
+            }
+        std::snprintf(tried, (size_t)tried_len, "%s", t.c_str());
+    }
+    return PM_OK;
+}
+
 extern "C" int pm_time_kernels(pm_ctx *c, int n, pm_kernel_timing *out, int max_out, int *n_out) {
     if (!c || !out || !n_out) return fail(PM_E_ARG, "null argument");
     if (!c->loaded) return fail(PM_E_STATE, "no geometry loaded");
@@ -690,6 +856,7 @@ int pm_fail(int code, const char *fmt, ...) {
     return fail(code, "%s", buf);
 }
 int pm_one_sweep(pm_ctx *c) { return one_sweep(c); }
+int pm_settle(pm_ctx *c) { return settle(c); }
 int pm_reduce_state(pm_ctx *c, bool full, pm::RedSums *out) { return reduce(c, full, out); }
 double pm_convergence_from(const pm::RedSums &r, double *kprev) { return convergence_from(r, kprev); }
 void pm_permeability_from(const pm::RedSums &r, const int *dom, double dx, double phi, double *out) {

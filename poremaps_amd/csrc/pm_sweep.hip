@@ -8,6 +8,8 @@
 #include "pm_kernels.h"
 #include "pm_voxel.h"
 
+#include <algorithm>
+
 #pragma clang fp contract(off)
 
 namespace pm {
@@ -51,6 +53,326 @@ __global__ __launch_bounds__(kListBlock) void k_jac_pres(Grid g, const int32_t *
     if (t >= n) return;
     const int64_t c = list[t];
     pres_voxel<T, HAS_R>(g, c, pc[c], p0, u1, v1, w1, p1, q1, Rp, cp, pin_out);
+}
+
+// First sweep from a settled state (p^n given): velocities into set 1 and
+// p^n carried over, so that set 1 holds (u^{n+1}, p^n) for the lagged passes.
+template <typename T, bool HAS_R, bool SEL>
+__global__ __launch_bounds__(kListBlock) void k_jac_first(Grid g, const int32_t *__restrict__ list, int n,
+                                                          const uint8_t *__restrict__ pc,
+                                                          const T *__restrict__ p0, const T *__restrict__ u0,
+                                                          const T *__restrict__ v0, const T *__restrict__ w0,
+                                                          const T *__restrict__ q0, T *__restrict__ p1,
+                                                          T *__restrict__ u1, T *__restrict__ v1,
+                                                          T *__restrict__ w1, const float *__restrict__ Ru,
+                                                          const float *__restrict__ Rv,
+                                                          const float *__restrict__ Rw, T cu) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const int64_t c = list[t];
+    vel_voxel<T, HAS_R, SEL>(g, c, pc[c], p0, u0, v0, w0, q0, u1, v1, w1, Ru, Rv, Rw, cu);
+    p1[c] = p0[c];
+}
+
+template <typename T, bool HAS_R, bool SEL>
+__global__ __launch_bounds__(kListBlock) void k_jac_lag(Grid g, const int32_t *__restrict__ list, int n,
+                                                        const uint8_t *__restrict__ pc,
+                                                        const T *__restrict__ p0, const T *__restrict__ u0,
+                                                        const T *__restrict__ v0, const T *__restrict__ w0,
+                                                        const T *__restrict__ q0, T *__restrict__ p1,
+                                                        T *__restrict__ u1, T *__restrict__ v1,
+                                                        T *__restrict__ w1, const float *__restrict__ Rp,
+                                                        const float *__restrict__ Ru,
+                                                        const float *__restrict__ Rv,
+                                                        const float *__restrict__ Rw, T cu, T cp, T pin_out) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const int64_t c = list[t];
+    lag_voxel<T, HAS_R, SEL>(g, c, pc[c], pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp,
+                             pin_out);
+}
+
+// ---------------------------------------------------------------------------
+// Dense 2.5-D sweep (first or lagged Jacobi pass, pm_voxel.h semantics).
+//
+// A workgroup owns a tile of kSX x kSY cells in (x, y) and marches along z
+// over a segment of planes.  Each thread keeps its own column's window of the
+// state in registers (u, v at z-2..z+2, w at z-1..z+1: the z-derivatives), the
+// tile's current plane of u, v, w with a 2-cell ring goes through LDS (the x-
+// and y-derivatives), and the plane of lagged pressures p^n of the tile and
+// its -1 ring is computed once into LDS, so a cell costs about ten coalesced
+// global loads instead of the ~40 gathers of the list kernels.  Interior solid
+// cells are skipped (their velocities are exactly +0 and no fluid cell reads
+// their pressure), so fully solid lines are never fetched.  The arithmetic is
+// pm_voxel.h's, value for value: the results are bit-equal to the list kernels.
+// ---------------------------------------------------------------------------
+constexpr int kSX = 64, kSY = 8, kSN = kSX * kSY;   // tile, threads per workgroup
+constexpr int kLX = kSX + 4, kLY = kSY + 4;          // LDS plane with a 2-cell ring
+constexpr int kPX = kSX + 1, kPY = kSY + 1;          // pressure plane with the -1 ring
+constexpr int kSHalo = 4 * kLX + 4 * kSY;            // ring cells of the LDS plane
+constexpr int kSRing = kSX + 1 + kSY;                // ring cells of the pressure plane
+
+__device__ __forceinline__ bool interior(const Grid &g, int i, int j, int k) {
+    return i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
+}
+
+__device__ __forceinline__ uint8_t pc_at(const Grid &g, const uint8_t *__restrict__ pc, int i, int j, int k) {
+    return (i < g.NZ && j < g.NY && k < g.NX) ? pc[g.at(i, j, k)] : kSolid;
+}
+
+// State value at (i, j, k): 0 outside the padded field and at interior solid
+// cells (exact: their velocities are +0), else loaded.
+template <typename T>
+__device__ __forceinline__ T st_ld(const Grid &g, const T *__restrict__ f, int i, int j, int k, uint8_t pcv) {
+    if (j >= g.NY || k >= g.NX) return (T)0;
+    if (pcv == kSolid && interior(g, i, j, k)) return (T)0;
+    return f[g.at(i, j, k)];
+}
+
+template <typename T, bool HAS_R, int MODE>
+__global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty, int zs,
+                                                     const uint8_t *__restrict__ pc, const T *__restrict__ p0,
+                                                     const T *__restrict__ u0, const T *__restrict__ v0,
+                                                     const T *__restrict__ w0, const T *__restrict__ q0,
+                                                     T *__restrict__ p1, T *__restrict__ u1,
+                                                     T *__restrict__ v1, T *__restrict__ w1,
+                                                     const float *__restrict__ Rp, const float *__restrict__ Ru,
+                                                     const float *__restrict__ Rv, const float *__restrict__ Rw,
+                                                     T cu, T cp, T pin_out) {
+    __shared__ T Lu[2][kLY][kLX], Lv[2][kLY][kLX], Lw[2][kLY][kLX], LP[2][kPY][kPX];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int txt = b % ntx, rest = b / ntx, tyt = rest % nty, seg = rest / nty;
+    const int k0 = 2 + txt * kSX, j0 = 2 + tyt * kSY;
+    const int i0 = 2 + seg * zs, i1 = min(i0 + zs, g.nz + 2);
+    const int tid = threadIdx.x, tx = tid % kSX, ty = tid / kSX;
+    const int j = j0 + ty, k = k0 + tx;
+    const bool own_in = j < g.ny + 2 && k < g.nx + 2;
+    const T INV_RE = (T)(1.0 / kRe), C2DT = (T)(kC2 * kDt);
+
+    // LDS-ring cell loaded by this thread
+    int hly = -1, hlx = -1;
+    if (tid < kSHalo) {
+        if (tid < 2 * kLX) { hly = tid / kLX; hlx = tid % kLX; }
+        else if (tid < 4 * kLX) { const int h = tid - 2 * kLX; hly = kSY + 2 + h / kLX; hlx = h % kLX; }
+        else { const int h = tid - 4 * kLX; hly = 2 + h / 4; const int s = h % 4; hlx = s < 2 ? s : kSX + s; }
+    }
+    const int hj = j0 - 2 + hly, hk = k0 - 2 + hlx;
+    const bool h_on = hly >= 0 && hj < g.NY && hk < g.NX;
+    // pressure-ring cell computed by this thread (corner (j0-1, k0-1) unused)
+    int rly = -1, rlx = -1;
+    if (tid < kSRing) {
+        if (tid < kSX + 1) { rly = 1; rlx = 1 + tid; }
+        else { rly = 2 + (tid - kSX - 1); rlx = 1; }
+    }
+    const int rj = j0 - 2 + rly, rk = k0 - 2 + rlx;
+    const bool r_on = rly >= 0 && rj <= g.ny + 1 && rk <= g.nx + 1 && !(rj < 2 && rk < 2);
+    const bool r_int = r_on && rj >= 2 && rk >= 2;
+
+    // Software pipeline: everything step i reads from global memory was
+    // requested during step i-1 (or the prologue), with the solid-cell
+    // predicates taken from codes fetched one step earlier still.
+    uint8_t c0 = pc_at(g, pc, i0, j, k), c1 = pc_at(g, pc, i0 + 1, j, k), c2 = pc_at(g, pc, i0 + 2, j, k),
+            c3 = pc_at(g, pc, i0 + 3, j, k);
+    const uint8_t cm2 = pc_at(g, pc, i0 - 2, j, k), cm1 = pc_at(g, pc, i0 - 1, j, k);
+    T um2 = st_ld(g, u0, i0 - 2, j, k, cm2), vm2 = st_ld(g, v0, i0 - 2, j, k, cm2);
+    T um1 = st_ld(g, u0, i0 - 1, j, k, cm1), vm1 = st_ld(g, v0, i0 - 1, j, k, cm1);
+    T wm1 = st_ld(g, w0, i0 - 1, j, k, cm1);
+    T uc = st_ld(g, u0, i0, j, k, c0), vc = st_ld(g, v0, i0, j, k, c0), wc = st_ld(g, w0, i0, j, k, c0);
+    T up1 = st_ld(g, u0, i0 + 1, j, k, c1), vp1 = st_ld(g, v0, i0 + 1, j, k, c1);
+    T wp1 = st_ld(g, w0, i0 + 1, j, k, c1);
+    T up2 = st_ld(g, u0, i0 + 2, j, k, c2), vp2 = st_ld(g, v0, i0 + 2, j, k, c2);
+    T pcur = (T)0, rpc = (T)0, ruc = (T)0, rvc = (T)0, rwc = (T)0;
+    if (own_in && c0 != kSolid) {
+        const int64_t c = g.at(i0, j, k);
+        pcur = p0[c];
+        if (HAS_R) { rpc = (T)Rp[c]; ruc = (T)Ru[c]; rvc = (T)Rv[c]; rwc = (T)Rw[c]; }
+    }
+    T hu = (T)0, hv = (T)0, hw = (T)0;
+    uint8_t hc1 = kSolid;
+    if (h_on) {
+        const uint8_t hc0 = pc[g.at(i0, hj, hk)];
+        hu = st_ld(g, u0, i0, hj, hk, hc0);
+        hv = st_ld(g, v0, i0, hj, hk, hc0);
+        hw = st_ld(g, w0, i0, hj, hk, hc0);
+        hc1 = pc[g.at(i0 + 1, hj, hk)];
+    }
+    uint8_t rc0 = kSolid, rc1 = kSolid;
+    T rp0 = (T)0, rrp = (T)0, rw1 = (T)0;
+    if (r_int) {
+        const int64_t r = g.at(i0, rj, rk);
+        rc0 = pc[r];
+        if (rc0 != kSolid) {
+            rp0 = p0[r];
+            if (HAS_R) rrp = (T)Rp[r];
+            rw1 = w0[r + g.sy];
+        }
+        rc1 = pc[r + g.sy];
+    }
+    // pressure of the own column one plane below the segment
+    T Pprev = (T)0;
+    if (own_in) {
+        if (i0 == 2 || MODE == JAC_FIRST) Pprev = p0[g.at(i0 - 1, j, k)];
+        else if (cm1 != kSolid) Pprev = lag_p<T, HAS_R>(g, g.at(i0 - 1, j, k), cm1, p0, u0, v0, w0, Rp, cp);
+    }
+
+    for (int i = i0; i < i1; i++) {
+        const int buf = i & 1;
+        const bool out = i == g.nz + 1;
+        const bool more = i + 1 < i1;
+
+        // requests for step i+1
+        uint8_t c4 = kSolid, hc2 = kSolid, rc2 = kSolid;
+        T u3 = (T)0, v3 = (T)0, w2 = (T)0, pn = (T)0, rpn = (T)0, run = (T)0, rvn = (T)0, rwn = (T)0;
+        T hun = (T)0, hvn = (T)0, hwn = (T)0, rp0n = (T)0, rrpn = (T)0, rw1n = (T)0;
+        if (more) {
+            c4 = pc_at(g, pc, i + 4, j, k);
+            u3 = st_ld(g, u0, i + 3, j, k, c3);
+            v3 = st_ld(g, v0, i + 3, j, k, c3);
+            w2 = st_ld(g, w0, i + 2, j, k, c2);
+            if (own_in && c1 != kSolid) {
+                const int64_t n = g.at(i + 1, j, k);
+                pn = p0[n];
+                if (HAS_R) { rpn = (T)Rp[n]; run = (T)Ru[n]; rvn = (T)Rv[n]; rwn = (T)Rw[n]; }
+            }
+            if (h_on) {
+                hun = st_ld(g, u0, i + 1, hj, hk, hc1);
+                hvn = st_ld(g, v0, i + 1, hj, hk, hc1);
+                hwn = st_ld(g, w0, i + 1, hj, hk, hc1);
+                hc2 = pc[g.at(i + 2, hj, hk)];
+            }
+            if (r_int) {
+                const int64_t r = g.at(i + 1, rj, rk);
+                if (rc1 != kSolid) {
+                    rp0n = p0[r];
+                    if (HAS_R) rrpn = (T)Rp[r];
+                    rw1n = w0[r + g.sy];
+                }
+                rc2 = pc[r + g.sy];
+            }
+        }
+
+        Lu[buf][ty + 2][tx + 2] = uc;
+        Lv[buf][ty + 2][tx + 2] = vc;
+        Lw[buf][ty + 2][tx + 2] = wc;
+        if (hly >= 0) {
+            Lu[buf][hly][hlx] = hu;
+            Lv[buf][hly][hlx] = hv;
+            Lw[buf][hly][hlx] = hw;
+        }
+        __syncthreads();
+
+        // lagged pressure (first pass: the given one) of the own cell and the
+        // -1 ring; on the outlet plane the ring/neighbour value is the buffer q
+        const bool own_fluid = own_in && c0 != kSolid;
+        T Pc = (T)0, lp = (T)0;
+        if (own_fluid) {
+            if (MODE == JAC_FIRST) {
+                Pc = pcur;
+                lp = out ? q0[g.qat(j, k)] : Pc;
+            } else {
+                const Cases cs = decode(c0);
+                const T wz = fwd_solid(cs.cz) ? (T)0 : wp1;
+                const T vy = fwd_solid(cs.cy) ? (T)0 : Lv[buf][ty + 3][tx + 2];
+                const T ux = fwd_solid(cs.cx) ? (T)0 : Lu[buf][ty + 2][tx + 3];
+                const T d0 = ((((wz - wc) + vy) - vc) + ux) - uc;
+                if (out) {
+                    Pc = pin_out;
+                    lp = pin_out - d0 * C2DT;
+                } else {
+                    T div = d0;
+                    if (HAS_R) div = div + rpc;
+                    Pc = pcur - div * cp;
+                    lp = Pc;
+                }
+            }
+        }
+        LP[buf][ty + 1][tx + 1] = lp;
+        if (rly >= 0) {
+            T val = (T)0;
+            if (r_int) {
+                if (rc0 != kSolid) {
+                    if (MODE == JAC_FIRST) {
+                        val = out ? q0[g.qat(rj, rk)] : rp0;
+                    } else {
+                        const Cases rs = decode(rc0);
+                        const T wz = fwd_solid(rs.cz) ? (T)0 : rw1;
+                        const T vy = fwd_solid(rs.cy) ? (T)0 : Lv[buf][rly + 1][rlx];
+                        const T ux = fwd_solid(rs.cx) ? (T)0 : Lu[buf][rly][rlx + 1];
+                        const T d0 = ((((wz - Lw[buf][rly][rlx]) + vy) - Lv[buf][rly][rlx]) + ux) -
+                                     Lu[buf][rly][rlx];
+                        if (out) {
+                            val = pin_out - d0 * C2DT;
+                        } else {
+                            T div = d0;
+                            if (HAS_R) div = div + rrp;
+                            val = rp0 - div * cp;
+                        }
+                    }
+                }
+            } else if (r_on) {
+                // ring cell in the domain's x or y halo
+                const bool xh = rk < 2;
+                const bool per = xh ? g.px : g.py;
+                if (MODE == JAC_FIRST || !per) {
+                    val = out ? q0[g.qat(rj, rk)] : p0[g.at(i, rj, rk)];
+                } else {
+                    const int wj = xh ? rj : rj + g.ny, wk = xh ? rk + g.nx : rk;
+                    const int64_t n = g.at(i, wj, wk);
+                    const uint8_t nc = pc[n];
+                    if (nc != kSolid)
+                        val = out ? lag_q<T>(g, n, nc, u0, v0, w0, pin_out)
+                                  : lag_p<T, HAS_R>(g, n, nc, p0, u0, v0, w0, Rp, cp);
+                }
+            }
+            LP[buf][rly - 1][rlx - 1] = val;
+        }
+        __syncthreads();
+
+        if (own_fluid) {
+            const int64_t c = g.at(i, j, k);
+            const Cases cs = decode(c0);
+            const T dpx = cs.bx ? (T)0 : Pc - LP[buf][ty + 1][tx];
+            const T dpy = cs.by ? (T)0 : Pc - LP[buf][ty][tx + 1];
+            const T dpz = cs.bz ? (T)0 : Pc - Pprev;
+            const int ly = ty + 2, lx = tx + 2;
+            const T lxx = ((cs.bx ? (T)0 : Lu[buf][ly][lx - 1]) - (T)2.0 * uc) +
+                          (fwd_solid(cs.cx) ? (T)0 : Lu[buf][ly][lx + 1]);
+            const T lyy = ((cs.by ? (T)0 : Lv[buf][ly - 1][lx]) - (T)2.0 * vc) +
+                          (fwd_solid(cs.cy) ? (T)0 : Lv[buf][ly + 1][lx]);
+            const T lzz = ((cs.bz ? (T)0 : wm1) - (T)2.0 * wc) + (fwd_solid(cs.cz) ? (T)0 : wp1);
+            const T dyx = d2<T>(cs.cx, Lv[buf][ly][lx - 2], Lv[buf][ly][lx - 1], vc, Lv[buf][ly][lx + 1],
+                                Lv[buf][ly][lx + 2]);
+            const T dzx = d2<T>(cs.cx, Lw[buf][ly][lx - 2], Lw[buf][ly][lx - 1], wc, Lw[buf][ly][lx + 1],
+                                Lw[buf][ly][lx + 2]);
+            const T dxy = d2<T>(cs.cy, Lu[buf][ly - 2][lx], Lu[buf][ly - 1][lx], uc, Lu[buf][ly + 1][lx],
+                                Lu[buf][ly + 2][lx]);
+            const T dzy = d2<T>(cs.cy, Lw[buf][ly - 2][lx], Lw[buf][ly - 1][lx], wc, Lw[buf][ly + 1][lx],
+                                Lw[buf][ly + 2][lx]);
+            const T dxz = d2<T>(cs.cz, um2, um1, uc, up1, up2);
+            const T dyz = d2<T>(cs.cz, vm2, vm1, vc, vp1, vp2);
+            const T su = (lxx + dxy) + dxz;
+            const T sv = (lyy + dyz) + dyx;
+            const T sw = (lzz + dzx) + dzy;
+            T fu = cs.bx ? (T)0 : su * INV_RE - dpx;
+            T fv = cs.by ? (T)0 : sv * INV_RE - dpy;
+            T fw = cs.bz ? (T)0 : sw * INV_RE - dpz;
+            if (HAS_R) { fu = fu + ruc; fv = fv + rvc; fw = fw + rwc; }
+            u1[c] = cs.bx ? (T)0 : uc + fu * cu;
+            v1[c] = cs.by ? (T)0 : vc + fv * cu;
+            w1[c] = cs.bz ? (T)0 : wc + fw * cu;
+            p1[c] = Pc;
+        }
+
+        // advance the pipeline one plane
+        Pprev = Pc;
+        um2 = um1; um1 = uc; uc = up1; up1 = up2; up2 = u3;
+        vm2 = vm1; vm1 = vc; vc = vp1; vp1 = vp2; vp2 = v3;
+        wm1 = wc; wc = wp1; wp1 = w2;
+        c0 = c1; c1 = c2; c2 = c3; c3 = c4;
+        pcur = pn; rpc = rpn; ruc = run; rvc = rvn; rwc = rwn;
+        hu = hun; hv = hvn; hw = hwn; hc1 = hc2;
+        rc0 = rc1; rc1 = rc2; rp0 = rp0n; rrp = rrpn; rw1 = rw1n;
+    }
 }
 
 template <typename T, bool HAS_R, bool SEL>
@@ -150,6 +472,85 @@ template void launch_jacobi<float>(const Grid &, const FluidLists &, const uint8
                                    const float *, const float *, const float *, const float *, float *, float *,
                                    float *, float *, float *, float, float, float, const float *, const float *,
                                    const float *, const float *, hipStream_t);
+
+template <typename T, bool HAS_R, bool SEL>
+static void jacobi_stage_t(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0,
+                           const T *u0, const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1,
+                           T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv,
+                           const float *Rw, hipStream_t s) {
+    const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
+    if (stage == JAC_FIRST)
+        hipLaunchKernelGGL((k_jac_first<T, HAS_R, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0,
+                           q0, p1, u1, v1, w1, Ru, Rv, Rw, cu);
+    else if (stage == JAC_LAG)
+        hipLaunchKernelGGL((k_jac_lag<T, HAS_R, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, q0,
+                           p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
+    else
+        hipLaunchKernelGGL((k_jac_pres<T, HAS_R>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, p1,
+                           q1, Rp, cp, pin_out);
+}
+
+template <typename T>
+void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0,
+                         const T *u0, const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1,
+                         T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+                         hipStream_t s) {
+    if (L.n_all == 0) return;
+    if (Rp && L.select_loads)
+        jacobi_stage_t<T, true, true>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, Rp,
+                                      Ru, Rv, Rw, s);
+    else if (Rp)
+        jacobi_stage_t<T, true, false>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out,
+                                       Rp, Ru, Rv, Rw, s);
+    else if (L.select_loads)
+        jacobi_stage_t<T, false, true>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out,
+                                       Rp, Ru, Rv, Rw, s);
+    else
+        jacobi_stage_t<T, false, false>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out,
+                                        Rp, Ru, Rv, Rw, s);
+}
+template void launch_jacobi_stage<double>(int, const Grid &, const FluidLists &, const uint8_t *, const double *,
+                                          const double *, const double *, const double *, const double *,
+                                          double *, double *, double *, double *, double *, double, double, double,
+                                          const float *, const float *, const float *, const float *,
+                                          hipStream_t);
+template void launch_jacobi_stage<float>(int, const Grid &, const FluidLists &, const uint8_t *, const float *,
+                                         const float *, const float *, const float *, const float *, float *,
+                                         float *, float *, float *, float *, float, float, float, const float *,
+                                         const float *, const float *, const float *, hipStream_t);
+
+template <typename T>
+void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const T *p0, const T *u0,
+                          const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T cu, T cp, T pin_out,
+                          const float *Rp, const float *Ru, const float *Rv, const float *Rw, hipStream_t s) {
+    const int ntx = (g.nx + kSX - 1) / kSX, nty = (g.ny + kSY - 1) / kSY;
+    zs = std::max(1, zs);
+    const int nseg = (g.nz + zs - 1) / zs;
+    const dim3 grid((unsigned)(ntx * nty * nseg)), blk(kSN);
+    if (Rp) {
+        if (stage == JAC_FIRST)
+            hipLaunchKernelGGL((k_jac_stream<T, true, JAC_FIRST>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0,
+                               w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
+        else
+            hipLaunchKernelGGL((k_jac_stream<T, true, JAC_LAG>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0,
+                               w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
+    } else {
+        if (stage == JAC_FIRST)
+            hipLaunchKernelGGL((k_jac_stream<T, false, JAC_FIRST>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0,
+                               v0, w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
+        else
+            hipLaunchKernelGGL((k_jac_stream<T, false, JAC_LAG>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0,
+                               w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
+    }
+}
+template void launch_jacobi_stream<double>(int, int, const Grid &, const uint8_t *, const double *, const double *,
+                                           const double *, const double *, const double *, double *, double *,
+                                           double *, double *, double, double, double, const float *,
+                                           const float *, const float *, const float *, hipStream_t);
+template void launch_jacobi_stream<float>(int, int, const Grid &, const uint8_t *, const float *, const float *,
+                                          const float *, const float *, const float *, float *, float *, float *,
+                                          float *, float, float, float, const float *, const float *,
+                                          const float *, const float *, hipStream_t);
 
 template <typename T, bool HAS_R, bool SEL>
 static void colour_stage(const Grid &g, const int32_t *list, int n, const uint8_t *pc, T *p, T *u, T *v, T *w,

@@ -69,6 +69,140 @@ __device__ __forceinline__ void pres_voxel(const Grid &g, int64_t c, uint8_t pcv
     if (i == g.nz + 1) q1[g.qat(j, k)] = pin_out - div0 * (T)(kC2 * kDt);
 }
 
+// ---------------------------------------------------------------------------
+// Symplectic Jacobi in lagged form.  A sweep n maps (u^n, p^n) to
+// (u^{n+1}, p^{n+1}) with u^{n+1} = u^n + cu*F(u^n, p^n) and
+// p^{n+1} = p^n - cp*(div u^{n+1} + Rp).  The lagged pass maps
+// (u^n, p^{n-1}) to (u^{n+1}, p^n): it recomputes p^n = p^{n-1} -
+// cp*(div u^n + Rp) at the cell and at its -x/-y/-z neighbours from the
+// input state (halos filled) instead of reading a pressure written by a
+// second kernel, so one pass over the fluid list does the work of the
+// velocity and pressure kernels.  Every value is produced by the same
+// expressions as vel_voxel / pres_voxel, so the lagged sequence is bit-equal
+// to the two-kernel one; a final pres_voxel pass ("settle") supplies p of the
+// last sweep.
+// ---------------------------------------------------------------------------
+
+// Viscous sums (su, sv, sw) of the momentum forces at c: the stencil of
+// fast_force_p (SEL false: case-predicated loads) or fast_force_s (SEL true).
+template <typename T, bool SEL>
+__device__ __forceinline__ void visc_sums(const Cases &cs, int64_t c, int64_t sj, int64_t si,
+                                          const T *__restrict__ u, const T *__restrict__ v,
+                                          const T *__restrict__ w, T uc, T vc, T wc, T &su, T &sv, T &sw) {
+    T lxx, lyy, lzz, dyx, dzx, dxy, dzy, dxz, dyz;
+    if (SEL) {
+        const T ul = u[c - 1], ur = u[c + 1], vl = v[c - sj], vr = v[c + sj], wl = w[c - si], wr = w[c + si];
+        lxx = ((cs.bx ? (T)0 : ul) - (T)2.0 * uc) + (fwd_solid(cs.cx) ? (T)0 : ur);
+        lyy = ((cs.by ? (T)0 : vl) - (T)2.0 * vc) + (fwd_solid(cs.cy) ? (T)0 : vr);
+        lzz = ((cs.bz ? (T)0 : wl) - (T)2.0 * wc) + (fwd_solid(cs.cz) ? (T)0 : wr);
+        dyx = d2_sel<T>(cs.cx, v, c, 1, vc);
+        dzx = d2_sel<T>(cs.cx, w, c, 1, wc);
+        dxy = d2_sel<T>(cs.cy, u, c, sj, uc);
+        dzy = d2_sel<T>(cs.cy, w, c, sj, wc);
+        dxz = d2_sel<T>(cs.cz, u, c, si, uc);
+        dyz = d2_sel<T>(cs.cz, v, c, si, vc);
+    } else {
+        lxx = lap_load<T>(cs.cx, u, c, 1, uc);
+        lyy = lap_load<T>(cs.cy, v, c, sj, vc);
+        lzz = lap_load<T>(cs.cz, w, c, si, wc);
+        dyx = d2_load<T>(cs.cx, v, c, 1, vc);
+        dzx = d2_load<T>(cs.cx, w, c, 1, wc);
+        dxy = d2_load<T>(cs.cy, u, c, sj, uc);
+        dzy = d2_load<T>(cs.cy, w, c, sj, wc);
+        dxz = d2_load<T>(cs.cz, u, c, si, uc);
+        dyz = d2_load<T>(cs.cz, v, c, si, vc);
+    }
+    su = (lxx + dxy) + dxz;
+    sv = (lyy + dyz) + dyx;
+    sw = (lzz + dzx) + dzy;
+}
+
+// Divergence of the velocities at fluid cell n as pres_voxel forms it (the +1
+// neighbours read through the filled halos, 0 where solid).
+template <typename T>
+__device__ __forceinline__ T lag_div0(const Grid &g, int64_t n, uint8_t pcn, const T *__restrict__ u,
+                                      const T *__restrict__ v, const T *__restrict__ w) {
+    const Cases cs = decode(pcn);
+    const T wz = fwd_solid(cs.cz) ? (T)0 : w[n + g.sy];
+    const T vy = fwd_solid(cs.cy) ? (T)0 : v[n + g.sx];
+    const T ux = fwd_solid(cs.cx) ? (T)0 : u[n + 1];
+    return ((((wz - w[n]) + vy) - v[n]) + ux) - u[n];
+}
+
+// p^n at a fluid interior cell n off the outlet plane.
+template <typename T, bool HAS_R>
+__device__ __forceinline__ T lag_p(const Grid &g, int64_t n, uint8_t pcn, const T *__restrict__ p0,
+                                   const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+                                   const float *__restrict__ Rp, T cp) {
+    T div = lag_div0<T>(g, n, pcn, u, v, w);
+    if (HAS_R) div = div + (T)Rp[n];
+    return p0[n] - div * cp;
+}
+
+// Outlet-plane buffer q^n at a fluid interior cell n of z = nz+1.
+template <typename T>
+__device__ __forceinline__ T lag_q(const Grid &g, int64_t n, uint8_t pcn, const T *__restrict__ u,
+                                   const T *__restrict__ v, const T *__restrict__ w, T pin_out) {
+    return pin_out - lag_div0<T>(g, n, pcn, u, v, w) * (T)(kC2 * kDt);
+}
+
+// Pressure seen by fluid cell c from its fluid -1 neighbour along an in-plane
+// axis (stride s, index idx of c on that axis, n_ax interior cells, periodic
+// flag per): recomputed at the neighbour or at its periodic image; a wall
+// halo keeps its stored value (p, or q on the outlet plane).
+template <typename T, bool HAS_R>
+__device__ __forceinline__ T lag_back(const Grid &g, int64_t c, int64_t s, int idx, int n_ax, bool per, bool out,
+                                      int64_t qn, const uint8_t *__restrict__ pc, const T *__restrict__ p0,
+                                      const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+                                      const T *__restrict__ q0, const float *__restrict__ Rp, T cp, T pin_out) {
+    int64_t n = c - s;
+    if (idx == 2) {
+        if (!per) return out ? q0[qn] : p0[n];
+        n += (int64_t)n_ax * s;
+    }
+    return out ? lag_q<T>(g, n, pc[n], u, v, w, pin_out) : lag_p<T, HAS_R>(g, n, pc[n], p0, u, v, w, Rp, cp);
+}
+
+// One lagged pass at fluid cell c: reads (p^{n-1}, u^n, v^n, w^n) = set 0,
+// writes (p^n, u^{n+1}, v^{n+1}, w^{n+1}) into set 1.
+template <typename T, bool HAS_R, bool SEL>
+__device__ __forceinline__ void lag_voxel(const Grid &g, int64_t c, uint8_t pcv, const uint8_t *__restrict__ pc,
+                                          const T *__restrict__ p0, const T *__restrict__ u0,
+                                          const T *__restrict__ v0, const T *__restrict__ w0,
+                                          const T *__restrict__ q0, T *__restrict__ p1, T *__restrict__ u1,
+                                          T *__restrict__ v1, T *__restrict__ w1, const float *__restrict__ Rp,
+                                          const float *__restrict__ Ru, const float *__restrict__ Rv,
+                                          const float *__restrict__ Rw, T cu, T cp, T pin_out) {
+    const Cases cs = decode(pcv);
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const bool out = i == g.nz + 1;
+    const T Pc = out ? pin_out : lag_p<T, HAS_R>(g, c, pcv, p0, u0, v0, w0, Rp, cp);
+    T dpx = (T)0, dpy = (T)0, dpz = (T)0;
+    if (!cs.bx)
+        dpx = Pc - lag_back<T, HAS_R>(g, c, 1, k, g.nx, g.px, out, g.qat(j, k - 1), pc, p0, u0, v0, w0, q0, Rp,
+                                      cp, pin_out);
+    if (!cs.by)
+        dpy = Pc - lag_back<T, HAS_R>(g, c, g.sx, j, g.ny, g.py, out, g.qat(j - 1, k), pc, p0, u0, v0, w0, q0,
+                                      Rp, cp, pin_out);
+    if (!cs.bz) {
+        const int64_t n = c - g.sy;
+        dpz = Pc - (i == 2 ? p0[n] : lag_p<T, HAS_R>(g, n, pc[n], p0, u0, v0, w0, Rp, cp));
+    }
+    const T uc = u0[c], vc = v0[c], wc = w0[c];
+    T su, sv, sw;
+    visc_sums<T, SEL>(cs, c, g.sx, g.sy, u0, v0, w0, uc, vc, wc, su, sv, sw);
+    const T INV_RE = (T)(1.0 / kRe);
+    T fu = cs.bx ? (T)0 : su * INV_RE - dpx;
+    T fv = cs.by ? (T)0 : sv * INV_RE - dpy;
+    T fw = cs.bz ? (T)0 : sw * INV_RE - dpz;
+    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
+    u1[c] = cs.bx ? (T)0 : uc + fu * cu;
+    v1[c] = cs.by ? (T)0 : vc + fv * cu;
+    w1[c] = cs.bz ? (T)0 : wc + fw * cu;
+    p1[c] = Pc;
+}
+
 // 3-colour SOR: one cell of the current colour, in place.
 template <typename T, bool HAS_R, bool SEL>
 __device__ __forceinline__ void colour_voxel(const Grid &g, int64_t c, uint8_t pcv, T *p, T *u, T *v, T *w,

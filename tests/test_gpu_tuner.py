@@ -1,0 +1,60 @@
+"""Jacobi kernel forms and the autotuner on the GPU.
+
+The velocity + pressure pair, the lagged single pass over the fluid list and
+the lagged dense 2.5-D pass (two z-segment lengths) must produce the same
+iterates bit for bit, and a solve whose first sweeps the autotuner spreads
+over all of them must end exactly where a fixed-form solve ends."""
+import os
+
+import numpy as np
+import pytest
+
+import poremaps_amd as pm
+from conftest import random_geometry
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def solve(raw, bc, precision, form=None, zs=None):
+    old = {k: os.environ.get(k) for k in ("POREMAPS_JACOBI", "POREMAPS_STREAM_ZS")}
+    try:
+        if form:
+            os.environ["POREMAPS_JACOBI"] = form
+        else:
+            os.environ.pop("POREMAPS_JACOBI", None)
+        if zs:
+            os.environ["POREMAPS_STREAM_ZS"] = str(zs)
+        else:
+            os.environ.pop("POREMAPS_STREAM_ZS", None)
+        with pm.Solver(raw, boundary_method=bc, mode="jacobi", precision=precision, omega=pm.OMEGA,
+                       omega_p=pm.OMEGA_P, eps=1e-6, voxel_size=1e-5, max_iter=200_000) as s:
+            res, _ = s.solve()
+            fields = [s.field(n) for n in ("velx", "vely", "velz", "press")]
+            return res, fields, s.kernel_choice()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("precision,bc", [("f32dc", 0), ("f32dc", 1), ("f64", 2), ("f64", 4)])
+def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
+    # 70 x 20 x 36: two x tiles and three y tiles of the dense kernel, a
+    # partial tile on each axis and (zs 16) three z segments
+    raw = random_geometry((36, 20, 70), 0.4, 71 + bc)
+    say("kernel forms", precision, bc)
+    ref, ref_fields, _ = solve(raw, bc, precision, "pair")
+    for form, zs in (("list", None), ("stream", 16), ("stream", 64), (None, None)):
+        res, fields, (name, tried) = solve(raw, bc, precision, form, zs)
+        say(form, zs, name, res.sweeps)
+        assert res.sweeps == ref.sweeps and res.last.wk33 == ref.last.wk33, (form, zs)
+        for a, b, n in zip(fields, ref_fields, ("velx", "vely", "velz", "press")):
+            assert np.array_equal(a, b), (form, zs, n)
+        if form is None:
+            assert len(tried) == 5 and name in tried
