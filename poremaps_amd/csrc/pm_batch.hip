@@ -39,6 +39,8 @@ struct BSample {
     float *R[4];
     double *X[4];
     double *qX;
+    // defect correction: interleaved correction sets and residual
+    float4 *A4, *B4, *R4;
     int chunk_begin, chunk_end;
 };
 
@@ -80,6 +82,66 @@ __global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, co
                       fld<T>(dst, 0), q1, DC ? s.R[0] : nullptr, cp, DC ? (T)0 : (T)3.0);
 }
 
+// Interleaved correction state (defect correction): velocity and pressure
+// passes of the pair, ghost fill, fold (over the fluid list: X += d, d = 0;
+// the halo copies of d are refilled with zeros by the following ghost fill)
+// and residual.
+__global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
+                                               float cu) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    vel_voxel4(s.g, c, s.pc[c], parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA), parity ? s.A4 : s.B4,
+               s.R4, cu);
+}
+
+__global__ __launch_bounds__(kB) void k_b_pres4(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
+                                                float cp) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    pres_voxel4(s.g, c, s.pc[c], parity ? s.B4 : s.A4, parity ? s.A4 : s.B4, (float *)(parity ? s.qA : s.qB), s.R4,
+                cp, 0.f);
+}
+
+__global__ __launch_bounds__(kB) void k_b_ghost4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                                 int which) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int64_t t = (int64_t)ch.start + threadIdx.x;
+    if (t >= ghost_lines(s.g)) return;
+    ghost_line4(s.g, t, s.G, which ? s.B4 : s.A4, (float *)(which ? s.qB : s.qA));
+}
+
+__global__ __launch_bounds__(kB) void k_b_fold4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                                int which) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    float4 *d = which ? s.B4 : s.A4;
+    const float4 v = d[c];
+    s.X[0][c] = s.X[0][c] + (double)v.x;
+    s.X[1][c] = s.X[1][c] + (double)v.y;
+    s.X[2][c] = s.X[2][c] + (double)v.z;
+    s.X[3][c] = s.X[3][c] + (double)v.w;
+    d[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kB) void k_b_residual4(const BSample *__restrict__ S, const BChunk *__restrict__ C) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    residual_voxel4(s.g, c, s.pc[c], s.X[0], s.X[1], s.X[2], s.X[3], s.qX, s.R4);
+}
+
 // which: 0 = set A, 1 = set B (the swept state, correction pins 0 in the
 // defect-correction mode), 2 = the FP64 solution X (solution pins).
 template <typename T, bool DC>
@@ -102,22 +164,6 @@ __global__ __launch_bounds__(kB) void k_b_ghost(const BSample *__restrict__ S, c
     }
 }
 
-// Defect-correction fold over the fluid list: X += d, d = 0 (the halo copies of
-// d are refilled with zeros by the following ghost fill).
-__global__ __launch_bounds__(kB) void k_b_fold(const BSample *__restrict__ S, const BChunk *__restrict__ C, int which) {
-    const BChunk ch = C[blockIdx.x];
-    const BSample &s = S[ch.s];
-    const int t = ch.start + threadIdx.x;
-    if (t >= s.n) return;
-    const int64_t c = s.list[t];
-    void *const *d = which ? s.B : s.A;
-    for (int q = 0; q < 4; q++) {
-        float *dq = fld<float>(d, q);
-        s.X[q][c] = s.X[q][c] + (double)dq[c];
-        dq[c] = 0.f;
-    }
-}
-
 __global__ __launch_bounds__(kB) void k_b_outlet(const BSample *__restrict__ S, const BChunk *__restrict__ C) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
@@ -136,14 +182,6 @@ __global__ __launch_bounds__(kB) void k_b_zero_q(const BSample *__restrict__ S, 
     ((float *)s.qB)[e] = 0.f;
 }
 
-__global__ __launch_bounds__(kB) void k_b_residual(const BSample *__restrict__ S, const BChunk *__restrict__ C) {
-    const BChunk ch = C[blockIdx.x];
-    const BSample &s = S[ch.s];
-    const int t = ch.start + threadIdx.x;
-    if (t >= s.n) return;
-    const int64_t c = s.list[t];
-    residual_voxel(s.g, c, s.pc[c], s.X[0], s.X[1], s.X[2], s.X[3], s.qX, s.R[0], s.R[1], s.R[2], s.R[3]);
-}
 
 // Per-chunk partial sums of the solution state (whole-domain part of
 // src/evaluation.cc:54-262).  which: 0/1 = set A/B (FP64 mode), 2 = X.
@@ -302,6 +340,9 @@ extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int
     int rc = pm_ctx_create(&q, &c);
     if (rc != PM_OK) return rc;
     rc = pm_load_geometry(c, raw);
+    // the batch runs its own kernels: the first sweep uses the pair, untuned
+    c->tuner = Tuner();
+    c->jac_impl = JI_PAIR;
     // the first sweep (and, in the defect-correction mode, the fold after
     // sweep 0) runs on the sample's own stream; afterwards every interval
     // ends on an iteration index that is a multiple of it_eval
@@ -368,6 +409,9 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
             d.R[q] = c->R[q];
             d.X[q] = c->X[q];
         }
+        d.A4 = c->S4[0];
+        d.B4 = c->S4[1];
+        d.R4 = c->R4;
         d.qA = b->dc() ? (void *)c->qD[0] : (void *)c->qX[0];
         d.qB = b->dc() ? (void *)c->qD[1] : (void *)c->qX[1];
         d.qX = c->qX[0];
@@ -393,16 +437,16 @@ static void launch_interval(pm_batch *b) {
         const float cu = (float)(om * kDt), cp = (float)(wp * kC2 * kDt);
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
-            hipLaunchKernelGGL((k_b_vel<float, true>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
-            hipLaunchKernelGGL((k_b_pres<float, true>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cp);
-            hipLaunchKernelGGL((k_b_ghost<float, true>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
+            hipLaunchKernelGGL(k_b_vel4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
+            hipLaunchKernelGGL(k_b_pres4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cp);
+            hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
         }
         const int cur = n & 1;
-        hipLaunchKernelGGL(k_b_fold, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur);
-        hipLaunchKernelGGL((k_b_ghost<float, true>), dim3(b->nG), blk, 0, st, b->dS, b->dG, cur);
+        hipLaunchKernelGGL(k_b_fold4, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur);
+        hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, cur);
         hipLaunchKernelGGL((k_b_ghost<double, true>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 2);
         hipLaunchKernelGGL(k_b_outlet, dim3(b->nP), blk, 0, st, b->dS, b->dP);
-        hipLaunchKernelGGL(k_b_residual, dim3(b->nC), blk, 0, st, b->dS, b->dC);
+        hipLaunchKernelGGL(k_b_residual4, dim3(b->nC), blk, 0, st, b->dS, b->dC);
         hipLaunchKernelGGL(k_b_zero_q, dim3(b->nP), blk, 0, st, b->dS, b->dP);
         hipLaunchKernelGGL(k_b_reduce, dim3(b->nC), blk, 0, st, b->dS, b->dC, 2, b->dPart);
     } else {
@@ -429,7 +473,7 @@ static void drop_graph(pm_batch *b) {
 // swap the context's pointers as pm_one_sweep does.
 static void swap_current(pm_batch *b, pm_ctx *c) {
     if (b->dc()) {
-        for (int q = 0; q < 4; q++) std::swap(c->D[q], c->D2[q]);
+        std::swap(c->S4[0], c->S4[1]);
         std::swap(c->qD[0], c->qD[1]);
     } else {
         for (int q = 0; q < 4; q++) std::swap(c->X[q], c->X2[q]);

@@ -58,6 +58,10 @@ struct pm_ctx {
     float *D2[4] = {nullptr, nullptr, nullptr, nullptr};
     float *qD[2] = {nullptr, nullptr};
     float *R[4] = {nullptr, nullptr, nullptr, nullptr};
+    // defect-correction Jacobi: the correction state and residual interleaved
+    // ({p, u, v, w} per cell), two sets for the out-of-place sweeps
+    float4 *S4[2] = {nullptr, nullptr};
+    float4 *R4 = nullptr;
     // reductions
     pm::FluidLists lists;   // fluid cells of the interior (fast modes)
     void *red_scratch = nullptr;
@@ -82,6 +86,7 @@ struct pm_ctx {
     bool fast() const { return prm.mode != PM_MODE_REFERENCE; }
     bool dc() const { return fast() && prm.precision == PM_PREC_F32DC; }
     bool jacobi() const { return prm.mode == PM_MODE_JACOBI; }
+    bool aos() const { return dc() && jacobi(); }
 };
 
 

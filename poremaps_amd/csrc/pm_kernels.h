@@ -59,7 +59,19 @@ void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const 
 // (u^n, p^{n-1}) to set 1 = (u^{n+1}, p^n); JAC_SETTLE completes set 0 in
 // place: p^n and the outlet buffer from set 0's velocities (set 1 pointers:
 // p1 = set 0's p, q1 = set 0's q).
-enum { JAC_FIRST = 0, JAC_LAG = 1, JAC_SETTLE = 2 };
+enum { JAC_FIRST = 0, JAC_LAG = 1, JAC_SETTLE = 2, JAC_PAIR = 3 };
+
+// Interleaved FP32 correction state (one float4 {p, u, v, w} per cell) of
+// the defect-correction Jacobi mode: stage JAC_FIRST / JAC_LAG / JAC_PAIR
+// (velocity + pressure kernels, S1 complete) from S0 into S1; JAC_SETTLE
+// completes S0 in place (pass S1 = S0, q1 = its outlet buffer).
+void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
+                    const float *q0, float4 *S1, float *q1, const float4 *R, float cu, float cp, hipStream_t s);
+void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
+                      const double *v, const double *w, const double *q, float4 *R, hipStream_t s);
+void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStream_t s);
+void launch_fold4(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d,
+                  hipStream_t s);
 template <typename T>
 void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0,
                          const T *u0, const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1,

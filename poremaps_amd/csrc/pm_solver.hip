@@ -106,7 +106,8 @@ static void free_all(pm_ctx *c) {
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     f(c->G); f(c->pc); f(c->code);
     for (int q = 0; q < 4; q++) { f(c->X[q]); f(c->X2[q]); f(c->D[q]); f(c->D2[q]); f(c->R[q]); }
-    for (int q = 0; q < 2; q++) { f(c->qX[q]); f(c->qD[q]); }
+    for (int q = 0; q < 2; q++) { f(c->qX[q]); f(c->qD[q]); f(c->S4[q]); c->S4[q] = nullptr; }
+    f(c->R4); c->R4 = nullptr;
     f(c->red_scratch); f(c->red_dev);
     f(c->lists.all);
     for (int q = 0; q < 3; q++) f(c->lists.col[q]);
@@ -204,8 +205,11 @@ static int dc_prepare(pm_ctx *c) {
     const pm::Grid &g = c->g;
     tb(c, TC_RESIDUAL);
     pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
-    pm::launch_residual(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R[0], c->R[1], c->R[2],
-                        c->R[3], c->stream);
+    if (c->aos())
+        pm::launch_residual4(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R4, c->stream);
+    else
+        pm::launch_residual(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R[0], c->R[1],
+                            c->R[2], c->R[3], c->stream);
     te(c);
     RC(fill_f32(c, c->qD[0], g.qtotal(), 0.f));
     RC(fill_f32(c, c->qD[1], g.qtotal(), 0.f));
@@ -234,7 +238,11 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     for (int q = 0; q < 2; q++) RC(dalloc(&c->qX[q], g.qtotal()));
     if (c->fast() && !c->dc() && c->jacobi())
         for (int q = 0; q < 4; q++) RC(dalloc(&c->X2[q], g.total));
-    if (c->dc()) {
+    if (c->aos()) {
+        for (int q = 0; q < 2; q++) RC(dalloc(&c->S4[q], g.total));
+        RC(dalloc(&c->R4, g.total));
+        for (int q = 0; q < 2; q++) RC(dalloc(&c->qD[q], g.qtotal()));
+    } else if (c->dc()) {
         for (int q = 0; q < 4; q++) RC(dalloc(&c->D[q], g.total));
         for (int q = 0; q < 4; q++) RC(dalloc(&c->R[q], g.total));
         if (c->jacobi())
@@ -308,7 +316,11 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         for (int q = 0; q < 4; q++)
             HIPCHK(hipMemcpyAsync(c->X2[q], c->X[q], sizeof(double) * (size_t)g.total, hipMemcpyDeviceToDevice,
                                   c->stream));
-    if (c->dc()) {
+    if (c->aos()) {
+        for (int q = 0; q < 2; q++) RC(fill_f32(c, (float *)c->S4[q], 4 * g.total, 0.f));
+        RC(fill_f32(c, (float *)c->R4, 4 * g.total, 0.f));
+        RC(dc_prepare(c));
+    } else if (c->dc()) {
         for (int q = 0; q < 4; q++) {
             RC(fill_f32(c, c->D[q], g.total, 0.f));
             RC(fill_f32(c, c->R[q], g.total, 0.f));
@@ -340,7 +352,11 @@ static int settle(pm_ctx *c) {
     hipStream_t s = c->stream;
     const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
     const double cu = om * pm::kDt, cp = wp * pm::kC2 * pm::kDt;
-    if (c->dc()) {
+    if (c->aos()) {
+        pm::launch_jacobi4(pm::JAC_SETTLE, g, c->lists, c->pc, c->S4[0], c->qD[0], c->S4[0], c->qD[0], c->R4,
+                           (float)cu, (float)cp, s);
+        pm::launch_ghost4(g, c->G, c->S4[0], c->qD[0], s);
+    } else if (c->dc()) {
         pm::launch_jacobi_stage<float>(pm::JAC_SETTLE, g, c->lists, c->pc, c->D[0], c->D[1], c->D[2], c->D[3],
                                        c->qD[0], c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], (float)cu,
                                        (float)cp, 0.f, c->R[0], c->R[1], c->R[2], c->R[3], s);
@@ -381,14 +397,25 @@ static void jacobi_pass(pm_ctx *c, T *const *A, T *qa, T *const *B, T *qb, T cu,
     c->lagged = true;
 }
 
+// X += d on the fluid interior, d = 0 (defect correction).
+static void fold_now(pm_ctx *c) {
+    const pm::Grid &g = c->g;
+    if (c->aos())
+        pm::launch_fold4(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->S4[0], c->stream);
+    else
+        pm::launch_fold(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->D[0], c->D[1], c->D[2], c->D[3],
+                        c->stream);
+}
+
 static void apply_variant(pm_ctx *c, const SweepVariant &v) {
     c->jac_impl = v.impl;
     c->lists.select_loads = v.select_loads;
     c->stream_zs = v.zs;
 }
 
-static const char *variant_name(const SweepVariant &v, char *buf, size_t len) {
-    if (v.impl == JI_STREAM) std::snprintf(buf, len, "dense-2.5d-zs%d", v.zs);
+static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *buf, size_t len) {
+    if (c->aos()) std::snprintf(buf, len, "interleaved-%s", v.impl == JI_PAIR ? "pair" : "lagged");
+    else if (v.impl == JI_STREAM) std::snprintf(buf, len, "dense-2.5d-zs%d", v.zs);
     else std::snprintf(buf, len, "%s-%s", v.impl == JI_PAIR ? "list-pair" : "list-lagged",
                        v.select_loads ? "select" : "branch");
     return buf;
@@ -405,9 +432,13 @@ static void tuner_setup(pm_ctx *c) {
         return;
     // the first window (caches and clocks still cold after the load) is run
     // with the last candidate and not scored
-    const SweepVariant cands[] = {{JI_STREAM, true, 16}, {JI_PAIR, true, 64}, {JI_PAIR, false, 64},
-                                  {JI_LIST, true, 64}, {JI_STREAM, true, 64}, {JI_STREAM, true, 16}};
-    for (const auto &v : cands) tn.cand[tn.n++] = v;
+    const SweepVariant soa[] = {{JI_STREAM, true, 16}, {JI_PAIR, true, 64}, {JI_PAIR, false, 64},
+                                {JI_LIST, true, 64}, {JI_STREAM, true, 64}, {JI_STREAM, true, 16}};
+    const SweepVariant aos[] = {{JI_LIST, true, 64}, {JI_PAIR, true, 64}, {JI_LIST, true, 64}};
+    if (c->aos())
+        for (const auto &v : aos) tn.cand[tn.n++] = v;
+    else
+        for (const auto &v : soa) tn.cand[tn.n++] = v;
     tn.first_scored = 1;
     tn.active = true;
 }
@@ -495,13 +526,20 @@ static int one_sweep(pm_ctx *c) {
         if (c->jacobi()) {
             RC(tuner_begin(c));
             tb(c, TC_SWEEP);
-            jacobi_pass<float>(c, c->D, c->qD[0], c->D2, c->qD[1], fcu, fcp, 0.f, c->R);
+            if (c->jac_impl == JI_PAIR) {
+                RC(settle(c));
+                pm::launch_jacobi4(pm::JAC_PAIR, g, c->lists, c->pc, c->S4[0], c->qD[0], c->S4[1], c->qD[1], c->R4,
+                                   fcu, fcp, s);
+            } else {
+                pm::launch_jacobi4(c->lagged ? pm::JAC_LAG : pm::JAC_FIRST, g, c->lists, c->pc, c->S4[0], c->qD[0],
+                                   c->S4[1], c->qD[1], c->R4, fcu, fcp, s);
+                c->lagged = true;
+            }
             te(c);
             tb(c, TC_GHOST);
-            pm::launch_ghost_fast<float>(g, c->G, c->D2[0], c->D2[1], c->D2[2], c->D2[3], c->qD[1], 0.f, 0.f, 0.f,
-                                         s);
+            pm::launch_ghost4(g, c->G, c->S4[1], c->qD[1], s);
             te(c);
-            for (int q = 0; q < 4; q++) std::swap(c->D[q], c->D2[q]);
+            std::swap(c->S4[0], c->S4[1]);
             std::swap(c->qD[0], c->qD[1]);
             RC(tuner_end(c));
         } else {
@@ -516,7 +554,7 @@ static int one_sweep(pm_ctx *c) {
         if (c->t % c->prm.it_eval == 0) {
             tb(c, TC_FOLD);
             RC(settle(c));
-            pm::launch_fold(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->D[0], c->D[1], c->D[2], c->D[3], s);
+            fold_now(c);
             pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr,
                                           (double)(g.nz + 2), 2.0, 3.0, s);
             te(c);
@@ -534,7 +572,7 @@ static int dc_flush(pm_ctx *c) {
     RC(settle(c));
     if (!c->dc()) return PM_OK;
     const pm::Grid &g = c->g;
-    pm::launch_fold(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->D[0], c->D[1], c->D[2], c->D[3], c->stream);
+    fold_now(c);
     pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr, (double)(g.nz + 2), 2.0,
                                   3.0, c->stream);
     RC(dc_prepare(c));
@@ -796,7 +834,7 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
     if (!c || !name || len <= 0) return fail(PM_E_ARG, "null argument");
     char buf[64];
     const SweepVariant cur = {c->jac_impl, c->lists.select_loads, c->stream_zs};
-    std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(cur, buf, sizeof buf) : "n/a");
+    std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(c, cur, buf, sizeof buf) : "n/a");
     if (tried && tried_len > 0) {
         std::string t;
         const Tuner &tn = c->tuner;
@@ -804,7 +842,7 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
             for (int q = tn.first_scored; q < tn.n; q++) {
                 char e[96];
                 std::snprintf(e, sizeof e, "%s%s=%.4f", q > tn.first_scored ? ";" : "",
-                              variant_name(tn.cand[q], buf, sizeof buf), tn.ms[q]);
+                              variant_name(c, tn.cand[q], buf, sizeof buf), tn.ms[q]);
                 t += e;
             }
         std::snprintf(tried, (size_t)tried_len, "%s", t.c_str());

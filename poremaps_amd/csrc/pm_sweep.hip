@@ -92,6 +92,47 @@ __global__ __launch_bounds__(kListBlock) void k_jac_lag(Grid g, const int32_t *_
                              pin_out);
 }
 
+// Interleaved FP32 correction state (defect-correction Jacobi): stage 0 =
+// first pass (velocities, p carried), 1 = lagged pass, 2 = pair velocity
+// pass, 3 = pair pressure pass / settle.
+template <int STAGE>
+__global__ __launch_bounds__(kListBlock) void k_jac4(Grid g, const int32_t *__restrict__ list, int n,
+                                                     const uint8_t *__restrict__ pc, const float4 *__restrict__ S0,
+                                                     const float *__restrict__ q0, float4 *S1, float *q1,
+                                                     const float4 *__restrict__ R, float cu, float cp) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const int64_t c = list[t];
+    if (STAGE == 0 || STAGE == 2) vel_voxel4(g, c, pc[c], S0, q0, S1, R, cu);
+    else if (STAGE == 1) lag_voxel4(g, c, pc[c], pc, S0, q0, S1, R, cu, cp, 0.f);
+    else pres_voxel4(g, c, pc[c], S0, S1, q1, R, cp, 0.f);
+}
+
+__global__ __launch_bounds__(kListBlock) void k_residual4(Grid g, const int32_t *__restrict__ list, int n,
+                                                          const uint8_t *__restrict__ pc,
+                                                          const double *__restrict__ p,
+                                                          const double *__restrict__ u,
+                                                          const double *__restrict__ v,
+                                                          const double *__restrict__ w,
+                                                          const double *__restrict__ q, float4 *__restrict__ R) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const int64_t c = list[t];
+    residual_voxel4(g, c, pc[c], p, u, v, w, q, R);
+}
+
+__global__ void k_ghost4(Grid g, const uint8_t *__restrict__ G, float4 *S, float *q) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    ghost_line4(g, t, G, S, q);
+}
+
+__global__ void k_fold4(Grid g, const uint8_t *__restrict__ pc, double *Xp, double *Xu, double *Xv, double *Xw,
+                        float4 *d) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.total) return;
+    fold_cell4(g, c, pc, Xp, Xu, Xv, Xw, d);
+}
+
 // ---------------------------------------------------------------------------
 // Dense 2.5-D sweep (first or lagged Jacobi pass, pm_voxel.h semantics).
 //
@@ -518,6 +559,38 @@ template void launch_jacobi_stage<float>(int, const Grid &, const FluidLists &, 
                                          const float *, const float *, const float *, const float *, float *,
                                          float *, float *, float *, float *, float, float, float, const float *,
                                          const float *, const float *, const float *, hipStream_t);
+
+void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
+                    const float *q0, float4 *S1, float *q1, const float4 *R, float cu, float cp, hipStream_t s) {
+    if (L.n_all == 0) return;
+    const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
+    if (stage == JAC_FIRST) {
+        hipLaunchKernelGGL(k_jac4<0>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
+    } else if (stage == JAC_LAG) {
+        hipLaunchKernelGGL(k_jac4<1>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
+    } else if (stage == JAC_PAIR) {
+        hipLaunchKernelGGL(k_jac4<2>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
+        hipLaunchKernelGGL(k_jac4<3>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
+    } else {
+        hipLaunchKernelGGL(k_jac4<3>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
+    }
+}
+
+void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
+                      const double *v, const double *w, const double *q, float4 *R, hipStream_t s) {
+    if (L.n_all == 0) return;
+    hipLaunchKernelGGL(k_residual4, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc, p, u, v,
+                       w, q, R);
+}
+
+void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStream_t s) {
+    hipLaunchKernelGGL(k_ghost4, dim3(nblocks(ghost_lines(g), 256)), dim3(256), 0, s, g, G, S, q);
+}
+
+void launch_fold4(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_fold4, dim3(nblocks(g.total, 256)), dim3(256), 0, s, g, pc, Xp, Xu, Xv, Xw, d);
+}
 
 template <typename T>
 void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const T *p0, const T *u0,

@@ -203,6 +203,247 @@ __device__ __forceinline__ void lag_voxel(const Grid &g, int64_t c, uint8_t pcv,
     p1[c] = Pc;
 }
 
+// ---------------------------------------------------------------------------
+// FP32 correction state of the defect-correction Jacobi mode, interleaved:
+// one float4 {p, u, v, w} per cell (and the FP32 residual likewise), so that a
+// neighbour costs one 16-byte load instead of up to four 4-byte gathers.  Same
+// expressions, in the same order, as the per-field functions above.
+// ---------------------------------------------------------------------------
+
+// Viscous sums at c from the interleaved state (the stencil of fast_force_s).
+__device__ __forceinline__ void visc_sums4(const Cases &cs, int64_t c, int64_t sj, int64_t si,
+                                           const float4 *__restrict__ S, const float4 &C, float &su, float &sv,
+                                           float &sw) {
+    const float4 XM = S[c - 1], XP = S[c + 1], XM2 = S[c - 2], XP2 = S[c + 2];
+    const float4 YM = S[c - sj], YP = S[c + sj], YM2 = S[c - 2 * sj], YP2 = S[c + 2 * sj];
+    const float4 ZM = S[c - si], ZP = S[c + si], ZM2 = S[c - 2 * si], ZP2 = S[c + 2 * si];
+    const float lxx = ((cs.bx ? 0.f : XM.y) - 2.0f * C.y) + (fwd_solid(cs.cx) ? 0.f : XP.y);
+    const float lyy = ((cs.by ? 0.f : YM.z) - 2.0f * C.z) + (fwd_solid(cs.cy) ? 0.f : YP.z);
+    const float lzz = ((cs.bz ? 0.f : ZM.w) - 2.0f * C.w) + (fwd_solid(cs.cz) ? 0.f : ZP.w);
+    const float dyx = d2<float>(cs.cx, XM2.z, XM.z, C.z, XP.z, XP2.z);
+    const float dzx = d2<float>(cs.cx, XM2.w, XM.w, C.w, XP.w, XP2.w);
+    const float dxy = d2<float>(cs.cy, YM2.y, YM.y, C.y, YP.y, YP2.y);
+    const float dzy = d2<float>(cs.cy, YM2.w, YM.w, C.w, YP.w, YP2.w);
+    const float dxz = d2<float>(cs.cz, ZM2.y, ZM.y, C.y, ZP.y, ZP2.y);
+    const float dyz = d2<float>(cs.cz, ZM2.z, ZM.z, C.z, ZP.z, ZP2.z);
+    su = (lxx + dxy) + dxz;
+    sv = (lyy + dyz) + dyx;
+    sw = (lzz + dzx) + dzy;
+}
+
+// Velocities from the forces (vel_voxel's tail); returns the new cell.
+__device__ __forceinline__ float4 vel_from4(const Cases &cs, const float4 &C, float su, float sv, float sw,
+                                            float dpx, float dpy, float dpz, const float4 &Rc, float p_out,
+                                            float cu) {
+    const float INV_RE = (float)(1.0 / kRe);
+    float fu = cs.bx ? 0.f : su * INV_RE - dpx;
+    float fv = cs.by ? 0.f : sv * INV_RE - dpy;
+    float fw = cs.bz ? 0.f : sw * INV_RE - dpz;
+    fu = fu + Rc.y; fv = fv + Rc.z; fw = fw + Rc.w;
+    float4 o;
+    o.x = p_out;
+    o.y = cs.bx ? 0.f : C.y + fu * cu;
+    o.z = cs.by ? 0.f : C.z + fv * cu;
+    o.w = cs.bz ? 0.f : C.w + fw * cu;
+    return o;
+}
+
+// Velocity part of the pair (vel_voxel): writes u, v, w of S1[c] and carries
+// p over when first is set (the first pass of a lagged sequence); the pair's
+// pressure pass overwrites p.
+__device__ __forceinline__ void vel_voxel4(const Grid &g, int64_t c, uint8_t pcv, const float4 *__restrict__ S0,
+                                           const float *__restrict__ q0, float4 *__restrict__ S1,
+                                           const float4 *__restrict__ R, float cu) {
+    const Cases cs = decode(pcv);
+    const bool out = on_outlet(g, c);
+    const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
+    const float4 C = S0[c];
+    const float pxm = out ? q0[qc - 1] : S0[c - 1].x;
+    const float pym = out ? q0[qc - g.sx] : S0[c - g.sx].x;
+    const float pzm = S0[c - g.sy].x;
+    const float dpx = cs.bx ? 0.f : C.x - pxm;
+    const float dpy = cs.by ? 0.f : C.x - pym;
+    const float dpz = cs.bz ? 0.f : C.x - pzm;
+    float su, sv, sw;
+    visc_sums4(cs, c, g.sx, g.sy, S0, C, su, sv, sw);
+    S1[c] = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, R[c], C.x, cu);
+}
+
+// Divergence of the interleaved velocities at fluid cell n (pres_voxel /
+// lag_div0 order), +1 neighbours through the filled halos.
+__device__ __forceinline__ float div4(const Grid &g, int64_t n, uint8_t pcn, const float4 *__restrict__ S,
+                                      const float4 &N) {
+    const Cases cs = decode(pcn);
+    const float wz = fwd_solid(cs.cz) ? 0.f : S[n + g.sy].w;
+    const float vy = fwd_solid(cs.cy) ? 0.f : S[n + g.sx].z;
+    const float ux = fwd_solid(cs.cx) ? 0.f : S[n + 1].y;
+    return ((((wz - N.w) + vy) - N.z) + ux) - N.y;
+}
+
+// Pressure part of the pair (pres_voxel), in S1 (velocities already new), or
+// in place for the settle pass: p of S1[c] and, on the outlet plane, q1.
+__device__ __forceinline__ void pres_voxel4(const Grid &g, int64_t c, uint8_t pcv, const float4 *__restrict__ S0,
+                                            float4 *S1, float *__restrict__ q1, const float4 *__restrict__ R,
+                                            float cp, float pin_out) {
+    const Cases cs = decode(pcv);
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const int64_t cxp = (k == g.nx + 1 && g.px) ? c + 1 - g.nx : c + 1;
+    const int64_t cyp = (j == g.ny + 1 && g.py) ? c + g.sx - (int64_t)g.ny * g.sx : c + g.sx;
+    const int64_t czp = (i == g.nz + 1 && g.pz) ? c + g.sy - (int64_t)g.nz * g.sy : c + g.sy;
+    const float4 N = S1[c];
+    const float wz = fwd_solid(cs.cz) ? 0.f : S1[czp].w;
+    const float vy = fwd_solid(cs.cy) ? 0.f : S1[cyp].z;
+    const float ux = fwd_solid(cs.cx) ? 0.f : S1[cxp].y;
+    const float div0 = ((((wz - N.w) + vy) - N.z) + ux) - N.y;
+    const float div = div0 + R[c].x;
+    S1[c].x = S0[c].x - div * cp;
+    if (i == g.nz + 1) q1[g.qat(j, k)] = pin_out - div0 * (float)(kC2 * kDt);
+}
+
+// Lagged pressure / outlet buffer at a neighbour n (lag_back on the
+// interleaved state).
+__device__ __forceinline__ float lag_back4(const Grid &g, int64_t c, int64_t s, int idx, int n_ax, bool per, bool out,
+                                           int64_t qn, const uint8_t *__restrict__ pc, const float4 *__restrict__ S0,
+                                           const float *__restrict__ q0, const float4 *__restrict__ R, float cp,
+                                           float pin_out) {
+    int64_t n = c - s;
+    if (idx == 2) {
+        if (!per) return out ? q0[qn] : S0[n].x;
+        n += (int64_t)n_ax * s;
+    }
+    const float4 N = S0[n];
+    const float d0 = div4(g, n, pc[n], S0, N);
+    if (out) return pin_out - d0 * (float)(kC2 * kDt);
+    return N.x - (d0 + R[n].x) * cp;
+}
+
+// One lagged pass at fluid cell c on the interleaved state (lag_voxel).
+__device__ __forceinline__ void lag_voxel4(const Grid &g, int64_t c, uint8_t pcv, const uint8_t *__restrict__ pc,
+                                           const float4 *__restrict__ S0, const float *__restrict__ q0,
+                                           float4 *__restrict__ S1, const float4 *__restrict__ R, float cu, float cp,
+                                           float pin_out) {
+    const Cases cs = decode(pcv);
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const bool out = i == g.nz + 1;
+    const float4 C = S0[c];
+    const float4 Rc = R[c];
+    const float Pc = out ? pin_out : C.x - (div4(g, c, pcv, S0, C) + Rc.x) * cp;
+    float dpx = 0.f, dpy = 0.f, dpz = 0.f;
+    if (!cs.bx)
+        dpx = Pc - lag_back4(g, c, 1, k, g.nx, g.px, out, g.qat(j, k - 1), pc, S0, q0, R, cp, pin_out);
+    if (!cs.by)
+        dpy = Pc - lag_back4(g, c, g.sx, j, g.ny, g.py, out, g.qat(j - 1, k), pc, S0, q0, R, cp, pin_out);
+    if (!cs.bz) {
+        const int64_t n = c - g.sy;
+        if (i == 2) {
+            dpz = Pc - S0[n].x;
+        } else {
+            const float4 N = S0[n];
+            dpz = Pc - (N.x - (div4(g, n, pc[n], S0, N) + R[n].x) * cp);
+        }
+    }
+    float su, sv, sw;
+    visc_sums4(cs, c, g.sx, g.sy, S0, C, su, sv, sw);
+    S1[c] = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rc, Pc, cu);
+}
+
+// Residual of the FP64 state at c into the interleaved FP32 residual.
+__device__ __forceinline__ void residual_voxel4(const Grid &g, int64_t c, uint8_t pcv, const double *__restrict__ p,
+                                                const double *__restrict__ u, const double *__restrict__ v,
+                                                const double *__restrict__ w, const double *__restrict__ q,
+                                                float4 *__restrict__ R) {
+    const Cases cs = decode(pcv);
+    const bool out = on_outlet(g, c);
+    double fu, fv, fw;
+    fast_force_p<double>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, c - (int64_t)(g.nz + 1) * g.sy, fu,
+                         fv, fw);
+    const double wz = fwd_solid(cs.cz) ? 0.0 : w[c + g.sy];
+    const double vy = fwd_solid(cs.cy) ? 0.0 : v[c + g.sx];
+    const double ux = fwd_solid(cs.cx) ? 0.0 : u[c + 1];
+    const double div = ((((wz - w[c]) + vy) - v[c]) + ux) - u[c];
+    float4 r;
+    r.x = (float)div; r.y = (float)fu; r.z = (float)fv; r.w = (float)fw;
+    R[c] = r;
+}
+
+// Ghost fill of the interleaved correction state (ghost_line with float4
+// copies; the pins of a correction state are 0).
+__device__ __forceinline__ void ghost_line4(const Grid &g, int64_t t, const uint8_t *__restrict__ G, float4 *S,
+                                            float *q) {
+    const int64_t nX = (int64_t)g.nz * g.ny, nY = (int64_t)g.nz * g.nx, nZ = (int64_t)g.ny * g.nx;
+    const int iout = g.nz + 1;
+    if (t < nX) {
+        if (!g.px) return;
+        const int i = (int)(t / g.ny) + 2, j = (int)(t % g.ny) + 2;
+        const int64_t r = g.at(i, j, 0);
+        const int src[4] = {g.nx, g.nx + 1, 2, 3};
+        const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
+        for (int e = 0; e < 4; e++) {
+            const int64_t s = r + src[e], d = r + dst[e];
+            float4 v = S[s];
+            if (i == iout && G[s] == 0) v.x = 0.f;
+            S[d] = v;
+            if (q && i == iout) q[g.qat(j, dst[e])] = q[g.qat(j, src[e])];
+        }
+    } else if (t < nX + nY) {
+        if (!g.py) return;
+        const int64_t tt = t - nX;
+        const int i = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
+        const int src[4] = {g.ny, g.ny + 1, 2, 3};
+        const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
+        for (int e = 0; e < 4; e++) {
+            const int64_t s = g.at(i, src[e], k), d = g.at(i, dst[e], k);
+            float4 v = S[s];
+            if (i == iout && G[s] == 0) v.x = 0.f;
+            S[d] = v;
+            if (q && i == iout) q[g.qat(dst[e], k)] = q[g.qat(src[e], k)];
+        }
+    } else if (t < nX + nY + nZ) {
+        const int64_t tt = t - nX - nY;
+        const int j = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
+        const int64_t cout = g.at(iout, j, k);
+        const bool out_fluid = G[cout] == 0;
+        if (g.pz) {
+            const int src[4] = {g.nz, g.nz + 1, 2, 3};
+            const int dst[4] = {0, 1, g.nz + 2, g.nz + 3};
+            for (int e = 0; e < 4; e++) {
+                const int64_t s = g.at(src[e], j, k), d = g.at(dst[e], j, k);
+                float4 v = S[s];
+                if (src[e] == iout && out_fluid) v.x = 0.f;
+                S[d] = v;
+            }
+        }
+        const int64_t c1 = g.at(1, j, k), chi = g.at(g.nz + 2, j, k);
+        if (G[c1] == 0) S[c1].x = 0.f;
+        if (G[chi] == 0) S[chi].x = 0.f;
+        if (out_fluid) S[cout].x = 0.f;
+    }
+}
+
+// Fold of the interleaved correction into the FP64 state: X += d on the
+// fluid interior, d = 0 everywhere.
+__device__ __forceinline__ void fold_cell4(const Grid &g, int64_t c, const uint8_t *__restrict__ pc, double *Xp,
+                                           double *Xu, double *Xv, double *Xw, float4 *d) {
+    const int64_t e = c - g.base;
+    bool in = false;
+    if (e >= 0) {
+        const int i = (int)(e / g.sy);
+        const int64_t r = e % g.sy;
+        const int j = (int)(r / g.sx), k = (int)(r % g.sx);
+        in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
+    }
+    if (in && pc[c] != kSolid) {
+        const float4 v = d[c];
+        Xp[c] = Xp[c] + (double)v.x;
+        Xu[c] = Xu[c] + (double)v.y;
+        Xv[c] = Xv[c] + (double)v.z;
+        Xw[c] = Xw[c] + (double)v.w;
+    }
+    d[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // 3-colour SOR: one cell of the current colour, in place.
 template <typename T, bool HAS_R, bool SEL>
 __device__ __forceinline__ void colour_voxel(const Grid &g, int64_t c, uint8_t pcv, T *p, T *u, T *v, T *w,
