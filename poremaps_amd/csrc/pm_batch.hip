@@ -40,7 +40,8 @@ struct BSample {
     double *X[4];
     double *qX;
     // defect correction: interleaved correction sets and residual
-    float4 *A4, *B4, *R4;
+    float4 *A4, *B4;
+    Res4 R4;
     int chunk_begin, chunk_end;
 };
 
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, co
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
     const int64_t c = s.list[t];
-    vel_voxel4(s.g, c, s.pc[c], parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA), parity ? s.A4 : s.B4,
+    vel_voxel4(s.g, c, t, s.pc[c], parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA), parity ? s.A4 : s.B4,
                s.R4, cu);
 }
 
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(kB) void k_b_residual4(const BSample *__restrict__ 
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
     const int64_t c = s.list[t];
-    residual_voxel4(s.g, c, s.pc[c], s.X[0], s.X[1], s.X[2], s.X[3], s.qX, s.R4);
+    residual_voxel4(s.g, c, t, s.pc[c], s.X[0], s.X[1], s.X[2], s.X[3], s.qX, s.R4);
 }
 
 // which: 0 = set A, 1 = set B (the swept state, correction pins 0 in the

@@ -61,7 +61,7 @@ struct pm_ctx {
     // defect-correction Jacobi: the correction state and residual interleaved
     // ({p, u, v, w} per cell), two sets for the out-of-place sweeps
     float4 *S4[2] = {nullptr, nullptr};
-    float4 *R4 = nullptr;
+    pm::Res4 R4 = {nullptr, nullptr};
     // reductions
     pm::FluidLists lists;   // fluid cells of the interior (fast modes)
     void *red_scratch = nullptr;

@@ -36,6 +36,16 @@ struct Grid {
     int64_t qtotal() const { return base + sy; }
 };
 
+// FP32 residual of the interleaved defect-correction state: the continuity
+// part Rp per cell (it is read at the cell and at its -1 neighbours) and the
+// momentum part (Ru, Rv, Rw, 0) per fluid voxel in fluid-list order (read only
+// by the voxel's own thread), which keeps the residual traffic of a sweep at
+// fluid size.
+struct Res4 {
+    float *p;
+    float4 *m;
+};
+
 inline Grid make_grid(int nx, int ny, int nz, int bc) {
     Grid g;
     g.nx = nx; g.ny = ny; g.nz = nz;

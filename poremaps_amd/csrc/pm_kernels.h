@@ -66,9 +66,9 @@ enum { JAC_FIRST = 0, JAC_LAG = 1, JAC_SETTLE = 2, JAC_PAIR = 3 };
 // (velocity + pressure kernels, S1 complete) from S0 into S1; JAC_SETTLE
 // completes S0 in place (pass S1 = S0, q1 = its outlet buffer).
 void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
-                    const float *q0, float4 *S1, float *q1, const float4 *R, float cu, float cp, hipStream_t s);
+                    const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s);
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
-                      const double *v, const double *w, const double *q, float4 *R, hipStream_t s);
+                      const double *v, const double *w, const double *q, Res4 R, hipStream_t s);
 void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStream_t s);
 void launch_fold4(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d,
                   hipStream_t s);

@@ -107,7 +107,7 @@ static void free_all(pm_ctx *c) {
     f(c->G); f(c->pc); f(c->code);
     for (int q = 0; q < 4; q++) { f(c->X[q]); f(c->X2[q]); f(c->D[q]); f(c->D2[q]); f(c->R[q]); }
     for (int q = 0; q < 2; q++) { f(c->qX[q]); f(c->qD[q]); f(c->S4[q]); c->S4[q] = nullptr; }
-    f(c->R4); c->R4 = nullptr;
+    f(c->R4.p); f(c->R4.m); c->R4 = pm::Res4{nullptr, nullptr};
     f(c->red_scratch); f(c->red_dev);
     f(c->lists.all);
     for (int q = 0; q < 3; q++) f(c->lists.col[q]);
@@ -240,7 +240,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         for (int q = 0; q < 4; q++) RC(dalloc(&c->X2[q], g.total));
     if (c->aos()) {
         for (int q = 0; q < 2; q++) RC(dalloc(&c->S4[q], g.total));
-        RC(dalloc(&c->R4, g.total));
+        RC(dalloc(&c->R4.p, g.total));  // the momentum part follows the fluid list
         for (int q = 0; q < 2; q++) RC(dalloc(&c->qD[q], g.qtotal()));
     } else if (c->dc()) {
         for (int q = 0; q < 4; q++) RC(dalloc(&c->D[q], g.total));
@@ -280,6 +280,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         c->lists.select_loads = !(ld && std::strcmp(ld, "branch") == 0);
         c->lists.n_all = (int)all.size();
         RC(dalloc(&c->lists.all, std::max<int64_t>(1, c->lists.n_all)));
+        if (c->aos()) RC(dalloc(&c->R4.m, std::max<int64_t>(1, c->lists.n_all)));
         HIPCHK(hipMemcpy(c->lists.all, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
         if (c->prm.mode == PM_MODE_SOR3)
             for (int q = 0; q < 3; q++) {
@@ -318,7 +319,8 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
                                   c->stream));
     if (c->aos()) {
         for (int q = 0; q < 2; q++) RC(fill_f32(c, (float *)c->S4[q], 4 * g.total, 0.f));
-        RC(fill_f32(c, (float *)c->R4, 4 * g.total, 0.f));
+        RC(fill_f32(c, c->R4.p, g.total, 0.f));
+        RC(fill_f32(c, (float *)c->R4.m, 4 * std::max<int64_t>(1, c->lists.n_all), 0.f));
         RC(dc_prepare(c));
     } else if (c->dc()) {
         for (int q = 0; q < 4; q++) {

@@ -99,12 +99,12 @@ template <int STAGE>
 __global__ __launch_bounds__(kListBlock) void k_jac4(Grid g, const int32_t *__restrict__ list, int n,
                                                      const uint8_t *__restrict__ pc, const float4 *__restrict__ S0,
                                                      const float *__restrict__ q0, float4 *S1, float *q1,
-                                                     const float4 *__restrict__ R, float cu, float cp) {
+                                                     Res4 R, float cu, float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     const int64_t c = list[t];
-    if (STAGE == 0 || STAGE == 2) vel_voxel4(g, c, pc[c], S0, q0, S1, R, cu);
-    else if (STAGE == 1) lag_voxel4(g, c, pc[c], pc, S0, q0, S1, R, cu, cp, 0.f);
+    if (STAGE == 0 || STAGE == 2) vel_voxel4(g, c, t, pc[c], S0, q0, S1, R, cu);
+    else if (STAGE == 1) lag_voxel4(g, c, t, pc[c], pc, S0, q0, S1, R, cu, cp, 0.f);
     else pres_voxel4(g, c, pc[c], S0, S1, q1, R, cp, 0.f);
 }
 
@@ -114,11 +114,11 @@ __global__ __launch_bounds__(kListBlock) void k_residual4(Grid g, const int32_t 
                                                           const double *__restrict__ u,
                                                           const double *__restrict__ v,
                                                           const double *__restrict__ w,
-                                                          const double *__restrict__ q, float4 *__restrict__ R) {
+                                                          const double *__restrict__ q, Res4 R) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     const int64_t c = list[t];
-    residual_voxel4(g, c, pc[c], p, u, v, w, q, R);
+    residual_voxel4(g, c, t, pc[c], p, u, v, w, q, R);
 }
 
 __global__ void k_ghost4(Grid g, const uint8_t *__restrict__ G, float4 *S, float *q) {
@@ -561,7 +561,7 @@ template void launch_jacobi_stage<float>(int, const Grid &, const FluidLists &, 
                                          const float *, const float *, const float *, hipStream_t);
 
 void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
-                    const float *q0, float4 *S1, float *q1, const float4 *R, float cu, float cp, hipStream_t s) {
+                    const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s) {
     if (L.n_all == 0) return;
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
     if (stage == JAC_FIRST) {
@@ -577,7 +577,7 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
 }
 
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
-                      const double *v, const double *w, const double *q, float4 *R, hipStream_t s) {
+                      const double *v, const double *w, const double *q, Res4 R, hipStream_t s) {
     if (L.n_all == 0) return;
     hipLaunchKernelGGL(k_residual4, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc, p, u, v,
                        w, q, R);

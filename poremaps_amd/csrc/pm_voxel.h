@@ -233,13 +233,13 @@ __device__ __forceinline__ void visc_sums4(const Cases &cs, int64_t c, int64_t s
 
 // Velocities from the forces (vel_voxel's tail); returns the new cell.
 __device__ __forceinline__ float4 vel_from4(const Cases &cs, const float4 &C, float su, float sv, float sw,
-                                            float dpx, float dpy, float dpz, const float4 &Rc, float p_out,
+                                            float dpx, float dpy, float dpz, const float4 &Rm, float p_out,
                                             float cu) {
     const float INV_RE = (float)(1.0 / kRe);
     float fu = cs.bx ? 0.f : su * INV_RE - dpx;
     float fv = cs.by ? 0.f : sv * INV_RE - dpy;
     float fw = cs.bz ? 0.f : sw * INV_RE - dpz;
-    fu = fu + Rc.y; fv = fv + Rc.z; fw = fw + Rc.w;
+    fu = fu + Rm.x; fv = fv + Rm.y; fw = fw + Rm.z;
     float4 o;
     o.x = p_out;
     o.y = cs.bx ? 0.f : C.y + fu * cu;
@@ -251,9 +251,9 @@ __device__ __forceinline__ float4 vel_from4(const Cases &cs, const float4 &C, fl
 // Velocity part of the pair (vel_voxel): writes u, v, w of S1[c] and carries
 // p over when first is set (the first pass of a lagged sequence); the pair's
 // pressure pass overwrites p.
-__device__ __forceinline__ void vel_voxel4(const Grid &g, int64_t c, uint8_t pcv, const float4 *__restrict__ S0,
-                                           const float *__restrict__ q0, float4 *__restrict__ S1,
-                                           const float4 *__restrict__ R, float cu) {
+__device__ __forceinline__ void vel_voxel4(const Grid &g, int64_t c, int t, uint8_t pcv,
+                                           const float4 *__restrict__ S0, const float *__restrict__ q0,
+                                           float4 *__restrict__ S1, Res4 R, float cu) {
     const Cases cs = decode(pcv);
     const bool out = on_outlet(g, c);
     const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
@@ -266,7 +266,7 @@ __device__ __forceinline__ void vel_voxel4(const Grid &g, int64_t c, uint8_t pcv
     const float dpz = cs.bz ? 0.f : C.x - pzm;
     float su, sv, sw;
     visc_sums4(cs, c, g.sx, g.sy, S0, C, su, sv, sw);
-    S1[c] = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, R[c], C.x, cu);
+    S1[c] = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, R.m[t], C.x, cu);
 }
 
 // Divergence of the interleaved velocities at fluid cell n (pres_voxel /
@@ -283,8 +283,7 @@ __device__ __forceinline__ float div4(const Grid &g, int64_t n, uint8_t pcn, con
 // Pressure part of the pair (pres_voxel), in S1 (velocities already new), or
 // in place for the settle pass: p of S1[c] and, on the outlet plane, q1.
 __device__ __forceinline__ void pres_voxel4(const Grid &g, int64_t c, uint8_t pcv, const float4 *__restrict__ S0,
-                                            float4 *S1, float *__restrict__ q1, const float4 *__restrict__ R,
-                                            float cp, float pin_out) {
+                                            float4 *S1, float *__restrict__ q1, Res4 R, float cp, float pin_out) {
     const Cases cs = decode(pcv);
     int i, j, k;
     cell_ijk(g, c, i, j, k);
@@ -296,7 +295,7 @@ __device__ __forceinline__ void pres_voxel4(const Grid &g, int64_t c, uint8_t pc
     const float vy = fwd_solid(cs.cy) ? 0.f : S1[cyp].z;
     const float ux = fwd_solid(cs.cx) ? 0.f : S1[cxp].y;
     const float div0 = ((((wz - N.w) + vy) - N.z) + ux) - N.y;
-    const float div = div0 + R[c].x;
+    const float div = div0 + R.p[c];
     S1[c].x = S0[c].x - div * cp;
     if (i == g.nz + 1) q1[g.qat(j, k)] = pin_out - div0 * (float)(kC2 * kDt);
 }
@@ -305,8 +304,7 @@ __device__ __forceinline__ void pres_voxel4(const Grid &g, int64_t c, uint8_t pc
 // interleaved state).
 __device__ __forceinline__ float lag_back4(const Grid &g, int64_t c, int64_t s, int idx, int n_ax, bool per, bool out,
                                            int64_t qn, const uint8_t *__restrict__ pc, const float4 *__restrict__ S0,
-                                           const float *__restrict__ q0, const float4 *__restrict__ R, float cp,
-                                           float pin_out) {
+                                           const float *__restrict__ q0, Res4 R, float cp, float pin_out) {
     int64_t n = c - s;
     if (idx == 2) {
         if (!per) return out ? q0[qn] : S0[n].x;
@@ -315,21 +313,20 @@ __device__ __forceinline__ float lag_back4(const Grid &g, int64_t c, int64_t s, 
     const float4 N = S0[n];
     const float d0 = div4(g, n, pc[n], S0, N);
     if (out) return pin_out - d0 * (float)(kC2 * kDt);
-    return N.x - (d0 + R[n].x) * cp;
+    return N.x - (d0 + R.p[n]) * cp;
 }
 
 // One lagged pass at fluid cell c on the interleaved state (lag_voxel).
-__device__ __forceinline__ void lag_voxel4(const Grid &g, int64_t c, uint8_t pcv, const uint8_t *__restrict__ pc,
-                                           const float4 *__restrict__ S0, const float *__restrict__ q0,
-                                           float4 *__restrict__ S1, const float4 *__restrict__ R, float cu, float cp,
-                                           float pin_out) {
+__device__ __forceinline__ void lag_voxel4(const Grid &g, int64_t c, int t, uint8_t pcv,
+                                           const uint8_t *__restrict__ pc, const float4 *__restrict__ S0,
+                                           const float *__restrict__ q0, float4 *__restrict__ S1, Res4 R, float cu,
+                                           float cp, float pin_out) {
     const Cases cs = decode(pcv);
     int i, j, k;
     cell_ijk(g, c, i, j, k);
     const bool out = i == g.nz + 1;
     const float4 C = S0[c];
-    const float4 Rc = R[c];
-    const float Pc = out ? pin_out : C.x - (div4(g, c, pcv, S0, C) + Rc.x) * cp;
+    const float Pc = out ? pin_out : C.x - (div4(g, c, pcv, S0, C) + R.p[c]) * cp;
     float dpx = 0.f, dpy = 0.f, dpz = 0.f;
     if (!cs.bx)
         dpx = Pc - lag_back4(g, c, 1, k, g.nx, g.px, out, g.qat(j, k - 1), pc, S0, q0, R, cp, pin_out);
@@ -341,19 +338,19 @@ __device__ __forceinline__ void lag_voxel4(const Grid &g, int64_t c, uint8_t pcv
             dpz = Pc - S0[n].x;
         } else {
             const float4 N = S0[n];
-            dpz = Pc - (N.x - (div4(g, n, pc[n], S0, N) + R[n].x) * cp);
+            dpz = Pc - (N.x - (div4(g, n, pc[n], S0, N) + R.p[n]) * cp);
         }
     }
     float su, sv, sw;
     visc_sums4(cs, c, g.sx, g.sy, S0, C, su, sv, sw);
-    S1[c] = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rc, Pc, cu);
+    S1[c] = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, R.m[t], Pc, cu);
 }
 
 // Residual of the FP64 state at c into the interleaved FP32 residual.
-__device__ __forceinline__ void residual_voxel4(const Grid &g, int64_t c, uint8_t pcv, const double *__restrict__ p,
-                                                const double *__restrict__ u, const double *__restrict__ v,
-                                                const double *__restrict__ w, const double *__restrict__ q,
-                                                float4 *__restrict__ R) {
+__device__ __forceinline__ void residual_voxel4(const Grid &g, int64_t c, int t, uint8_t pcv,
+                                                const double *__restrict__ p, const double *__restrict__ u,
+                                                const double *__restrict__ v, const double *__restrict__ w,
+                                                const double *__restrict__ q, Res4 R) {
     const Cases cs = decode(pcv);
     const bool out = on_outlet(g, c);
     double fu, fv, fw;
@@ -363,9 +360,8 @@ __device__ __forceinline__ void residual_voxel4(const Grid &g, int64_t c, uint8_
     const double vy = fwd_solid(cs.cy) ? 0.0 : v[c + g.sx];
     const double ux = fwd_solid(cs.cx) ? 0.0 : u[c + 1];
     const double div = ((((wz - w[c]) + vy) - v[c]) + ux) - u[c];
-    float4 r;
-    r.x = (float)div; r.y = (float)fu; r.z = (float)fv; r.w = (float)fw;
-    R[c] = r;
+    R.p[c] = (float)div;
+    R.m[t] = make_float4((float)fu, (float)fv, (float)fw, 0.f);
 }
 
 // Ghost fill of the interleaved correction state (ghost_line with float4
