@@ -320,7 +320,15 @@ def main():
                           (f"{a.precision}_{a.mode}_w{a.omega}_wp{a.omega_p}",
                            dict(mode=a.mode, precision=a.precision, omega=a.omega, omega_p=a.omega_p))):
             s = pm.Solver(g, **dict(kw, **mk))
-            res, _ = s.solve()
+            last = [time.time()]
+
+            def progress(it, conv, secs, label=label):
+                if time.time() - last[0] > 20:
+                    last[0] = time.time()
+                    print(f"[bench] tte {label}: sweep {it}, conv {conv:.3e}, {secs:.0f}s", file=sys.stderr,
+                          flush=True)
+
+            res, _ = s.solve(progress=progress)
             s.close()
             tte[label] = {"seconds": round(res.seconds, 3), "sweeps": res.sweeps,
                           "wk33": res.last.wk33, "glups": round(res.glups, 2)}

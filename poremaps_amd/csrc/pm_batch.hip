@@ -83,8 +83,8 @@ __global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, co
                       fld<T>(dst, 0), q1, DC ? s.R[0] : nullptr, cp, DC ? (T)0 : (T)3.0);
 }
 
-// Interleaved correction state (defect correction): velocity and pressure
-// passes of the pair, ghost fill, fold (over the fluid list: X += d, d = 0;
+// Interleaved correction state (defect correction): the first (velocity)
+// pass of an interval, ghost fill, fold (over the fluid list: X += d, d = 0;
 // the halo copies of d are refilled with zeros by the following ghost fill)
 // and residual.
 __global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
@@ -98,15 +98,28 @@ __global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, co
                s.R4, cu);
 }
 
-__global__ __launch_bounds__(kB) void k_b_pres4(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
-                                                float cp) {
+// Lagged single pass (the sweeps after the first of an interval) and the
+// settle pass that completes the pressure at the interval's end, in place.
+__global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
+                                               float cu, float cp) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
     const int64_t c = s.list[t];
-    pres_voxel4(s.g, c, s.pc[c], parity ? s.B4 : s.A4, parity ? s.A4 : s.B4, (float *)(parity ? s.qA : s.qB), s.R4,
-                cp, 0.f);
+    lag_voxel4(s.g, c, t, s.pc[c], s.pc, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
+               parity ? s.A4 : s.B4, s.R4, cu, cp, 0.f);
+}
+
+__global__ __launch_bounds__(kB) void k_b_settle4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                                  int which, float cp) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    float4 *set = which ? s.B4 : s.A4;
+    pres_voxel4(s.g, c, s.pc[c], set, set, (float *)(which ? s.qB : s.qA), s.R4, cp, 0.f);
 }
 
 __global__ __launch_bounds__(kB) void k_b_ghost4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
@@ -436,13 +449,17 @@ static void launch_interval(pm_batch *b) {
     const dim3 blk(kB);
     if (b->dc()) {
         const float cu = (float)(om * kDt), cp = (float)(wp * kC2 * kDt);
+        // symplectic Jacobi in lagged form: the first sweep from the settled
+        // state, then one lagged pass per sweep, then the settle pass
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
-            hipLaunchKernelGGL(k_b_vel4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
-            hipLaunchKernelGGL(k_b_pres4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cp);
+            if (s == 0) hipLaunchKernelGGL(k_b_vel4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
+            else hipLaunchKernelGGL(k_b_lag4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu, cp);
             hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
         }
         const int cur = n & 1;
+        hipLaunchKernelGGL(k_b_settle4, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur, cp);
+        hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, cur);
         hipLaunchKernelGGL(k_b_fold4, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur);
         hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, cur);
         hipLaunchKernelGGL((k_b_ghost<double, true>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 2);
