@@ -248,6 +248,8 @@ def main():
     ap.add_argument("--omega-p", type=float, default=6.0, help="pressure relaxation of the fast modes")
     ap.add_argument("--it-eval", type=int, default=100)
     ap.add_argument("--tte", action="store_true")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="with --tte: FP64 at the same relaxation and an eps 1e-9 reference solve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--count", type=int, default=None, help="media in the batch / serving workloads")
     ap.add_argument("--rate", type=float, default=4.0, help="serving arrival rate, requests/s")
@@ -314,11 +316,19 @@ def main():
                          "GBps": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k2, v in kt.items()},
         "sweep_kernel": {"form": choice, "autotuner_ms_per_sweep": tuned},
     }
-    if a.tte:
+    if a.tte or a.accuracy:
         tte = {}
-        for label, mk in (("fp64_plain_jacobi", dict(mode="jacobi", precision="f64", omega=1.0, omega_p=0.0)),
-                          (f"{a.precision}_{a.mode}_w{a.omega}_wp{a.omega_p}",
-                           dict(mode=a.mode, precision=a.precision, omega=a.omega, omega_p=a.omega_p))):
+        runs = [("fp64_plain_jacobi", dict(mode="jacobi", precision="f64", omega=1.0, omega_p=0.0)),
+                (f"{a.precision}_{a.mode}_w{a.omega}_wp{a.omega_p}",
+                 dict(mode=a.mode, precision=a.precision, omega=a.omega, omega_p=a.omega_p))]
+        if a.accuracy:
+            # the FP64 solver with the same relaxation (the oracle's semantics,
+            # bit-exact on small cases) and a tightly converged reference
+            runs += [(f"fp64_{a.mode}_w{a.omega}_wp{a.omega_p}",
+                      dict(mode=a.mode, precision="f64", omega=a.omega, omega_p=a.omega_p)),
+                     ("reference_eps1e-9", dict(mode=a.mode, precision=a.precision, omega=a.omega,
+                                                omega_p=a.omega_p, eps=1e-9))]
+        for label, mk in runs:
             s = pm.Solver(g, **dict(kw, **mk))
             last = [time.time()]
 
@@ -333,6 +343,10 @@ def main():
             tte[label] = {"seconds": round(res.seconds, 3), "sweeps": res.sweeps,
                           "wk33": res.last.wk33, "glups": round(res.glups, 2)}
             print(f"[bench] tte {label}: {tte[label]}", file=sys.stderr, flush=True)
+        if a.accuracy:
+            ref = tte["reference_eps1e-9"]["wk33"]
+            for v in tte.values():
+                v["wk33_rel_err_vs_eps1e-9"] = (v["wk33"] - ref) / ref
         line["time_to_eps"] = tte
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
