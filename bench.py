@@ -20,7 +20,9 @@ Workloads (synthetic, generated from a seed):
 roofline: the dominant kernel group ("sweep") timed with HIP events on the
 library's own stream over the timed region; achieved = algorithmic bytes per
 launch (8*S+1 B per voxel for the state, + 16 B of FP32 residual in the
-defect-correction mode) / average launch duration; peak = 8 TB/s HBM.
+defect-correction mode) / average launch duration; peak = 8 TB/s HBM;
+traffic = the measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+calibrated) from the committed counter summary when the run matches it.
 cpu_baseline: the CPU oracle's reference-order sweep (FP64, one thread) timed
 on this host on a bounded sample of the same field.
 --tte: time-to-eps (eps 1e-6) of the plain FP64 Jacobi solver and of the
@@ -31,6 +33,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -110,6 +113,20 @@ def time_kernels(s, n):
         raise RuntimeError(L.pm_last_error().decode())
     return {out[i].name.decode(): dict(launches=out[i].launches, avg_ms=out[i].avg_ms,
                                        bytes=out[i].bytes_per_launch) for i in range(cnt.value)}
+
+
+def measured_traffic(spec, form, precision):
+    """HBM bytes per launch of the sweep kernel, from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a kernel of known
+    traffic), when the run uses the workload and kernel form it was measured
+    on; else None."""
+    if spec["name"] != "grf_512" or form != "interleaved-lagged" or precision != "f32dc":
+        return None
+    path = os.path.join(ROOT, "profiles", "jacobi_f32dc_grf512_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch k_jac4<1>: (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
 
 
 def _solve_alone(raw, bc, dx, **kw):
@@ -309,7 +326,8 @@ def main():
                    "mode": a.mode, "precision": a.precision, "omega": a.omega, "omega_p": a.omega_p,
                    "sweeps_per_step": a.it_eval, "porosity": float((g == 0).mean())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": measured_traffic(spec, choice, a.precision),
                      "kernel": "sweep", "avg_ms": round(sw["avg_ms"], 4),
                      "bytes_per_launch": sw["bytes"]},
         "kernels": {k2: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"],
