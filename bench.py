@@ -219,8 +219,8 @@ def bench_serve(a):
         print(f"[bench] t={t:.1f}s completed {done}/{len(reqs)}, {running} solves running", file=sys.stderr,
               flush=True)
 
-    resp, st = pm.serve(reqs, realtime=True, progress=progress, precision=a.precision, omega=a.omega, omega_p=a.omega_p,
-                        it_eval=a.it_eval, max_iter=a.max_iter)
+    resp, st = pm.serve(reqs, realtime=True, progress=progress, max_solves=a.max_solves, precision=a.precision,
+                        omega=a.omega, omega_p=a.omega_p, it_eval=a.it_eval, max_iter=a.max_iter)
     print(f"[bench] served: {json.dumps(st.as_dict())}", file=sys.stderr, flush=True)
     line = {
         "metric": METRIC, "value": round(st.solves_per_hour, 1), "unit": "solves/h", "n_gpus": 1,
@@ -233,7 +233,8 @@ def bench_serve(a):
                    "it_eval": a.it_eval},
         "latency_p50_s": round(st.latency_p50_s, 3), "latency_p99_s": round(st.latency_p99_s, 3),
         "latency_max_s": round(st.latency_max_s, 3), "wall_s": round(st.wall_s, 3),
-        "completed": st.completed, "max_running_solves": st.max_running,
+        "completed": st.completed, "max_running_solves": st.max_running, "solve_cap": a.max_solves,
+        "offered_solves_per_hour": round(3 * a.rate * 3600.0, 1),
         "peak_device_GB": round(st.peak_bytes / 1e9, 2),
     }
     if a.compare:
@@ -272,6 +273,7 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0, help="serving arrival rate, requests/s")
     ap.add_argument("--compare", action="store_true", help="also time the one-at-a-time baselines")
     ap.add_argument("--baseline-requests", type=int, default=0)
+    ap.add_argument("--max-solves", type=int, default=0, help="serving: most solves on the GPU at once")
     ap.add_argument("--max-iter", type=int, default=400_000, help="sweep cap per solve (batch, serving)")
     a = ap.parse_args()
     if a.gpus != 1:

@@ -60,6 +60,9 @@ typedef struct {
     int realtime;            /* 1: requests become visible at arrival_s; 0: all at once */
     pm_serve_progress_fn progress;  /* may be NULL */
     void *user;
+    int max_solves;          /* most solves on the GPU at once (0: limited by memory only);
+                                a cap trades throughput under overload for shorter waits of
+                                the requests already admitted */
 } pm_serve_params;
 
 typedef struct {

@@ -28,7 +28,7 @@ PRECISIONS = {"f64": 0, "f32dc": 1}
 FIELDS = {"velx": 0, "vely": 1, "velz": 2, "press": 3, "nhood": 4}
 # Default relaxation of the batch and serving paths (velocities, pressure):
 # the symplectic Jacobi sweep is stable while OMEGA * OMEGA_P stays below
-# about 20 (wave CFL of the pressure-velocity coupling), and the slow Darcy
+# about 18 (wave CFL of the pressure-velocity coupling), and the slow Darcy
 # pressure mode decays in proportion to OMEGA_P.
 OMEGA, OMEGA_P = 2.5, 6.0
 
@@ -360,7 +360,7 @@ SERVE_PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_
 
 class ServeParams(C.Structure):
     _fields_ = [("batch", BatchParams), ("realtime", C.c_int), ("progress", SERVE_PROGRESS_FN),
-                ("user", C.c_void_p)]
+                ("user", C.c_void_p), ("max_solves", C.c_int)]
 
 
 class ServeStats(C.Structure):
@@ -410,7 +410,7 @@ def tensor(raw: np.ndarray, *, boundary_method=0, voxel_size=1.0, porosity=-1.0,
     return np.array(k[:]).reshape(3, 3), list(res)
 
 
-def serve(requests, *, realtime=True, progress=None, **batch_kw):
+def serve(requests, *, realtime=True, progress=None, max_solves=0, **batch_kw):
     """Serve permeability-tensor requests: `requests` is a sequence of
     (raw, boundary_method, voxel_size, eps, arrival_s) (eps <= 0: the batch's
     eps).  ``progress(completed, running, t_s)`` is called as requests
@@ -425,7 +425,7 @@ def serve(requests, *, realtime=True, progress=None, **batch_kw):
     resp = (Response * max(n, 1))()
     st = ServeStats()
     cb = SERVE_PROGRESS_FN(lambda d, r, t, _u: int(bool(progress(d, r, t)))) if progress else SERVE_PROGRESS_FN(0)
-    sp = ServeParams(_batch_params(**batch_kw), int(realtime), cb, None)
+    sp = ServeParams(_batch_params(**batch_kw), int(realtime), cb, None, int(max_solves))
     _check(_serve_lib().pm_serve(C.byref(sp), req, n, resp, C.byref(st)))
     return list(resp)[:n], st
 

@@ -121,6 +121,7 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
                 need += pm_batch_sample_bytes(&p->batch, d[0], d[1], d[2]);
             }
             if (used + need > budget && running > 0) break;  // back-pressure
+            if (p->max_solves > 0 && running > 0 && running + 3 > p->max_solves) break;
             const int id = order[next];
             resp[id].admit_s = now();
             for (int r = 0; r < 3 && rc == PM_OK; r++) {
