@@ -367,7 +367,7 @@ class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("completed", C.c_int), ("latency_p50_s", C.c_double),
                 ("latency_p99_s", C.c_double), ("latency_mean_s", C.c_double), ("latency_max_s", C.c_double),
                 ("solves_per_hour", C.c_double), ("max_running", C.c_int), ("peak_bytes", C.c_int64),
-                ("intervals", C.c_int)]
+                ("intervals", C.c_int), ("admit_s", C.c_double), ("step_s", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -106,6 +106,7 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
     double last_report = 0.0;
     int reported = 0;
     int64_t peak = 0;
+    double admit_time = 0.0, step_time = 0.0;
     std::vector<uint8_t> rot;
     while (rc == PM_OK && done < n) {
         // admission: first come, first served, while the three solves fit
@@ -124,6 +125,7 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
             if (p->max_solves > 0 && running > 0 && running + 3 > p->max_solves) break;
             const int id = order[next];
             resp[id].admit_s = now();
+            const double t_admit = now();
             for (int r = 0; r < 3 && rc == PM_OK; r++) {
                 int d[3], slot = -1;
                 rotate(q.raw, q.nx, q.ny, q.nz, r, rot, d);
@@ -136,6 +138,7 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
                 running++;
             }
             next++;
+            admit_time += now() - t_admit;
             int64_t u2;
             pm_batch_memory(b, &u2, nullptr);
             peak = std::max(peak, u2);
@@ -150,7 +153,9 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
             continue;
         }
         int still = 0;
+        const double t_step = now();
         rc = pm_batch_step(b, &still);
+        step_time += now() - t_step;
         intervals++;
         if (rc != PM_OK) break;
         bool progressed = still > 0;
@@ -211,6 +216,8 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
         stats->max_running = max_running;
         stats->peak_bytes = peak;
         stats->intervals = intervals;
+        stats->admit_s = admit_time;
+        stats->step_s = step_time;
     }
     return rc;
 }
