@@ -263,7 +263,7 @@ def main():
     ap.add_argument("--mode", default="jacobi")
     ap.add_argument("--precision", default="f32dc")
     ap.add_argument("--omega", type=float, default=2.5, help="velocity relaxation of the fast modes")
-    ap.add_argument("--omega-p", type=float, default=6.0, help="pressure relaxation of the fast modes")
+    ap.add_argument("--omega-p", type=float, default=7.0, help="pressure relaxation of the fast modes")
     ap.add_argument("--it-eval", type=int, default=100)
     ap.add_argument("--tte", action="store_true")
     ap.add_argument("--accuracy", action="store_true",

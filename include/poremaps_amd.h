@@ -68,7 +68,8 @@ typedef struct {
     int device;              /* HIP device ordinal */
     double omega_p;          /* pressure relaxation of the fast modes; 0: omega.  The fixed
                                 point does not depend on it; the Jacobi sweep is stable while
-                                omega * omega_p stays below about 20 */
+                                omega * omega_p stays below about 19 (a divergent solve
+                                restarts with omega_p cut by a quarter: pm_result.restarts) */
 } pm_params;
 
 /* One log row (src/output.cc:193-230 column order). */
@@ -86,6 +87,9 @@ typedef struct {
     double glups;            /* nx*ny*nz*sweeps / seconds / 1e9 */
     int n_log_rows;
     pm_log_row last;         /* permeability at the last sweep */
+    int restarts;            /* fast modes: restarts from the initial state with omega_p
+                                reduced by a quarter after a divergence (sweeps counts the
+                                last attempt only) */
 } pm_result;
 
 /* Called at every log event (t > 1000, t % it_eval == 0, t % it_write == 0);

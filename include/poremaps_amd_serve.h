@@ -73,6 +73,8 @@ typedef struct {
     int max_running;         /* most solves sharing the GPU at once */
     int64_t peak_bytes;      /* most device memory held by admitted solves */
     int intervals;           /* batch steps */
+    double admit_s;          /* host time spent admitting requests (rotation, upload, first sweep) */
+    double step_s;           /* host time spent in batch steps (GPU intervals) */
 } pm_serve_stats;
 
 int pm_serve(const pm_serve_params *p, const pm_request *req, int n, pm_response *resp, pm_serve_stats *stats);

@@ -30,7 +30,7 @@ FIELDS = {"velx": 0, "vely": 1, "velz": 2, "press": 3, "nhood": 4}
 # the symplectic Jacobi sweep is stable while OMEGA * OMEGA_P stays below
 # about 18 (wave CFL of the pressure-velocity coupling), and the slow Darcy
 # pressure mode decays in proportion to OMEGA_P.
-OMEGA, OMEGA_P = 2.5, 6.0
+OMEGA, OMEGA_P = 2.5, 7.0
 
 
 class Params(C.Structure):
@@ -57,7 +57,7 @@ class LogRow(C.Structure):
 class Result(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("conv", C.c_double), ("porosity", C.c_double),
                 ("seconds", C.c_double), ("glups", C.c_double), ("n_log_rows", C.c_int),
-                ("last", LogRow)]
+                ("last", LogRow), ("restarts", C.c_int)]
 
 
 class InpExtra(C.Structure):

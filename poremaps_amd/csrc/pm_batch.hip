@@ -42,6 +42,8 @@ struct BSample {
     // defect correction: interleaved correction sets and residual
     float4 *A4, *B4;
     Res4 R4;
+    // relaxation of this sample: omega*dt, omega_p*c2*dt
+    double cu, cp;
     int chunk_begin, chunk_end;
 };
 
@@ -53,8 +55,8 @@ template <typename T>
 __device__ __forceinline__ T *fld(void *const *set, int q) { return (T *)set[q]; }
 
 template <typename T, bool DC>
-__global__ __launch_bounds__(kB) void k_b_vel(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
-                                              T cu) {
+__global__ __launch_bounds__(kB) void k_b_vel(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                              int parity) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
@@ -65,12 +67,12 @@ __global__ __launch_bounds__(kB) void k_b_vel(const BSample *__restrict__ S, con
     const T *q0 = (const T *)(parity ? s.qB : s.qA);
     vel_voxel<T, DC, true>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(src, 1), fld<T>(src, 2), fld<T>(src, 3), q0,
                            fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3), DC ? s.R[1] : nullptr,
-                           DC ? s.R[2] : nullptr, DC ? s.R[3] : nullptr, cu);
+                           DC ? s.R[2] : nullptr, DC ? s.R[3] : nullptr, (T)s.cu);
 }
 
 template <typename T, bool DC>
-__global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
-                                               T cp) {
+__global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                               int parity) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
@@ -80,46 +82,46 @@ __global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, co
     void *const *dst = parity ? s.A : s.B;
     T *q1 = (T *)(parity ? s.qA : s.qB);
     pres_voxel<T, DC>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3),
-                      fld<T>(dst, 0), q1, DC ? s.R[0] : nullptr, cp, DC ? (T)0 : (T)3.0);
+                      fld<T>(dst, 0), q1, DC ? s.R[0] : nullptr, (T)s.cp, DC ? (T)0 : (T)3.0);
 }
 
 // Interleaved correction state (defect correction): the first (velocity)
 // pass of an interval, ghost fill, fold (over the fluid list: X += d, d = 0;
 // the halo copies of d are refilled with zeros by the following ghost fill)
 // and residual.
-__global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
-                                               float cu) {
+__global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                               int parity) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
     const int64_t c = s.list[t];
     vel_voxel4(s.g, c, t, s.pc[c], parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA), parity ? s.A4 : s.B4,
-               s.R4, cu);
+               s.R4, (float)s.cu);
 }
 
 // Lagged single pass (the sweeps after the first of an interval) and the
 // settle pass that completes the pressure at the interval's end, in place.
-__global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, const BChunk *__restrict__ C, int parity,
-                                               float cu, float cp) {
+__global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                               int parity) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
     const int64_t c = s.list[t];
     lag_voxel4(s.g, c, t, s.pc[c], s.pc, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-               parity ? s.A4 : s.B4, s.R4, cu, cp, 0.f);
+               parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
 }
 
 __global__ __launch_bounds__(kB) void k_b_settle4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
-                                                  int which, float cp) {
+                                                  int which) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
     const int64_t c = s.list[t];
     float4 *set = which ? s.B4 : s.A4;
-    pres_voxel4(s.g, c, s.pc[c], set, set, (float *)(which ? s.qB : s.qA), s.R4, cp, 0.f);
+    pres_voxel4(s.g, c, s.pc[c], set, set, (float *)(which ? s.qB : s.qA), s.R4, (float)s.cp, 0.f);
 }
 
 __global__ __launch_bounds__(kB) void k_b_ghost4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
@@ -272,7 +274,7 @@ extern "C" void pm_batch_params_default(pm_batch_params *p) {
     std::memset(p, 0, sizeof *p);
     p->precision = PM_PREC_F32DC;
     p->omega = 2.5;
-    p->omega_p = 6.0;
+    p->omega_p = 7.0;
     p->it_eval = 100;
     p->it_write = 100;
     p->max_iter = 100000000;
@@ -341,6 +343,19 @@ extern "C" int64_t pm_batch_sample_bytes(const pm_batch_params *p, int nx, int n
     return pm_device_bytes(&q);
 }
 
+// First sweep of a sample on its own stream (and, in the defect-correction
+// mode, the fold after sweep 0), so that every batch interval afterwards ends
+// on an iteration index that is a multiple of it_eval.  The batch runs its own
+// kernels, so the context's autotuner is off and the sweep uses the pair.
+static int prologue(pm_ctx *c) {
+    c->tuner = Tuner();
+    c->jac_impl = JI_PAIR;
+    int rc = pm_one_sweep(c);
+    if (rc == PM_OK) rc = pm_settle(c);
+    if (rc == PM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = pm_fail(PM_E_HIP, "first sweep failed");
+    return rc;
+}
+
 extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int nz, int bc, double dx, double phi,
                             int *slot) {
     if (!b || !raw || !slot) return pm_fail(PM_E_ARG, "null argument");
@@ -354,15 +369,7 @@ extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int
     int rc = pm_ctx_create(&q, &c);
     if (rc != PM_OK) return rc;
     rc = pm_load_geometry(c, raw);
-    // the batch runs its own kernels: the first sweep uses the pair, untuned
-    c->tuner = Tuner();
-    c->jac_impl = JI_PAIR;
-    // the first sweep (and, in the defect-correction mode, the fold after
-    // sweep 0) runs on the sample's own stream; afterwards every interval
-    // ends on an iteration index that is a multiple of it_eval
-    if (rc == PM_OK) rc = pm_one_sweep(c);
-    if (rc == PM_OK) rc = pm_settle(c);
-    if (rc == PM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = pm_fail(PM_E_HIP, "first sweep failed");
+    if (rc == PM_OK) rc = prologue(c);
     if (rc != PM_OK) {
         pm_destroy(c);
         return rc;
@@ -426,6 +433,9 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
         d.A4 = c->S4[0];
         d.B4 = c->S4[1];
         d.R4 = c->R4;
+        const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
+        d.cu = om * pm::kDt;
+        d.cp = wp * pm::kC2 * pm::kDt;
         d.qA = b->dc() ? (void *)c->qD[0] : (void *)c->qX[0];
         d.qB = b->dc() ? (void *)c->qD[1] : (void *)c->qX[1];
         d.qX = c->qX[0];
@@ -444,21 +454,20 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
 static void launch_interval(pm_batch *b) {
     using namespace pm;
     hipStream_t st = b->stream;
-    const double om = b->prm.omega, wp = b->prm.omega_p > 0.0 ? b->prm.omega_p : om;
     const int n = b->prm.it_eval;
     const dim3 blk(kB);
+    // the relaxation factors are per sample (sample descriptors)
     if (b->dc()) {
-        const float cu = (float)(om * kDt), cp = (float)(wp * kC2 * kDt);
         // symplectic Jacobi in lagged form: the first sweep from the settled
         // state, then one lagged pass per sweep, then the settle pass
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
-            if (s == 0) hipLaunchKernelGGL(k_b_vel4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
-            else hipLaunchKernelGGL(k_b_lag4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu, cp);
+            if (s == 0) hipLaunchKernelGGL(k_b_vel4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
+            else hipLaunchKernelGGL(k_b_lag4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
             hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
         }
         const int cur = n & 1;
-        hipLaunchKernelGGL(k_b_settle4, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur, cp);
+        hipLaunchKernelGGL(k_b_settle4, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur);
         hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, cur);
         hipLaunchKernelGGL(k_b_fold4, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur);
         hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, cur);
@@ -468,11 +477,10 @@ static void launch_interval(pm_batch *b) {
         hipLaunchKernelGGL(k_b_zero_q, dim3(b->nP), blk, 0, st, b->dS, b->dP);
         hipLaunchKernelGGL(k_b_reduce, dim3(b->nC), blk, 0, st, b->dS, b->dC, 2, b->dPart);
     } else {
-        const double cu = om * kDt, cp = wp * kC2 * kDt;
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
-            hipLaunchKernelGGL((k_b_vel<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cu);
-            hipLaunchKernelGGL((k_b_pres<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par, cp);
+            hipLaunchKernelGGL((k_b_vel<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
+            hipLaunchKernelGGL((k_b_pres<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
             hipLaunchKernelGGL((k_b_ghost<double, false>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
         }
         hipLaunchKernelGGL(k_b_reduce, dim3(b->nC), blk, 0, st, b->dS, b->dC, n & 1, b->dPart);
@@ -576,8 +584,17 @@ extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
         }
         row.conv = b->conv[slot];
         int st = PM_SAMPLE_RUNNING;
-        if (!std::isfinite(b->conv[slot])) st = PM_SAMPLE_DIVERGED;
-        else if (b->conv[slot] < c->prm.eps) st = PM_SAMPLE_CONVERGED;
+        if (!std::isfinite(b->conv[slot])) {
+            // relaxation beyond the stability bound: this sample starts over
+            // with a smaller pressure relaxation (the others are unaffected)
+            if (pm_restart_slower(c) == PM_OK && prologue(c) == PM_OK) {
+                b->conv[slot] = 1.0 + c->prm.eps;
+                b->shape_dirty = true;
+                running++;
+                continue;
+            }
+            st = PM_SAMPLE_DIVERGED;
+        } else if (b->conv[slot] < c->prm.eps) st = PM_SAMPLE_CONVERGED;
         else if (c->t >= b->prm.max_iter) st = PM_SAMPLE_MAX_ITER;
         b->state[slot] = st;
         if (st != PM_SAMPLE_RUNNING) b->shape_dirty = true;
@@ -605,6 +622,7 @@ extern "C" int pm_batch_status(pm_batch *b, int slot, int *state, pm_result *res
         res->porosity = pm_used_porosity(c);
         res->n_log_rows = b->n_rows[slot];
         res->last = b->last[slot];
+        res->restarts = c->restarts;
     }
     return PM_OK;
 }

@@ -84,6 +84,8 @@ struct pm_ctx {
     Tuner tuner;
     // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence
     bool lagged = false;
+    // restarts after a divergence (pm_restart_slower)
+    int restarts = 0;
 
     bool fast() const { return prm.mode != PM_MODE_REFERENCE; }
     bool dc() const { return fast() && prm.precision == PM_PREC_F32DC; }
@@ -96,6 +98,7 @@ struct pm_ctx {
 int pm_fail(int code, const char *fmt, ...);
 int pm_one_sweep(pm_ctx *c);
 int pm_settle(pm_ctx *c);
+int pm_restart_slower(pm_ctx *c);
 int pm_reduce_state(pm_ctx *c, bool full, pm::RedSums *out);
 double pm_convergence_from(const pm::RedSums &r, double *kprev);
 void pm_permeability_from(const pm::RedSums &r, const int *dom, double dx, double phi, double *out);

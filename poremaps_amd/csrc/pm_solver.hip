@@ -217,6 +217,57 @@ static int dc_prepare(pm_ctx *c) {
     return PM_OK;
 }
 
+// The initial state of src/initialize.cc:69-121 (linear pressure, zero
+// velocities, pins) and the mode's derived buffers; the geometry is loaded.
+static int init_state(pm_ctx *c) {
+    const pm::Grid &g = c->g;
+    for (int q = 0; q < 4; q++) HIPCHK(hipMemsetAsync(c->X[q], 0, sizeof(double) * (size_t)g.total, c->stream));
+    pm::launch_init_fields(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->stream);
+    // src/calc_flow.cc:105-108: the pins are applied before sweep 0
+    pm::launch_pins<double>(g, c->G, c->X[0], (double)(g.nz + 2), 2.0, 3.0, c->stream);
+    if (c->fast()) {
+        pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
+        HIPCHK(hipMemcpyAsync(c->qX[1], c->qX[0], sizeof(double) * (size_t)g.qtotal(),
+                              hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (c->X2[0])
+        for (int q = 0; q < 4; q++)
+            HIPCHK(hipMemcpyAsync(c->X2[q], c->X[q], sizeof(double) * (size_t)g.total, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    if (c->aos()) {
+        for (int q = 0; q < 2; q++) RC(fill_f32(c, (float *)c->S4[q], 4 * g.total, 0.f));
+        RC(fill_f32(c, c->R4.p, g.total, 0.f));
+        RC(fill_f32(c, (float *)c->R4.m, 4 * std::max<int64_t>(1, c->lists.n_all), 0.f));
+        RC(dc_prepare(c));
+    } else if (c->dc()) {
+        for (int q = 0; q < 4; q++) {
+            RC(fill_f32(c, c->D[q], g.total, 0.f));
+            RC(fill_f32(c, c->R[q], g.total, 0.f));
+            if (c->D2[q]) RC(fill_f32(c, c->D2[q], g.total, 0.f));
+        }
+        RC(dc_prepare(c));
+    }
+    c->t = 0;
+    c->kprev = 0.0;
+    c->lagged = false;
+    HIPCHK(hipGetLastError());
+    return PM_OK;
+}
+
+// Start a fast-mode solve again from the initial state with a smaller
+// pressure relaxation, after it diverged: the relaxation pair crossed the
+// stability bound of the symplectic sweep (omega * omega_p near 19 on every
+// medium probed).  Returns PM_E_DIVERGED when the pair is already small.
+int pm_restart_slower(pm_ctx *c) {
+    const double wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : c->prm.omega;
+    if (!c->fast() || c->prm.omega * wp <= 8.0) return PM_E_DIVERGED;
+    c->prm.omega_p = 0.75 * wp;
+    c->restarts++;
+    RC(init_state(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
 extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     if (!c || !raw) return fail(PM_E_ARG, "null argument");
     HIPCHK(hipSetDevice(c->prm.device));
@@ -304,32 +355,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     HIPCHK(hipMemsetAsync(c->pc, 0xff, (size_t)g.total, c->stream));
     HIPCHK(hipMemsetAsync(c->code, 0, sizeof(int32_t) * (size_t)g.total, c->stream));
     pm::launch_eval_geometry(g, c->G, c->code, c->pc, c->stream);
-    for (int q = 0; q < 4; q++) HIPCHK(hipMemsetAsync(c->X[q], 0, sizeof(double) * (size_t)g.total, c->stream));
-    pm::launch_init_fields(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->stream);
-    // src/calc_flow.cc:105-108: the pins are applied before sweep 0
-    pm::launch_pins<double>(g, c->G, c->X[0], (double)(g.nz + 2), 2.0, 3.0, c->stream);
-    if (c->fast()) {
-        pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
-        HIPCHK(hipMemcpyAsync(c->qX[1], c->qX[0], sizeof(double) * (size_t)g.qtotal(),
-                              hipMemcpyDeviceToDevice, c->stream));
-    }
-    if (c->X2[0])
-        for (int q = 0; q < 4; q++)
-            HIPCHK(hipMemcpyAsync(c->X2[q], c->X[q], sizeof(double) * (size_t)g.total, hipMemcpyDeviceToDevice,
-                                  c->stream));
-    if (c->aos()) {
-        for (int q = 0; q < 2; q++) RC(fill_f32(c, (float *)c->S4[q], 4 * g.total, 0.f));
-        RC(fill_f32(c, c->R4.p, g.total, 0.f));
-        RC(fill_f32(c, (float *)c->R4.m, 4 * std::max<int64_t>(1, c->lists.n_all), 0.f));
-        RC(dc_prepare(c));
-    } else if (c->dc()) {
-        for (int q = 0; q < 4; q++) {
-            RC(fill_f32(c, c->D[q], g.total, 0.f));
-            RC(fill_f32(c, c->R[q], g.total, 0.f));
-            if (c->D2[q]) RC(fill_f32(c, c->D2[q], g.total, 0.f));
-        }
-        RC(dc_prepare(c));
-    }
+    RC(init_state(c));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     (void)hipFree(raw_dev);
@@ -704,7 +730,13 @@ extern "C" int pm_solve(pm_ctx *c, pm_log_fn log_fn, pm_progress_fn progress_fn,
             t_mark_it = c->t;
             if (progress_fn(it, conv, dt / nit, user) != 0) { status = fail(PM_E_ABORTED, "progress callback stopped the solve"); break; }
         }
-        if (!std::isfinite(conv)) break;
+        if (!std::isfinite(conv)) {
+            // a fast-mode relaxation beyond the stability bound: start over slower
+            if (pm_restart_slower(c) != PM_OK) break;
+            conv = 1.0 + P.eps;
+            t_mark = clk::now();
+            t_mark_it = c->t;
+        }
     }
     RC(dc_flush(c));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -721,6 +753,7 @@ extern "C" int pm_solve(pm_ctx *c, pm_log_fn log_fn, pm_progress_fn progress_fn,
         res->seconds = secs;
         res->glups = (double)c->g.nx * c->g.ny * c->g.nz * c->t / std::max(secs, 1e-12) / 1e9;
         res->n_log_rows = nrows;
+        res->restarts = c->restarts;
         fill_row(&res->last, c->t - 1, conv, 0.0, k);
     }
     if (status != PM_OK) return status;

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import poremaps_amd as pm
+from poremaps_amd import geometry as geo
 from conftest import random_geometry
 
 pytestmark = pytest.mark.gpu
@@ -60,3 +61,27 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
         if form is None:
             # FP32+DC runs on the interleaved state: pair and lagged forms
             assert len(tried) == (2 if precision == "f32dc" else 5) and name in tried
+
+
+def test_divergent_relaxation_restarts_slower():
+    """A relaxation pair beyond the stability bound (omega * omega_p = 22.5)
+    diverges; the solve starts over from the initial state with omega_p cut
+    by a quarter and ends exactly where a solve started with that pair ends
+    -- alone and in a batch."""
+    # open pores: the unstable mode is a high-frequency mode of bulk fluid
+    raw = geo.sphere_pack((20, 16, 18), 0.55, 2, 4, seed=3)
+    say("restart after divergence")
+    kw = dict(boundary_method=0, voxel_size=1e-5, mode="jacobi", precision="f32dc", eps=1e-6, max_iter=100_000)
+    with pm.Solver(raw, omega=2.5, omega_p=9.0, **kw) as s:
+        res, _ = s.solve()
+    with pm.Solver(raw, omega=2.5, omega_p=9.0 * 0.75, **kw) as s:
+        ref, _ = s.solve()
+    assert res.restarts == 1 and ref.restarts == 0
+    assert res.sweeps == ref.sweeps and res.last.wk33 == ref.last.wk33
+    with pm.Batch(omega=2.5, omega_p=9.0) as b:
+        slot = b.add(raw, boundary_method=0, voxel_size=1e-5)
+        while b.step():
+            pass
+        state, bres = b.status(slot)
+    assert state == "converged" and bres.restarts == 1
+    assert bres.sweeps == ref.sweeps and bres.last.wk33 == ref.last.wk33
