@@ -100,17 +100,17 @@ __global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, co
                s.R4, (float)s.cu);
 }
 
-// Lagged single pass (the sweeps after the first of an interval) and the
-// settle pass that completes the pressure at the interval's end, in place.
+// Lagged single pass (the sweeps after the first of an interval; every load
+// issued up front) and the settle pass that completes the pressure at the
+// interval's end, in place.
 __global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
                                                int parity) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    const int64_t c = s.list[t];
-    lag_voxel4(s.g, c, t, s.pc[c], s.pc, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-               parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+    lag_voxel4_eager(s.g, s.list[t], t, s.pc, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
+                     parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
 }
 
 __global__ __launch_bounds__(kB) void k_b_settle4(const BSample *__restrict__ S, const BChunk *__restrict__ C,

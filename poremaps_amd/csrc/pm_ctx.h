@@ -72,10 +72,12 @@ struct pm_ctx {
     double kprev = 0.0;   // permeability of the previous convergence test
     double phi = 0.0;
     int64_t fluid = 0;
-    // fluid-list slab height requested for the next pm_load_geometry (0: plain
-    // order); 64 rows measured best on 256^3 and 512^3 media (512^3 GRF
-    // interleaved lagged pass: 1.43 ms plain order, 1.36 ms with 64-row slabs)
+    // fluid-list column (rows x cells) for the next pm_load_geometry (0: whole
+    // extent); 64 x 64 measured best (flat from 16 x 32 to 128 x 128) on the
+    // 512^3 field: interleaved lagged pass 0.96 ms against 1.05 ms with whole
+    // rows and 1.43 ms in plain (z, y, x) order
     int prm_tile_y = 64;
+    int prm_tile_x = 64;
     // Jacobi kernels: velocity + pressure pair over the fluid list, one lagged
     // pass over the fluid list, or one lagged dense 2.5-D pass (z segments of
     // stream_zs planes)

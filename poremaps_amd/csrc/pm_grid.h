@@ -124,13 +124,11 @@ __device__ __forceinline__ T lap_load(int cs, const T *__restrict__ f, int64_t c
     return (l - (T)2.0 * vc) + r;
 }
 
-// get_2nd_derivation with all four neighbours loaded up front and the case's
-// expression picked by select: no divergent branches around the loads, the
-// same arithmetic as d2_load.
+// d2 with every case's expression evaluated and the case's value picked by
+// select (no branches; the same arithmetic as d2).
 template <typename T>
-__device__ __forceinline__ T d2_sel(int cs, const T *__restrict__ f, int64_t c, int64_t s, T vc) {
+__device__ __forceinline__ T d2_pick(int cs, T ll, T l, T vc, T r, T rr) {
     const T K83 = (T)(8.0 / 3.0), K163 = (T)(16.0 / 3.0), K15 = (T)(1.0 / 5.0);
-    const T l = f[c - s], r = f[c + s], ll = f[c - 2 * s], rr = f[c + 2 * s];
     const T r1 = (T)-8.0 * vc;
     const T r2 = K83 * r - K163 * vc;
     const T r3 = K83 * l - K163 * vc;
@@ -144,6 +142,14 @@ __device__ __forceinline__ T d2_sel(int cs, const T *__restrict__ f, int64_t c, 
     o = cs == 4 ? r4 : o;
     o = cs == 5 ? r5 : o;
     return o;
+}
+
+// get_2nd_derivation with all four neighbours loaded up front and the case's
+// expression picked by select: no divergent branches around the loads, the
+// same arithmetic as d2_load.
+template <typename T>
+__device__ __forceinline__ T d2_sel(int cs, const T *__restrict__ f, int64_t c, int64_t s, T vc) {
+    return d2_pick<T>(cs, f[c - 2 * s], f[c - s], vc, f[c + s], f[c + 2 * s]);
 }
 
 // Momentum forces, every neighbour loaded (branch-free), same arithmetic as

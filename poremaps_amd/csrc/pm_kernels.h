@@ -35,8 +35,10 @@ void launch_pins(const Grid &g, const uint8_t *G, T *p, T pin_in, T pin_hi, T pi
 void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
                        double *v, double *w, int solver, int nsweeps, bool one_wg, hipStream_t s);
 
-// Device lists of the interior fluid cells' element offsets, ascending
-// (z, y, x): all of them, and split by colour (i+j+k) mod 3.
+// Device lists of the interior fluid cells' element offsets in column order
+// (columns of tile_y rows x tile_x cells, each marching along z; within a
+// plane of a column ascending y, x): all of them, and split by colour
+// (i+j+k) mod 3.
 struct FluidLists {
     int32_t *all = nullptr;
     int n_all = 0;
@@ -45,8 +47,8 @@ struct FluidLists {
     // stencil loads: every neighbour up front + select (true) or only the
     // neighbours the case needs, behind branches (false)
     bool select_loads = true;
-    // rows per slab of the list order (0: plain z, y, x order)
-    int tile_y = 0;
+    // column of the list order, rows x cells (0: whole extent)
+    int tile_y = 0, tile_x = 0;
 };
 
 template <typename T>

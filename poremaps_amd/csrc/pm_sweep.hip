@@ -94,7 +94,8 @@ __global__ __launch_bounds__(kListBlock) void k_jac_lag(Grid g, const int32_t *_
 
 // Interleaved FP32 correction state (defect-correction Jacobi): stage 0 =
 // first pass (velocities, p carried), 1 = lagged pass, 2 = pair velocity
-// pass, 3 = pair pressure pass / settle.
+// pass, 3 = pair pressure pass / settle, 4 = lagged pass with every load
+// issued up front.
 template <int STAGE>
 __global__ __launch_bounds__(kListBlock) void k_jac4(Grid g, const int32_t *__restrict__ list, int n,
                                                      const uint8_t *__restrict__ pc, const float4 *__restrict__ S0,
@@ -105,6 +106,7 @@ __global__ __launch_bounds__(kListBlock) void k_jac4(Grid g, const int32_t *__re
     const int64_t c = list[t];
     if (STAGE == 0 || STAGE == 2) vel_voxel4(g, c, t, pc[c], S0, q0, S1, R, cu);
     else if (STAGE == 1) lag_voxel4(g, c, t, pc[c], pc, S0, q0, S1, R, cu, cp, 0.f);
+    else if (STAGE == 4) lag_voxel4_eager(g, c, t, pc, S0, q0, S1, R, cu, cp, 0.f);
     else pres_voxel4(g, c, pc[c], S0, S1, q1, R, cp, 0.f);
 }
 
@@ -566,6 +568,8 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
     if (stage == JAC_FIRST) {
         hipLaunchKernelGGL(k_jac4<0>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
+    } else if (stage == JAC_LAG && L.select_loads) {
+        hipLaunchKernelGGL(k_jac4<4>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (stage == JAC_LAG) {
         hipLaunchKernelGGL(k_jac4<1>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (stage == JAC_PAIR) {

@@ -210,13 +210,26 @@ __device__ __forceinline__ void lag_voxel(const Grid &g, int64_t c, uint8_t pcv,
 // expressions, in the same order, as the per-field functions above.
 // ---------------------------------------------------------------------------
 
-// Viscous sums at c from the interleaved state (the stencil of fast_force_s).
-__device__ __forceinline__ void visc_sums4(const Cases &cs, int64_t c, int64_t sj, int64_t si,
-                                           const float4 *__restrict__ S, const float4 &C, float &su, float &sv,
+// The 12 stencil neighbours of a cell (+-1, +-2 on each axis).
+struct Star4 {
+    float4 XM, XP, XM2, XP2, YM, YP, YM2, YP2, ZM, ZP, ZM2, ZP2;
+};
+
+__device__ __forceinline__ Star4 load_star4(const float4 *__restrict__ S, int64_t c, int64_t sj, int64_t si) {
+    Star4 s;
+    s.XM = S[c - 1]; s.XP = S[c + 1]; s.XM2 = S[c - 2]; s.XP2 = S[c + 2];
+    s.YM = S[c - sj]; s.YP = S[c + sj]; s.YM2 = S[c - 2 * sj]; s.YP2 = S[c + 2 * sj];
+    s.ZM = S[c - si]; s.ZP = S[c + si]; s.ZM2 = S[c - 2 * si]; s.ZP2 = S[c + 2 * si];
+    return s;
+}
+
+// Viscous sums at a cell from its value C and its stencil (the stencil of
+// fast_force_s).
+__device__ __forceinline__ void visc_from4(const Cases &cs, const Star4 &s, const float4 &C, float &su, float &sv,
                                            float &sw) {
-    const float4 XM = S[c - 1], XP = S[c + 1], XM2 = S[c - 2], XP2 = S[c + 2];
-    const float4 YM = S[c - sj], YP = S[c + sj], YM2 = S[c - 2 * sj], YP2 = S[c + 2 * sj];
-    const float4 ZM = S[c - si], ZP = S[c + si], ZM2 = S[c - 2 * si], ZP2 = S[c + 2 * si];
+    const float4 &XM = s.XM, &XP = s.XP, &XM2 = s.XM2, &XP2 = s.XP2;
+    const float4 &YM = s.YM, &YP = s.YP, &YM2 = s.YM2, &YP2 = s.YP2;
+    const float4 &ZM = s.ZM, &ZP = s.ZP, &ZM2 = s.ZM2, &ZP2 = s.ZP2;
     const float lxx = ((cs.bx ? 0.f : XM.y) - 2.0f * C.y) + (fwd_solid(cs.cx) ? 0.f : XP.y);
     const float lyy = ((cs.by ? 0.f : YM.z) - 2.0f * C.z) + (fwd_solid(cs.cy) ? 0.f : YP.z);
     const float lzz = ((cs.bz ? 0.f : ZM.w) - 2.0f * C.w) + (fwd_solid(cs.cz) ? 0.f : ZP.w);
@@ -229,6 +242,13 @@ __device__ __forceinline__ void visc_sums4(const Cases &cs, int64_t c, int64_t s
     su = (lxx + dxy) + dxz;
     sv = (lyy + dyz) + dyx;
     sw = (lzz + dzx) + dzy;
+}
+
+// Viscous sums at c from the interleaved state.
+__device__ __forceinline__ void visc_sums4(const Cases &cs, int64_t c, int64_t sj, int64_t si,
+                                           const float4 *__restrict__ S, const float4 &C, float &su, float &sv,
+                                           float &sw) {
+    visc_from4(cs, load_star4(S, c, sj, si), C, su, sv, sw);
 }
 
 // Velocities from the forces (vel_voxel's tail); returns the new cell.
@@ -344,6 +364,76 @@ __device__ __forceinline__ void lag_voxel4(const Grid &g, int64_t c, int t, uint
     float su, sv, sw;
     visc_sums4(cs, c, g.sx, g.sy, S0, C, su, sv, sw);
     S1[c] = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, R.m[t], Pc, cu);
+}
+
+// Divergence at a cell from its value N, the +1 neighbours' components and
+// its code (div4 with the loads done by the caller).
+__device__ __forceinline__ float div_from4(uint8_t pcn, const float4 &N, float ux, float vy, float wz) {
+    const Cases cs = decode(pcn);
+    const float a = fwd_solid(cs.cz) ? 0.f : wz;
+    const float b = fwd_solid(cs.cy) ? 0.f : vy;
+    const float d = fwd_solid(cs.cx) ? 0.f : ux;
+    return ((((a - N.w) + b) - N.z) + d) - N.y;
+}
+
+// lag_voxel4 with every load of the voxel issued before any condition is
+// evaluated (on a 512^3 field: 1.07 ms per pass against 1.29-1.38 ms): the pressures of the -x, -y and -z neighbours are recomputed
+// whether or not the face is open and selected afterwards, so the loads wait
+// on the list entry only, not on the cell's code.  Same expressions in the
+// same order, so the same values.  Every address stays inside the 2-cell halo
+// (c is an interior cell; a periodic image is an interior cell too).
+__device__ __forceinline__ void lag_voxel4_eager(const Grid &g, int64_t c, int t, const uint8_t *__restrict__ pc,
+                                                 const float4 *__restrict__ S0, const float *__restrict__ q0,
+                                                 float4 *__restrict__ S1, Res4 R, float cu, float cp,
+                                                 float pin_out) {
+    const int64_t sx = g.sx, sy = g.sy;
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const bool out = i == g.nz + 1;
+    // the -x / -y neighbour whose pressure is recomputed: the cell before, or
+    // its periodic image at the first interior index
+    const bool wx = k == 2 && g.px, wy = j == 2 && g.py;
+    const int64_t nxn = wx ? c - 1 + g.nx : c - 1;
+    const int64_t nyn = wy ? c - sx + (int64_t)g.ny * sx : c - sx;
+    const int64_t nzn = c - sy;
+    const uint8_t pcv = pc[c], pcx = pc[nxn], pcy = pc[nyn], pcz = pc[nzn];
+    const float rpc = R.p[c], rpx = R.p[nxn], rpy = R.p[nyn], rpz = R.p[nzn];
+    const float4 C = S0[c];
+    const Star4 s = load_star4(S0, c, sx, sy);
+    const float vyx = S0[nxn + sx].z, wzx = S0[nxn + sy].w;
+    const float uxy = S0[nyn + 1].y, wzy = S0[nyn + sy].w;
+    const float uxz = S0[nzn + 1].y, vyz = S0[nzn + sx].z;
+    // the residual entry and the new cell are touched once per pass: streamed
+    // (non-temporal), so that the stencil's lines stay in cache
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
+    const float4 Rm = make_float4(r.x, r.y, r.z, r.w);
+    // without wrap the neighbour and its +1 cell are already loaded
+    float4 NX = s.XM, NY = s.YM;
+    float uxx = C.y, vyy = C.z;
+    if (wx) { NX = S0[nxn]; uxx = S0[nxn + 1].y; }
+    if (wy) { NY = S0[nyn]; vyy = S0[nyn + sx].z; }
+
+    const Cases cs = decode(pcv);
+    const float Pc = out ? pin_out : C.x - (div_from4(pcv, C, s.XP.y, s.YP.z, s.ZP.w) + rpc) * cp;
+    const float c2dt = (float)(kC2 * kDt);
+    const float dx0 = div_from4(pcx, NX, uxx, vyx, wzx);
+    const float dy0 = div_from4(pcy, NY, uxy, vyy, wzy);
+    const float dz0 = div_from4(pcz, s.ZM, uxz, vyz, C.w);
+    float px = out ? pin_out - dx0 * c2dt : NX.x - (dx0 + rpx) * cp;
+    float py = out ? pin_out - dy0 * c2dt : NY.x - (dy0 + rpy) * cp;
+    const float pz = i == 2 ? s.ZM.x : s.ZM.x - (dz0 + rpz) * cp;
+    // a wall halo keeps its stored value (q on the outlet plane)
+    if (k == 2 && !g.px) px = out ? q0[g.qat(j, k - 1)] : s.XM.x;
+    if (j == 2 && !g.py) py = out ? q0[g.qat(j - 1, k)] : s.YM.x;
+    const float dpx = cs.bx ? 0.f : Pc - px;
+    const float dpy = cs.by ? 0.f : Pc - py;
+    const float dpz = cs.bz ? 0.f : Pc - pz;
+    float su, sv, sw;
+    visc_from4(cs, s, C, su, sv, sw);
+    const float4 o = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rm, Pc, cu);
+    const v4f ov = {o.x, o.y, o.z, o.w};
+    __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c]));
 }
 
 // Residual of the FP64 state at c into the interleaved FP32 residual.
