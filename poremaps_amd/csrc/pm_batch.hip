@@ -26,9 +26,9 @@ namespace pm {
 
 constexpr int kB = 256;  // fluid voxels / halo lines / plane cells per chunk
 
-// Device descriptor of one running sample.  A/B: the swept state (FP32
-// correction in the defect-correction mode, FP64 solution otherwise), A the
-// current one at the start of an interval.
+// Device descriptor of one running sample.  A/B: the swept FP64 state of the
+// FP64 mode (A4/B4 in the defect-correction mode), A the current one at the
+// start of an interval.
 struct BSample {
     Grid g;
     const int32_t *list;
@@ -36,7 +36,6 @@ struct BSample {
     const uint8_t *G, *pc;
     void *A[4], *B[4];
     void *qA, *qB;
-    float *R[4];
     double *X[4];
     double *qX;
     // defect correction: interleaved correction sets and residual
@@ -54,7 +53,7 @@ struct BChunk {
 template <typename T>
 __device__ __forceinline__ T *fld(void *const *set, int q) { return (T *)set[q]; }
 
-template <typename T, bool DC>
+template <typename T>
 __global__ __launch_bounds__(kB) void k_b_vel(const BSample *__restrict__ S, const BChunk *__restrict__ C,
                                               int parity) {
     const BChunk ch = C[blockIdx.x];
@@ -65,12 +64,12 @@ __global__ __launch_bounds__(kB) void k_b_vel(const BSample *__restrict__ S, con
     void *const *src = parity ? s.B : s.A;
     void *const *dst = parity ? s.A : s.B;
     const T *q0 = (const T *)(parity ? s.qB : s.qA);
-    vel_voxel<T, DC, true>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(src, 1), fld<T>(src, 2), fld<T>(src, 3), q0,
-                           fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3), DC ? s.R[1] : nullptr,
-                           DC ? s.R[2] : nullptr, DC ? s.R[3] : nullptr, (T)s.cu);
+    vel_voxel<T, false, true>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(src, 1), fld<T>(src, 2), fld<T>(src, 3),
+                              q0, fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3), nullptr, nullptr, nullptr,
+                              (T)s.cu);
 }
 
-template <typename T, bool DC>
+template <typename T>
 __global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, const BChunk *__restrict__ C,
                                                int parity) {
     const BChunk ch = C[blockIdx.x];
@@ -81,8 +80,8 @@ __global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, co
     void *const *src = parity ? s.B : s.A;
     void *const *dst = parity ? s.A : s.B;
     T *q1 = (T *)(parity ? s.qA : s.qB);
-    pres_voxel<T, DC>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3),
-                      fld<T>(dst, 0), q1, DC ? s.R[0] : nullptr, (T)s.cp, DC ? (T)0 : (T)3.0);
+    pres_voxel<T, false>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3),
+                         fld<T>(dst, 0), q1, nullptr, (T)s.cp, (T)3.0);
 }
 
 // Interleaved correction state (defect correction): the first (velocity)
@@ -425,9 +424,9 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
         d.G = c->G;
         d.pc = c->pc;
         for (int q = 0; q < 4; q++) {
-            d.A[q] = b->dc() ? (void *)c->D[q] : (void *)c->X[q];
-            d.B[q] = b->dc() ? (void *)c->D2[q] : (void *)c->X2[q];
-            d.R[q] = c->R[q];
+            // the FP64 mode's two sets (the defect-correction mode uses A4 / B4)
+            d.A[q] = c->X[q];
+            d.B[q] = c->X2[q];
             d.X[q] = c->X[q];
         }
         d.A4 = c->S4[0];
@@ -479,8 +478,8 @@ static void launch_interval(pm_batch *b) {
     } else {
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
-            hipLaunchKernelGGL((k_b_vel<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
-            hipLaunchKernelGGL((k_b_pres<double, false>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
+            hipLaunchKernelGGL((k_b_vel<double>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
+            hipLaunchKernelGGL((k_b_pres<double>), dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
             hipLaunchKernelGGL((k_b_ghost<double, false>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
         }
         hipLaunchKernelGGL(k_b_reduce, dim3(b->nC), blk, 0, st, b->dS, b->dC, n & 1, b->dPart);

@@ -53,14 +53,11 @@ struct pm_ctx {
     double *X[4] = {nullptr, nullptr, nullptr, nullptr};   // p, u, v, w
     double *X2[4] = {nullptr, nullptr, nullptr, nullptr};  // Jacobi second buffer
     double *qX[2] = {nullptr, nullptr};
-    // FP32 correction state
-    float *D[4] = {nullptr, nullptr, nullptr, nullptr};
-    float *D2[4] = {nullptr, nullptr, nullptr, nullptr};
-    float *qD[2] = {nullptr, nullptr};
-    float *R[4] = {nullptr, nullptr, nullptr, nullptr};
-    // defect-correction Jacobi: the correction state and residual interleaved
-    // ({p, u, v, w} per cell), two sets for the out-of-place sweeps
+    // FP32 correction state of the defect-correction modes, interleaved
+    // ({p, u, v, w} per cell): two sets for Jacobi's out-of-place sweeps, one
+    // for SOR's in-place sweeps; its outlet buffers and the FP32 residual
     float4 *S4[2] = {nullptr, nullptr};
+    float *qD[2] = {nullptr, nullptr};
     pm::Res4 R4 = {nullptr, nullptr};
     // reductions
     pm::FluidLists lists;   // fluid cells of the interior (fast modes)
@@ -73,11 +70,13 @@ struct pm_ctx {
     double phi = 0.0;
     int64_t fluid = 0;
     // fluid-list column (rows x cells) for the next pm_load_geometry (0: whole
-    // extent); 64 x 64 measured best (flat from 16 x 32 to 128 x 128) on the
-    // 512^3 field: interleaved lagged pass 0.96 ms against 1.05 ms with whole
-    // rows and 1.43 ms in plain (z, y, x) order
+    // extent; prm_tile_x -1: by state layout).  On the 512^3 field 64 x 64
+    // is best for the interleaved FP32 state (flat from 16 x 32 to 128 x 128;
+    // lagged pass 0.96 ms against 1.05 ms with whole rows and 1.43 ms in plain
+    // (z, y, x) order) and 64 rows of whole rows for the per-field FP64 state
+    // (sweep 1.93 ms against 2.09 ms in 64 x 64 columns)
     int prm_tile_y = 64;
-    int prm_tile_x = 64;
+    int prm_tile_x = -1;
     // Jacobi kernels: velocity + pressure pair over the fluid list, one lagged
     // pass over the fluid list, or one lagged dense 2.5-D pass (z segments of
     // stream_zs planes)
@@ -92,7 +91,6 @@ struct pm_ctx {
     bool fast() const { return prm.mode != PM_MODE_REFERENCE; }
     bool dc() const { return fast() && prm.precision == PM_PREC_F32DC; }
     bool jacobi() const { return prm.mode == PM_MODE_JACOBI; }
-    bool aos() const { return dc() && jacobi(); }
 };
 
 

@@ -38,12 +38,16 @@ void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, dou
 // Device lists of the interior fluid cells' element offsets in column order
 // (columns of tile_y rows x tile_x cells, each marching along z; within a
 // plane of a column ascending y, x): all of them, and split by colour
-// (i+j+k) mod 3.
+// (i+j+k) mod 3.  For 3-colour SOR on the interleaved state the colour lists
+// lie back to back in `all` (colour q from off_col[q]) and col[q] point
+// into it (col_in_all).
 struct FluidLists {
     int32_t *all = nullptr;
     int n_all = 0;
     int32_t *col[3] = {nullptr, nullptr, nullptr};
     int n_col[3] = {0, 0, 0};
+    int off_col[3] = {0, 0, 0};
+    bool col_in_all = false;
     // stencil loads: every neighbour up front + select (true) or only the
     // neighbours the case needs, behind branches (false)
     bool select_loads = true;
@@ -51,11 +55,11 @@ struct FluidLists {
     int tile_y = 0, tile_x = 0;
 };
 
-template <typename T>
-void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0, const T *u0,
-                   const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu,
-                   T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
-                   hipStream_t s);
+// FP64 fast modes (per-field state p, u, v, w): the velocity + pressure
+// kernel pair of one symplectic Jacobi sweep from set 0 into set 1.
+void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0, const double *u0,
+                   const double *v0, const double *w0, const double *q0, double *p1, double *u1, double *v1,
+                   double *w1, double *q1, double cu, double cp, double pin_out, hipStream_t s);
 // Symplectic Jacobi in lagged form (pm_voxel.h): JAC_FIRST maps a settled
 // set 0 (u^n, p^n) to set 1 = (u^{n+1}, p^n); JAC_LAG maps set 0 =
 // (u^n, p^{n-1}) to set 1 = (u^{n+1}, p^n); JAC_SETTLE completes set 0 in
@@ -64,34 +68,33 @@ void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const 
 enum { JAC_FIRST = 0, JAC_LAG = 1, JAC_SETTLE = 2, JAC_PAIR = 3 };
 
 // Interleaved FP32 correction state (one float4 {p, u, v, w} per cell) of
-// the defect-correction Jacobi mode: stage JAC_FIRST / JAC_LAG / JAC_PAIR
+// the defect-correction modes.  Jacobi: stage JAC_FIRST / JAC_LAG / JAC_PAIR
 // (velocity + pressure kernels, S1 complete) from S0 into S1; JAC_SETTLE
-// completes S0 in place (pass S1 = S0, q1 = its outlet buffer).
+// completes S0 in place (pass S1 = S0, q1 = its outlet buffer).  3-colour
+// SOR: the three colour launches, in place.
 void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s);
+void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
+                    float cp, hipStream_t s);
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
                       const double *v, const double *w, const double *q, Res4 R, hipStream_t s);
 void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStream_t s);
 void launch_fold4(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d,
                   hipStream_t s);
-template <typename T>
-void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0,
-                         const T *u0, const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1,
-                         T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0,
+                         const double *u0, const double *v0, const double *w0, const double *q0, double *p1,
+                         double *u1, double *v1, double *w1, double *q1, double cu, double cp, double pin_out,
                          hipStream_t s);
 // The JAC_FIRST / JAC_LAG passes as the dense 2.5-D kernel: 64 x 8 tiles
 // marching along z in segments of zs planes (pm_sweep.hip).
-template <typename T>
-void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const T *p0, const T *u0,
-                          const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T cu, T cp, T pin_out,
-                          const float *Rp, const float *Ru, const float *Rv, const float *Rw, hipStream_t s);
-template <typename T>
-void launch_colour(const Grid &g, const FluidLists &L, const uint8_t *pc, T *p, T *u, T *v, T *w, T *q,
-                   T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv,
-                   const float *Rw, hipStream_t s);
-template <typename T>
-void launch_ghost_fast(const Grid &g, const uint8_t *G, T *p, T *u, T *v, T *w, T *q, T pin_in, T pin_hi,
-                       T pin_out, hipStream_t s);
+void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const double *p0, const double *u0,
+                          const double *v0, const double *w0, const double *q0, double *p1, double *u1, double *v1,
+                          double *w1, double cu, double cp, double pin_out, hipStream_t s);
+// 3-colour SOR, FP64 per-field state, in place: the three colour launches.
+void launch_colour(const Grid &g, const FluidLists &L, const uint8_t *pc, double *p, double *u, double *v, double *w,
+                   double *q, double cu, double cp, double pin_out, hipStream_t s);
+void launch_ghost_fast(const Grid &g, const uint8_t *G, double *p, double *u, double *v, double *w, double *q,
+                       double pin_in, double pin_hi, double pin_out, hipStream_t s);
 
 void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const double *u, const double *v,
                    const double *w, const DomBox &box, bool full, void *scratch, RedSums *out,
@@ -100,10 +103,5 @@ size_t reduce_scratch_bytes();
 
 void launch_outlet_plane(const Grid &g, const uint8_t *pc, const double *p, const double *u,
                          const double *v, const double *w, double pin_out, double *q, hipStream_t s);
-void launch_residual(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
-                     const double *v, const double *w, const double *q, float *Rp, float *Ru, float *Rv,
-                     float *Rw, hipStream_t s);
-void launch_fold(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw,
-                 float *dp, float *du, float *dv, float *dw, hipStream_t s);
 
 }  // namespace pm

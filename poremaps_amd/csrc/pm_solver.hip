@@ -97,24 +97,29 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     const bool dc = fast && p->precision == PM_PREC_F32DC;
     const bool jac = p->mode == PM_MODE_JACOBI;
     if (fast && !dc && jac) per += 32;
-    if (dc) per += 16 + 16 + (jac ? 16 : 0);
-    if (fast) per += jac ? 4 : 8;  // fluid lists (upper bound: every cell fluid)
+    // interleaved correction sets (16 B each: two for Jacobi, one for SOR),
+    // continuity residual (4 B) and momentum residual (16 B per fluid cell)
+    if (dc) per += (jac ? 32 : 16) + 4 + 16;
+    // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
+    // keeps separate colour lists
+    if (fast) per += (jac || dc) ? 4 : 8;
     return per * g.total + 4 * g.qtotal() * 8 + (int64_t)pm::reduce_scratch_bytes() + (1 << 20);
 }
 
 static void free_all(pm_ctx *c) {
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     f(c->G); f(c->pc); f(c->code);
-    for (int q = 0; q < 4; q++) { f(c->X[q]); f(c->X2[q]); f(c->D[q]); f(c->D2[q]); f(c->R[q]); }
+    for (int q = 0; q < 4; q++) { f(c->X[q]); f(c->X2[q]); }
     for (int q = 0; q < 2; q++) { f(c->qX[q]); f(c->qD[q]); f(c->S4[q]); c->S4[q] = nullptr; }
     f(c->R4.p); f(c->R4.m); c->R4 = pm::Res4{nullptr, nullptr};
     f(c->red_scratch); f(c->red_dev);
     f(c->lists.all);
-    for (int q = 0; q < 3; q++) f(c->lists.col[q]);
+    if (!c->lists.col_in_all)
+        for (int q = 0; q < 3; q++) f(c->lists.col[q]);
     c->lists = pm::FluidLists();
     if (c->red_host) (void)hipHostFree(c->red_host);
     c->G = c->pc = nullptr; c->code = nullptr;
-    for (int q = 0; q < 4; q++) c->X[q] = c->X2[q] = nullptr, c->D[q] = c->D2[q] = c->R[q] = nullptr;
+    for (int q = 0; q < 4; q++) c->X[q] = c->X2[q] = nullptr;
     c->qX[0] = c->qX[1] = nullptr; c->qD[0] = c->qD[1] = nullptr;
     c->red_scratch = nullptr; c->red_dev = nullptr; c->red_host = nullptr;
 }
@@ -206,11 +211,7 @@ static int dc_prepare(pm_ctx *c) {
     const pm::Grid &g = c->g;
     tb(c, TC_RESIDUAL);
     pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
-    if (c->aos())
-        pm::launch_residual4(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R4, c->stream);
-    else
-        pm::launch_residual(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R[0], c->R[1],
-                            c->R[2], c->R[3], c->stream);
+    pm::launch_residual4(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R4, c->stream);
     te(c);
     RC(fill_f32(c, c->qD[0], g.qtotal(), 0.f));
     RC(fill_f32(c, c->qD[1], g.qtotal(), 0.f));
@@ -235,17 +236,11 @@ static int init_state(pm_ctx *c) {
         for (int q = 0; q < 4; q++)
             HIPCHK(hipMemcpyAsync(c->X2[q], c->X[q], sizeof(double) * (size_t)g.total, hipMemcpyDeviceToDevice,
                                   c->stream));
-    if (c->aos()) {
-        for (int q = 0; q < 2; q++) RC(fill_f32(c, (float *)c->S4[q], 4 * g.total, 0.f));
+    if (c->dc()) {
+        for (int q = 0; q < 2; q++)
+            if (c->S4[q]) RC(fill_f32(c, (float *)c->S4[q], 4 * g.total, 0.f));
         RC(fill_f32(c, c->R4.p, g.total, 0.f));
         RC(fill_f32(c, (float *)c->R4.m, 4 * std::max<int64_t>(1, c->lists.n_all), 0.f));
-        RC(dc_prepare(c));
-    } else if (c->dc()) {
-        for (int q = 0; q < 4; q++) {
-            RC(fill_f32(c, c->D[q], g.total, 0.f));
-            RC(fill_f32(c, c->R[q], g.total, 0.f));
-            if (c->D2[q]) RC(fill_f32(c, c->D2[q], g.total, 0.f));
-        }
         RC(dc_prepare(c));
     }
     c->t = 0;
@@ -290,15 +285,11 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     for (int q = 0; q < 2; q++) RC(dalloc(&c->qX[q], g.qtotal()));
     if (c->fast() && !c->dc() && c->jacobi())
         for (int q = 0; q < 4; q++) RC(dalloc(&c->X2[q], g.total));
-    if (c->aos()) {
-        for (int q = 0; q < 2; q++) RC(dalloc(&c->S4[q], g.total));
+    if (c->dc()) {
+        // interleaved correction state: Jacobi ping-pongs two sets, SOR
+        // updates one in place
+        for (int q = 0; q < (c->jacobi() ? 2 : 1); q++) RC(dalloc(&c->S4[q], g.total));
         RC(dalloc(&c->R4.p, g.total));  // the momentum part follows the fluid list
-        for (int q = 0; q < 2; q++) RC(dalloc(&c->qD[q], g.qtotal()));
-    } else if (c->dc()) {
-        for (int q = 0; q < 4; q++) RC(dalloc(&c->D[q], g.total));
-        for (int q = 0; q < 4; q++) RC(dalloc(&c->R[q], g.total));
-        if (c->jacobi())
-            for (int q = 0; q < 4; q++) RC(dalloc(&c->D2[q], g.total));
         for (int q = 0; q < 2; q++) RC(dalloc(&c->qD[q], g.qtotal()));
     }
     RC(dalloc((uint8_t **)&c->red_scratch, (int64_t)pm::reduce_scratch_bytes()));
@@ -316,7 +307,8 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         // plain (z, y, x) order, 1.05 ms in 64-row slabs of whole rows and
         // 0.96 ms in 64 x 64 columns.  0: whole extent.
         const int ty = c->prm_tile_y > 0 ? std::min(c->prm_tile_y, g.ny) : g.ny;
-        const int tx = c->prm_tile_x > 0 ? std::min(c->prm_tile_x, g.nx) : g.nx;
+        const int want_tx = c->prm_tile_x >= 0 ? c->prm_tile_x : (c->dc() ? 64 : 0);
+        const int tx = want_tx > 0 ? std::min(want_tx, g.nx) : g.nx;
         std::vector<int32_t> all;
         std::vector<int32_t> col[3];
         all.reserve((size_t)fluid);
@@ -334,14 +326,28 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
                     }
                 }
         c->lists.tile_y = c->prm_tile_y > 0 ? ty : 0;
-        c->lists.tile_x = c->prm_tile_x > 0 ? tx : 0;
+        c->lists.tile_x = want_tx > 0 ? tx : 0;
         const char *ld = std::getenv("POREMAPS_SWEEP_LOADS");
         c->lists.select_loads = !(ld && std::strcmp(ld, "branch") == 0);
+        const bool sor = c->prm.mode == PM_MODE_SOR3;
+        if (sor && c->dc()) {
+            // the colour lists back to back: the residual's momentum part,
+            // indexed by list position, then serves every colour pass
+            all.clear();
+            for (int q = 0; q < 3; q++) {
+                c->lists.off_col[q] = (int)all.size();
+                c->lists.n_col[q] = (int)col[q].size();
+                all.insert(all.end(), col[q].begin(), col[q].end());
+            }
+        }
         c->lists.n_all = (int)all.size();
         RC(dalloc(&c->lists.all, std::max<int64_t>(1, c->lists.n_all)));
-        if (c->aos()) RC(dalloc(&c->R4.m, std::max<int64_t>(1, c->lists.n_all)));
+        if (c->dc()) RC(dalloc(&c->R4.m, std::max<int64_t>(1, c->lists.n_all)));
         HIPCHK(hipMemcpy(c->lists.all, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
-        if (c->prm.mode == PM_MODE_SOR3)
+        if (sor && c->dc()) {
+            c->lists.col_in_all = true;
+            for (int q = 0; q < 3; q++) c->lists.col[q] = c->lists.all + c->lists.off_col[q];
+        } else if (sor)
             for (int q = 0; q < 3; q++) {
                 c->lists.n_col[q] = (int)col[q].size();
                 RC(dalloc(&c->lists.col[q], std::max<int64_t>(1, c->lists.n_col[q])));
@@ -388,59 +394,46 @@ static int settle(pm_ctx *c) {
     hipStream_t s = c->stream;
     const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
     const double cu = om * pm::kDt, cp = wp * pm::kC2 * pm::kDt;
-    if (c->aos()) {
+    if (c->dc()) {
         pm::launch_jacobi4(pm::JAC_SETTLE, g, c->lists, c->pc, c->S4[0], c->qD[0], c->S4[0], c->qD[0], c->R4,
                            (float)cu, (float)cp, s);
         pm::launch_ghost4(g, c->G, c->S4[0], c->qD[0], s);
-    } else if (c->dc()) {
-        pm::launch_jacobi_stage<float>(pm::JAC_SETTLE, g, c->lists, c->pc, c->D[0], c->D[1], c->D[2], c->D[3],
-                                       c->qD[0], c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], (float)cu,
-                                       (float)cp, 0.f, c->R[0], c->R[1], c->R[2], c->R[3], s);
-        pm::launch_ghost_fast<float>(g, c->G, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], 0.f, 0.f, 0.f, s);
     } else {
-        pm::launch_jacobi_stage<double>(pm::JAC_SETTLE, g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3],
-                                        c->qX[0], c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0,
-                                        nullptr, nullptr, nullptr, nullptr, s);
-        pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], (double)(g.nz + 2),
+        pm::launch_jacobi_stage(pm::JAC_SETTLE, g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0],
+                                c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0, s);
+        pm::launch_ghost_fast(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], (double)(g.nz + 2),
                                       2.0, 3.0, s);
     }
     HIPCHK(hipGetLastError());
     return PM_OK;
 }
 
-// One Jacobi sweep from set A (p, u, v, w; outlet buffer qa) into set B with
-// the context's kernels: the velocity + pressure pair, or one lagged pass
-// (dense 2.5-D or fluid-list form).
-template <typename T>
-static void jacobi_pass(pm_ctx *c, T *const *A, T *qa, T *const *B, T *qb, T cu, T cp, T pin_out,
-                        float *const *R) {
+// One FP64 Jacobi sweep from set A (p, u, v, w; outlet buffer qa) into set B
+// with the context's kernels: the velocity + pressure pair, or one lagged
+// pass (dense 2.5-D or fluid-list form).
+static void jacobi_pass(pm_ctx *c, double *const *A, double *qa, double *const *B, double *qb, double cu,
+                        double cp, double pin_out) {
     const pm::Grid &g = c->g;
-    const float *Rp = R ? R[0] : nullptr, *Ru = R ? R[1] : nullptr, *Rv = R ? R[2] : nullptr,
-                *Rw = R ? R[3] : nullptr;
     if (c->jac_impl == JI_PAIR) {
         (void)settle(c);  // the pair reads a complete state
-        pm::launch_jacobi<T>(g, c->lists, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3], qb, cu, cp,
-                             pin_out, Rp, Ru, Rv, Rw, c->stream);
+        pm::launch_jacobi(g, c->lists, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3], qb, cu, cp,
+                          pin_out, c->stream);
         return;
     }
     const int st = c->lagged ? pm::JAC_LAG : pm::JAC_FIRST;
     if (c->jac_impl == JI_STREAM)
-        pm::launch_jacobi_stream<T>(st, c->stream_zs, g, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3],
-                                    cu, cp, pin_out, Rp, Ru, Rv, Rw, c->stream);
+        pm::launch_jacobi_stream(st, c->stream_zs, g, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3],
+                                 cu, cp, pin_out, c->stream);
     else
-        pm::launch_jacobi_stage<T>(st, g, c->lists, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3], qb,
-                                   cu, cp, pin_out, Rp, Ru, Rv, Rw, c->stream);
+        pm::launch_jacobi_stage(st, g, c->lists, c->pc, A[0], A[1], A[2], A[3], qa, B[0], B[1], B[2], B[3], qb,
+                                cu, cp, pin_out, c->stream);
     c->lagged = true;
 }
 
 // X += d on the fluid interior, d = 0 (defect correction).
 static void fold_now(pm_ctx *c) {
     const pm::Grid &g = c->g;
-    if (c->aos())
-        pm::launch_fold4(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->S4[0], c->stream);
-    else
-        pm::launch_fold(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->D[0], c->D[1], c->D[2], c->D[3],
-                        c->stream);
+    pm::launch_fold4(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->S4[0], c->stream);
 }
 
 static void apply_variant(pm_ctx *c, const SweepVariant &v) {
@@ -450,8 +443,8 @@ static void apply_variant(pm_ctx *c, const SweepVariant &v) {
 }
 
 static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *buf, size_t len) {
-    if (c->aos() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
-    else if (c->aos()) std::snprintf(buf, len, "interleaved-lagged-%s", v.select_loads ? "eager" : "branch");
+    if (c->dc() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
+    else if (c->dc()) std::snprintf(buf, len, "interleaved-lagged-%s", v.select_loads ? "eager" : "branch");
     else if (v.impl == JI_STREAM) std::snprintf(buf, len, "dense-2.5d-zs%d", v.zs);
     else std::snprintf(buf, len, "%s-%s", v.impl == JI_PAIR ? "list-pair" : "list-lagged",
                        v.select_loads ? "select" : "branch");
@@ -472,7 +465,7 @@ static void tuner_setup(pm_ctx *c) {
     const SweepVariant soa[] = {{JI_STREAM, true, 16}, {JI_PAIR, true, 64}, {JI_PAIR, false, 64},
                                 {JI_LIST, true, 64}, {JI_STREAM, true, 64}, {JI_STREAM, true, 16}};
     const SweepVariant aos[] = {{JI_LIST, true, 64}, {JI_PAIR, true, 64}, {JI_LIST, false, 64}, {JI_LIST, true, 64}};
-    if (c->aos())
+    if (c->dc())
         for (const auto &v : aos) tn.cand[tn.n++] = v;
     else
         for (const auto &v : soa) tn.cand[tn.n++] = v;
@@ -539,10 +532,10 @@ static int one_sweep(pm_ctx *c) {
         if (c->jacobi()) {
             RC(tuner_begin(c));
             tb(c, TC_SWEEP);
-            jacobi_pass<double>(c, c->X, c->qX[0], c->X2, c->qX[1], cu, cp, 3.0, nullptr);
+            jacobi_pass(c, c->X, c->qX[0], c->X2, c->qX[1], cu, cp, 3.0);
             te(c);
             tb(c, TC_GHOST);
-            pm::launch_ghost_fast<double>(g, c->G, c->X2[0], c->X2[1], c->X2[2], c->X2[3], c->qX[1], pin_in, 2.0,
+            pm::launch_ghost_fast(g, c->G, c->X2[0], c->X2[1], c->X2[2], c->X2[3], c->qX[1], pin_in, 2.0,
                                           3.0, s);
             te(c);
             for (int q = 0; q < 4; q++) std::swap(c->X[q], c->X2[q]);
@@ -550,11 +543,10 @@ static int one_sweep(pm_ctx *c) {
             RC(tuner_end(c));
         } else {
             tb(c, TC_SWEEP);
-            pm::launch_colour<double>(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0,
-                                      nullptr, nullptr, nullptr, nullptr, s);
+            pm::launch_colour(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0, s);
             te(c);
             tb(c, TC_GHOST);
-            pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], pin_in, 2.0,
+            pm::launch_ghost_fast(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], pin_in, 2.0,
                                           3.0, s);
             te(c);
         }
@@ -581,18 +573,17 @@ static int one_sweep(pm_ctx *c) {
             RC(tuner_end(c));
         } else {
             tb(c, TC_SWEEP);
-            pm::launch_colour<float>(g, c->lists, c->pc, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], fcu, fcp, 0.f,
-                                     c->R[0], c->R[1], c->R[2], c->R[3], s);
+            pm::launch_colour4(g, c->lists, c->pc, c->S4[0], c->qD[0], c->R4, fcu, fcp, s);
             te(c);
             tb(c, TC_GHOST);
-            pm::launch_ghost_fast<float>(g, c->G, c->D[0], c->D[1], c->D[2], c->D[3], c->qD[0], 0.f, 0.f, 0.f, s);
+            pm::launch_ghost4(g, c->G, c->S4[0], c->qD[0], s);
             te(c);
         }
         if (c->t % c->prm.it_eval == 0) {
             tb(c, TC_FOLD);
             RC(settle(c));
             fold_now(c);
-            pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr,
+            pm::launch_ghost_fast(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr,
                                           (double)(g.nz + 2), 2.0, 3.0, s);
             te(c);
             RC(dc_prepare(c));
@@ -610,7 +601,7 @@ static int dc_flush(pm_ctx *c) {
     if (!c->dc()) return PM_OK;
     const pm::Grid &g = c->g;
     fold_now(c);
-    pm::launch_ghost_fast<double>(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr, (double)(g.nz + 2), 2.0,
+    pm::launch_ghost_fast(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], nullptr, (double)(g.nz + 2), 2.0,
                                   3.0, c->stream);
     RC(dc_prepare(c));
     HIPCHK(hipGetLastError());

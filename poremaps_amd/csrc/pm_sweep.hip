@@ -110,6 +110,16 @@ __global__ __launch_bounds__(kListBlock) void k_jac4(Grid g, const int32_t *__re
     else pres_voxel4(g, c, pc[c], S0, S1, q1, R, cp, 0.f);
 }
 
+// 3-colour SOR on the interleaved state: one colour's cells, positions t0.. of
+// the fluid list (the colour lists lie back to back in it).
+__global__ __launch_bounds__(kListBlock) void k_colour4(Grid g, const int32_t *__restrict__ list, int n, int t0,
+                                                        const uint8_t *__restrict__ pc, float4 *S, float *q,
+                                                        Res4 R, float cu, float cp) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    colour_voxel4(g, list[t0 + t], t0 + t, pc, S, q, R, cu, cp, 0.f);
+}
+
 __global__ __launch_bounds__(kListBlock) void k_residual4(Grid g, const int32_t *__restrict__ list, int n,
                                                           const uint8_t *__restrict__ pc,
                                                           const double *__restrict__ p,
@@ -432,21 +442,6 @@ __global__ __launch_bounds__(kListBlock) void k_colour(Grid g, const int32_t *__
     colour_voxel<T, HAS_R, SEL>(g, c, pc[c], p, u, v, w, q, cu, cp, pin_out, Rp, Ru, Rv, Rw);
 }
 
-__global__ __launch_bounds__(kListBlock) void k_residual(Grid g, const int32_t *__restrict__ list, int n,
-                                                         const uint8_t *__restrict__ pc,
-                                                         const double *__restrict__ p,
-                                                         const double *__restrict__ u,
-                                                         const double *__restrict__ v,
-                                                         const double *__restrict__ w,
-                                                         const double *__restrict__ q, float *__restrict__ Rp,
-                                                         float *__restrict__ Ru, float *__restrict__ Rv,
-                                                         float *__restrict__ Rw) {
-    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
-    if (t >= n) return;
-    const int64_t c = list[t];
-    residual_voxel(g, c, pc[c], p, u, v, w, q, Rp, Ru, Rv, Rw);
-}
-
 template <typename T>
 __global__ void k_ghost_fast(Grid g, const uint8_t *__restrict__ G, T *p, T *u, T *v, T *w, T *q, T pin_in,
                              T pin_hi, T pin_out) {
@@ -463,13 +458,6 @@ __global__ void k_outlet_plane(Grid g, const uint8_t *__restrict__ pc, const dou
     outlet_cell(g, j, k, pc, p, u, v, w, pin_out, q);
 }
 
-__global__ void k_fold(Grid g, const uint8_t *__restrict__ pc, double *Xp, double *Xu, double *Xv, double *Xw,
-                       float *dp, float *du, float *dv, float *dw) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.total) return;
-    fold_cell(g, c, pc, Xp, Xu, Xv, Xw, dp, du, dv, dw);
-}
-
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -477,90 +465,55 @@ __global__ void k_fold(Grid g, const uint8_t *__restrict__ pc, double *Xp, doubl
 static inline unsigned lblocks(int n) { return (unsigned)((n + kListBlock - 1) / kListBlock); }
 static inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-template <typename T, bool HAS_R, bool SEL>
-static void jacobi_pair(const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0, const T *u0,
-                        const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu, T cp,
-                        T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
-                        hipStream_t s) {
-    hipLaunchKernelGGL((k_jac_vel<T, HAS_R, SEL>), dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all,
-                       L.n_all, pc, p0, u0, v0, w0, q0, u1, v1, w1, Ru, Rv, Rw, cu);
-    hipLaunchKernelGGL((k_jac_pres<T, HAS_R>), dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all,
-                       pc, p0, u1, v1, w1, p1, q1, Rp, cp, pin_out);
+// The FP64 fast modes' launchers (per-field state, no residual); the
+// kernels above are generic in the value type and the residual term.
+template <bool SEL>
+static void jacobi_pair(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0, const double *u0,
+                        const double *v0, const double *w0, const double *q0, double *p1, double *u1, double *v1,
+                        double *w1, double *q1, double cu, double cp, double pin_out, hipStream_t s) {
+    hipLaunchKernelGGL((k_jac_vel<double, false, SEL>), dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all,
+                       L.n_all, pc, p0, u0, v0, w0, q0, u1, v1, w1, nullptr, nullptr, nullptr, cu);
+    hipLaunchKernelGGL((k_jac_pres<double, false>), dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all,
+                       L.n_all, pc, p0, u1, v1, w1, p1, q1, nullptr, cp, pin_out);
 }
 
-template <typename T>
-void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0, const T *u0,
-                   const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1, T cu, T cp,
-                   T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
-                   hipStream_t s) {
+void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0, const double *u0,
+                   const double *v0, const double *w0, const double *q0, double *p1, double *u1, double *v1,
+                   double *w1, double *q1, double cu, double cp, double pin_out, hipStream_t s) {
     if (L.n_all == 0) return;
-    if (Rp && L.select_loads)
-        jacobi_pair<T, true, true>(g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, Rp, Ru, Rv,
-                                   Rw, s);
-    else if (Rp)
-        jacobi_pair<T, true, false>(g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, Rp, Ru, Rv,
-                                    Rw, s);
-    else if (L.select_loads)
-        jacobi_pair<T, false, true>(g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, Rp, Ru,
-                                    Rv, Rw, s);
+    if (L.select_loads)
+        jacobi_pair<true>(g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, s);
     else
-        jacobi_pair<T, false, false>(g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, Rp, Ru,
-                                     Rv, Rw, s);
+        jacobi_pair<false>(g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, s);
 }
-template void launch_jacobi<double>(const Grid &, const FluidLists &, const uint8_t *, const double *,
-                                    const double *, const double *, const double *, const double *, double *,
-                                    double *, double *, double *, double *, double, double, double,
-                                    const float *, const float *, const float *, const float *, hipStream_t);
-template void launch_jacobi<float>(const Grid &, const FluidLists &, const uint8_t *, const float *,
-                                   const float *, const float *, const float *, const float *, float *, float *,
-                                   float *, float *, float *, float, float, float, const float *, const float *,
-                                   const float *, const float *, hipStream_t);
 
-template <typename T, bool HAS_R, bool SEL>
-static void jacobi_stage_t(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0,
-                           const T *u0, const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1,
-                           T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv,
-                           const float *Rw, hipStream_t s) {
+template <bool SEL>
+static void jacobi_stage_t(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0,
+                           const double *u0, const double *v0, const double *w0, const double *q0, double *p1,
+                           double *u1, double *v1, double *w1, double *q1, double cu, double cp, double pin_out,
+                           hipStream_t s) {
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
     if (stage == JAC_FIRST)
-        hipLaunchKernelGGL((k_jac_first<T, HAS_R, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0,
-                           q0, p1, u1, v1, w1, Ru, Rv, Rw, cu);
+        hipLaunchKernelGGL((k_jac_first<double, false, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0,
+                           w0, q0, p1, u1, v1, w1, nullptr, nullptr, nullptr, cu);
     else if (stage == JAC_LAG)
-        hipLaunchKernelGGL((k_jac_lag<T, HAS_R, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, q0,
-                           p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
+        hipLaunchKernelGGL((k_jac_lag<double, false, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0,
+                           q0, p1, u1, v1, w1, nullptr, nullptr, nullptr, nullptr, cu, cp, pin_out);
     else
-        hipLaunchKernelGGL((k_jac_pres<T, HAS_R>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, p1,
-                           q1, Rp, cp, pin_out);
+        hipLaunchKernelGGL((k_jac_pres<double, false>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0,
+                           p1, q1, nullptr, cp, pin_out);
 }
 
-template <typename T>
-void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const T *p0,
-                         const T *u0, const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T *q1,
-                         T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
+void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0,
+                         const double *u0, const double *v0, const double *w0, const double *q0, double *p1,
+                         double *u1, double *v1, double *w1, double *q1, double cu, double cp, double pin_out,
                          hipStream_t s) {
     if (L.n_all == 0) return;
-    if (Rp && L.select_loads)
-        jacobi_stage_t<T, true, true>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, Rp,
-                                      Ru, Rv, Rw, s);
-    else if (Rp)
-        jacobi_stage_t<T, true, false>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out,
-                                       Rp, Ru, Rv, Rw, s);
-    else if (L.select_loads)
-        jacobi_stage_t<T, false, true>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out,
-                                       Rp, Ru, Rv, Rw, s);
+    if (L.select_loads)
+        jacobi_stage_t<true>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, s);
     else
-        jacobi_stage_t<T, false, false>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out,
-                                        Rp, Ru, Rv, Rw, s);
+        jacobi_stage_t<false>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, s);
 }
-template void launch_jacobi_stage<double>(int, const Grid &, const FluidLists &, const uint8_t *, const double *,
-                                          const double *, const double *, const double *, const double *,
-                                          double *, double *, double *, double *, double *, double, double, double,
-                                          const float *, const float *, const float *, const float *,
-                                          hipStream_t);
-template void launch_jacobi_stage<float>(int, const Grid &, const FluidLists &, const uint8_t *, const float *,
-                                         const float *, const float *, const float *, const float *, float *,
-                                         float *, float *, float *, float *, float, float, float, const float *,
-                                         const float *, const float *, const float *, hipStream_t);
 
 void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s) {
@@ -580,6 +533,16 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
     }
 }
 
+void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
+                    float cp, hipStream_t s) {
+    for (int col = 0; col < 3; col++) {
+        const int n = L.n_col[col];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_colour4, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.all, n, L.off_col[col], pc, S,
+                           q, R, cu, cp);
+    }
+}
+
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
                       const double *v, const double *w, const double *q, Res4 R, hipStream_t s) {
     if (L.n_all == 0) return;
@@ -596,100 +559,50 @@ void launch_fold4(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, doub
     hipLaunchKernelGGL(k_fold4, dim3(nblocks(g.total, 256)), dim3(256), 0, s, g, pc, Xp, Xu, Xv, Xw, d);
 }
 
-template <typename T>
-void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const T *p0, const T *u0,
-                          const T *v0, const T *w0, const T *q0, T *p1, T *u1, T *v1, T *w1, T cu, T cp, T pin_out,
-                          const float *Rp, const float *Ru, const float *Rv, const float *Rw, hipStream_t s) {
+void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const double *p0, const double *u0,
+                          const double *v0, const double *w0, const double *q0, double *p1, double *u1, double *v1,
+                          double *w1, double cu, double cp, double pin_out, hipStream_t s) {
     const int ntx = (g.nx + kSX - 1) / kSX, nty = (g.ny + kSY - 1) / kSY;
     zs = std::max(1, zs);
     const int nseg = (g.nz + zs - 1) / zs;
     const dim3 grid((unsigned)(ntx * nty * nseg)), blk(kSN);
-    if (Rp) {
-        if (stage == JAC_FIRST)
-            hipLaunchKernelGGL((k_jac_stream<T, true, JAC_FIRST>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0,
-                               w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
-        else
-            hipLaunchKernelGGL((k_jac_stream<T, true, JAC_LAG>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0,
-                               w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
-    } else {
-        if (stage == JAC_FIRST)
-            hipLaunchKernelGGL((k_jac_stream<T, false, JAC_FIRST>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0,
-                               v0, w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
-        else
-            hipLaunchKernelGGL((k_jac_stream<T, false, JAC_LAG>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0,
-                               w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp, pin_out);
-    }
-}
-template void launch_jacobi_stream<double>(int, int, const Grid &, const uint8_t *, const double *, const double *,
-                                           const double *, const double *, const double *, double *, double *,
-                                           double *, double *, double, double, double, const float *,
-                                           const float *, const float *, const float *, hipStream_t);
-template void launch_jacobi_stream<float>(int, int, const Grid &, const uint8_t *, const float *, const float *,
-                                          const float *, const float *, const float *, float *, float *, float *,
-                                          float *, float, float, float, const float *, const float *,
-                                          const float *, const float *, hipStream_t);
-
-template <typename T, bool HAS_R, bool SEL>
-static void colour_stage(const Grid &g, const int32_t *list, int n, const uint8_t *pc, T *p, T *u, T *v, T *w,
-                         T *q, T cu, T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv,
-                         const float *Rw, hipStream_t s) {
-    hipLaunchKernelGGL((k_colour<T, HAS_R, SEL>), dim3(lblocks(n)), dim3(kListBlock), 0, s, g, list, n, pc, p, u,
-                       v, w, q, cu, cp, pin_out, Rp, Ru, Rv, Rw);
+    if (stage == JAC_FIRST)
+        hipLaunchKernelGGL((k_jac_stream<double, false, JAC_FIRST>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0,
+                           v0, w0, q0, p1, u1, v1, w1, nullptr, nullptr, nullptr, nullptr, cu, cp, pin_out);
+    else
+        hipLaunchKernelGGL((k_jac_stream<double, false, JAC_LAG>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0,
+                           w0, q0, p1, u1, v1, w1, nullptr, nullptr, nullptr, nullptr, cu, cp, pin_out);
 }
 
-template <typename T>
-void launch_colour(const Grid &g, const FluidLists &L, const uint8_t *pc, T *p, T *u, T *v, T *w, T *q, T cu,
-                   T cp, T pin_out, const float *Rp, const float *Ru, const float *Rv, const float *Rw,
-                   hipStream_t s) {
+template <bool SEL>
+static void colour_stage(const Grid &g, const int32_t *list, int n, const uint8_t *pc, double *p, double *u,
+                         double *v, double *w, double *q, double cu, double cp, double pin_out, hipStream_t s) {
+    hipLaunchKernelGGL((k_colour<double, false, SEL>), dim3(lblocks(n)), dim3(kListBlock), 0, s, g, list, n, pc, p,
+                       u, v, w, q, cu, cp, pin_out, nullptr, nullptr, nullptr, nullptr);
+}
+
+void launch_colour(const Grid &g, const FluidLists &L, const uint8_t *pc, double *p, double *u, double *v, double *w,
+                   double *q, double cu, double cp, double pin_out, hipStream_t s) {
     for (int col = 0; col < 3; col++) {
         const int n = L.n_col[col];
         if (n == 0) continue;
-        if (Rp && L.select_loads)
-            colour_stage<T, true, true>(g, L.col[col], n, pc, p, u, v, w, q, cu, cp, pin_out, Rp, Ru, Rv, Rw, s);
-        else if (Rp)
-            colour_stage<T, true, false>(g, L.col[col], n, pc, p, u, v, w, q, cu, cp, pin_out, Rp, Ru, Rv, Rw, s);
-        else if (L.select_loads)
-            colour_stage<T, false, true>(g, L.col[col], n, pc, p, u, v, w, q, cu, cp, pin_out, Rp, Ru, Rv, Rw, s);
+        if (L.select_loads)
+            colour_stage<true>(g, L.col[col], n, pc, p, u, v, w, q, cu, cp, pin_out, s);
         else
-            colour_stage<T, false, false>(g, L.col[col], n, pc, p, u, v, w, q, cu, cp, pin_out, Rp, Ru, Rv, Rw, s);
+            colour_stage<false>(g, L.col[col], n, pc, p, u, v, w, q, cu, cp, pin_out, s);
     }
 }
-template void launch_colour<double>(const Grid &, const FluidLists &, const uint8_t *, double *, double *,
-                                    double *, double *, double *, double, double, double, const float *,
-                                    const float *, const float *, const float *, hipStream_t);
-template void launch_colour<float>(const Grid &, const FluidLists &, const uint8_t *, float *, float *, float *,
-                                   float *, float *, float, float, float, const float *, const float *,
-                                   const float *, const float *, hipStream_t);
 
-void launch_residual(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
-                     const double *v, const double *w, const double *q, float *Rp, float *Ru, float *Rv,
-                     float *Rw, hipStream_t s) {
-    if (L.n_all == 0) return;
-    hipLaunchKernelGGL(k_residual, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc, p, u, v,
-                       w, q, Rp, Ru, Rv, Rw);
+void launch_ghost_fast(const Grid &g, const uint8_t *G, double *p, double *u, double *v, double *w, double *q,
+                       double pin_in, double pin_hi, double pin_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_ghost_fast<double>, dim3(nblocks(ghost_lines(g), 256)), dim3(256), 0, s, g, G, p, u, v, w,
+                       q, pin_in, pin_hi, pin_out);
 }
-
-template <typename T>
-void launch_ghost_fast(const Grid &g, const uint8_t *G, T *p, T *u, T *v, T *w, T *q, T pin_in, T pin_hi,
-                       T pin_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_ghost_fast<T>, dim3(nblocks(ghost_lines(g), 256)), dim3(256), 0, s, g, G, p, u, v, w, q,
-                       pin_in, pin_hi, pin_out);
-}
-template void launch_ghost_fast<double>(const Grid &, const uint8_t *, double *, double *, double *, double *,
-                                        double *, double, double, double, hipStream_t);
-template void launch_ghost_fast<float>(const Grid &, const uint8_t *, float *, float *, float *, float *,
-                                       float *, float, float, float, hipStream_t);
 
 void launch_outlet_plane(const Grid &g, const uint8_t *pc, const double *p, const double *u, const double *v,
                          const double *w, double pin_out, double *q, hipStream_t s) {
     hipLaunchKernelGGL(k_outlet_plane, dim3(nblocks(g.NX, 64), (unsigned)g.NY), dim3(64), 0, s, g, pc, p, u, v,
                        w, pin_out, q);
-}
-
-void launch_fold(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw, float *dp,
-                 float *du, float *dv, float *dw, hipStream_t s) {
-    hipLaunchKernelGGL(k_fold, dim3(nblocks(g.total, 256)), dim3(256), 0, s, g, pc, Xp, Xu, Xv, Xw, dp, du, dv,
-                       dw);
 }
 
 }  // namespace pm

@@ -436,6 +436,42 @@ __device__ __forceinline__ void lag_voxel4_eager(const Grid &g, int64_t c, int t
     __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c]));
 }
 
+// One 3-colour SOR update at fluid cell c on the interleaved state, in place
+// (colour_voxel): velocities from the current state, then the pressure from
+// the new velocities at c and the current ones of its +1 neighbours.  Every
+// stencil neighbour has another colour, so nothing read here is written by the
+// same launch.  t is the cell's position in the fluid list the residual's
+// momentum part follows.  All loads are issued before any condition.
+__device__ __forceinline__ void colour_voxel4(const Grid &g, int64_t c, int t, const uint8_t *__restrict__ pc,
+                                              float4 *S, float *q, Res4 R, float cu, float cp, float pin_out) {
+    const uint8_t pcv = pc[c];
+    const float4 C = S[c];
+    const Star4 s = load_star4(S, c, g.sx, g.sy);
+    const float rp = R.p[c];
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
+    const float4 Rm = make_float4(r.x, r.y, r.z, r.w);
+    const bool out = on_outlet(g, c);
+    const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
+    float pxm = s.XM.x, pym = s.YM.x;
+    if (out) { pxm = q[qc - 1]; pym = q[qc - g.sx]; }
+    const Cases cs = decode(pcv);
+    const float dpx = cs.bx ? 0.f : C.x - pxm;
+    const float dpy = cs.by ? 0.f : C.x - pym;
+    const float dpz = cs.bz ? 0.f : C.x - s.ZM.x;
+    float su, sv, sw;
+    visc_from4(cs, s, C, su, sv, sw);
+    float4 N = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rm, C.x, cu);
+    const float wz = fwd_solid(cs.cz) ? 0.f : s.ZP.w;
+    const float vy = fwd_solid(cs.cy) ? 0.f : s.YP.z;
+    const float ux = fwd_solid(cs.cx) ? 0.f : s.XP.y;
+    const float div0 = ((((wz - N.w) + vy) - N.z) + ux) - N.y;
+    const float div = div0 + rp;
+    N.x = C.x - div * cp;
+    S[c] = N;
+    if (out) q[qc] = pin_out - div0 * (float)(kC2 * kDt);
+}
+
 // Residual of the FP64 state at c into the interleaved FP32 residual.
 __device__ __forceinline__ void residual_voxel4(const Grid &g, int64_t c, int t, uint8_t pcv,
                                                 const double *__restrict__ p, const double *__restrict__ u,
@@ -557,26 +593,6 @@ __device__ __forceinline__ void colour_voxel(const Grid &g, int64_t c, uint8_t p
     if (out) q[qc] = pin_out - div0 * (T)(kC2 * kDt);
 }
 
-// Momentum and continuity residual of a FP64 state at c, rounded to FP32.
-__device__ __forceinline__ void residual_voxel(const Grid &g, int64_t c, uint8_t pcv,
-                                               const double *__restrict__ p, const double *__restrict__ u,
-                                               const double *__restrict__ v, const double *__restrict__ w,
-                                               const double *__restrict__ q, float *__restrict__ Rp,
-                                               float *__restrict__ Ru, float *__restrict__ Rv,
-                                               float *__restrict__ Rw) {
-    const Cases cs = decode(pcv);
-    const bool out = on_outlet(g, c);
-    double fu, fv, fw;
-    fast_force_p<double>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, c - (int64_t)(g.nz + 1) * g.sy, fu,
-                         fv, fw);
-    Ru[c] = (float)fu; Rv[c] = (float)fv; Rw[c] = (float)fw;
-    const double wz = fwd_solid(cs.cz) ? 0.0 : w[c + g.sy];
-    const double vy = fwd_solid(cs.cy) ? 0.0 : v[c + g.sx];
-    const double ux = fwd_solid(cs.cx) ? 0.0 : u[c + 1];
-    const double div = ((((wz - w[c]) + vy) - v[c]) + ux) - u[c];
-    Rp[c] = (float)div;
-}
-
 // Number of halo lines the fast-mode ghost fill walks (x-face, y-face and
 // z-face lines of the interior).
 __host__ __device__ inline int64_t ghost_lines(const Grid &g) {
@@ -658,28 +674,6 @@ __device__ __forceinline__ void outlet_cell(const Grid &g, int j, int k, const u
         val = p[c];
     }
     q[g.qat(j, k)] = val;
-}
-
-// Defect-correction fold at element c of the padded array: X += d on the
-// fluid interior, d = 0 everywhere.
-__device__ __forceinline__ void fold_cell(const Grid &g, int64_t c, const uint8_t *__restrict__ pc, double *Xp,
-                                          double *Xu, double *Xv, double *Xw, float *dp, float *du, float *dv,
-                                          float *dw) {
-    const int64_t e = c - g.base;
-    bool in = false;
-    if (e >= 0) {
-        const int i = (int)(e / g.sy);
-        const int64_t r = e % g.sy;
-        const int j = (int)(r / g.sx), k = (int)(r % g.sx);
-        in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
-    }
-    if (in && pc[c] != kSolid) {
-        Xp[c] = Xp[c] + (double)dp[c];
-        Xu[c] = Xu[c] + (double)du[c];
-        Xv[c] = Xv[c] + (double)dv[c];
-        Xw[c] = Xw[c] + (double)dw[c];
-    }
-    dp[c] = 0.f; du[c] = 0.f; dv[c] = 0.f; dw[c] = 0.f;
 }
 
 }  // namespace pm
