@@ -75,6 +75,9 @@ struct pm_ctx {
     // lagged pass 0.96 ms against 1.05 ms with whole rows and 1.43 ms in plain
     // (z, y, x) order) and 64 rows of whole rows for the per-field FP64 state
     // (sweep 1.93 ms against 2.09 ms in 64 x 64 columns)
+    // exact-order sweep: LDS-tiled wavefront (default) or the single-workgroup
+    // / per-hyperplane forms (POREMAPS_REF=tiles|onewg|planes)
+    int ref_impl = 0;
     int prm_tile_y = 64;
     int prm_tile_x = -1;
     // Jacobi kernels: velocity + pressure pair over the fluid list, one lagged

@@ -31,9 +31,12 @@ void launch_init_fields(const Grid &g, const uint8_t *G, double *p, double *u, d
 template <typename T>
 void launch_pins(const Grid &g, const uint8_t *G, T *p, T pin_in, T pin_hi, T pin_out, hipStream_t s);
 
-// Reference order: `nsweeps` complete sweeps (hyperplanes, exchange, pins).
+// Reference order: `nsweeps` complete sweeps (cells in lexicographic order,
+// exchange, pins), as a wavefront of LDS tiles (REF_TILES), in one workgroup
+// (REF_ONE_WG) or one launch per hyperplane (REF_PLANES); the same bits.
+enum { REF_TILES = 0, REF_ONE_WG = 1, REF_PLANES = 2 };
 void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
-                       double *v, double *w, int solver, int nsweeps, bool one_wg, hipStream_t s);
+                       double *v, double *w, int solver, int nsweeps, int impl, hipStream_t s);
 
 // Device lists of the interior fluid cells' element offsets in column order
 // (columns of tile_y rows x tile_x cells, each marching along z; within a

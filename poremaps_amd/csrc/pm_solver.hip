@@ -140,6 +140,11 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     c->g = pm::make_grid(p->nx, p->ny, p->nz, p->boundary_method);
     if (const char *ty = std::getenv("POREMAPS_LIST_TILE")) c->prm_tile_y = std::max(0, std::atoi(ty));
     if (const char *tx = std::getenv("POREMAPS_LIST_TILE_X")) c->prm_tile_x = std::max(0, std::atoi(tx));
+    if (const char *ri = std::getenv("POREMAPS_REF")) {
+        if (std::strcmp(ri, "tiles") == 0) c->ref_impl = pm::REF_TILES;
+        else if (std::strcmp(ri, "onewg") == 0) c->ref_impl = pm::REF_ONE_WG;
+        else if (std::strcmp(ri, "planes") == 0) c->ref_impl = pm::REF_PLANES;
+    }
     if (const char *jm = std::getenv("POREMAPS_JACOBI")) {
         if (std::strcmp(jm, "pair") == 0) c->jac_impl = JI_PAIR;
         else if (std::strcmp(jm, "list") == 0) c->jac_impl = JI_LIST;
@@ -522,10 +527,9 @@ static int one_sweep(pm_ctx *c) {
     const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
     const double cu = om * pm::kDt, cp = wp * pm::kC2 * pm::kDt;
     if (!c->fast()) {
-        const bool one_wg = (int64_t)g.nx * g.ny * g.nz <= 400000;
         tb(c, TC_REF);
         pm::launch_ref_sweeps(g, c->G, c->code, c->X[0], c->X[1], c->X[2], c->X[3], c->prm.solving_algorithm,
-                              1, one_wg, s);
+                              1, c->ref_impl, s);
         te(c);
     } else if (!c->dc()) {
         const double pin_in = (double)(g.nz + 2);

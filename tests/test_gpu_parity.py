@@ -6,6 +6,7 @@ sweeps; the reductions are compensated parallel sums, compared to 1e-12
 relative and on the log's printed digits.  Full-size runs are checked through
 sweep counts and k against the reference binary's known answers.
 """
+import contextlib
 import os
 import subprocess
 import sys
@@ -92,12 +93,46 @@ def test_reference_sweeps_bit_exact(solver, bc):
                 assert np.array_equal(a, b), f"{name} differs after {done} sweeps"
 
 
-def test_reference_sweeps_multi_launch_path_bit_exact():
-    # above the single-workgroup size limit: one launch per hyperplane
+@contextlib.contextmanager
+def ref_form(form):
+    """The exact-order sweep's kernel form for solvers created inside."""
+    old = os.environ.get("POREMAPS_REF")
+    os.environ["POREMAPS_REF"] = form
+    try:
+        yield
+    finally:
+        if old is None:
+            os.environ.pop("POREMAPS_REF", None)
+        else:
+            os.environ["POREMAPS_REF"] = old
+
+
+@pytest.mark.parametrize("form", ["onewg", "planes"])
+@pytest.mark.parametrize("solver,bc", [(2, 0), (3, 2), (1, 4)])
+def test_reference_sweep_forms_bit_exact(form, solver, bc):
+    # the default LDS-tiled wavefront is covered above; the single-workgroup
+    # and per-hyperplane forms must give the same bits
+    raw = random_geometry((13, 9, 11), 0.4, 7 * bc + solver)
+    say("reference form", form, solver, bc)
+    G, code, st = oracle_state(raw, bc)
+    with ref_form(form), pm.Solver(raw, boundary_method=bc, solving_algorithm=solver) as s:
+        for _ in range(7):
+            O.sweep_ref(solver, G, code, *st)
+            for f in st:
+                O.exchange(f, bc)
+            O.pins(G, st[0])
+        s.step(7)
+        for name, a, b in zip("puvw", s.state_halo(), st):
+            assert np.array_equal(a, b), f"{name} differs ({form})"
+
+
+@pytest.mark.parametrize("form", ["tiles", "planes"])
+def test_reference_sweeps_large_domain_bit_exact(form):
+    # many tiles per diagonal, partial tiles on every axis (60 x 70 x 98)
     raw = random_geometry((60, 70, 98), 0.35, 5)
-    say("reference multi-launch", raw.shape)
+    say("reference large domain", form, raw.shape)
     G, code, st = oracle_state(raw, 0)
-    with pm.Solver(raw, boundary_method=0, solving_algorithm=2) as s:
+    with ref_form(form), pm.Solver(raw, boundary_method=0, solving_algorithm=2) as s:
         for _ in range(2):
             O.sweep_ref(2, G, code, *st)
             for f in st:
