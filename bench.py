@@ -72,7 +72,12 @@ def workload(cfg: int):
 
 
 def cpu_baseline(g, bc, budget_s=20.0):
-    """Reference-order FP64 sweeps of the CPU oracle on this host (1 thread)."""
+    """Reference-order FP64 sweeps (solving_algorithm 2) of the CPU oracle on
+    this host: the interior cut into z-slabs, one thread each, the way
+    `mpirun -np P` with dom_decomposition 1 1 P runs the reference, on all
+    host cores (at most 16, the GPU box's share); the rate is the difference
+    of a 1-sweep and an n-sweep run, so that copying the state into the slabs
+    does not count.  The one-thread rate of the serial sweep is kept beside."""
     import oracle as O
 
     G = O.geom_halo(g, bc)
@@ -80,6 +85,24 @@ def cpu_baseline(g, bc, budget_s=20.0):
     st = [a.copy() for a in O.init_fields(G, bc)]
     O.pins(G, st[0])
     n = g.size
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    P = max(1, min(cores, 16))
+
+    def run(k):
+        t0 = time.perf_counter()
+        O.sweeps_ref_slabs(2, G, code, *st, bc, k, P)
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    t3 = run(3)
+    per = max((t3 - t1) / 2, 1e-3)
+    k = int(max(4, min(400, (budget_s * 0.6) / per)))
+    tk = run(k)
+    rate = n * (k - 1) / (tk - t1) / 1e9
+    # one thread, serial sweep
     done, t0 = 0, time.perf_counter()
     while True:
         O.sweep_ref(2, G, code, *st)
@@ -88,11 +111,14 @@ def cpu_baseline(g, bc, budget_s=20.0):
         O.pins(G, st[0])
         done += 1
         el = time.perf_counter() - t0
-        if el > budget_s * 0.5 or done >= 50:
+        if el > budget_s * 0.25 or done >= 50:
             break
-    return {"value": round(n * done / el / 1e9, 5), "unit": "GLUP/s", "cores": 1, "kind": "port",
-            "sample": f"{done} reference-order (solving_algorithm 2) FP64 sweeps of the same "
-                      f"{g.shape[2]}x{g.shape[1]}x{g.shape[0]} field, CPU oracle, 1 thread, {el:.1f} s"}
+    return {"value": round(rate, 4), "unit": "GLUP/s", "cores": P, "kind": "port",
+            "sample": f"{k} minus 1 reference-order (solving_algorithm 2) FP64 sweeps of the same "
+                      f"{g.shape[2]}x{g.shape[1]}x{g.shape[0]} field, CPU oracle in {P} z-slabs on {P} threads "
+                      f"(as mpirun -np {P}, dom_decomposition 1 1 {P}), {tk - t1:.1f} s",
+            "value_1core": round(n * done / el / 1e9, 5),
+            "sample_1core": f"{done} serial sweeps on 1 thread, {el:.1f} s"}
 
 
 def time_kernels(s, n):

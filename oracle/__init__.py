@@ -75,6 +75,8 @@ def _declare(L):
     L.orc_exchange_f64.argtypes = [_f64p, _I, _I, _I, _I]
     L.orc_pins.argtypes = [_u8p, _I, _I, _I, _f64p]
     L.orc_sweep_ref.argtypes = [_I, _u8p, _i32p, _I, _I, _I, _f64p, _f64p, _f64p, _f64p]
+    L.orc_sweeps_ref_slabs.argtypes = [_I, _u8p, _i32p, _I, _I, _I, _I, _f64p, _f64p, _f64p, _f64p, _I, _I]
+    L.orc_sweeps_ref_slabs.restype = _I
     L.orc_convergence.argtypes = [_u8p, _f64p, _I, _I, _I, C.POINTER(C.c_double)]
     L.orc_convergence.restype = _D
     L.orc_permeability.argtypes = [_u8p, _f64p, _f64p, _f64p, _f64p, _I, _I, _I,
@@ -145,6 +147,14 @@ def pins(G: np.ndarray, p: np.ndarray) -> None:
 
 def sweep_ref(solver: int, G, code, p, u, v, w) -> None:
     lib().orc_sweep_ref(solver, G, code, *_dims(G), p, u, v, w)
+
+
+def sweeps_ref_slabs(solver: int, G, code, p, u, v, w, bc: int, nsweeps: int, threads: int) -> None:
+    """nsweeps reference sweeps, exchanges and pins with the interior cut into
+    `threads` z-slabs (one thread each), as mpirun -np P with
+    dom_decomposition 1 1 P runs them; threads = 1 is the serial sweep."""
+    if lib().orc_sweeps_ref_slabs(solver, G, code, *_dims(G), bc, p, u, v, w, nsweeps, threads) != 0:
+        raise MemoryError("orc_sweeps_ref_slabs: allocation failed")
 
 
 def convergence(G, w, kprev: float):

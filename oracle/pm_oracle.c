@@ -19,6 +19,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* src/constants.h:51-64 */
 #define ORC_RE 0.01
@@ -340,6 +343,117 @@ void orc_permeability(const uint8_t *G, const double *u, const double *v,
 }
 
 /* src/calc_flow.cc:82-165 at one rank. */
+/* The reference's sweep with the interior cut into P z-slabs, the way
+ * `mpirun -np P` runs it with dom_decomposition 1 1 P
+ * (src/parallelization.cc:193-318, src/boundary_conditions.cc:46-78): every
+ * slab sweeps its own planes in lexicographic order (orc_sweep_ref) against
+ * its own frozen copies of its neighbours' planes, then copies its x / y
+ * periodic halos, then takes its 2-plane z halos from the neighbouring slabs'
+ * edge planes (from the last / first slab across the periodic z boundary),
+ * then the slabs owning the pinned planes apply the pins.  One OpenMP thread
+ * per slab; slab k holds planes [z0, z0 + n) of the interior, the first
+ * nz % P slabs one plane more.  Geometry and codes are read in place (a slab's
+ * rows of the halo-including arrays); the state is copied into the slabs and
+ * back.  P = 1 gives orc_sweep_ref + exchange + pins exactly.  Used to time
+ * the CPU baseline on all host cores. */
+int orc_sweeps_ref_slabs(int solver, const uint8_t *G, const int32_t *code, int nx, int ny, int nz, int bc,
+                         double *p, double *u, double *v, double *w, int nsweeps, int P)
+{
+    const int NX = HX(nx), NY = HY(ny);
+    const size_t plane = (size_t)NX * NY;
+    int px, py, pz;
+    periodic_axes(bc, &px, &py, &pz);
+    if (P < 1) P = 1;
+    if (P > nz) P = nz;
+    int *z0 = malloc(sizeof(int) * (size_t)P), *nk = malloc(sizeof(int) * (size_t)P);
+    double **S = calloc((size_t)P * 4, sizeof(double *));
+    if (!z0 || !nk || !S) { free(z0); free(nk); free(S); return -1; }
+    int fail = 0;
+    for (int k = 0, z = 0; k < P; k++) {
+        nk[k] = nz / P + (k < nz % P ? 1 : 0);
+        z0[k] = z;
+        z += nk[k];
+        for (int q = 0; q < 4 && !fail; q++) {
+            S[4 * k + q] = malloc(sizeof(double) * plane * (size_t)(nk[k] + 4));
+            if (!S[4 * k + q]) fail = 1;
+        }
+    }
+    if (!fail) {
+        double *glob[4] = {p, u, v, w};
+#pragma omp parallel num_threads(P)
+        {
+#ifdef _OPENMP
+            const int tid = omp_get_thread_num(), nth = omp_get_num_threads();
+#else
+            const int tid = 0, nth = 1;
+#endif
+            /* every phase: the slabs tid, tid + nth, ... (one each when the
+             * runtime grants P threads) */
+            for (int k = tid; k < P; k += nth)
+                for (int q = 0; q < 4; q++)
+                    memcpy(S[4 * k + q], glob[q] + plane * (size_t)z0[k],
+                           sizeof(double) * plane * (size_t)(nk[k] + 4));
+#pragma omp barrier
+            for (int it = 0; it < nsweeps; it++) {
+                for (int k = tid; k < P; k += nth) {
+                    double **f = S + 4 * k;
+                    orc_sweep_ref(solver, G + plane * (size_t)z0[k], code + plane * (size_t)z0[k], nx, ny, nk[k],
+                                  f[0], f[1], f[2], f[3]);
+                    for (int q = 0; q < 4; q++) exchange_f64(f[q], nx, ny, nk[k], px, py, 0);
+                }
+#pragma omp barrier
+                /* z halos: planes 0, 1 from the slab below (its planes n, n+1),
+                 * planes n+2, n+3 from the slab above (its planes 2, 3) */
+                for (int k = tid; k < P; k += nth) {
+                    double **f = S + 4 * k;
+                    const int below = k > 0 ? k - 1 : (pz ? P - 1 : -1);
+                    const int above = k < P - 1 ? k + 1 : (pz ? 0 : -1);
+                    for (int q = 0; q < 4; q++) {
+                        if (below >= 0)
+                            memcpy(f[q], S[4 * below + q] + plane * (size_t)nk[below], sizeof(double) * plane * 2);
+                        if (above >= 0)
+                            memcpy(f[q] + plane * (size_t)(nk[k] + 2), S[4 * above + q] + plane * 2,
+                                   sizeof(double) * plane * 2);
+                    }
+                }
+#pragma omp barrier
+                /* pins: global plane 1 (slab 0's plane 1), planes nz+1 and
+                 * nz+2 (the last slab's planes n+1 and n+2) */
+                for (int k = tid; k < P; k += nth) {
+                    const uint8_t *Gk = G + plane * (size_t)z0[k];
+                    double *pk = S[4 * k];
+                    for (int j = 2; j < NY - 2; j++)
+                        for (int x = 2; x < NX - 2; x++) {
+                            const size_t r = (size_t)j * NX + x;
+                            if (k == 0 && Gk[plane + r] == 0) pk[plane + r] = (double)(nz + 2);
+                            if (k == P - 1) {
+                                const size_t hi = plane * (size_t)(nk[k] + 2) + r;
+                                const size_t out = plane * (size_t)(nk[k] + 1) + r;
+                                if (Gk[hi] == 0) pk[hi] = 2.0;
+                                if (Gk[out] == 0) pk[out] = 3.0;
+                            }
+                        }
+                }
+#pragma omp barrier
+            }
+            /* back: every slab's planes (x / y halos included), and the
+             * outer z-halo planes of the first and the last slab */
+            for (int k = tid; k < P; k += nth)
+                for (int q = 0; q < 4; q++) {
+                    memcpy(glob[q] + plane * (size_t)(z0[k] + 2), S[4 * k + q] + plane * 2,
+                           sizeof(double) * plane * (size_t)nk[k]);
+                    if (k == 0) memcpy(glob[q], S[q], sizeof(double) * plane * 2);
+                    if (k == P - 1)
+                        memcpy(glob[q] + plane * (size_t)(nz + 2), S[4 * k + q] + plane * (size_t)(nk[k] + 2),
+                               sizeof(double) * plane * 2);
+                }
+        }
+    }
+    for (int q = 0; q < 4 * P; q++) free(S[q]);
+    free(S); free(z0); free(nk);
+    return fail ? -1 : 0;
+}
+
 int orc_calc_flow(const uint8_t *G, const int32_t *code, int nx, int ny, int nz,
                   int bc, const orc_params *prm,
                   double *p, double *u, double *v, double *w,

@@ -33,6 +33,12 @@ void orc_pins(const uint8_t *G, int nx, int ny, int nz, double *p);
 void orc_sweep_ref(int solver, const uint8_t *G, const int32_t *code,
                    int nx, int ny, int nz,
                    double *p, double *u, double *v, double *w);
+/* nsweeps reference sweeps (+ exchange + pins) with the interior cut into P
+ * z-slabs swept by one thread each, as `mpirun -np P` with
+ * dom_decomposition 1 1 P runs them; P = 1: the serial sweep.  0 or -1
+ * (allocation failure). */
+int orc_sweeps_ref_slabs(int solver, const uint8_t *G, const int32_t *code, int nx, int ny, int nz, int bc,
+                         double *p, double *u, double *v, double *w, int nsweeps, int P);
 double orc_convergence(const uint8_t *G, const double *w, int nx, int ny, int nz,
                        double *kprev);
 /* out[0..7] = k13, k23, k33, wk13, wk23, wk33, wmax_velz, wmean_velz */
