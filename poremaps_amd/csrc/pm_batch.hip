@@ -138,14 +138,7 @@ __global__ __launch_bounds__(kB) void k_b_fold4(const BSample *__restrict__ S, c
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    const int64_t c = s.list[t];
-    float4 *d = which ? s.B4 : s.A4;
-    const float4 v = d[c];
-    s.X[0][c] = s.X[0][c] + (double)v.x;
-    s.X[1][c] = s.X[1][c] + (double)v.y;
-    s.X[2][c] = s.X[2][c] + (double)v.z;
-    s.X[3][c] = s.X[3][c] + (double)v.w;
-    d[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    fold_voxel4(s.list[t], s.X[0], s.X[1], s.X[2], s.X[3], which ? s.B4 : s.A4);
 }
 
 __global__ __launch_bounds__(kB) void k_b_residual4(const BSample *__restrict__ S, const BChunk *__restrict__ C) {

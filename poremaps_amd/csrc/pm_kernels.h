@@ -82,8 +82,8 @@ void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
                       const double *v, const double *w, const double *q, Res4 R, hipStream_t s);
 void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStream_t s);
-void launch_fold4(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d,
-                  hipStream_t s);
+// X += d, d = 0 over the fluid list; d's halo copies need a ghost fill after.
+void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s);
 void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0,
                          const double *u0, const double *v0, const double *w0, const double *q0, double *p1,
                          double *u1, double *v1, double *w1, double *q1, double cu, double cp, double pin_out,

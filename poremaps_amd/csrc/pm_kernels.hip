@@ -355,7 +355,7 @@ __global__ __launch_bounds__(1024) void k_ref_sweeps_1wg(Grid g, const uint8_t *
 // reductions (deterministic, compensated FP64)
 // ---------------------------------------------------------------------------
 
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = 8192;  // 32 waves per CU: enough loads in flight
 
 // FULL = false: only the sum of w and the fluid count (compute_convergence).
 template <bool FULL>
@@ -365,23 +365,28 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_state(
     DomBox box, RedAcc *__restrict__ partial) {
     RedAcc a;
     acc_zero(a);
-    // 64 lanes along x, 4 rows per block step: coalesced, no per-cell division
+    // 64 lanes along x, 4 rows per block step: coalesced, no per-cell
+    // division.  The operands are loaded whatever the cell and a solid cell
+    // adds 0.0, which leaves a compensated sum exactly as it was: the loads of
+    // a row do not wait for its geometry bytes.
     const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
     const int64_t rows = (int64_t)g.ny * g.nz;
-    for (int64_t r = (int64_t)blockIdx.x * 4 + sub; r < rows; r += (int64_t)gridDim.x * 4)
-    for (int k = lane; k < g.nx; k += 64) {
+    for (int64_t r = (int64_t)blockIdx.x * 4 + sub; r < rows; r += (int64_t)gridDim.x * 4) {
         const int j = (int)(r % g.ny), i = (int)(r / g.ny);
-        const int64_t c = g.at(i + 2, j + 2, k + 2);
-        if (G[c] != 0) continue;
+        const int64_t row = g.at(i + 2, j + 2, 2);
+#pragma unroll 4
+    for (int k = lane; k < g.nx; k += 64) {
+        const int64_t c = row + k;
+        const bool fl = G[c] == 0;
         const double wc = w[c];
-        a.n[0] += 1;
-        nadd(a.s[2], wc);
+        const double uc = FULL ? u[c] : 0.0, vc = FULL ? v[c] : 0.0;
+        a.n[0] += fl ? 1 : 0;
+        nadd(a.s[2], fl ? wc : 0.0);
         if (FULL) {
-            const double uc = u[c], vc = v[c];
-            nadd(a.s[0], uc);
-            nadd(a.s[1], vc);
-            if (wc > a.wmax) a.wmax = wc;
-            if (box.on && k >= box.d[0] && k <= box.d[1] && j >= box.d[2] && j <= box.d[3] &&
+            nadd(a.s[0], fl ? uc : 0.0);
+            nadd(a.s[1], fl ? vc : 0.0);
+            if (fl && wc > a.wmax) a.wmax = wc;
+            if (fl && box.on && k >= box.d[0] && k <= box.d[1] && j >= box.d[2] && j <= box.d[3] &&
                 i >= box.d[4] && i <= box.d[5]) {
                 a.n[1] += 1;
                 nadd(a.s[3], uc);
@@ -391,6 +396,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_state(
                 if (i == box.d[5]) { a.n[3] += 1; nadd(a.s[7], p[c]); }
             }
         }
+    }
     }
     block_reduce_store(a, partial + blockIdx.x);
 }

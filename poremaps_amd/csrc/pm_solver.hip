@@ -435,10 +435,12 @@ static void jacobi_pass(pm_ctx *c, double *const *A, double *qa, double *const *
     c->lagged = true;
 }
 
-// X += d on the fluid interior, d = 0 (defect correction).
+// X += d on the fluid interior, d = 0 (defect correction); the ghost fill
+// then refills d's periodic halo copies with zeros.
 static void fold_now(pm_ctx *c) {
     const pm::Grid &g = c->g;
-    pm::launch_fold4(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->S4[0], c->stream);
+    pm::launch_fold4(c->lists, c->X[0], c->X[1], c->X[2], c->X[3], c->S4[0], c->stream);
+    pm::launch_ghost4(g, c->G, c->S4[0], c->qD[0], c->stream);
 }
 
 static void apply_variant(pm_ctx *c, const SweepVariant &v) {

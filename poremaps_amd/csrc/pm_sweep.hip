@@ -138,11 +138,11 @@ __global__ void k_ghost4(Grid g, const uint8_t *__restrict__ G, float4 *S, float
     ghost_line4(g, t, G, S, q);
 }
 
-__global__ void k_fold4(Grid g, const uint8_t *__restrict__ pc, double *Xp, double *Xu, double *Xv, double *Xw,
-                        float4 *d) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.total) return;
-    fold_cell4(g, c, pc, Xp, Xu, Xv, Xw, d);
+__global__ __launch_bounds__(kListBlock) void k_fold4(const int32_t *__restrict__ list, int n, double *Xp,
+                                                      double *Xu, double *Xv, double *Xw, float4 *d) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    fold_voxel4(list[t], Xp, Xu, Xv, Xw, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -554,9 +554,9 @@ void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStre
     hipLaunchKernelGGL(k_ghost4, dim3(nblocks(ghost_lines(g), 256)), dim3(256), 0, s, g, G, S, q);
 }
 
-void launch_fold4(const Grid &g, const uint8_t *pc, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d,
-                  hipStream_t s) {
-    hipLaunchKernelGGL(k_fold4, dim3(nblocks(g.total, 256)), dim3(256), 0, s, g, pc, Xp, Xu, Xv, Xw, d);
+void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s) {
+    if (L.n_all == 0) return;
+    hipLaunchKernelGGL(k_fold4, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, L.all, L.n_all, Xp, Xu, Xv, Xw, d);
 }
 
 void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const double *p0, const double *u0,

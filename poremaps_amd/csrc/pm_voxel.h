@@ -544,25 +544,15 @@ __device__ __forceinline__ void ghost_line4(const Grid &g, int64_t t, const uint
     }
 }
 
-// Fold of the interleaved correction into the FP64 state: X += d on the
-// fluid interior, d = 0 everywhere.
-__device__ __forceinline__ void fold_cell4(const Grid &g, int64_t c, const uint8_t *__restrict__ pc, double *Xp,
-                                           double *Xu, double *Xv, double *Xw, float4 *d) {
-    const int64_t e = c - g.base;
-    bool in = false;
-    if (e >= 0) {
-        const int i = (int)(e / g.sy);
-        const int64_t r = e % g.sy;
-        const int j = (int)(r / g.sx), k = (int)(r % g.sx);
-        in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
-    }
-    if (in && pc[c] != kSolid) {
-        const float4 v = d[c];
-        Xp[c] = Xp[c] + (double)v.x;
-        Xu[c] = Xu[c] + (double)v.y;
-        Xv[c] = Xv[c] + (double)v.z;
-        Xw[c] = Xw[c] + (double)v.w;
-    }
+// Fold of the interleaved correction into the FP64 state at fluid interior
+// cell c: X += d, d = 0.  Only fluid cells carry a correction; the caller
+// refills the halo copies of d (with zeros) by a ghost fill afterwards.
+__device__ __forceinline__ void fold_voxel4(int64_t c, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d) {
+    const float4 v = d[c];
+    Xp[c] = Xp[c] + (double)v.x;
+    Xu[c] = Xu[c] + (double)v.y;
+    Xv[c] = Xv[c] + (double)v.z;
+    Xw[c] = Xw[c] + (double)v.w;
     d[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
