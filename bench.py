@@ -146,12 +146,12 @@ def measured_traffic(spec, form, precision):
     counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a kernel of known
     traffic), when the run uses the workload and kernel form it was measured
     on; else None."""
-    if spec["name"] != "grf_512" or form != "interleaved-lagged-eager" or precision != "f32dc":
+    if spec["name"] != "grf_512" or form != "interleaved-lagged-records" or precision != "f32dc":
         return None
     path = os.path.join(ROOT, "profiles", "jacobi_f32dc_grf512_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch k_jac4<4>: (\d+)", open(path).read())
+    m = re.search(r"traffic_bytes_per_launch k_jac4_rec: (\d+)", open(path).read())
     return int(m.group(1)) if m else None
 
 

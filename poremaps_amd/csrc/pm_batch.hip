@@ -38,9 +38,11 @@ struct BSample {
     void *qA, *qB;
     double *X[4];
     double *qX;
-    // defect correction: interleaved correction sets and residual
+    // defect correction: interleaved correction sets and residual, and the
+    // lagged pass's per-voxel records
     float4 *A4, *B4;
     Res4 R4;
+    const uint2 *rec;
     // relaxation of this sample: omega*dt, omega_p*c2*dt
     double cu, cp;
     int chunk_begin, chunk_end;
@@ -99,17 +101,17 @@ __global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, co
                s.R4, (float)s.cu);
 }
 
-// Lagged single pass (the sweeps after the first of an interval; every load
-// issued up front) and the settle pass that completes the pressure at the
-// interval's end, in place.
+// Lagged single pass (the sweeps after the first of an interval; from the
+// per-voxel records, every load issued up front) and the settle pass that
+// completes the pressure at the interval's end, in place.
 __global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
                                                int parity) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    lag_voxel4_eager(s.g, s.list[t], t, s.pc, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-                     parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+    lag_voxel4_rec(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
+                   parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
 }
 
 __global__ __launch_bounds__(kB) void k_b_settle4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
@@ -148,6 +150,20 @@ __global__ __launch_bounds__(kB) void k_b_residual4(const BSample *__restrict__ 
     if (t >= s.n) return;
     const int64_t c = s.list[t];
     residual_voxel4(s.g, c, t, s.pc[c], s.X[0], s.X[1], s.X[2], s.X[3], s.qX, s.R4);
+}
+
+// The gathered continuity residuals of the lagged pass (k_gather_rq).
+__global__ __launch_bounds__(kB) void k_b_gather_rq(const BSample *__restrict__ S, const BChunk *__restrict__ C) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const int64_t c = s.list[t];
+    int i, j, k;
+    cell_ijk(s.g, c, i, j, k);
+    int64_t nxn, nyn, nzn;
+    lag_targets(s.g, c, i, j, k, nxn, nyn, nzn);
+    s.R4.q[t] = make_float4(s.R4.p[c], s.R4.p[nxn], s.R4.p[nyn], s.R4.p[nzn]);
 }
 
 // which: 0 = set A, 1 = set B (the swept state, correction pins 0 in the
@@ -425,6 +441,7 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
         d.A4 = c->S4[0];
         d.B4 = c->S4[1];
         d.R4 = c->R4;
+        d.rec = c->lists.rec;
         const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
         d.cu = om * pm::kDt;
         d.cp = wp * pm::kC2 * pm::kDt;
@@ -466,6 +483,7 @@ static void launch_interval(pm_batch *b) {
         hipLaunchKernelGGL((k_b_ghost<double, true>), dim3(b->nG), blk, 0, st, b->dS, b->dG, 2);
         hipLaunchKernelGGL(k_b_outlet, dim3(b->nP), blk, 0, st, b->dS, b->dP);
         hipLaunchKernelGGL(k_b_residual4, dim3(b->nC), blk, 0, st, b->dS, b->dC);
+        hipLaunchKernelGGL(k_b_gather_rq, dim3(b->nC), blk, 0, st, b->dS, b->dC);
         hipLaunchKernelGGL(k_b_zero_q, dim3(b->nP), blk, 0, st, b->dS, b->dP);
         hipLaunchKernelGGL(k_b_reduce, dim3(b->nC), blk, 0, st, b->dS, b->dC, 2, b->dPart);
     } else {

@@ -16,7 +16,9 @@ struct Timer {
 };
 
 enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_N };
-enum JacobiImpl { JI_PAIR = 0, JI_LIST = 1, JI_STREAM = 2 };
+// JI_RECORDS: the interleaved state's lagged pass reading per-voxel records
+// (codes of the cell and its recompute targets, gathered residuals).
+enum JacobiImpl { JI_PAIR = 0, JI_LIST = 1, JI_STREAM = 2, JI_RECORDS = 3 };
 static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep"};
 
 // One Jacobi kernel configuration.  Every configuration produces the same
@@ -58,7 +60,7 @@ struct pm_ctx {
     // for SOR's in-place sweeps; its outlet buffers and the FP32 residual
     float4 *S4[2] = {nullptr, nullptr};
     float *qD[2] = {nullptr, nullptr};
-    pm::Res4 R4 = {nullptr, nullptr};
+    pm::Res4 R4 = {nullptr, nullptr, nullptr};
     // reductions
     pm::FluidLists lists;   // fluid cells of the interior (fast modes)
     void *red_scratch = nullptr;

@@ -40,10 +40,14 @@ struct Grid {
 // part Rp per cell (it is read at the cell and at its -1 neighbours) and the
 // momentum part (Ru, Rv, Rw, 0) per fluid voxel in fluid-list order (read only
 // by the voxel's own thread), which keeps the residual traffic of a sweep at
-// fluid size.
+// fluid size.  q (Jacobi): per fluid voxel, the continuity part at the cell
+// and at the three cells whose pressure its lagged pass recomputes (-x, -y
+// neighbour or their periodic images, -z neighbour), gathered once per
+// residual so that a sweep reads them with one streamed load.
 struct Res4 {
     float *p;
     float4 *m;
+    float4 *q;
 };
 
 inline Grid make_grid(int nx, int ny, int nz, int bc) {

@@ -56,6 +56,11 @@ struct FluidLists {
     bool select_loads = true;
     // column of the list order, rows x cells (0: whole extent)
     int tile_y = 0, tile_x = 0;
+    // per-voxel records of the lagged pass (element offset + packed codes of
+    // the cell and its recompute targets), FP32+DC Jacobi; use_rec selects
+    // the record form of the lagged pass
+    uint2 *rec = nullptr;
+    bool use_rec = false;
 };
 
 // FP64 fast modes (per-field state p, u, v, w): the velocity + pressure
@@ -79,6 +84,8 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s);
 void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
                     float cp, hipStream_t s);
+void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, hipStream_t s);
+void launch_gather_rq(const Grid &g, const FluidLists &L, Res4 R, hipStream_t s);
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
                       const double *v, const double *w, const double *q, Res4 R, hipStream_t s);
 void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStream_t s);

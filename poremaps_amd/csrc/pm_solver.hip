@@ -98,8 +98,10 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     const bool jac = p->mode == PM_MODE_JACOBI;
     if (fast && !dc && jac) per += 32;
     // interleaved correction sets (16 B each: two for Jacobi, one for SOR),
-    // continuity residual (4 B) and momentum residual (16 B per fluid cell)
-    if (dc) per += (jac ? 32 : 16) + 4 + 16;
+    // continuity residual (4 B) and momentum residual (16 B per fluid cell);
+    // Jacobi: the lagged pass's records (8 B) and gathered residuals (16 B)
+    // per fluid cell
+    if (dc) per += (jac ? 32 : 16) + 4 + 16 + (jac ? 24 : 0);
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
     // keeps separate colour lists
     if (fast) per += (jac || dc) ? 4 : 8;
@@ -111,9 +113,10 @@ static void free_all(pm_ctx *c) {
     f(c->G); f(c->pc); f(c->code);
     for (int q = 0; q < 4; q++) { f(c->X[q]); f(c->X2[q]); }
     for (int q = 0; q < 2; q++) { f(c->qX[q]); f(c->qD[q]); f(c->S4[q]); c->S4[q] = nullptr; }
-    f(c->R4.p); f(c->R4.m); c->R4 = pm::Res4{nullptr, nullptr};
+    f(c->R4.p); f(c->R4.m); f(c->R4.q); c->R4 = pm::Res4{nullptr, nullptr, nullptr};
     f(c->red_scratch); f(c->red_dev);
     f(c->lists.all);
+    f(c->lists.rec);
     if (!c->lists.col_in_all)
         for (int q = 0; q < 3; q++) f(c->lists.col[q]);
     c->lists = pm::FluidLists();
@@ -149,6 +152,7 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
         if (std::strcmp(jm, "pair") == 0) c->jac_impl = JI_PAIR;
         else if (std::strcmp(jm, "list") == 0) c->jac_impl = JI_LIST;
         else if (std::strcmp(jm, "stream") == 0) c->jac_impl = JI_STREAM;
+        else if (std::strcmp(jm, "records") == 0) c->jac_impl = JI_RECORDS;
     }
     if (const char *zs = std::getenv("POREMAPS_STREAM_ZS")) c->stream_zs = std::max(1, std::atoi(zs));
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -217,6 +221,7 @@ static int dc_prepare(pm_ctx *c) {
     tb(c, TC_RESIDUAL);
     pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
     pm::launch_residual4(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R4, c->stream);
+    pm::launch_gather_rq(g, c->lists, c->R4, c->stream);
     te(c);
     RC(fill_f32(c, c->qD[0], g.qtotal(), 0.f));
     RC(fill_f32(c, c->qD[1], g.qtotal(), 0.f));
@@ -334,6 +339,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         c->lists.tile_x = want_tx > 0 ? tx : 0;
         const char *ld = std::getenv("POREMAPS_SWEEP_LOADS");
         c->lists.select_loads = !(ld && std::strcmp(ld, "branch") == 0);
+        c->lists.use_rec = c->jac_impl == JI_RECORDS;
         const bool sor = c->prm.mode == PM_MODE_SOR3;
         if (sor && c->dc()) {
             // the colour lists back to back: the residual's momentum part,
@@ -348,6 +354,10 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         c->lists.n_all = (int)all.size();
         RC(dalloc(&c->lists.all, std::max<int64_t>(1, c->lists.n_all)));
         if (c->dc()) RC(dalloc(&c->R4.m, std::max<int64_t>(1, c->lists.n_all)));
+        if (c->dc() && c->jacobi()) {
+            RC(dalloc(&c->R4.q, std::max<int64_t>(1, c->lists.n_all)));
+            RC(dalloc(&c->lists.rec, std::max<int64_t>(1, c->lists.n_all)));
+        }
         HIPCHK(hipMemcpy(c->lists.all, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
         if (sor && c->dc()) {
             c->lists.col_in_all = true;
@@ -374,6 +384,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     HIPCHK(hipMemsetAsync(c->pc, 0xff, (size_t)g.total, c->stream));
     HIPCHK(hipMemsetAsync(c->code, 0, sizeof(int32_t) * (size_t)g.total, c->stream));
     pm::launch_eval_geometry(g, c->G, c->code, c->pc, c->stream);
+    pm::launch_lag_records(g, c->lists, c->pc, c->stream);
     RC(init_state(c));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -447,11 +458,14 @@ static void apply_variant(pm_ctx *c, const SweepVariant &v) {
     c->jac_impl = v.impl;
     c->lists.select_loads = v.select_loads;
     c->stream_zs = v.zs;
+    c->lists.use_rec = v.impl == JI_RECORDS;
 }
 
 static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *buf, size_t len) {
     if (c->dc() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
-    else if (c->dc()) std::snprintf(buf, len, "interleaved-lagged-%s", v.select_loads ? "eager" : "branch");
+    else if (c->dc())
+        std::snprintf(buf, len, "interleaved-lagged-%s",
+                      v.impl == JI_RECORDS ? "records" : v.select_loads ? "eager" : "branch");
     else if (v.impl == JI_STREAM) std::snprintf(buf, len, "dense-2.5d-zs%d", v.zs);
     else std::snprintf(buf, len, "%s-%s", v.impl == JI_PAIR ? "list-pair" : "list-lagged",
                        v.select_loads ? "select" : "branch");
@@ -471,7 +485,8 @@ static void tuner_setup(pm_ctx *c) {
     // with the last candidate and not scored
     const SweepVariant soa[] = {{JI_STREAM, true, 16}, {JI_PAIR, true, 64}, {JI_PAIR, false, 64},
                                 {JI_LIST, true, 64}, {JI_STREAM, true, 64}, {JI_STREAM, true, 16}};
-    const SweepVariant aos[] = {{JI_LIST, true, 64}, {JI_PAIR, true, 64}, {JI_LIST, false, 64}, {JI_LIST, true, 64}};
+    const SweepVariant aos[] = {{JI_RECORDS, true, 64}, {JI_PAIR, true, 64}, {JI_LIST, false, 64},
+                                {JI_LIST, true, 64}, {JI_RECORDS, true, 64}};
     if (c->dc())
         for (const auto &v : aos) tn.cand[tn.n++] = v;
     else

@@ -120,6 +120,35 @@ __global__ __launch_bounds__(kListBlock) void k_colour4(Grid g, const int32_t *_
     colour_voxel4(g, list[t0 + t], t0 + t, pc, S, q, R, cu, cp, 0.f);
 }
 
+// Lagged pass from the per-voxel records (lag_voxel4_rec).
+__global__ __launch_bounds__(kListBlock) void k_jac4_rec(Grid g, const uint2 *__restrict__ rec, int n,
+                                                         const float4 *__restrict__ S0, const float *__restrict__ q0,
+                                                         float4 *S1, Res4 R, float cu, float cp) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    lag_voxel4_rec(g, t, rec, S0, q0, S1, R, cu, cp, 0.f);
+}
+
+__global__ __launch_bounds__(kListBlock) void k_lag_records(Grid g, const int32_t *__restrict__ list, int n,
+                                                            const uint8_t *__restrict__ pc, uint2 *rec) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    rec[t] = lag_record(g, list[t], pc);
+}
+
+// R.q from R.p: the continuity residual at each voxel and its recompute
+// targets.
+__global__ __launch_bounds__(kListBlock) void k_gather_rq(Grid g, const int32_t *__restrict__ list, int n, Res4 R) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const int64_t c = list[t];
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    int64_t nxn, nyn, nzn;
+    lag_targets(g, c, i, j, k, nxn, nyn, nzn);
+    R.q[t] = make_float4(R.p[c], R.p[nxn], R.p[nyn], R.p[nzn]);
+}
+
 __global__ __launch_bounds__(kListBlock) void k_residual4(Grid g, const int32_t *__restrict__ list, int n,
                                                           const uint8_t *__restrict__ pc,
                                                           const double *__restrict__ p,
@@ -521,6 +550,8 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
     if (stage == JAC_FIRST) {
         hipLaunchKernelGGL(k_jac4<0>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
+    } else if (stage == JAC_LAG && L.select_loads && L.rec && L.use_rec) {
+        hipLaunchKernelGGL(k_jac4_rec, grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
     } else if (stage == JAC_LAG && L.select_loads) {
         hipLaunchKernelGGL(k_jac4<4>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (stage == JAC_LAG) {
@@ -541,6 +572,16 @@ void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float
         hipLaunchKernelGGL(k_colour4, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.all, n, L.off_col[col], pc, S,
                            q, R, cu, cp);
     }
+}
+
+void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, hipStream_t s) {
+    if (L.n_all == 0 || !L.rec) return;
+    hipLaunchKernelGGL(k_lag_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc, L.rec);
+}
+
+void launch_gather_rq(const Grid &g, const FluidLists &L, Res4 R, hipStream_t s) {
+    if (L.n_all == 0 || !R.q) return;
+    hipLaunchKernelGGL(k_gather_rq, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, R);
 }
 
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,

@@ -436,6 +436,84 @@ __device__ __forceinline__ void lag_voxel4_eager(const Grid &g, int64_t c, int t
     __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c]));
 }
 
+// The cells whose pressure the lagged pass at c recomputes: the -x and -y
+// neighbours, or their periodic images when c is on the first interior
+// index of a periodic axis, and the -z neighbour.
+__device__ __forceinline__ void lag_targets(const Grid &g, int64_t c, int i, int j, int k, int64_t &nxn,
+                                            int64_t &nyn, int64_t &nzn) {
+    (void)i;
+    nxn = (k == 2 && g.px) ? c - 1 + g.nx : c - 1;
+    nyn = (j == 2 && g.py) ? c - g.sx + (int64_t)g.ny * g.sx : c - g.sx;
+    nzn = c - g.sy;
+}
+
+// Per-voxel record of the lagged pass: the cell's element offset and the
+// packed codes of the cell and of its three recompute targets (8 bits each,
+// cell lowest).
+__device__ __forceinline__ uint2 lag_record(const Grid &g, int64_t c, const uint8_t *__restrict__ pc) {
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    int64_t nxn, nyn, nzn;
+    lag_targets(g, c, i, j, k, nxn, nyn, nzn);
+    const uint32_t codes = (uint32_t)pc[c] | ((uint32_t)pc[nxn] << 8) | ((uint32_t)pc[nyn] << 16) |
+                           ((uint32_t)pc[nzn] << 24);
+    return make_uint2((uint32_t)c, codes);
+}
+
+// lag_voxel4_eager with the codes from the voxel's record and the four
+// continuity residuals from R.q: two streamed loads instead of eight gathers,
+// the same values.
+__device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2 *__restrict__ rec,
+                                               const float4 *__restrict__ S0, const float *__restrict__ q0,
+                                               float4 *__restrict__ S1, Res4 R, float cu, float cp, float pin_out) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const v2u rc = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&rec[t]));
+    const int64_t c = (int64_t)rc.x;
+    const uint8_t pcv = (uint8_t)(rc.y & 0xff), pcx = (uint8_t)((rc.y >> 8) & 0xff),
+                  pcy = (uint8_t)((rc.y >> 16) & 0xff), pcz = (uint8_t)(rc.y >> 24);
+    const int64_t sx = g.sx, sy = g.sy;
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const bool out = i == g.nz + 1;
+    const bool wx = k == 2 && g.px, wy = j == 2 && g.py;
+    int64_t nxn, nyn, nzn;
+    lag_targets(g, c, i, j, k, nxn, nyn, nzn);
+    const v4f rq = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.q[t]));
+    const float rpc = rq.x, rpx = rq.y, rpy = rq.z, rpz = rq.w;
+    const float4 C = S0[c];
+    const Star4 s = load_star4(S0, c, sx, sy);
+    const float vyx = S0[nxn + sx].z, wzx = S0[nxn + sy].w;
+    const float uxy = S0[nyn + 1].y, wzy = S0[nyn + sy].w;
+    const float uxz = S0[nzn + 1].y, vyz = S0[nzn + sx].z;
+    const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
+    const float4 Rm = make_float4(r.x, r.y, r.z, r.w);
+    float4 NX = s.XM, NY = s.YM;
+    float uxx = C.y, vyy = C.z;
+    if (wx) { NX = S0[nxn]; uxx = S0[nxn + 1].y; }
+    if (wy) { NY = S0[nyn]; vyy = S0[nyn + sx].z; }
+
+    const Cases cs = decode(pcv);
+    const float Pc = out ? pin_out : C.x - (div_from4(pcv, C, s.XP.y, s.YP.z, s.ZP.w) + rpc) * cp;
+    const float c2dt = (float)(kC2 * kDt);
+    const float dx0 = div_from4(pcx, NX, uxx, vyx, wzx);
+    const float dy0 = div_from4(pcy, NY, uxy, vyy, wzy);
+    const float dz0 = div_from4(pcz, s.ZM, uxz, vyz, C.w);
+    float px = out ? pin_out - dx0 * c2dt : NX.x - (dx0 + rpx) * cp;
+    float py = out ? pin_out - dy0 * c2dt : NY.x - (dy0 + rpy) * cp;
+    const float pz = i == 2 ? s.ZM.x : s.ZM.x - (dz0 + rpz) * cp;
+    if (k == 2 && !g.px) px = out ? q0[g.qat(j, k - 1)] : s.XM.x;
+    if (j == 2 && !g.py) py = out ? q0[g.qat(j - 1, k)] : s.YM.x;
+    const float dpx = cs.bx ? 0.f : Pc - px;
+    const float dpy = cs.by ? 0.f : Pc - py;
+    const float dpz = cs.bz ? 0.f : Pc - pz;
+    float su, sv, sw;
+    visc_from4(cs, s, C, su, sv, sw);
+    const float4 o = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rm, Pc, cu);
+    const v4f ov = {o.x, o.y, o.z, o.w};
+    __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c]));
+}
+
 // One 3-colour SOR update at fluid cell c on the interleaved state, in place
 // (colour_voxel): velocities from the current state, then the pressure from
 // the new velocities at c and the current ones of its +1 neighbours.  Every

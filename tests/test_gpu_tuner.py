@@ -1,7 +1,8 @@
 """Jacobi kernel forms and the autotuner on the GPU.
 
 The velocity + pressure pair, the lagged single pass over the fluid list
-(all loads up front, or case-predicated) and the lagged dense 2.5-D pass (two
+(all loads up front, case-predicated, or from per-voxel records) and the
+lagged dense 2.5-D pass (two
 z-segment lengths) -- in FP32+DC the pair and lagged passes over the
 interleaved state -- must produce the same
 iterates bit for bit, and a solve whose first sweeps the autotuner spreads
@@ -57,17 +58,17 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
     raw = random_geometry((36, 20, 70), 0.4, 71 + bc)
     say("kernel forms", precision, bc)
     ref, ref_fields, _ = solve(raw, bc, precision, "pair")
-    for form, zs, loads in (("list", None, None), ("list", None, "branch"), ("stream", 16, None),
-                            ("stream", 64, None), (None, None, None)):
+    for form, zs, loads in (("list", None, None), ("list", None, "branch"), ("records", None, None),
+                            ("stream", 16, None), ("stream", 64, None), (None, None, None)):
         res, fields, (name, tried) = solve(raw, bc, precision, form, zs, loads)
         say(form, zs, loads, name, res.sweeps)
         assert res.sweeps == ref.sweeps and res.last.wk33 == ref.last.wk33, (form, zs)
         for a, b, n in zip(fields, ref_fields, ("velx", "vely", "velz", "press")):
             assert np.array_equal(a, b), (form, zs, n)
         if form is None:
-            # FP32+DC runs on the interleaved state: pair and the two lagged
-            # forms (eager, branch)
-            assert len(tried) == (3 if precision == "f32dc" else 5) and name in tried
+            # FP32+DC runs on the interleaved state: pair and the three lagged
+            # forms (eager, branch, records)
+            assert len(tried) == (4 if precision == "f32dc" else 5) and name in tried
 
 
 def test_divergent_relaxation_restarts_slower():
