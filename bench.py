@@ -21,10 +21,12 @@ roofline: the dominant kernel group ("sweep") timed with HIP events on the
 library's own stream over the timed region; achieved = algorithmic bytes per
 launch (8*S+1 B per voxel for the state, + 16 B of FP32 residual in the
 defect-correction mode) / average launch duration; peak = 8 TB/s HBM;
-traffic = the measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-calibrated) from the committed counter summary when the run matches it.
-cpu_baseline: the CPU oracle's reference-order sweep (FP64, one thread) timed
-on this host on a bounded sample of the same field.
+traffic = the measured HBM bytes per launch (rocprofv3 read requests by size
++ WRITE_SIZE, calibrated on a device copy of known size) from the committed
+counter summary when the run matches it.
+cpu_baseline: the CPU oracle's reference-order sweep (FP64) timed on this
+host on a bounded sample of the same field, in z-slabs on all host cores (at
+most 16), with the one-thread rate beside.
 --tte: time-to-eps (eps 1e-6) of the plain FP64 Jacobi solver and of the
 selected mode, reported beside the line.
 """
@@ -143,9 +145,9 @@ def time_kernels(s, n):
 
 def measured_traffic(spec, form, precision):
     """HBM bytes per launch of the sweep kernel, from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a kernel of known
-    traffic), when the run uses the workload and kernel form it was measured
-    on; else None."""
+    counter summary (read requests by size + WRITE_SIZE, calibrated on a
+    device copy of known size), when the run uses the workload and kernel
+    form it was measured on; else None."""
     if spec["name"] != "grf_512" or form != "interleaved-lagged-records" or precision != "f32dc":
         return None
     path = os.path.join(ROOT, "profiles", "jacobi_f32dc_grf512_hbm_traffic.md")
