@@ -155,12 +155,13 @@ int pm_time_kernels(pm_ctx *ctx, int n, pm_kernel_timing *out, int max_out, int 
 /* Jacobi sweep kernels in use and the autotuner's measurements.  The fast
  * Jacobi mode has several kernel forms that produce the same iterates bit
  * for bit (velocity + pressure kernel pair over the fluid list with select or
- * case-predicated neighbour loads, one lagged pass over the fluid list, one
- * lagged dense 2.5-D pass); the first sweeps of a solve run each in turn and
- * the fastest is kept (environment: POREMAPS_TUNE=0 disables,
- * POREMAPS_JACOBI=pair|list|stream fixes the form).  name receives the form
- * in use; tried (optional) "form=ms/sweep;..." once tuning has finished.
- * No reference counterpart. */
+ * case-predicated neighbour loads; one lagged pass over the fluid list with
+ * case-predicated loads, with every load issued up front, or reading
+ * per-voxel records; one lagged dense 2.5-D pass); the first sweeps of a
+ * solve run each in turn and the fastest is kept (environment:
+ * POREMAPS_TUNE=0 disables, POREMAPS_JACOBI=pair|list|records|stream fixes the
+ * form).  name receives the form in use; tried (optional)
+ * "form=ms/sweep;..." once tuning has finished.  No reference counterpart. */
 int pm_kernel_choice(pm_ctx *ctx, char *name, int len, char *tried, int tried_len);
 
 void pm_destroy(pm_ctx *ctx);
