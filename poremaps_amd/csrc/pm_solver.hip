@@ -263,10 +263,12 @@ static int init_state(pm_ctx *c) {
 // Start a fast-mode solve again from the initial state with a smaller
 // pressure relaxation, after it diverged: the relaxation pair crossed the
 // stability bound of the symplectic sweep (omega * omega_p near 19 on every
-// medium probed).  Returns PM_E_DIVERGED when the pair is already small.
+// medium probed).  Returns PM_E_DIVERGED when the pair is already small, or
+// when the medium has no fluid (the convergence value is then 0/0, as in the
+// reference, and no relaxation changes that).
 int pm_restart_slower(pm_ctx *c) {
     const double wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : c->prm.omega;
-    if (!c->fast() || c->prm.omega * wp <= 8.0) return PM_E_DIVERGED;
+    if (!c->fast() || c->fluid == 0 || c->prm.omega * wp <= 8.0) return PM_E_DIVERGED;
     c->prm.omega_p = 0.75 * wp;
     c->restarts++;
     RC(init_state(c));

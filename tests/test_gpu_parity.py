@@ -268,6 +268,72 @@ def test_fast_f32dc_sweeps_bit_exact(mode, bc, omega, omega_p):
 
 
 # ---------------------------------------------------------------------------
+# edge shapes: the smallest domains, thin ones, all fluid, all solid
+# ---------------------------------------------------------------------------
+EDGE_SHAPES = [
+    ("minimal all-fluid 2x2x2", lambda: np.zeros((2, 2, 2), np.uint8)),
+    ("thin random 2x3x9", lambda: random_geometry((2, 3, 9), 0.3, 5)),
+    ("all-fluid 5x4x6", lambda: np.zeros((5, 4, 6), np.uint8)),
+    ("thin random 3x7x2", lambda: random_geometry((3, 7, 2), 0.3, 6)),
+]
+
+
+@pytest.mark.parametrize("name,make", EDGE_SHAPES, ids=[e[0] for e in EDGE_SHAPES])
+@pytest.mark.parametrize("bc", [0, 2, 4])
+def test_edge_shapes_reference_bit_exact(name, make, bc):
+    raw = make()
+    say("edge reference", name, bc)
+    G, code, st = oracle_state(raw, bc)
+    with pm.Solver(raw, boundary_method=bc, solving_algorithm=2) as s:
+        for _ in range(7):
+            O.sweep_ref(2, G, code, *st)
+            for f in st:
+                O.exchange(f, bc)
+            O.pins(G, st[0])
+        s.step(7)
+        for n_, a, b in zip("puvw", s.state_halo(), st):
+            assert np.array_equal(a, b), f"{n_} differs ({name}, bc {bc})"
+
+
+@pytest.mark.parametrize("name,make", EDGE_SHAPES, ids=[e[0] for e in EDGE_SHAPES])
+@pytest.mark.parametrize("mode,prec", [("jacobi", "f64"), ("jacobi", "f32dc"), ("sor3", "f32dc")])
+@pytest.mark.parametrize("bc", [0, 1, 3])
+def test_edge_shapes_fast_modes_bit_exact(name, make, mode, prec, bc):
+    raw = make()
+    say("edge fast", name, mode, prec, bc)
+    # FP32+DC: 11 sweeps with a fold every 5, so that the last sweep (index
+    # 10) ends with a fold and both sides hold the FP64 state
+    n = 13 if prec == "f64" else 11
+    if prec == "f64":
+        ref = oracle_fast(raw, bc, mode, 2.0, n, 5.0)
+        kw = {}
+    else:
+        ref = oracle_dc(raw, bc, mode, 2.0, n, 5, 5.0)
+        kw = dict(precision="f32dc", it_eval=5)
+    with pm.Solver(raw, boundary_method=bc, mode=mode, omega=2.0, omega_p=5.0, **kw) as s:
+        s.step(n)
+        m = nonc(ref[0])
+        for n_, a, b in zip("puvw", s.state_halo(), ref):
+            assert np.array_equal(a[m], b[m]), f"{n_} differs ({name}, {mode} {prec}, bc {bc})"
+
+
+@pytest.mark.parametrize("mode,prec", [("reference", "f64"), ("jacobi", "f64"), ("jacobi", "f32dc"),
+                                       ("sor3", "f32dc")])
+def test_all_solid_medium_stops_at_first_convergence_test(mode, prec):
+    """No fluid: the convergence value is 0/0 at the first test (sweep 1100);
+    the reference's loop condition is false for it and the run ends there.
+    The solver ends at the same point with PM_E_DIVERGED and no restart."""
+    raw = np.ones((6, 5, 7), np.uint8)
+    say("all solid", mode, prec)
+    kw = {} if mode == "reference" else dict(mode=mode, precision=prec, omega=2.5, omega_p=7.0)
+    with pm.Solver(raw, boundary_method=0, eps=1e-6, max_iter=50_000, **kw) as s:
+        with pytest.raises(pm.PoremapsError) as e:
+            s.solve()
+        assert e.value.code == -7 and "iteration 1100" in str(e.value)
+        assert s.porosity == 0.0
+
+
+# ---------------------------------------------------------------------------
 # fast modes, to convergence: the reference's fixed point
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("mode,omega,prec,omega_p", [("jacobi", 1.0, "f64", 0.0), ("jacobi", 2.5, "f32dc", 0.0),
