@@ -66,9 +66,8 @@ __global__ __launch_bounds__(kB) void k_b_vel(const BSample *__restrict__ S, con
     void *const *src = parity ? s.B : s.A;
     void *const *dst = parity ? s.A : s.B;
     const T *q0 = (const T *)(parity ? s.qB : s.qA);
-    vel_voxel<T, false, true>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(src, 1), fld<T>(src, 2), fld<T>(src, 3),
-                              q0, fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3), nullptr, nullptr, nullptr,
-                              (T)s.cu);
+    vel_voxel<T, true>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(src, 1), fld<T>(src, 2), fld<T>(src, 3), q0,
+                       fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3), (T)s.cu);
 }
 
 template <typename T>
@@ -82,8 +81,8 @@ __global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, co
     void *const *src = parity ? s.B : s.A;
     void *const *dst = parity ? s.A : s.B;
     T *q1 = (T *)(parity ? s.qA : s.qB);
-    pres_voxel<T, false>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3),
-                         fld<T>(dst, 0), q1, nullptr, (T)s.cp, (T)3.0);
+    pres_voxel<T>(s.g, c, s.pc[c], fld<T>(src, 0), fld<T>(dst, 1), fld<T>(dst, 2), fld<T>(dst, 3), fld<T>(dst, 0),
+                  q1, (T)s.cp, (T)3.0);
 }
 
 // Interleaved correction state (defect correction): the first (velocity)

@@ -26,70 +26,60 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-template <typename T, bool HAS_R, bool SEL>
+template <typename T, bool SEL>
 __global__ __launch_bounds__(kListBlock) void k_jac_vel(Grid g, const int32_t *__restrict__ list, int n,
                                                         const uint8_t *__restrict__ pc,
                                                         const T *__restrict__ p0, const T *__restrict__ u0,
                                                         const T *__restrict__ v0, const T *__restrict__ w0,
                                                         const T *__restrict__ q0, T *__restrict__ u1,
-                                                        T *__restrict__ v1, T *__restrict__ w1,
-                                                        const float *__restrict__ Ru,
-                                                        const float *__restrict__ Rv,
-                                                        const float *__restrict__ Rw, T cu) {
+                                                        T *__restrict__ v1, T *__restrict__ w1, T cu) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     const int64_t c = list[t];
-    vel_voxel<T, HAS_R, SEL>(g, c, pc[c], p0, u0, v0, w0, q0, u1, v1, w1, Ru, Rv, Rw, cu);
+    vel_voxel<T, SEL>(g, c, pc[c], p0, u0, v0, w0, q0, u1, v1, w1, cu);
 }
 
-template <typename T, bool HAS_R>
+template <typename T>
 __global__ __launch_bounds__(kListBlock) void k_jac_pres(Grid g, const int32_t *__restrict__ list, int n,
                                                          const uint8_t *__restrict__ pc,
                                                          const T *__restrict__ p0, const T *__restrict__ u1,
                                                          const T *__restrict__ v1, const T *__restrict__ w1,
-                                                         T *__restrict__ p1, T *__restrict__ q1,
-                                                         const float *__restrict__ Rp, T cp, T pin_out) {
+                                                         T *__restrict__ p1, T *__restrict__ q1, T cp, T pin_out) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     const int64_t c = list[t];
-    pres_voxel<T, HAS_R>(g, c, pc[c], p0, u1, v1, w1, p1, q1, Rp, cp, pin_out);
+    pres_voxel<T>(g, c, pc[c], p0, u1, v1, w1, p1, q1, cp, pin_out);
 }
 
 // First sweep from a settled state (p^n given): velocities into set 1 and
 // p^n carried over, so that set 1 holds (u^{n+1}, p^n) for the lagged passes.
-template <typename T, bool HAS_R, bool SEL>
+template <typename T, bool SEL>
 __global__ __launch_bounds__(kListBlock) void k_jac_first(Grid g, const int32_t *__restrict__ list, int n,
                                                           const uint8_t *__restrict__ pc,
                                                           const T *__restrict__ p0, const T *__restrict__ u0,
                                                           const T *__restrict__ v0, const T *__restrict__ w0,
                                                           const T *__restrict__ q0, T *__restrict__ p1,
                                                           T *__restrict__ u1, T *__restrict__ v1,
-                                                          T *__restrict__ w1, const float *__restrict__ Ru,
-                                                          const float *__restrict__ Rv,
-                                                          const float *__restrict__ Rw, T cu) {
+                                                          T *__restrict__ w1, T cu) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     const int64_t c = list[t];
-    vel_voxel<T, HAS_R, SEL>(g, c, pc[c], p0, u0, v0, w0, q0, u1, v1, w1, Ru, Rv, Rw, cu);
+    vel_voxel<T, SEL>(g, c, pc[c], p0, u0, v0, w0, q0, u1, v1, w1, cu);
     p1[c] = p0[c];
 }
 
-template <typename T, bool HAS_R, bool SEL>
+template <typename T, bool SEL>
 __global__ __launch_bounds__(kListBlock) void k_jac_lag(Grid g, const int32_t *__restrict__ list, int n,
                                                         const uint8_t *__restrict__ pc,
                                                         const T *__restrict__ p0, const T *__restrict__ u0,
                                                         const T *__restrict__ v0, const T *__restrict__ w0,
                                                         const T *__restrict__ q0, T *__restrict__ p1,
                                                         T *__restrict__ u1, T *__restrict__ v1,
-                                                        T *__restrict__ w1, const float *__restrict__ Rp,
-                                                        const float *__restrict__ Ru,
-                                                        const float *__restrict__ Rv,
-                                                        const float *__restrict__ Rw, T cu, T cp, T pin_out) {
+                                                        T *__restrict__ w1, T cu, T cp, T pin_out) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     const int64_t c = list[t];
-    lag_voxel<T, HAS_R, SEL>(g, c, pc[c], pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, Rp, Ru, Rv, Rw, cu, cp,
-                             pin_out);
+    lag_voxel<T, SEL>(g, c, pc[c], pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, cu, cp, pin_out);
 }
 
 // Interleaved FP32 correction state (defect-correction Jacobi): stage 0 =
@@ -211,15 +201,13 @@ __device__ __forceinline__ T st_ld(const Grid &g, const T *__restrict__ f, int i
     return f[g.at(i, j, k)];
 }
 
-template <typename T, bool HAS_R, int MODE>
+template <typename T, int MODE>
 __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty, int zs,
                                                      const uint8_t *__restrict__ pc, const T *__restrict__ p0,
                                                      const T *__restrict__ u0, const T *__restrict__ v0,
                                                      const T *__restrict__ w0, const T *__restrict__ q0,
                                                      T *__restrict__ p1, T *__restrict__ u1,
                                                      T *__restrict__ v1, T *__restrict__ w1,
-                                                     const float *__restrict__ Rp, const float *__restrict__ Ru,
-                                                     const float *__restrict__ Rv, const float *__restrict__ Rw,
                                                      T cu, T cp, T pin_out) {
     __shared__ T Lu[2][kLY][kLX], Lv[2][kLY][kLX], Lw[2][kLY][kLX], LP[2][kPY][kPX];
     const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -263,12 +251,8 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
     T up1 = st_ld(g, u0, i0 + 1, j, k, c1), vp1 = st_ld(g, v0, i0 + 1, j, k, c1);
     T wp1 = st_ld(g, w0, i0 + 1, j, k, c1);
     T up2 = st_ld(g, u0, i0 + 2, j, k, c2), vp2 = st_ld(g, v0, i0 + 2, j, k, c2);
-    T pcur = (T)0, rpc = (T)0, ruc = (T)0, rvc = (T)0, rwc = (T)0;
-    if (own_in && c0 != kSolid) {
-        const int64_t c = g.at(i0, j, k);
-        pcur = p0[c];
-        if (HAS_R) { rpc = (T)Rp[c]; ruc = (T)Ru[c]; rvc = (T)Rv[c]; rwc = (T)Rw[c]; }
-    }
+    T pcur = (T)0;
+    if (own_in && c0 != kSolid) pcur = p0[g.at(i0, j, k)];
     T hu = (T)0, hv = (T)0, hw = (T)0;
     uint8_t hc1 = kSolid;
     if (h_on) {
@@ -279,13 +263,12 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
         hc1 = pc[g.at(i0 + 1, hj, hk)];
     }
     uint8_t rc0 = kSolid, rc1 = kSolid;
-    T rp0 = (T)0, rrp = (T)0, rw1 = (T)0;
+    T rp0 = (T)0, rw1 = (T)0;
     if (r_int) {
         const int64_t r = g.at(i0, rj, rk);
         rc0 = pc[r];
         if (rc0 != kSolid) {
             rp0 = p0[r];
-            if (HAS_R) rrp = (T)Rp[r];
             rw1 = w0[r + g.sy];
         }
         rc1 = pc[r + g.sy];
@@ -294,7 +277,7 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
     T Pprev = (T)0;
     if (own_in) {
         if (i0 == 2 || MODE == JAC_FIRST) Pprev = p0[g.at(i0 - 1, j, k)];
-        else if (cm1 != kSolid) Pprev = lag_p<T, HAS_R>(g, g.at(i0 - 1, j, k), cm1, p0, u0, v0, w0, Rp, cp);
+        else if (cm1 != kSolid) Pprev = lag_p<T>(g, g.at(i0 - 1, j, k), cm1, p0, u0, v0, w0, cp);
     }
 
     for (int i = i0; i < i1; i++) {
@@ -304,18 +287,14 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
 
         // requests for step i+1
         uint8_t c4 = kSolid, hc2 = kSolid, rc2 = kSolid;
-        T u3 = (T)0, v3 = (T)0, w2 = (T)0, pn = (T)0, rpn = (T)0, run = (T)0, rvn = (T)0, rwn = (T)0;
-        T hun = (T)0, hvn = (T)0, hwn = (T)0, rp0n = (T)0, rrpn = (T)0, rw1n = (T)0;
+        T u3 = (T)0, v3 = (T)0, w2 = (T)0, pn = (T)0;
+        T hun = (T)0, hvn = (T)0, hwn = (T)0, rp0n = (T)0, rw1n = (T)0;
         if (more) {
             c4 = pc_at(g, pc, i + 4, j, k);
             u3 = st_ld(g, u0, i + 3, j, k, c3);
             v3 = st_ld(g, v0, i + 3, j, k, c3);
             w2 = st_ld(g, w0, i + 2, j, k, c2);
-            if (own_in && c1 != kSolid) {
-                const int64_t n = g.at(i + 1, j, k);
-                pn = p0[n];
-                if (HAS_R) { rpn = (T)Rp[n]; run = (T)Ru[n]; rvn = (T)Rv[n]; rwn = (T)Rw[n]; }
-            }
+            if (own_in && c1 != kSolid) pn = p0[g.at(i + 1, j, k)];
             if (h_on) {
                 hun = st_ld(g, u0, i + 1, hj, hk, hc1);
                 hvn = st_ld(g, v0, i + 1, hj, hk, hc1);
@@ -326,7 +305,6 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
                 const int64_t r = g.at(i + 1, rj, rk);
                 if (rc1 != kSolid) {
                     rp0n = p0[r];
-                    if (HAS_R) rrpn = (T)Rp[r];
                     rw1n = w0[r + g.sy];
                 }
                 rc2 = pc[r + g.sy];
@@ -361,9 +339,7 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
                     Pc = pin_out;
                     lp = pin_out - d0 * C2DT;
                 } else {
-                    T div = d0;
-                    if (HAS_R) div = div + rpc;
-                    Pc = pcur - div * cp;
+                    Pc = pcur - d0 * cp;
                     lp = Pc;
                 }
             }
@@ -385,9 +361,7 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
                         if (out) {
                             val = pin_out - d0 * C2DT;
                         } else {
-                            T div = d0;
-                            if (HAS_R) div = div + rrp;
-                            val = rp0 - div * cp;
+                            val = rp0 - d0 * cp;
                         }
                     }
                 }
@@ -403,7 +377,7 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
                     const uint8_t nc = pc[n];
                     if (nc != kSolid)
                         val = out ? lag_q<T>(g, n, nc, u0, v0, w0, pin_out)
-                                  : lag_p<T, HAS_R>(g, n, nc, p0, u0, v0, w0, Rp, cp);
+                                  : lag_p<T>(g, n, nc, p0, u0, v0, w0, cp);
                 }
             }
             LP[buf][rly - 1][rlx - 1] = val;
@@ -438,7 +412,6 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
             T fu = cs.bx ? (T)0 : su * INV_RE - dpx;
             T fv = cs.by ? (T)0 : sv * INV_RE - dpy;
             T fw = cs.bz ? (T)0 : sw * INV_RE - dpz;
-            if (HAS_R) { fu = fu + ruc; fv = fv + rvc; fw = fw + rwc; }
             u1[c] = cs.bx ? (T)0 : uc + fu * cu;
             v1[c] = cs.by ? (T)0 : vc + fv * cu;
             w1[c] = cs.bz ? (T)0 : wc + fw * cu;
@@ -451,24 +424,20 @@ __global__ __launch_bounds__(kSN, 4) void k_jac_stream(Grid g, int ntx, int nty,
         vm2 = vm1; vm1 = vc; vc = vp1; vp1 = vp2; vp2 = v3;
         wm1 = wc; wc = wp1; wp1 = w2;
         c0 = c1; c1 = c2; c2 = c3; c3 = c4;
-        pcur = pn; rpc = rpn; ruc = run; rvc = rvn; rwc = rwn;
+        pcur = pn;
         hu = hun; hv = hvn; hw = hwn; hc1 = hc2;
-        rc0 = rc1; rc1 = rc2; rp0 = rp0n; rrp = rrpn; rw1 = rw1n;
+        rc0 = rc1; rc1 = rc2; rp0 = rp0n; rw1 = rw1n;
     }
 }
 
-template <typename T, bool HAS_R, bool SEL>
+template <typename T, bool SEL>
 __global__ __launch_bounds__(kListBlock) void k_colour(Grid g, const int32_t *__restrict__ list, int n,
                                                        const uint8_t *__restrict__ pc, T *p, T *u, T *v, T *w,
-                                                       T *q, T cu, T cp, T pin_out,
-                                                       const float *__restrict__ Rp,
-                                                       const float *__restrict__ Ru,
-                                                       const float *__restrict__ Rv,
-                                                       const float *__restrict__ Rw) {
+                                                       T *q, T cu, T cp, T pin_out) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     const int64_t c = list[t];
-    colour_voxel<T, HAS_R, SEL>(g, c, pc[c], p, u, v, w, q, cu, cp, pin_out, Rp, Ru, Rv, Rw);
+    colour_voxel<T, SEL>(g, c, pc[c], p, u, v, w, q, cu, cp, pin_out);
 }
 
 template <typename T>
@@ -494,16 +463,15 @@ __global__ void k_outlet_plane(Grid g, const uint8_t *__restrict__ pc, const dou
 static inline unsigned lblocks(int n) { return (unsigned)((n + kListBlock - 1) / kListBlock); }
 static inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// The FP64 fast modes' launchers (per-field state, no residual); the
-// kernels above are generic in the value type and the residual term.
+// The FP64 fast modes' launchers (per-field state).
 template <bool SEL>
 static void jacobi_pair(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0, const double *u0,
                         const double *v0, const double *w0, const double *q0, double *p1, double *u1, double *v1,
                         double *w1, double *q1, double cu, double cp, double pin_out, hipStream_t s) {
-    hipLaunchKernelGGL((k_jac_vel<double, false, SEL>), dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all,
-                       L.n_all, pc, p0, u0, v0, w0, q0, u1, v1, w1, nullptr, nullptr, nullptr, cu);
-    hipLaunchKernelGGL((k_jac_pres<double, false>), dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all,
-                       L.n_all, pc, p0, u1, v1, w1, p1, q1, nullptr, cp, pin_out);
+    hipLaunchKernelGGL((k_jac_vel<double, SEL>), dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all,
+                       pc, p0, u0, v0, w0, q0, u1, v1, w1, cu);
+    hipLaunchKernelGGL(k_jac_pres<double>, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc, p0,
+                       u1, v1, w1, p1, q1, cp, pin_out);
 }
 
 void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0, const double *u0,
@@ -523,14 +491,14 @@ static void jacobi_stage_t(int stage, const Grid &g, const FluidLists &L, const 
                            hipStream_t s) {
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
     if (stage == JAC_FIRST)
-        hipLaunchKernelGGL((k_jac_first<double, false, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0,
-                           w0, q0, p1, u1, v1, w1, nullptr, nullptr, nullptr, cu);
+        hipLaunchKernelGGL((k_jac_first<double, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, q0,
+                           p1, u1, v1, w1, cu);
     else if (stage == JAC_LAG)
-        hipLaunchKernelGGL((k_jac_lag<double, false, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0,
-                           q0, p1, u1, v1, w1, nullptr, nullptr, nullptr, nullptr, cu, cp, pin_out);
+        hipLaunchKernelGGL((k_jac_lag<double, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, q0, p1,
+                           u1, v1, w1, cu, cp, pin_out);
     else
-        hipLaunchKernelGGL((k_jac_pres<double, false>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0,
-                           p1, q1, nullptr, cp, pin_out);
+        hipLaunchKernelGGL(k_jac_pres<double>, grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, p1, q1, cp,
+                           pin_out);
 }
 
 void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0,
@@ -608,18 +576,18 @@ void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, c
     const int nseg = (g.nz + zs - 1) / zs;
     const dim3 grid((unsigned)(ntx * nty * nseg)), blk(kSN);
     if (stage == JAC_FIRST)
-        hipLaunchKernelGGL((k_jac_stream<double, false, JAC_FIRST>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0,
-                           v0, w0, q0, p1, u1, v1, w1, nullptr, nullptr, nullptr, nullptr, cu, cp, pin_out);
+        hipLaunchKernelGGL((k_jac_stream<double, JAC_FIRST>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0, w0,
+                           q0, p1, u1, v1, w1, cu, cp, pin_out);
     else
-        hipLaunchKernelGGL((k_jac_stream<double, false, JAC_LAG>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0,
-                           w0, q0, p1, u1, v1, w1, nullptr, nullptr, nullptr, nullptr, cu, cp, pin_out);
+        hipLaunchKernelGGL((k_jac_stream<double, JAC_LAG>), grid, blk, 0, s, g, ntx, nty, zs, pc, p0, u0, v0, w0, q0,
+                           p1, u1, v1, w1, cu, cp, pin_out);
 }
 
 template <bool SEL>
 static void colour_stage(const Grid &g, const int32_t *list, int n, const uint8_t *pc, double *p, double *u,
                          double *v, double *w, double *q, double cu, double cp, double pin_out, hipStream_t s) {
-    hipLaunchKernelGGL((k_colour<double, false, SEL>), dim3(lblocks(n)), dim3(kListBlock), 0, s, g, list, n, pc, p,
-                       u, v, w, q, cu, cp, pin_out, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_colour<double, SEL>), dim3(lblocks(n)), dim3(kListBlock), 0, s, g, list, n, pc, p, u, v, w,
+                       q, cu, cp, pin_out);
 }
 
 void launch_colour(const Grid &g, const FluidLists &L, const uint8_t *pc, double *p, double *u, double *v, double *w,

@@ -25,20 +25,17 @@ __device__ __forceinline__ bool on_outlet(const Grid &g, int64_t c) {
 }
 
 // Symplectic Jacobi, velocity part: (p0,u0,v0,w0) -> (u1,v1,w1) at c.
-template <typename T, bool HAS_R, bool SEL>
+template <typename T, bool SEL>
 __device__ __forceinline__ void vel_voxel(const Grid &g, int64_t c, uint8_t pcv, const T *__restrict__ p0,
                                           const T *__restrict__ u0, const T *__restrict__ v0,
                                           const T *__restrict__ w0, const T *__restrict__ q0,
-                                          T *__restrict__ u1, T *__restrict__ v1, T *__restrict__ w1,
-                                          const float *__restrict__ Ru, const float *__restrict__ Rv,
-                                          const float *__restrict__ Rw, T cu) {
+                                          T *__restrict__ u1, T *__restrict__ v1, T *__restrict__ w1, T cu) {
     const Cases cs = decode(pcv);
     const bool out = on_outlet(g, c);
     const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
     T fu, fv, fw;
     if (SEL) fast_force_s<T>(cs, c, g.sx, g.sy, p0, u0, v0, w0, out ? q0 : nullptr, qc, fu, fv, fw);
     else fast_force_p<T>(cs, c, g.sx, g.sy, p0, u0, v0, w0, out ? q0 : nullptr, qc, fu, fv, fw);
-    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
     u1[c] = cs.bx ? (T)0 : u0[c] + fu * cu;
     v1[c] = cs.by ? (T)0 : v0[c] + fv * cu;
     w1[c] = cs.bz ? (T)0 : w0[c] + fw * cu;
@@ -47,11 +44,11 @@ __device__ __forceinline__ void vel_voxel(const Grid &g, int64_t c, uint8_t pcv,
 // Symplectic Jacobi, pressure part: divergence of the new velocities (+1
 // neighbours wrapped on periodic axes, 0 where solid) and the old pressure;
 // on the outlet plane also q1 = pin_out - c2*dt*div (residual excluded).
-template <typename T, bool HAS_R>
+template <typename T>
 __device__ __forceinline__ void pres_voxel(const Grid &g, int64_t c, uint8_t pcv, const T *__restrict__ p0,
                                            const T *__restrict__ u1, const T *__restrict__ v1,
                                            const T *__restrict__ w1, T *__restrict__ p1, T *__restrict__ q1,
-                                           const float *__restrict__ Rp, T cp, T pin_out) {
+                                           T cp, T pin_out) {
     const Cases cs = decode(pcv);
     int i, j, k;
     cell_ijk(g, c, i, j, k);
@@ -63,9 +60,7 @@ __device__ __forceinline__ void pres_voxel(const Grid &g, int64_t c, uint8_t pcv
     const T vy = fwd_solid(cs.cy) ? (T)0 : v1[cyp];
     const T ux = fwd_solid(cs.cx) ? (T)0 : u1[cxp];
     const T div0 = ((((wz - wc) + vy) - vc) + ux) - uc;
-    T div = div0;
-    if (HAS_R) div = div + (T)Rp[c];
-    p1[c] = p0[c] - div * cp;
+    p1[c] = p0[c] - div0 * cp;
     if (i == g.nz + 1) q1[g.qat(j, k)] = pin_out - div0 * (T)(kC2 * kDt);
 }
 
@@ -130,13 +125,10 @@ __device__ __forceinline__ T lag_div0(const Grid &g, int64_t n, uint8_t pcn, con
 }
 
 // p^n at a fluid interior cell n off the outlet plane.
-template <typename T, bool HAS_R>
+template <typename T>
 __device__ __forceinline__ T lag_p(const Grid &g, int64_t n, uint8_t pcn, const T *__restrict__ p0,
-                                   const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
-                                   const float *__restrict__ Rp, T cp) {
-    T div = lag_div0<T>(g, n, pcn, u, v, w);
-    if (HAS_R) div = div + (T)Rp[n];
-    return p0[n] - div * cp;
+                                   const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w, T cp) {
+    return p0[n] - lag_div0<T>(g, n, pcn, u, v, w) * cp;
 }
 
 // Outlet-plane buffer q^n at a fluid interior cell n of z = nz+1.
@@ -150,44 +142,41 @@ __device__ __forceinline__ T lag_q(const Grid &g, int64_t n, uint8_t pcn, const 
 // axis (stride s, index idx of c on that axis, n_ax interior cells, periodic
 // flag per): recomputed at the neighbour or at its periodic image; a wall
 // halo keeps its stored value (p, or q on the outlet plane).
-template <typename T, bool HAS_R>
+template <typename T>
 __device__ __forceinline__ T lag_back(const Grid &g, int64_t c, int64_t s, int idx, int n_ax, bool per, bool out,
                                       int64_t qn, const uint8_t *__restrict__ pc, const T *__restrict__ p0,
                                       const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
-                                      const T *__restrict__ q0, const float *__restrict__ Rp, T cp, T pin_out) {
+                                      const T *__restrict__ q0, T cp, T pin_out) {
     int64_t n = c - s;
     if (idx == 2) {
         if (!per) return out ? q0[qn] : p0[n];
         n += (int64_t)n_ax * s;
     }
-    return out ? lag_q<T>(g, n, pc[n], u, v, w, pin_out) : lag_p<T, HAS_R>(g, n, pc[n], p0, u, v, w, Rp, cp);
+    return out ? lag_q<T>(g, n, pc[n], u, v, w, pin_out) : lag_p<T>(g, n, pc[n], p0, u, v, w, cp);
 }
 
 // One lagged pass at fluid cell c: reads (p^{n-1}, u^n, v^n, w^n) = set 0,
 // writes (p^n, u^{n+1}, v^{n+1}, w^{n+1}) into set 1.
-template <typename T, bool HAS_R, bool SEL>
+template <typename T, bool SEL>
 __device__ __forceinline__ void lag_voxel(const Grid &g, int64_t c, uint8_t pcv, const uint8_t *__restrict__ pc,
                                           const T *__restrict__ p0, const T *__restrict__ u0,
                                           const T *__restrict__ v0, const T *__restrict__ w0,
                                           const T *__restrict__ q0, T *__restrict__ p1, T *__restrict__ u1,
-                                          T *__restrict__ v1, T *__restrict__ w1, const float *__restrict__ Rp,
-                                          const float *__restrict__ Ru, const float *__restrict__ Rv,
-                                          const float *__restrict__ Rw, T cu, T cp, T pin_out) {
+                                          T *__restrict__ v1, T *__restrict__ w1, T cu, T cp, T pin_out) {
     const Cases cs = decode(pcv);
     int i, j, k;
     cell_ijk(g, c, i, j, k);
     const bool out = i == g.nz + 1;
-    const T Pc = out ? pin_out : lag_p<T, HAS_R>(g, c, pcv, p0, u0, v0, w0, Rp, cp);
+    const T Pc = out ? pin_out : lag_p<T>(g, c, pcv, p0, u0, v0, w0, cp);
     T dpx = (T)0, dpy = (T)0, dpz = (T)0;
     if (!cs.bx)
-        dpx = Pc - lag_back<T, HAS_R>(g, c, 1, k, g.nx, g.px, out, g.qat(j, k - 1), pc, p0, u0, v0, w0, q0, Rp,
-                                      cp, pin_out);
+        dpx = Pc - lag_back<T>(g, c, 1, k, g.nx, g.px, out, g.qat(j, k - 1), pc, p0, u0, v0, w0, q0, cp, pin_out);
     if (!cs.by)
-        dpy = Pc - lag_back<T, HAS_R>(g, c, g.sx, j, g.ny, g.py, out, g.qat(j - 1, k), pc, p0, u0, v0, w0, q0,
-                                      Rp, cp, pin_out);
+        dpy = Pc - lag_back<T>(g, c, g.sx, j, g.ny, g.py, out, g.qat(j - 1, k), pc, p0, u0, v0, w0, q0, cp,
+                               pin_out);
     if (!cs.bz) {
         const int64_t n = c - g.sy;
-        dpz = Pc - (i == 2 ? p0[n] : lag_p<T, HAS_R>(g, n, pc[n], p0, u0, v0, w0, Rp, cp));
+        dpz = Pc - (i == 2 ? p0[n] : lag_p<T>(g, n, pc[n], p0, u0, v0, w0, cp));
     }
     const T uc = u0[c], vc = v0[c], wc = w0[c];
     T su, sv, sw;
@@ -196,7 +185,6 @@ __device__ __forceinline__ void lag_voxel(const Grid &g, int64_t c, uint8_t pcv,
     T fu = cs.bx ? (T)0 : su * INV_RE - dpx;
     T fv = cs.by ? (T)0 : sv * INV_RE - dpy;
     T fw = cs.bz ? (T)0 : sw * INV_RE - dpz;
-    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
     u1[c] = cs.bx ? (T)0 : uc + fu * cu;
     v1[c] = cs.by ? (T)0 : vc + fv * cu;
     w1[c] = cs.bz ? (T)0 : wc + fw * cu;
@@ -635,18 +623,15 @@ __device__ __forceinline__ void fold_voxel4(int64_t c, double *Xp, double *Xu, d
 }
 
 // 3-colour SOR: one cell of the current colour, in place.
-template <typename T, bool HAS_R, bool SEL>
+template <typename T, bool SEL>
 __device__ __forceinline__ void colour_voxel(const Grid &g, int64_t c, uint8_t pcv, T *p, T *u, T *v, T *w,
-                                             T *q, T cu, T cp, T pin_out, const float *__restrict__ Rp,
-                                             const float *__restrict__ Ru, const float *__restrict__ Rv,
-                                             const float *__restrict__ Rw) {
+                                             T *q, T cu, T cp, T pin_out) {
     const Cases cs = decode(pcv);
     const bool out = on_outlet(g, c);
     const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
     T fu, fv, fw;
     if (SEL) fast_force_s<T>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, qc, fu, fv, fw);
     else fast_force_p<T>(cs, c, g.sx, g.sy, p, u, v, w, out ? q : nullptr, qc, fu, fv, fw);
-    if (HAS_R) { fu = fu + (T)Ru[c]; fv = fv + (T)Rv[c]; fw = fw + (T)Rw[c]; }
     const T un = cs.bx ? (T)0 : u[c] + fu * cu;
     const T vn = cs.by ? (T)0 : v[c] + fv * cu;
     const T wn = cs.bz ? (T)0 : w[c] + fw * cu;
@@ -655,9 +640,7 @@ __device__ __forceinline__ void colour_voxel(const Grid &g, int64_t c, uint8_t p
     const T vy = fwd_solid(cs.cy) ? (T)0 : v[c + g.sx];
     const T ux = fwd_solid(cs.cx) ? (T)0 : u[c + 1];
     const T div0 = ((((wz - wn) + vy) - vn) + ux) - un;
-    T div = div0;
-    if (HAS_R) div = div + (T)Rp[c];
-    p[c] = p[c] - div * cp;
+    p[c] = p[c] - div0 * cp;
     if (out) q[qc] = pin_out - div0 * (T)(kC2 * kDt);
 }
 
