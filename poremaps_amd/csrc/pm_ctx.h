@@ -71,17 +71,16 @@ struct pm_ctx {
     double kprev = 0.0;   // permeability of the previous convergence test
     double phi = 0.0;
     int64_t fluid = 0;
-    // fluid-list column (rows x cells) for the next pm_load_geometry (0: whole
-    // extent; prm_tile_x -1: by state layout).  On the 512^3 field 64 x 64
-    // is best for the interleaved FP32 state (flat from 16 x 32 to 128 x 128;
-    // lagged pass 0.96 ms against 1.05 ms with whole rows and 1.43 ms in plain
-    // (z, y, x) order) and 64 rows of whole rows for the per-field FP64 state
-    // (sweep 1.93 ms against 2.09 ms in 64 x 64 columns)
     // exact-order sweep: LDS-tiled wavefront (default) or the single-workgroup
     // / per-hyperplane forms (POREMAPS_REF=tiles|onewg|planes)
     int ref_impl = 0;
+    // fluid-list column (rows x cells) for the next pm_load_geometry (0: whole
+    // extent).  On the 512^3 field 64 x 64 is best: interleaved FP32 lagged
+    // pass 0.96 ms against 1.05 ms with whole rows and 1.43 ms in plain
+    // (z, y, x) order (flat from 16 x 32 to 128 x 128); FP64 record pass
+    // 1.60 ms against 1.64 ms with whole rows
     int prm_tile_y = 64;
-    int prm_tile_x = -1;
+    int prm_tile_x = 64;
     // Jacobi kernels: velocity + pressure pair over the fluid list, one lagged
     // pass over the fluid list, or one lagged dense 2.5-D pass (z segments of
     // stream_zs planes)

@@ -99,9 +99,10 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     if (fast && !dc && jac) per += 32;
     // interleaved correction sets (16 B each: two for Jacobi, one for SOR),
     // continuity residual (4 B) and momentum residual (16 B per fluid cell);
-    // Jacobi: the lagged pass's records (8 B) and gathered residuals (16 B)
-    // per fluid cell
-    if (dc) per += (jac ? 32 : 16) + 4 + 16 + (jac ? 24 : 0);
+    // Jacobi: the gathered residuals (16 B per fluid cell)
+    if (dc) per += (jac ? 32 : 16) + 4 + 16 + (jac ? 16 : 0);
+    // Jacobi: the lagged pass's records (8 B per fluid cell)
+    if (fast && jac) per += 8;
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
     // keeps separate colour lists
     if (fast) per += (jac || dc) ? 4 : 8;
@@ -319,8 +320,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         // plain (z, y, x) order, 1.05 ms in 64-row slabs of whole rows and
         // 0.96 ms in 64 x 64 columns.  0: whole extent.
         const int ty = c->prm_tile_y > 0 ? std::min(c->prm_tile_y, g.ny) : g.ny;
-        const int want_tx = c->prm_tile_x >= 0 ? c->prm_tile_x : (c->dc() ? 64 : 0);
-        const int tx = want_tx > 0 ? std::min(want_tx, g.nx) : g.nx;
+        const int tx = c->prm_tile_x > 0 ? std::min(c->prm_tile_x, g.nx) : g.nx;
         std::vector<int32_t> all;
         std::vector<int32_t> col[3];
         all.reserve((size_t)fluid);
@@ -338,7 +338,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
                     }
                 }
         c->lists.tile_y = c->prm_tile_y > 0 ? ty : 0;
-        c->lists.tile_x = want_tx > 0 ? tx : 0;
+        c->lists.tile_x = c->prm_tile_x > 0 ? tx : 0;
         const char *ld = std::getenv("POREMAPS_SWEEP_LOADS");
         c->lists.select_loads = !(ld && std::strcmp(ld, "branch") == 0);
         c->lists.use_rec = c->jac_impl == JI_RECORDS;
@@ -356,10 +356,8 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         c->lists.n_all = (int)all.size();
         RC(dalloc(&c->lists.all, std::max<int64_t>(1, c->lists.n_all)));
         if (c->dc()) RC(dalloc(&c->R4.m, std::max<int64_t>(1, c->lists.n_all)));
-        if (c->dc() && c->jacobi()) {
-            RC(dalloc(&c->R4.q, std::max<int64_t>(1, c->lists.n_all)));
-            RC(dalloc(&c->lists.rec, std::max<int64_t>(1, c->lists.n_all)));
-        }
+        if (c->jacobi()) RC(dalloc(&c->lists.rec, std::max<int64_t>(1, c->lists.n_all)));
+        if (c->dc() && c->jacobi()) RC(dalloc(&c->R4.q, std::max<int64_t>(1, c->lists.n_all)));
         HIPCHK(hipMemcpy(c->lists.all, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
         if (sor && c->dc()) {
             c->lists.col_in_all = true;
@@ -469,6 +467,7 @@ static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *bu
         std::snprintf(buf, len, "interleaved-lagged-%s",
                       v.impl == JI_RECORDS ? "records" : v.select_loads ? "eager" : "branch");
     else if (v.impl == JI_STREAM) std::snprintf(buf, len, "dense-2.5d-zs%d", v.zs);
+    else if (v.impl == JI_RECORDS) std::snprintf(buf, len, "list-lagged-records");
     else std::snprintf(buf, len, "%s-%s", v.impl == JI_PAIR ? "list-pair" : "list-lagged",
                        v.select_loads ? "select" : "branch");
     return buf;
@@ -486,7 +485,8 @@ static void tuner_setup(pm_ctx *c) {
     // the first window (caches and clocks still cold after the load) is run
     // with the last candidate and not scored
     const SweepVariant soa[] = {{JI_STREAM, true, 16}, {JI_PAIR, true, 64}, {JI_PAIR, false, 64},
-                                {JI_LIST, true, 64}, {JI_STREAM, true, 64}, {JI_STREAM, true, 16}};
+                                {JI_LIST, true, 64}, {JI_RECORDS, true, 64}, {JI_STREAM, true, 64},
+                                {JI_STREAM, true, 16}};
     const SweepVariant aos[] = {{JI_RECORDS, true, 64}, {JI_PAIR, true, 64}, {JI_LIST, false, 64},
                                 {JI_LIST, true, 64}, {JI_RECORDS, true, 64}};
     if (c->dc())

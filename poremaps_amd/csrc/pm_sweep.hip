@@ -110,6 +110,19 @@ __global__ __launch_bounds__(kListBlock) void k_colour4(Grid g, const int32_t *_
     colour_voxel4(g, list[t0 + t], t0 + t, pc, S, q, R, cu, cp, 0.f);
 }
 
+// FP64 per-field lagged pass from the per-voxel records (lag_voxel_rec).
+template <typename T>
+__global__ __launch_bounds__(kListBlock) void k_jac_rec(Grid g, const uint2 *__restrict__ rec, int n,
+                                                        const T *__restrict__ p0, const T *__restrict__ u0,
+                                                        const T *__restrict__ v0, const T *__restrict__ w0,
+                                                        const T *__restrict__ q0, T *__restrict__ p1,
+                                                        T *__restrict__ u1, T *__restrict__ v1, T *__restrict__ w1,
+                                                        T cu, T cp, T pin_out) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    lag_voxel_rec<T>(g, t, rec, p0, u0, v0, w0, q0, p1, u1, v1, w1, cu, cp, pin_out);
+}
+
 // Lagged pass from the per-voxel records (lag_voxel4_rec).
 __global__ __launch_bounds__(kListBlock) void k_jac4_rec(Grid g, const uint2 *__restrict__ rec, int n,
                                                          const float4 *__restrict__ S0, const float *__restrict__ q0,
@@ -493,6 +506,9 @@ static void jacobi_stage_t(int stage, const Grid &g, const FluidLists &L, const 
     if (stage == JAC_FIRST)
         hipLaunchKernelGGL((k_jac_first<double, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, q0,
                            p1, u1, v1, w1, cu);
+    else if (stage == JAC_LAG && L.rec && L.use_rec)
+        hipLaunchKernelGGL(k_jac_rec<double>, grid, blk, 0, s, g, L.rec, L.n_all, p0, u0, v0, w0, q0, p1, u1, v1, w1,
+                           cu, cp, pin_out);
     else if (stage == JAC_LAG)
         hipLaunchKernelGGL((k_jac_lag<double, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, q0, p1,
                            u1, v1, w1, cu, cp, pin_out);

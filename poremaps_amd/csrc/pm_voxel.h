@@ -502,6 +502,106 @@ __device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2
     __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c]));
 }
 
+// lag_voxel (per-field state, SEL form) from the voxel's record, every load
+// issued before any condition: the state of the cell, its 12 stencil
+// neighbours per field, the pressures of the three recompute targets and the
+// six diagonal velocities their divergences need.  The same expressions in the
+// same order as lag_voxel, so the same values.
+template <typename T>
+__device__ __forceinline__ void lag_voxel_rec(const Grid &g, int t, const uint2 *__restrict__ rec,
+                                              const T *__restrict__ p0, const T *__restrict__ u0,
+                                              const T *__restrict__ v0, const T *__restrict__ w0,
+                                              const T *__restrict__ q0, T *__restrict__ p1, T *__restrict__ u1,
+                                              T *__restrict__ v1, T *__restrict__ w1, T cu, T cp, T pin_out) {
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const v2u rc = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&rec[t]));
+    const int64_t c = (int64_t)rc.x;
+    const uint8_t pcv = (uint8_t)(rc.y & 0xff), pcx = (uint8_t)((rc.y >> 8) & 0xff),
+                  pcy = (uint8_t)((rc.y >> 16) & 0xff), pcz = (uint8_t)(rc.y >> 24);
+    const int64_t sx = g.sx, sy = g.sy;
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const bool out = i == g.nz + 1;
+    const bool wx = k == 2 && g.px, wy = j == 2 && g.py;
+    int64_t nxn, nyn, nzn;
+    lag_targets(g, c, i, j, k, nxn, nyn, nzn);
+    // the cell and its stencil
+    const T pc_ = p0[c], uc = u0[c], vc = v0[c], wc = w0[c];
+    const T uxm = u0[c - 1], uxp = u0[c + 1];
+    const T uym2 = u0[c - 2 * sx], uym = u0[c - sx], uyp = u0[c + sx], uyp2 = u0[c + 2 * sx];
+    const T uzm2 = u0[c - 2 * sy], uzm = u0[c - sy], uzp = u0[c + sy], uzp2 = u0[c + 2 * sy];
+    const T vym = v0[c - sx], vyp = v0[c + sx];
+    const T vxm2 = v0[c - 2], vxm = v0[c - 1], vxp = v0[c + 1], vxp2 = v0[c + 2];
+    const T vzm2 = v0[c - 2 * sy], vzm = v0[c - sy], vzp = v0[c + sy], vzp2 = v0[c + 2 * sy];
+    const T wzm = w0[c - sy], wzp = w0[c + sy];
+    const T wxm2 = w0[c - 2], wxm = w0[c - 1], wxp = w0[c + 1], wxp2 = w0[c + 2];
+    const T wym2 = w0[c - 2 * sx], wym = w0[c - sx], wyp = w0[c + sx], wyp2 = w0[c + 2 * sx];
+    // recompute targets: pressures and the +1 velocities their divergences need
+    const T pxt = p0[nxn], pyt = p0[nyn], pzt = p0[nzn];
+    const T vx_d = v0[nxn + sx], wx_d = w0[nxn + sy];
+    const T uy_d = u0[nyn + 1], wy_d = w0[nyn + sy];
+    const T uz_d = u0[nzn + 1], vz_d = v0[nzn + sx];
+    // the -x / -y targets' own velocities and +1 neighbour: already loaded
+    // unless the target is a periodic image
+    T uX = uxm, vX = vxm, wX = wxm, uXp = uc;
+    T uY = uym, vY = vym, wY = wym, vYp = vc;
+    if (wx) { uX = u0[nxn]; vX = v0[nxn]; wX = w0[nxn]; uXp = u0[nxn + 1]; }
+    if (wy) { uY = u0[nyn]; vY = v0[nyn]; wY = w0[nyn]; vYp = v0[nyn + sx]; }
+
+    const Cases cs = decode(pcv);
+    // lag_div0 at the cell and at the three targets
+    const auto div0 = [](uint8_t code, T u, T v, T w, T uxp1, T vyp1, T wzp1) {
+        const Cases d = decode(code);
+        const T wz = fwd_solid(d.cz) ? (T)0 : wzp1;
+        const T vy = fwd_solid(d.cy) ? (T)0 : vyp1;
+        const T ux = fwd_solid(d.cx) ? (T)0 : uxp1;
+        return ((((wz - w) + vy) - v) + ux) - u;
+    };
+    const T C2DT = (T)(kC2 * kDt);
+    const T Pc = out ? pin_out : pc_ - div0(pcv, uc, vc, wc, uxp, vyp, wzp) * cp;
+    T dpx = (T)0, dpy = (T)0, dpz = (T)0;
+    if (!cs.bx) {
+        T bx;
+        if (k == 2 && !g.px) bx = out ? q0[g.qat(j, k - 1)] : p0[c - 1];
+        else {
+            const T d = div0(pcx, uX, vX, wX, uXp, vx_d, wx_d);
+            bx = out ? pin_out - d * C2DT : pxt - d * cp;
+        }
+        dpx = Pc - bx;
+    }
+    if (!cs.by) {
+        T by;
+        if (j == 2 && !g.py) by = out ? q0[g.qat(j - 1, k)] : p0[c - sx];
+        else {
+            const T d = div0(pcy, uY, vY, wY, uy_d, vYp, wy_d);
+            by = out ? pin_out - d * C2DT : pyt - d * cp;
+        }
+        dpy = Pc - by;
+    }
+    if (!cs.bz) dpz = Pc - (i == 2 ? pzt : pzt - div0(pcz, uzm, vzm, wzm, uz_d, vz_d, wc) * cp);
+    // visc_sums, SEL form
+    const T lxx = ((cs.bx ? (T)0 : uxm) - (T)2.0 * uc) + (fwd_solid(cs.cx) ? (T)0 : uxp);
+    const T lyy = ((cs.by ? (T)0 : vym) - (T)2.0 * vc) + (fwd_solid(cs.cy) ? (T)0 : vyp);
+    const T lzz = ((cs.bz ? (T)0 : wzm) - (T)2.0 * wc) + (fwd_solid(cs.cz) ? (T)0 : wzp);
+    const T dyx = d2_pick<T>(cs.cx, vxm2, vxm, vc, vxp, vxp2);
+    const T dzx = d2_pick<T>(cs.cx, wxm2, wxm, wc, wxp, wxp2);
+    const T dxy = d2_pick<T>(cs.cy, uym2, uym, uc, uyp, uyp2);
+    const T dzy = d2_pick<T>(cs.cy, wym2, wym, wc, wyp, wyp2);
+    const T dxz = d2_pick<T>(cs.cz, uzm2, uzm, uc, uzp, uzp2);
+    const T dyz = d2_pick<T>(cs.cz, vzm2, vzm, vc, vzp, vzp2);
+    const T su = (lxx + dxy) + dxz;
+    const T sv = (lyy + dyz) + dyx;
+    const T sw = (lzz + dzx) + dzy;
+    const T INV_RE = (T)(1.0 / kRe);
+    const T fu = cs.bx ? (T)0 : su * INV_RE - dpx;
+    const T fv = cs.by ? (T)0 : sv * INV_RE - dpy;
+    const T fw = cs.bz ? (T)0 : sw * INV_RE - dpz;
+    u1[c] = cs.bx ? (T)0 : uc + fu * cu;
+    v1[c] = cs.by ? (T)0 : vc + fv * cu;
+    w1[c] = cs.bz ? (T)0 : wc + fw * cu;
+    p1[c] = Pc;
+}
+
 // One 3-colour SOR update at fluid cell c on the interleaved state, in place
 // (colour_voxel): velocities from the current state, then the pressure from
 // the new velocities at c and the current ones of its +1 neighbours.  Every

@@ -68,7 +68,7 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
         if form is None:
             # FP32+DC runs on the interleaved state: pair and the three lagged
             # forms (eager, branch, records)
-            assert len(tried) == (4 if precision == "f32dc" else 5) and name in tried
+            assert len(tried) == (4 if precision == "f32dc" else 6) and name in tried
 
 
 def test_divergent_relaxation_restarts_slower():
