@@ -144,16 +144,21 @@ def time_kernels(s, n):
 
 
 def measured_traffic(spec, form, precision):
-    """HBM bytes per launch of the sweep kernel, from the committed rocprofv3
-    counter summary (read requests by size + WRITE_SIZE, calibrated on a
-    device copy of known size), when the run uses the workload and kernel
+    """HBM bytes per sweep of the sweep kernels (one launch of the Jacobi
+    record pass, three colour launches of the SOR), from the committed
+    rocprofv3 counter summary (read requests by size + WRITE_SIZE, calibrated
+    on a device copy of known size), when the run uses the workload and kernel
     form it was measured on; else None."""
-    if spec["name"] != "grf_512" or form != "interleaved-lagged-records" or precision != "f32dc":
+    known = {"interleaved-lagged-records": ("jacobi_f32dc_grf512_hbm_traffic.md",
+                                            r"traffic_bytes_per_launch k_jac4_rec: (\d+)"),
+             "interleaved-colour-split": ("sor3_f32dc_grf512_hbm_traffic.md",
+                                          r"traffic_bytes_per_sweep k_colour4_split x3: (\d+)")}
+    if spec["name"] != "grf_512" or form not in known or precision != "f32dc":
         return None
-    path = os.path.join(ROOT, "profiles", "jacobi_f32dc_grf512_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", known[form][0])
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch k_jac4_rec: (\d+)", open(path).read())
+    m = re.search(known[form][1], open(path).read())
     return int(m.group(1)) if m else None
 
 

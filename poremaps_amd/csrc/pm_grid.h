@@ -1,10 +1,11 @@
 // pm_grid.h -- device layout and the per-voxel stencil shared by all kernels.
 //
 // Every field lives in HBM in the reference's halo-2 shape (nz+4, ny+4, nx+4),
-// x fastest, with the row pitch rounded up to 32 elements and a base offset of
-// 30 so that the first interior cell of every row (k = 2) starts a 128-byte
-// (FP32) / 256-byte (FP64) segment: the interior rows a wavefront streams are
-// aligned, the halo columns spill into the neighbouring segments.
+// x fastest, with the row pitch (3 ceil((nx+4)/3) >= nx+4 cells, room for a
+// colour-split row) rounded up to 32 elements and a base offset of 30 so that
+// the first interior cell of every row (k = 2) starts a 128-byte (FP32) /
+// 256-byte (FP64) segment: the interior rows a wavefront streams are aligned,
+// the halo columns spill into the neighbouring segments.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,6 +28,7 @@ struct Grid {
     int64_t base;     // element offset of cell (0,0,0)
     int64_t total;    // elements to allocate per field
     int px, py, pz;   // periodic axes
+    int cb;           // block length of a colour-split row, ceil(NX / 3)
 
     __host__ __device__ int64_t at(int i, int j, int k) const {
         return base + (int64_t)i * sy + (int64_t)j * sx + k;
@@ -34,6 +36,19 @@ struct Grid {
     // outlet-plane buffer: one pitched plane
     __host__ __device__ int64_t qat(int j, int k) const { return base + (int64_t)j * sx + k; }
     int64_t qtotal() const { return base + sy; }
+
+    // Colour-split rows (the 3-colour SOR correction state): a row's cells
+    // are stored as three blocks by k mod 3, cell k at b * cb + k / 3 with
+    // b = k mod 3.  The colour (i + j + k) mod 3 of a row's cells is one
+    // block, so the cells one colour pass updates lie contiguous in every
+    // row, and so do their -+y / -+z neighbours (same block of the
+    // neighbouring rows) and their -+x neighbours (the other two blocks).
+    // Same pitches, base and size as the natural layout (3 * cb <= sx).
+    __host__ __device__ int64_t ks(int k) const { return (int64_t)(k % 3) * cb + k / 3; }
+    __host__ __device__ int64_t at_s(int i, int j, int k) const {
+        return base + (int64_t)i * sy + (int64_t)j * sx + ks(k);
+    }
+    __host__ __device__ int64_t qat_s(int j, int k) const { return base + (int64_t)j * sx + ks(k); }
 };
 
 // FP32 residual of the interleaved defect-correction state: the continuity
@@ -54,7 +69,8 @@ inline Grid make_grid(int nx, int ny, int nz, int bc) {
     Grid g;
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.NX = nx + 4; g.NY = ny + 4; g.NZ = nz + 4;
-    g.sx = ((int64_t)g.NX + 31) / 32 * 32;
+    g.cb = (g.NX + 2) / 3;
+    g.sx = ((int64_t)3 * g.cb + 31) / 32 * 32;
     g.sy = g.sx * g.NY;
     g.base = 30;
     g.total = g.base + g.sy * g.NZ;

@@ -61,6 +61,10 @@ struct FluidLists {
     // the record form of the lagged pass
     uint2 *rec = nullptr;
     bool use_rec = false;
+    // FP32+DC 3-colour SOR: the correction state in colour-split rows
+    // (Grid::at_s) and rec holding the colour passes' records (colour-split
+    // offset + packed code, block, outlet flag)
+    bool split = false;
 };
 
 // FP64 fast modes (per-field state p, u, v, w): the velocity + pressure
@@ -84,11 +88,13 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s);
 void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
                     float cp, hipStream_t s);
+// The per-voxel records L.rec of the list: the lagged pass's, or the colour
+// passes' when L.split.
 void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, hipStream_t s);
 void launch_gather_rq(const Grid &g, const FluidLists &L, Res4 R, hipStream_t s);
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
                       const double *v, const double *w, const double *q, Res4 R, hipStream_t s);
-void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStream_t s);
+void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, bool split, hipStream_t s);
 // X += d, d = 0 over the fluid list; d's halo copies need a ghost fill after.
 void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s);
 void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0,

@@ -110,6 +110,22 @@ __global__ __launch_bounds__(kListBlock) void k_colour4(Grid g, const int32_t *_
     colour_voxel4(g, list[t0 + t], t0 + t, pc, S, q, R, cu, cp, 0.f);
 }
 
+// The same on the colour-split state, from the voxels' records
+// (colour_voxel4_split).
+__global__ __launch_bounds__(kListBlock) void k_colour4_split(Grid g, const uint2 *__restrict__ rec, int n, int t0,
+                                                              float4 *S, float *q, Res4 R, float cu, float cp) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    colour_voxel4_split(g, t0 + t, rec, S, q, R, cu, cp, 0.f);
+}
+
+__global__ __launch_bounds__(kListBlock) void k_colour_records(Grid g, const int32_t *__restrict__ list, int n,
+                                                               const uint8_t *__restrict__ pc, uint2 *rec) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    rec[t] = colour_record(g, list[t], pc);
+}
+
 // FP64 per-field lagged pass from the per-voxel records (lag_voxel_rec).
 template <typename T>
 __global__ __launch_bounds__(kListBlock) void k_jac_rec(Grid g, const uint2 *__restrict__ rec, int n,
@@ -152,22 +168,52 @@ __global__ __launch_bounds__(kListBlock) void k_gather_rq(Grid g, const int32_t 
     R.q[t] = make_float4(R.p[c], R.p[nxn], R.p[nyn], R.p[nzn]);
 }
 
+// Thread -> list position for the once-per-interval kernels.  Plain lists:
+// the identity.  Colour lists back to back (SOR): the i-th voxels of the three
+// colour lists go to three consecutive threads, so that a wavefront walks the
+// domain in space order instead of one colour at a time (the entries keep
+// their list positions, which index the momentum residual).
+struct ListWalk {
+    int off[3], n[3];
+    bool colours;
+};
+
+__device__ __forceinline__ int walk_pos(const ListWalk &W, int T, int n) {
+    if (!W.colours) return T < n ? T : -1;
+    const int q = T % 3, i = T / 3;
+    return i < W.n[q] ? W.off[q] + i : -1;
+}
+
+static ListWalk list_walk(const FluidLists &L, int &threads) {
+    ListWalk W;
+    W.colours = L.col_in_all;
+    int most = 0;
+    for (int q = 0; q < 3; q++) {
+        W.off[q] = L.off_col[q];
+        W.n[q] = L.n_col[q];
+        most = std::max(most, L.n_col[q]);
+    }
+    threads = W.colours ? 3 * most : L.n_all;
+    return W;
+}
+
 __global__ __launch_bounds__(kListBlock) void k_residual4(Grid g, const int32_t *__restrict__ list, int n,
-                                                          const uint8_t *__restrict__ pc,
+                                                          ListWalk W, const uint8_t *__restrict__ pc,
                                                           const double *__restrict__ p,
                                                           const double *__restrict__ u,
                                                           const double *__restrict__ v,
                                                           const double *__restrict__ w,
                                                           const double *__restrict__ q, Res4 R) {
-    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
-    if (t >= n) return;
+    const int t = walk_pos(W, xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x, n);
+    if (t < 0) return;
     const int64_t c = list[t];
     residual_voxel4(g, c, t, pc[c], p, u, v, w, q, R);
 }
 
+template <bool SPLIT>
 __global__ void k_ghost4(Grid g, const uint8_t *__restrict__ G, float4 *S, float *q) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    ghost_line4(g, t, G, S, q);
+    ghost_line4<SPLIT>(g, t, G, S, q);
 }
 
 __global__ __launch_bounds__(kListBlock) void k_fold4(const int32_t *__restrict__ list, int n, double *Xp,
@@ -175,6 +221,15 @@ __global__ __launch_bounds__(kListBlock) void k_fold4(const int32_t *__restrict_
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     fold_voxel4(list[t], Xp, Xu, Xv, Xw, d);
+}
+
+__global__ __launch_bounds__(kListBlock) void k_fold4_split(const int32_t *__restrict__ list,
+                                                            const uint2 *__restrict__ rec, int n, ListWalk W,
+                                                            double *Xp, double *Xu, double *Xv, double *Xw,
+                                                            float4 *d) {
+    const int t = walk_pos(W, xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x, n);
+    if (t < 0) return;
+    fold_voxel4_split(list[t], rec[t].x, Xp, Xu, Xv, Xw, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -553,14 +608,23 @@ void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float
     for (int col = 0; col < 3; col++) {
         const int n = L.n_col[col];
         if (n == 0) continue;
-        hipLaunchKernelGGL(k_colour4, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.all, n, L.off_col[col], pc, S,
-                           q, R, cu, cp);
+        if (L.split)
+            hipLaunchKernelGGL(k_colour4_split, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.rec, n,
+                               L.off_col[col], S, q, R, cu, cp);
+        else
+            hipLaunchKernelGGL(k_colour4, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.all, n, L.off_col[col], pc,
+                               S, q, R, cu, cp);
     }
 }
 
 void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, hipStream_t s) {
     if (L.n_all == 0 || !L.rec) return;
-    hipLaunchKernelGGL(k_lag_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc, L.rec);
+    if (L.split)
+        hipLaunchKernelGGL(k_colour_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc,
+                           L.rec);
+    else
+        hipLaunchKernelGGL(k_lag_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc,
+                           L.rec);
 }
 
 void launch_gather_rq(const Grid &g, const FluidLists &L, Res4 R, hipStream_t s) {
@@ -571,17 +635,30 @@ void launch_gather_rq(const Grid &g, const FluidLists &L, Res4 R, hipStream_t s)
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
                       const double *v, const double *w, const double *q, Res4 R, hipStream_t s) {
     if (L.n_all == 0) return;
-    hipLaunchKernelGGL(k_residual4, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc, p, u, v,
-                       w, q, R);
+    int threads = 0;
+    const ListWalk W = list_walk(L, threads);
+    hipLaunchKernelGGL(k_residual4, dim3(lblocks(threads)), dim3(kListBlock), 0, s, g, L.all, L.n_all, W, pc, p, u,
+                       v, w, q, R);
 }
 
-void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, hipStream_t s) {
-    hipLaunchKernelGGL(k_ghost4, dim3(nblocks(ghost_lines(g), 256)), dim3(256), 0, s, g, G, S, q);
+void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, bool split, hipStream_t s) {
+    if (split)
+        hipLaunchKernelGGL(k_ghost4<true>, dim3(nblocks(ghost_lines(g), 256)), dim3(256), 0, s, g, G, S, q);
+    else
+        hipLaunchKernelGGL(k_ghost4<false>, dim3(nblocks(ghost_lines(g), 256)), dim3(256), 0, s, g, G, S, q);
 }
 
 void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s) {
     if (L.n_all == 0) return;
-    hipLaunchKernelGGL(k_fold4, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, L.all, L.n_all, Xp, Xu, Xv, Xw, d);
+    if (L.split) {
+        int threads = 0;
+        const ListWalk W = list_walk(L, threads);
+        hipLaunchKernelGGL(k_fold4_split, dim3(lblocks(threads)), dim3(kListBlock), 0, s, L.all, L.rec, L.n_all, W,
+                           Xp, Xu, Xv, Xw, d);
+    }
+    else
+        hipLaunchKernelGGL(k_fold4, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, L.all, L.n_all, Xp, Xu, Xv, Xw,
+                           d);
 }
 
 void launch_jacobi_stream(int stage, int zs, const Grid &g, const uint8_t *pc, const double *p0, const double *u0,

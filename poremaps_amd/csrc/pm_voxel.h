@@ -638,6 +638,62 @@ __device__ __forceinline__ void colour_voxel4(const Grid &g, int64_t c, int t, c
     if (out) q[qc] = pin_out - div0 * (float)(kC2 * kDt);
 }
 
+// Record of a fluid voxel for the colour-split SOR pass: its colour-split
+// offset and its packed code, block k mod 3 (bits 8-9) and outlet flag (bit
+// 10).
+__device__ __forceinline__ uint2 colour_record(const Grid &g, int64_t c, const uint8_t *__restrict__ pc) {
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const uint32_t f = (uint32_t)pc[c] | ((uint32_t)(k % 3) << 8) | ((uint32_t)(i == g.nz + 1) << 10);
+    return make_uint2((uint32_t)g.at_s(i, j, k), f);
+}
+
+// colour_voxel4 on the colour-split state from the voxel's record: the same
+// values in the same order, from the same logical cells.  The x neighbours
+// lie in the other blocks of the row: k-1 is block b-1 (slot m-1 from block
+// 0), k+1 block b+1 (slot m+1 from block 2), k-2 and k+2 likewise.  The
+// continuity residual is read from the w lane of the momentum residual.
+__device__ __forceinline__ void colour_voxel4_split(const Grid &g, int t, const uint2 *__restrict__ rec,
+                                                    float4 *S, float *q, Res4 R, float cu, float cp,
+                                                    float pin_out) {
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v2u rr = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&rec[t]));
+    const int64_t c = rr.x;
+    const uint32_t f = rr.y;
+    const int b = (int)((f >> 8) & 3u);
+    const bool out = (f >> 10) & 1u;
+    const int64_t cb = g.cb, sj = g.sx, si = g.sy;
+    const int64_t oM1 = b == 0 ? 2 * cb - 1 : -cb, oP1 = b == 2 ? 1 - 2 * cb : cb;
+    const int64_t oM2 = b == 2 ? -2 * cb : cb - 1, oP2 = b == 0 ? 2 * cb : 1 - cb;
+    const float4 C = S[c];
+    Star4 s;
+    s.XM = S[c + oM1]; s.XP = S[c + oP1]; s.XM2 = S[c + oM2]; s.XP2 = S[c + oP2];
+    s.YM = S[c - sj]; s.YP = S[c + sj]; s.YM2 = S[c - 2 * sj]; s.YP2 = S[c + 2 * sj];
+    s.ZM = S[c - si]; s.ZP = S[c + si]; s.ZM2 = S[c - 2 * si]; s.ZP2 = S[c + 2 * si];
+    const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
+    const float4 Rm = make_float4(r.x, r.y, r.z, 0.f);
+    const float rp = r.w;
+    const int64_t qc = c - (int64_t)(g.nz + 1) * si;
+    float pxm = s.XM.x, pym = s.YM.x;
+    if (out) { pxm = q[qc + oM1]; pym = q[qc - sj]; }
+    const Cases cs = decode((uint8_t)(f & 0xffu));
+    const float dpx = cs.bx ? 0.f : C.x - pxm;
+    const float dpy = cs.by ? 0.f : C.x - pym;
+    const float dpz = cs.bz ? 0.f : C.x - s.ZM.x;
+    float su, sv, sw;
+    visc_from4(cs, s, C, su, sv, sw);
+    float4 N = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rm, C.x, cu);
+    const float wz = fwd_solid(cs.cz) ? 0.f : s.ZP.w;
+    const float vy = fwd_solid(cs.cy) ? 0.f : s.YP.z;
+    const float ux = fwd_solid(cs.cx) ? 0.f : s.XP.y;
+    const float div0 = ((((wz - N.w) + vy) - N.z) + ux) - N.y;
+    const float div = div0 + rp;
+    N.x = C.x - div * cp;
+    S[c] = N;
+    if (out) q[qc] = pin_out - div0 * (float)(kC2 * kDt);
+}
+
 // Residual of the FP64 state at c into the interleaved FP32 residual.
 __device__ __forceinline__ void residual_voxel4(const Grid &g, int64_t c, int t, uint8_t pcv,
                                                 const double *__restrict__ p, const double *__restrict__ u,
@@ -653,27 +709,29 @@ __device__ __forceinline__ void residual_voxel4(const Grid &g, int64_t c, int t,
     const double ux = fwd_solid(cs.cx) ? 0.0 : u[c + 1];
     const double div = ((((wz - w[c]) + vy) - v[c]) + ux) - u[c];
     R.p[c] = (float)div;
-    R.m[t] = make_float4((float)fu, (float)fv, (float)fw, 0.f);
+    R.m[t] = make_float4((float)fu, (float)fv, (float)fw, (float)div);
 }
 
 // Ghost fill of the interleaved correction state (ghost_line with float4
-// copies; the pins of a correction state are 0).
+// copies; the pins of a correction state are 0).  SPLIT: the state and its
+// outlet buffer in colour-split rows (the geometry G stays natural).
+template <bool SPLIT = false>
 __device__ __forceinline__ void ghost_line4(const Grid &g, int64_t t, const uint8_t *__restrict__ G, float4 *S,
                                             float *q) {
     const int64_t nX = (int64_t)g.nz * g.ny, nY = (int64_t)g.nz * g.nx, nZ = (int64_t)g.ny * g.nx;
     const int iout = g.nz + 1;
+    auto sat = [&](int i, int j, int k) { return SPLIT ? g.at_s(i, j, k) : g.at(i, j, k); };
+    auto sqat = [&](int j, int k) { return SPLIT ? g.qat_s(j, k) : g.qat(j, k); };
     if (t < nX) {
         if (!g.px) return;
         const int i = (int)(t / g.ny) + 2, j = (int)(t % g.ny) + 2;
-        const int64_t r = g.at(i, j, 0);
         const int src[4] = {g.nx, g.nx + 1, 2, 3};
         const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
         for (int e = 0; e < 4; e++) {
-            const int64_t s = r + src[e], d = r + dst[e];
-            float4 v = S[s];
-            if (i == iout && G[s] == 0) v.x = 0.f;
-            S[d] = v;
-            if (q && i == iout) q[g.qat(j, dst[e])] = q[g.qat(j, src[e])];
+            float4 v = S[sat(i, j, src[e])];
+            if (i == iout && G[g.at(i, j, src[e])] == 0) v.x = 0.f;
+            S[sat(i, j, dst[e])] = v;
+            if (q && i == iout) q[sqat(j, dst[e])] = q[sqat(j, src[e])];
         }
     } else if (t < nX + nY) {
         if (!g.py) return;
@@ -682,31 +740,27 @@ __device__ __forceinline__ void ghost_line4(const Grid &g, int64_t t, const uint
         const int src[4] = {g.ny, g.ny + 1, 2, 3};
         const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
         for (int e = 0; e < 4; e++) {
-            const int64_t s = g.at(i, src[e], k), d = g.at(i, dst[e], k);
-            float4 v = S[s];
-            if (i == iout && G[s] == 0) v.x = 0.f;
-            S[d] = v;
-            if (q && i == iout) q[g.qat(dst[e], k)] = q[g.qat(src[e], k)];
+            float4 v = S[sat(i, src[e], k)];
+            if (i == iout && G[g.at(i, src[e], k)] == 0) v.x = 0.f;
+            S[sat(i, dst[e], k)] = v;
+            if (q && i == iout) q[sqat(dst[e], k)] = q[sqat(src[e], k)];
         }
     } else if (t < nX + nY + nZ) {
         const int64_t tt = t - nX - nY;
         const int j = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
-        const int64_t cout = g.at(iout, j, k);
-        const bool out_fluid = G[cout] == 0;
+        const bool out_fluid = G[g.at(iout, j, k)] == 0;
         if (g.pz) {
             const int src[4] = {g.nz, g.nz + 1, 2, 3};
             const int dst[4] = {0, 1, g.nz + 2, g.nz + 3};
             for (int e = 0; e < 4; e++) {
-                const int64_t s = g.at(src[e], j, k), d = g.at(dst[e], j, k);
-                float4 v = S[s];
+                float4 v = S[sat(src[e], j, k)];
                 if (src[e] == iout && out_fluid) v.x = 0.f;
-                S[d] = v;
+                S[sat(dst[e], j, k)] = v;
             }
         }
-        const int64_t c1 = g.at(1, j, k), chi = g.at(g.nz + 2, j, k);
-        if (G[c1] == 0) S[c1].x = 0.f;
-        if (G[chi] == 0) S[chi].x = 0.f;
-        if (out_fluid) S[cout].x = 0.f;
+        if (G[g.at(1, j, k)] == 0) S[sat(1, j, k)].x = 0.f;
+        if (G[g.at(g.nz + 2, j, k)] == 0) S[sat(g.nz + 2, j, k)].x = 0.f;
+        if (out_fluid) S[sat(iout, j, k)].x = 0.f;
     }
 }
 
@@ -720,6 +774,18 @@ __device__ __forceinline__ void fold_voxel4(int64_t c, double *Xp, double *Xu, d
     Xv[c] = Xv[c] + (double)v.z;
     Xw[c] = Xw[c] + (double)v.w;
     d[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// fold_voxel4 for a colour-split correction state: c natural, cs its
+// colour-split offset.
+__device__ __forceinline__ void fold_voxel4_split(int64_t c, int64_t cs, double *Xp, double *Xu, double *Xv,
+                                                  double *Xw, float4 *d) {
+    const float4 v = d[cs];
+    Xp[c] = Xp[c] + (double)v.x;
+    Xu[c] = Xu[c] + (double)v.y;
+    Xv[c] = Xv[c] + (double)v.z;
+    Xw[c] = Xw[c] + (double)v.w;
+    d[cs] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // 3-colour SOR: one cell of the current colour, in place.

@@ -94,17 +94,22 @@ def test_reference_sweeps_bit_exact(solver, bc):
 
 
 @contextlib.contextmanager
-def ref_form(form):
-    """The exact-order sweep's kernel form for solvers created inside."""
-    old = os.environ.get("POREMAPS_REF")
-    os.environ["POREMAPS_REF"] = form
+def env(name, value):
+    """An environment knob of the library, for solvers created inside."""
+    old = os.environ.get(name)
+    os.environ[name] = value
     try:
         yield
     finally:
         if old is None:
-            os.environ.pop("POREMAPS_REF", None)
+            os.environ.pop(name, None)
         else:
-            os.environ["POREMAPS_REF"] = old
+            os.environ[name] = old
+
+
+def ref_form(form):
+    """The exact-order sweep's kernel form for solvers created inside."""
+    return env("POREMAPS_REF", form)
 
 
 @pytest.mark.parametrize("form", ["onewg", "planes"])
@@ -262,6 +267,26 @@ def test_fast_f32dc_sweeps_bit_exact(mode, bc, omega, omega_p):
     with pm.Solver(raw, boundary_method=bc, mode=mode, omega=omega, omega_p=omega_p, precision="f32dc",
                    it_eval=10) as s:
         s.step(31)
+        m = nonc(ref[0])
+        for name, a, b in zip("puvw", s.state_halo(), ref):
+            assert np.array_equal(a[m], b[m]), f"{name} differs"
+
+
+@pytest.mark.parametrize("layout", ["split", "natural"])
+@pytest.mark.parametrize("shape,bc", [((6, 5, 28), 0), ((5, 7, 27), 1), ((4, 6, 2), 2), ((7, 4, 60), 0),
+                                      ((5, 6, 13), 3)])
+def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
+    """FP32+DC 3-colour SOR with the correction state in colour-split rows
+    (the default) and in natural rows: both equal the oracle bit for bit.
+    nx 27, 28 and 60 are widths whose split rows need a wider row pitch than
+    the natural ones; nx 2 gives blocks of two cells."""
+    raw = random_geometry(shape, 0.3, sum(shape) + bc)
+    say("sor f32dc layout", layout, shape, bc)
+    ref = oracle_dc(raw, bc, "sor3", 1.8, 16, 5, 6.0)
+    with env("POREMAPS_SOR", layout), pm.Solver(raw, boundary_method=bc, mode="sor3", omega=1.8, omega_p=6.0,
+                                                precision="f32dc", it_eval=5) as s:
+        assert s.kernel_choice()[0] == ("interleaved-colour-split" if layout == "split" else "interleaved-colour")
+        s.step(16)
         m = nonc(ref[0])
         for name, a, b in zip("puvw", s.state_halo(), ref):
             assert np.array_equal(a[m], b[m]), f"{name} differs"
