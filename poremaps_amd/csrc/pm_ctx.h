@@ -77,8 +77,10 @@ struct pm_ctx {
     // fluid-list column (rows x cells) for the next pm_load_geometry (0: whole
     // extent).  On the 512^3 field 64 x 64 is best: interleaved FP32 lagged
     // pass 0.96 ms against 1.05 ms with whole rows and 1.43 ms in plain
-    // (z, y, x) order (flat from 16 x 32 to 128 x 128); FP64 record pass
-    // 1.60 ms against 1.64 ms with whole rows
+    // (z, y, x) order (flat from 16 x 32 to 128 x 128); FP32 record pass
+    // 0.860 ms (128 x 64: 0.862, 32 x 128: 0.879, 64 x whole rows: 0.929);
+    // FP64 record pass 1.60 ms against 1.64 ms with whole rows.  FP32+DC SOR
+    // uses whole rows (pm_ctx_create)
     int prm_tile_y = 64;
     int prm_tile_x = 64;
     // Jacobi kernels: velocity + pressure pair over the fluid list, one lagged

@@ -274,12 +274,13 @@ def test_fast_f32dc_sweeps_bit_exact(mode, bc, omega, omega_p):
 
 @pytest.mark.parametrize("layout", ["split", "natural"])
 @pytest.mark.parametrize("shape,bc", [((6, 5, 28), 0), ((5, 7, 27), 1), ((4, 6, 2), 2), ((7, 4, 60), 0),
-                                      ((5, 6, 13), 3)])
+                                      ((5, 6, 13), 3), ((9, 12, 130), 1), ((10, 9, 97), 0)])
 def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
     """FP32+DC 3-colour SOR with the correction state in colour-split rows
     (the default) and in natural rows: both equal the oracle bit for bit.
     nx 27, 28 and 60 are widths whose split rows need a wider row pitch than
-    the natural ones; nx 2 gives blocks of two cells."""
+    the natural ones; nx 2 gives blocks of two cells; nx 97 and 130 rows of
+    several lines per block."""
     raw = random_geometry(shape, 0.3, sum(shape) + bc)
     say("sor f32dc layout", layout, shape, bc)
     ref = oracle_dc(raw, bc, "sor3", 1.8, 16, 5, 6.0)
