@@ -53,9 +53,10 @@ struct Grid {
 
 // FP32 residual of the interleaved defect-correction state: the continuity
 // part Rp per cell (it is read at the cell and at its -1 neighbours) and the
-// momentum part (Ru, Rv, Rw, 0) per fluid voxel in fluid-list order (read only
-// by the voxel's own thread), which keeps the residual traffic of a sweep at
-// fluid size.  q (Jacobi): per fluid voxel, the continuity part at the cell
+// momentum part (Ru, Rv, Rw) per fluid voxel in fluid-list order, with Rp
+// again in its fourth lane (read only by the voxel's own thread; the SOR's
+// colour passes take Rp from there), which keeps the residual traffic of a
+// sweep at fluid size.  q (Jacobi): per fluid voxel, the continuity part at the cell
 // and at the three cells whose pressure its lagged pass recomputes (-x, -y
 // neighbour or their periodic images, -z neighbour), gathered once per
 // residual so that a sweep reads them with one streamed load.

@@ -274,7 +274,7 @@ def test_fast_f32dc_sweeps_bit_exact(mode, bc, omega, omega_p):
 
 @pytest.mark.parametrize("layout", ["split", "natural"])
 @pytest.mark.parametrize("shape,bc", [((6, 5, 28), 0), ((5, 7, 27), 1), ((4, 6, 2), 2), ((7, 4, 60), 0),
-                                      ((5, 6, 13), 3), ((9, 12, 130), 1), ((10, 9, 97), 0)])
+                                      ((5, 6, 13), 3), ((9, 12, 130), 1), ((10, 9, 97), 0), ((6, 7, 20), 4)])
 def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
     """FP32+DC 3-colour SOR with the correction state in colour-split rows
     (the default) and in natural rows: both equal the oracle bit for bit.
