@@ -61,6 +61,9 @@ struct FluidLists {
     // the record form of the lagged pass
     uint2 *rec = nullptr;
     bool use_rec = false;
+    // the FP32+DC record pass writes the periodic halo copies and halo pins of
+    // its cells itself (no ghost fill after it)
+    bool fill_halos = true;
     // FP32+DC 3-colour SOR: the correction state in colour-split rows
     // (Grid::at_s) and rec holding the colour passes' records (colour-split
     // offset + packed code, block, outlet flag)
@@ -86,6 +89,9 @@ enum { JAC_FIRST = 0, JAC_LAG = 1, JAC_SETTLE = 2, JAC_PAIR = 3 };
 // SOR: the three colour launches, in place.
 void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s);
+// Whether launch_jacobi4 of this stage leaves S1's halos filled (the record
+// pass with L.fill_halos): the caller then skips launch_ghost4.
+bool jacobi4_fills_halos(int stage, const FluidLists &L);
 void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
                     float cp, hipStream_t s);
 // The per-voxel records L.rec of the list: the lagged pass's, or the colour

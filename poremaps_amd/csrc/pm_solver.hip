@@ -345,6 +345,8 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
                 }
         c->lists.tile_y = c->prm_tile_y > 0 ? ty : 0;
         c->lists.tile_x = c->prm_tile_x > 0 ? tx : 0;
+        const char *hf = std::getenv("POREMAPS_HALOS");
+        c->lists.fill_halos = !(hf && std::strcmp(hf, "separate") == 0);
         const char *ld = std::getenv("POREMAPS_SWEEP_LOADS");
         c->lists.select_loads = !(ld && std::strcmp(ld, "branch") == 0);
         c->lists.use_rec = c->jac_impl == JI_RECORDS;
@@ -588,19 +590,23 @@ static int one_sweep(pm_ctx *c) {
         if (c->jacobi()) {
             RC(tuner_begin(c));
             tb(c, TC_SWEEP);
+            int stage = pm::JAC_PAIR;
             if (c->jac_impl == JI_PAIR) {
                 RC(settle(c));
                 pm::launch_jacobi4(pm::JAC_PAIR, g, c->lists, c->pc, c->S4[0], c->qD[0], c->S4[1], c->qD[1], c->R4,
                                    fcu, fcp, s);
             } else {
-                pm::launch_jacobi4(c->lagged ? pm::JAC_LAG : pm::JAC_FIRST, g, c->lists, c->pc, c->S4[0], c->qD[0],
-                                   c->S4[1], c->qD[1], c->R4, fcu, fcp, s);
+                stage = c->lagged ? pm::JAC_LAG : pm::JAC_FIRST;
+                pm::launch_jacobi4(stage, g, c->lists, c->pc, c->S4[0], c->qD[0], c->S4[1], c->qD[1], c->R4, fcu,
+                                   fcp, s);
                 c->lagged = true;
             }
             te(c);
-            tb(c, TC_GHOST);
-            pm::launch_ghost4(g, c->G, c->S4[1], c->qD[1], false, s);
-            te(c);
+            if (!pm::jacobi4_fills_halos(stage, c->lists)) {
+                tb(c, TC_GHOST);
+                pm::launch_ghost4(g, c->G, c->S4[1], c->qD[1], false, s);
+                te(c);
+            }
             std::swap(c->S4[0], c->S4[1]);
             std::swap(c->qD[0], c->qD[1]);
             RC(tuner_end(c));

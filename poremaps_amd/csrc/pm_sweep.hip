@@ -139,13 +139,15 @@ __global__ __launch_bounds__(kListBlock) void k_jac_rec(Grid g, const uint2 *__r
     lag_voxel_rec<T>(g, t, rec, p0, u0, v0, w0, q0, p1, u1, v1, w1, cu, cp, pin_out);
 }
 
-// Lagged pass from the per-voxel records (lag_voxel4_rec).
+// Lagged pass from the per-voxel records (lag_voxel4_rec); IMAGES: the pass
+// also writes the ghost fill's periodic copies and halo pins of its cells.
+template <bool IMAGES>
 __global__ __launch_bounds__(kListBlock) void k_jac4_rec(Grid g, const uint2 *__restrict__ rec, int n,
                                                          const float4 *__restrict__ S0, const float *__restrict__ q0,
                                                          float4 *S1, Res4 R, float cu, float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
-    lag_voxel4_rec(g, t, rec, S0, q0, S1, R, cu, cp, 0.f);
+    lag_voxel4_rec<IMAGES>(g, t, rec, S0, q0, S1, R, cu, cp, 0.f);
 }
 
 __global__ __launch_bounds__(kListBlock) void k_lag_records(Grid g, const int32_t *__restrict__ list, int n,
@@ -583,14 +585,20 @@ void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const ui
         jacobi_stage_t<false>(stage, g, L, pc, p0, u0, v0, w0, q0, p1, u1, v1, w1, q1, cu, cp, pin_out, s);
 }
 
+bool jacobi4_fills_halos(int stage, const FluidLists &L) {
+    return stage == JAC_LAG && L.select_loads && L.rec && L.use_rec && L.fill_halos;
+}
+
 void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s) {
     if (L.n_all == 0) return;
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
     if (stage == JAC_FIRST) {
         hipLaunchKernelGGL(k_jac4<0>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
+    } else if (jacobi4_fills_halos(stage, L)) {
+        hipLaunchKernelGGL(k_jac4_rec<true>, grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
     } else if (stage == JAC_LAG && L.select_loads && L.rec && L.use_rec) {
-        hipLaunchKernelGGL(k_jac4_rec, grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
+        hipLaunchKernelGGL(k_jac4_rec<false>, grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
     } else if (stage == JAC_LAG && L.select_loads) {
         hipLaunchKernelGGL(k_jac4<4>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (stage == JAC_LAG) {

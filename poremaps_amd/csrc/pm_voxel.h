@@ -451,6 +451,7 @@ __device__ __forceinline__ uint2 lag_record(const Grid &g, int64_t c, const uint
 // lag_voxel4_eager with the codes from the voxel's record and the four
 // continuity residuals from R.q: two streamed loads instead of eight gathers,
 // the same values.
+template <bool IMAGES = false>
 __device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2 *__restrict__ rec,
                                                const float4 *__restrict__ S0, const float *__restrict__ q0,
                                                float4 *__restrict__ S1, Res4 R, float cu, float cp, float pin_out) {
@@ -500,6 +501,37 @@ __device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2
     const float4 o = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rm, Pc, cu);
     const v4f ov = {o.x, o.y, o.z, o.w};
     __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c]));
+    if (IMAGES) {
+        // The ghost fill of the new state (ghost_line4), cell by cell: the
+        // cell's copies in the periodic halos, with the pressure pinned to 0
+        // on halo planes 1 and nz+2 (the copies of planes nz+1 and 2; the
+        // geometry's halos are periodic too, so the copy is fluid).  An outlet
+        // cell's pressure is already pin_out = 0.  Solid cells' copies stay 0,
+        // wall halos are never written, and the lagged pass leaves the outlet
+        // buffer alone, so nothing else of the fill changes the state.
+        if (g.pz) {
+            const int64_t zs = (int64_t)g.nz * sy;
+            if (i >= g.nz) {
+                v4f w = ov;
+                if (i == g.nz + 1) w.x = 0.f;
+                __builtin_nontemporal_store(w, reinterpret_cast<v4f *>(&S1[c - zs]));
+            }
+            if (i <= 3) {
+                v4f w = ov;
+                if (i == 2) w.x = 0.f;
+                __builtin_nontemporal_store(w, reinterpret_cast<v4f *>(&S1[c + zs]));
+            }
+        }
+        if (g.px) {
+            if (k >= g.nx) __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c - g.nx]));
+            if (k <= 3) __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c + g.nx]));
+        }
+        if (g.py) {
+            const int64_t ys = (int64_t)g.ny * sx;
+            if (j >= g.ny) __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c - ys]));
+            if (j <= 3) __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c + ys]));
+        }
+    }
 }
 
 // lag_voxel (per-field state, SEL form) from the voxel's record, every load

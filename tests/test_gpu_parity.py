@@ -293,6 +293,27 @@ def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
             assert np.array_equal(a[m], b[m]), f"{name} differs"
 
 
+@pytest.mark.parametrize("halos", ["fused", "separate"])
+@pytest.mark.parametrize("shape,bc", [((2, 5, 6), 0), ((3, 3, 7), 1), ((6, 2, 3), 0), ((5, 6, 7), 2),
+                                      ((4, 5, 6), 4), ((7, 6, 9), 3), ((9, 8, 40), 0)])
+def test_jacobi_f32dc_record_pass_halo_fill_bit_exact(halos, shape, bc):
+    """The FP32+DC record pass writing its cells' periodic halo copies and
+    halo pins itself (default) and followed by the separate ghost fill: both
+    equal the oracle bit for bit.  Sides of 2 and 3 give cells with two
+    copies on one axis (planes 2 and nz both, or 3 and nz+1)."""
+    raw = random_geometry(shape, 0.3, 3 * sum(shape) + bc)
+    say("record pass halos", halos, shape, bc)
+    ref = oracle_dc(raw, bc, "jacobi", 2.5, 16, 5, 6.0)
+    with env("POREMAPS_HALOS", "separate" if halos == "separate" else "fused"), env("POREMAPS_JACOBI", "records"), \
+            pm.Solver(raw, boundary_method=bc, mode="jacobi", omega=2.5, omega_p=6.0, precision="f32dc",
+                      it_eval=5) as s:
+        s.step(16)
+        assert s.kernel_choice()[0] == "interleaved-lagged-records"
+        m = nonc(ref[0])
+        for name, a, b in zip("puvw", s.state_halo(), ref):
+            assert np.array_equal(a[m], b[m]), f"{name} differs"
+
+
 # ---------------------------------------------------------------------------
 # edge shapes: the smallest domains, thin ones, all fluid, all solid
 # ---------------------------------------------------------------------------
