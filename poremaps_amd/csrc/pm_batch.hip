@@ -109,8 +109,8 @@ __global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, co
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    lag_voxel4_rec(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-                   parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+    lag_voxel4_rec<true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
+                         parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
 }
 
 __global__ __launch_bounds__(kB) void k_b_settle4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
@@ -467,12 +467,16 @@ static void launch_interval(pm_batch *b) {
     // the relaxation factors are per sample (sample descriptors)
     if (b->dc()) {
         // symplectic Jacobi in lagged form: the first sweep from the settled
-        // state, then one lagged pass per sweep, then the settle pass
+        // state (+ ghost fill), then one lagged pass per sweep (it writes its
+        // cells' halo copies itself), then the settle pass
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
-            if (s == 0) hipLaunchKernelGGL(k_b_vel4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
-            else hipLaunchKernelGGL(k_b_lag4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
-            hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
+            if (s == 0) {
+                hipLaunchKernelGGL(k_b_vel4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
+                hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
+            } else {
+                hipLaunchKernelGGL(k_b_lag4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
+            }
         }
         const int cur = n & 1;
         hipLaunchKernelGGL(k_b_settle4, dim3(b->nC), blk, 0, st, b->dS, b->dC, cur);
