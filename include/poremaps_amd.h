@@ -160,8 +160,11 @@ int pm_time_kernels(pm_ctx *ctx, int n, pm_kernel_timing *out, int max_out, int 
  * per-voxel records; one lagged dense 2.5-D pass); the first sweeps of a
  * solve run each in turn and the fastest is kept (environment:
  * POREMAPS_TUNE=0 disables, POREMAPS_JACOBI=pair|list|records|stream fixes the
- * form).  name receives the form in use; tried (optional)
- * "form=ms/sweep;..." once tuning has finished.  No reference counterpart. */
+ * form).  name receives the form in use ("n/a" in the reference mode; for
+ * 3-colour SOR "list-colour" (FP64), "interleaved-colour-split" or
+ * "interleaved-colour" (FP32+DC state in colour-split or natural rows));
+ * tried (optional) "form=ms/sweep;..." once tuning has finished.  No
+ * reference counterpart. */
 int pm_kernel_choice(pm_ctx *ctx, char *name, int len, char *tried, int tried_len);
 
 void pm_destroy(pm_ctx *ctx);
