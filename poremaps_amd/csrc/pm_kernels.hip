@@ -186,6 +186,46 @@ __device__ __forceinline__ void ref_update(const uint8_t *G, const int32_t *code
     p[c] = p[c] - kC2 * v_sum * kDt;
 }
 
+// ref_update split by field for the tiled wavefront: COMP 0/1/2 updates u/v/w
+// of cell c (each reads only its own field, the pressure and the geometry,
+// so the three run side by side on different waves), ref_pressure then p from
+// the new velocities (solvers 2, 3) or from the divergence vs_old taken
+// before any velocity of the cell changed (solver 1).  The same expressions
+// in the same order as ref_update, the case switch of get_2nd_derivation
+// replaced by its select form (same arithmetic), so the same values.
+template <int COMP>
+__device__ __forceinline__ void ref_velocity(const uint8_t *G, const int32_t *code, const double *p, double *f,
+                                             int solver, int c, int sj, int si) {
+    const int s = COMP == 0 ? 1 : COMP == 1 ? sj : si;  // the component's own axis
+    double dp;
+    if (G[c - s] == 0) dp = -p[c - s] + p[c]; else dp = 0.;
+    int num = code[c];
+    const int cz = num % 10; num /= 10;
+    const int cy = num % 10; num /= 10;
+    const int cx = num % 10;
+    // own-axis Laplacian term, then the other axes' get_2nd_derivation in the
+    // reference's summation order: u: x, y, z; v: y, z, x; w: z, x, y
+    const double own = (f[c - s] - 2 * f[c] + f[c + s]);
+    const int ca = COMP == 0 ? cy : COMP == 1 ? cz : cx;   // first other axis
+    const int cb = COMP == 0 ? cz : COMP == 1 ? cx : cy;   // second other axis
+    const int sa = COMP == 0 ? sj : COMP == 1 ? si : 1;
+    const int sb = COMP == 0 ? si : COMP == 1 ? 1 : sj;
+    const double da = d2_pick<double>(ca, f[c - 2 * sa], f[c - sa], f[c], f[c + sa], f[c + 2 * sa]);
+    const double db = d2_pick<double>(cb, f[c - 2 * sb], f[c - sb], f[c], f[c + sb], f[c + 2 * sb]);
+    const double sum = own + da + db;
+    if (solver == 1 || solver == 2) {
+        if (G[c - s] == 1) f[c] = 0.0;
+        else f[c] = f[c] + (-dp + sum / kRe) * kDt;
+    } else if (solver == 3) {
+        f[c] = (1.0 - kOmegaRef) * f[c] + kOmegaRef * (-dp + sum / kRe) * kDt;
+    }
+}
+
+__device__ __forceinline__ double ref_divergence(const double *u, const double *v, const double *w, int c, int sj,
+                                                 int si) {
+    return (w[c + si] - w[c] + v[c + sj] - v[c] + u[c + 1] - u[c]);
+}
+
 __device__ __forceinline__ void ref_voxel(const Grid &g, const uint8_t *__restrict__ G,
                                           const int32_t *__restrict__ code, double *p, double *u,
                                           double *v, double *w, int solver, int i, int j, int k) {
@@ -253,16 +293,34 @@ __global__ __launch_bounds__(kRThreads) void k_ref_tile(Grid g, const uint8_t *_
         Lc[l] = code[gc]; LG[l] = G[gc];
     }
     __syncthreads();
-    // the first kRT*kRT threads take one (z, y) line each; the others only
-    // help with the copies
-    const int a = (int)threadIdx.x / kRT, b = (int)threadIdx.x % kRT;
-    const bool line = (int)threadIdx.x < kRT * kRT;
+    // each of the first three waves takes one velocity field of the tile's
+    // current local hyperplane, one (z, y) line of the tile per lane; then
+    // the first wave updates the pressure from the new velocities.  The
+    // fourth wave only helps with the copies.
+    const int q = (int)threadIdx.x / (kRT * kRT), l = (int)threadIdx.x % (kRT * kRT);
+    const int a = l / kRT, b = l % kRT;
     const int smax = (tz - 1) + (ty - 1) + (tx - 1);
+    constexpr int sj = kRB, si = kRB * kRB;
     for (int sh = 0; sh <= smax; sh++) {
         const int x = sh - a - b;
-        if (line && a < tz && b < ty && x >= 0 && x < tx)
-            ref_update<int>(LG, Lc, Lp, Lu, Lv, Lw, solver, ((a + 2) * kRB + (b + 2)) * kRB + (x + 2), kRB,
-                            kRB * kRB);
+        const int c = ((a + 2) * kRB + (b + 2)) * kRB + (x + 2);
+        const bool fl = a < tz && b < ty && x >= 0 && x < tx && LG[c] == 0;
+        double vs = 0.0;
+        if (solver == 1) {
+            // src/solver.cc: the divergence before the cell's velocities change
+            if (q == 0 && fl) vs = ref_divergence(Lu, Lv, Lw, c, sj, si);
+            __syncthreads();
+        }
+        if (fl) {
+            if (q == 0) ref_velocity<0>(LG, Lc, Lp, Lu, solver, c, sj, si);
+            else if (q == 1) ref_velocity<1>(LG, Lc, Lp, Lv, solver, c, sj, si);
+            else if (q == 2) ref_velocity<2>(LG, Lc, Lp, Lw, solver, c, sj, si);
+        }
+        __syncthreads();
+        if (q == 0 && fl) {
+            if (solver == 2 || solver == 3) vs = ref_divergence(Lu, Lv, Lw, c, sj, si);
+            Lp[c] = Lp[c] - kC2 * vs * kDt;
+        }
         __syncthreads();
     }
     const int ntile = tz * ty * tx;
