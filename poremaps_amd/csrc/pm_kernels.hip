@@ -262,10 +262,11 @@ __global__ void k_ref_hyperplane(Grid g, const uint8_t *__restrict__ G,
 // rim are copied into LDS first: the rim belongs to tiles of diagonal D - 1
 // (finished), of diagonal D + 1 (not started) or to the halo (frozen during a
 // sweep), so it does not change while the tile runs.  One workgroup of four
-// waves per tile: all copy, the first wave computes, one (z, y) line of the
-// tile per thread.  54 x 54 x 50 tube: 0.62 ms per sweep (one workgroup for
-// the whole domain: 2.03 ms; one launch per hyperplane: 0.76 ms); 256^3:
-// 6.7 ms (8.3 ms).
+// waves per tile: all copy; per local hyperplane three waves update u, v, w
+// side by side (one (z, y) line of the tile per lane), then the first one the
+// pressure.  54 x 54 x 50 tube: 0.40 ms per sweep (one wave computing every
+// field: 0.62 ms; one workgroup for the whole domain: 2.03 ms; one launch per
+// hyperplane: 0.76 ms); 128^3 sphere pack 1.15 ms, 256^3 4.05 ms.
 constexpr int kRT = 8;                   // tile edge
 constexpr int kRB = kRT + 4;             // tile + rim
 constexpr int kRN = kRB * kRB * kRB;     // 1728 cells, 63.9 KB of LDS
