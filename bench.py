@@ -1,7 +1,7 @@
 """Benchmark of the POREMAPS Stokes sweep on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
-                    [--mode jacobi|sor3] [--precision f32dc|f64] [--omega w] [--omega-p wp]
+                    [--mode jacobi|sor3|reference] [--precision f32dc|f64] [--omega w] [--omega-p wp]
                     [--tte] [--no-cpu]
     python bench.py --config 3 [--count 64] [--compare]      (batch)
     python bench.py --config 4 [--count 24] [--rate 4] [--compare]  (serving)
@@ -11,6 +11,8 @@ with the defect-correction fold and residual of that interval) over the whole
 synthetic field, fields resident in HBM.  value = GLUP/s = nx*ny*nz*sweeps/s.
 
 Workloads (synthetic, generated from a seed):
+  config 0: the reference's own CPU case, a tube 54x54x50 (r 20, bc 0, dx 2e-5);
+            with --mode reference it times the exact-order FP64 sweep
   config 1: overlapping-sphere pack 256^3, phi 0.35, r 6-12, bc 2, dx 1e-5
   config 2: thresholded Gaussian random field 512^3, corr. length 8, phi 0.20,
             bc 1, dx 5e-6 (default: the largest single-GPU configuration)
@@ -51,6 +53,10 @@ PEAK_HBM_GBS = 8000.0
 def workload(cfg: int):
     from poremaps_amd import geometry as geo
 
+    if cfg == 0:
+        spec = dict(name="tube_54", n=54, bc=0, dx=2e-5,
+                    desc="tube 54x54x50, radius 20, bc 0 (periodic), the reference's own CPU case")
+        return geo.config0(), spec
     cache = f"/tmp/poremaps_bench_cfg{cfg}.npy"
     if cfg == 1:
         spec = dict(name="sphere_pack_256", n=256, bc=2, dx=1e-5,
@@ -333,6 +339,9 @@ def main():
     import poremaps_amd as pm
 
     g, spec = workload(a.config)
+    if a.mode == "reference":
+        # the exact-order FP64 sweep takes no precision or relaxation
+        a.precision, a.omega, a.omega_p = "f64", 1.0, 0.0
     n = g.size
     kw = dict(boundary_method=spec["bc"], voxel_size=spec["dx"], mode=a.mode, precision=a.precision,
               omega=a.omega, omega_p=a.omega_p, it_eval=a.it_eval, eps=1e-6)
@@ -350,7 +359,8 @@ def main():
     k = s.permeability()
     choice, tuned = s.kernel_choice()
     s.close()
-    sw = kt["sweep"]
+    group = "sweep" if "sweep" in kt else "reference_sweep"
+    sw = kt[group]
     achieved = sw["bytes"] / (sw["avg_ms"] * 1e-3) / 1e9
     print(f"[bench] {sweeps} sweeps in {el:.3f}s: {glups:.2f} GLUP/s; kernels {json.dumps(kt)}",
           file=sys.stderr, flush=True)
@@ -373,7 +383,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": measured_traffic(spec, choice, a.precision),
-                     "kernel": "sweep", "avg_ms": round(sw["avg_ms"], 4),
+                     "kernel": group, "avg_ms": round(sw["avg_ms"], 4),
                      "bytes_per_launch": sw["bytes"]},
         "kernels": {k2: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"],
                          "GBps": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k2, v in kt.items()},
