@@ -309,8 +309,11 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--mode", default="jacobi")
     ap.add_argument("--precision", default="f32dc")
-    ap.add_argument("--omega", type=float, default=2.5, help="velocity relaxation of the fast modes")
-    ap.add_argument("--omega-p", type=float, default=7.0, help="pressure relaxation of the fast modes")
+    ap.add_argument("--omega", type=float, default=None,
+                    help="velocity relaxation of the fast modes (default 2.5 for one field; batch and serving: "
+                         "0 = the pair chosen per geometry from its porosity)")
+    ap.add_argument("--omega-p", type=float, default=None,
+                    help="pressure relaxation of the fast modes (default 7.0 for one field; 0 with omega 0)")
     ap.add_argument("--it-eval", type=int, default=100)
     ap.add_argument("--tte", action="store_true")
     ap.add_argument("--accuracy", action="store_true",
@@ -329,6 +332,10 @@ def main():
         print(f"bench.py measures one GPU: this build solves one geometry per GPU and has no "
               f"multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
         sys.exit(2)
+    if a.omega is None:
+        a.omega = 0.0 if a.config in (3, 4) else 2.5
+    if a.omega_p is None:
+        a.omega_p = 0.0 if a.omega == 0.0 else 7.0
     if a.config == 3:
         a.count = a.count or 64
         return bench_batch(a)

@@ -64,7 +64,10 @@ typedef struct {
     /* build knobs */
     int mode;                /* PM_MODE_* */
     int precision;           /* PM_PREC_* */
-    double omega;            /* relaxation of PM_MODE_JACOBI / PM_MODE_SOR3 (velocities) */
+    double omega;            /* relaxation of PM_MODE_JACOBI / PM_MODE_SOR3 (velocities); 0:
+                                the pair (omega, omega_p) chosen at pm_load_geometry from the
+                                geometry's porosity: (0.6, 30) below 0.20, where the pressure
+                                mode limits convergence, else (2.5, 7) (pm_relaxation) */
     int device;              /* HIP device ordinal */
     double omega_p;          /* pressure relaxation of the fast modes; 0: omega.  The fixed
                                 point does not depend on it; the Jacobi sweep is stable while
@@ -166,6 +169,11 @@ int pm_time_kernels(pm_ctx *ctx, int n, pm_kernel_timing *out, int max_out, int 
  * tried (optional) "form=ms/sweep;..." once tuning has finished.  No
  * reference counterpart. */
 int pm_kernel_choice(pm_ctx *ctx, char *name, int len, char *tried, int tried_len);
+
+/* The relaxation pair in use (after the choice by porosity when
+ * pm_params.omega was 0, and after any restart that cut omega_p).  No
+ * reference counterpart. */
+int pm_relaxation(pm_ctx *ctx, double *omega, double *omega_p);
 
 void pm_destroy(pm_ctx *ctx);
 

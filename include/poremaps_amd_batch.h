@@ -29,13 +29,15 @@ typedef struct pm_batch pm_batch;
 
 typedef struct {
     int precision;           /* PM_PREC_F64 or PM_PREC_F32DC */
-    double omega;
+    double omega;            /* velocity relaxation; 0 (default): the pair chosen per sample
+                                from its porosity, as pm_params.omega = 0 */
     int it_eval, it_write, max_iter;
     double eps;
     int device;
     int use_graph;           /* capture each interval as a HIP graph (1) or launch directly (0) */
     int64_t budget_bytes;    /* admission limit for the samples' device memory; 0: 90 % of free HBM */
-    double omega_p;          /* pressure relaxation; 0: omega (as pm_params.omega_p) */
+    double omega_p;          /* pressure relaxation; 0: omega (as pm_params.omega_p); ignored
+                                when omega is 0 */
 } pm_batch_params;
 
 /* Sample states for pm_batch_status. */

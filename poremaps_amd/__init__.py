@@ -26,11 +26,15 @@ PM_OK = 0
 MODES = {"reference": 0, "jacobi": 1, "sor3": 2}
 PRECISIONS = {"f64": 0, "f32dc": 1}
 FIELDS = {"velx": 0, "vely": 1, "velz": 2, "press": 3, "nhood": 4}
-# Default relaxation of the batch and serving paths (velocities, pressure):
-# the symplectic Jacobi sweep is stable while OMEGA * OMEGA_P stays below
-# about 18 (wave CFL of the pressure-velocity coupling), and the slow Darcy
-# pressure mode decays in proportion to OMEGA_P.
+# Relaxation (velocities, pressure) of the fast modes: the symplectic sweep
+# is stable while omega * omega_p stays below about 19 (wave CFL of the
+# pressure-velocity coupling); omega speeds up the viscous relaxation of the
+# velocities, omega_p the slow Darcy diffusion of the pressure.  OMEGA,
+# OMEGA_P: the pair for open media; AUTO (omega = 0, the batch and serving
+# default): the library picks the pair per geometry from its porosity,
+# (0.6, 30) below 0.20 and (2.5, 7) otherwise.
 OMEGA, OMEGA_P = 2.5, 7.0
+AUTO = 0.0
 
 
 class Params(C.Structure):
@@ -213,6 +217,15 @@ class Solver:
         _check(lib().pm_porosity(self._h, C.byref(v)))
         return v.value
 
+    def relaxation(self):
+        """(omega, omega_p) in use: the pair chosen from the porosity when the
+        solver was made with omega=AUTO, after any restart."""
+        L = lib()
+        L.pm_relaxation.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        w, wp = C.c_double(), C.c_double()
+        _check(L.pm_relaxation(self._h, C.byref(w), C.byref(wp)))
+        return w.value, wp.value
+
     def kernel_choice(self):
         """(sweep kernel form in use, {form: ms per sweep} measured by the
         autotuner, empty until it has finished)."""
@@ -266,8 +279,8 @@ def _batch_lib():
 class Batch:
     """Many geometries on one GPU (include/poremaps_amd_batch.h)."""
 
-    def __init__(self, *, precision="f32dc", omega=OMEGA, it_eval=100, it_write=100, max_iter=100_000_000,
-                 eps=1e-6, device=0, use_graph=True, budget_bytes=0, omega_p=OMEGA_P):
+    def __init__(self, *, precision="f32dc", omega=AUTO, it_eval=100, it_write=100, max_iter=100_000_000,
+                 eps=1e-6, device=0, use_graph=True, budget_bytes=0, omega_p=0.0):
         L = _batch_lib()
         p = _batch_params(precision, omega, it_eval, it_write, max_iter, eps, device, use_graph, budget_bytes,
                           omega_p)
@@ -384,8 +397,8 @@ def _serve_lib():
     return L
 
 
-def _batch_params(precision="f32dc", omega=OMEGA, it_eval=100, it_write=100, max_iter=100_000_000, eps=1e-6,
-                  device=0, use_graph=True, budget_bytes=0, omega_p=OMEGA_P) -> BatchParams:
+def _batch_params(precision="f32dc", omega=AUTO, it_eval=100, it_write=100, max_iter=100_000_000, eps=1e-6,
+                  device=0, use_graph=True, budget_bytes=0, omega_p=0.0) -> BatchParams:
     p = BatchParams()
     _batch_lib().pm_batch_params_default(C.byref(p))
     p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision

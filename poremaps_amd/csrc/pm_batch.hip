@@ -280,8 +280,8 @@ struct pm_batch {
 extern "C" void pm_batch_params_default(pm_batch_params *p) {
     std::memset(p, 0, sizeof *p);
     p->precision = PM_PREC_F32DC;
-    p->omega = 2.5;
-    p->omega_p = 7.0;
+    p->omega = 0.0;  // the relaxation pair by each sample's porosity
+    p->omega_p = 0.0;
     p->it_eval = 100;
     p->it_write = 100;
     p->max_iter = 100000000;
@@ -322,7 +322,8 @@ extern "C" int pm_batch_create(const pm_batch_params *p, pm_batch **out) {
     if (p->precision != PM_PREC_F64 && p->precision != PM_PREC_F32DC)
         return pm_fail(PM_E_ARG, "unknown precision %d", p->precision);
     if (p->it_eval <= 0 || p->it_write <= 0) return pm_fail(PM_E_ARG, "it_eval and it_write must be positive");
-    if (!(p->omega > 0.0 && p->omega < 100.0)) return pm_fail(PM_E_ARG, "omega must be in (0, 100)");
+    if (!(p->omega >= 0.0 && p->omega < 100.0))
+        return pm_fail(PM_E_ARG, "omega must be in (0, 100), or 0 for the pair by porosity");
     if (!(p->omega_p >= 0.0 && p->omega_p < 100.0)) return pm_fail(PM_E_ARG, "omega_p must be in [0, 100)");
     BCHK(hipSetDevice(p->device));
     pm_batch *b = new (std::nothrow) pm_batch;

@@ -81,6 +81,9 @@ struct pm_ctx {
     // 0.860 ms (128 x 64: 0.862, 32 x 128: 0.879, 64 x whole rows: 0.929);
     // FP64 record pass 1.60 ms against 1.64 ms with whole rows.  FP32+DC SOR
     // uses whole rows (pm_ctx_create)
+    // pm_params.omega was 0: the relaxation pair is chosen at each geometry
+    // load from the porosity
+    bool auto_relax = false;
     int prm_tile_y = 64;
     int prm_tile_x = 64;
     // Jacobi kernels: velocity + pressure pair over the fluid list, one lagged

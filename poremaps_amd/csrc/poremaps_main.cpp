@@ -12,7 +12,7 @@
 // environment variables:
 //   --mode=reference|jacobi|sor3        POREMAPS_MODE
 //   --precision=f64|f32dc               POREMAPS_PRECISION
-//   --omega=<w>                         POREMAPS_OMEGA
+//   --omega=<w>                         POREMAPS_OMEGA (0: the pair by porosity)
 //   --omega-p=<w>                       POREMAPS_OMEGA_P (pressure relaxation; default omega)
 //   --device=<n>                        POREMAPS_DEVICE
 // Deviation: hard errors (missing file, bad .inp, bad geometry, HIP failure)

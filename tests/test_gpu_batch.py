@@ -96,3 +96,28 @@ def test_batch_admission_by_memory(channel_raw):
             b.add(channel_raw)
         used, budget = b.memory()
         assert used == one and budget == int(one * 1.5)
+
+
+def test_relaxation_by_porosity_and_batch_equals_solo():
+    """omega = 0: the pair is chosen at load from the porosity, (0.6, 30)
+    below 0.20 and (2.5, 7) otherwise, the same way alone and in a batch (the
+    batch default), so each batched sample still equals its solo solve."""
+    tight = geo.tube(20, 18, 24, 9.5, 8.5, 3.0)  # porosity 0.08
+    open_ = geo.sphere_pack((18, 16, 20), 0.5, 2, 4, seed=5)
+    say("relaxation by porosity")
+    with pm.Batch(precision="f32dc") as b:
+        slots = [b.add(m, boundary_method=0, voxel_size=1e-5) for m in (tight, open_)]
+        while b.step() > 0:
+            pass
+        st = [b.status(s)[1] for s in slots]
+    for m, r, pair in zip((tight, open_), st, ((0.6, 30.0), (2.5, 7.0))):
+        with pm.Solver(m, boundary_method=0, voxel_size=1e-5, mode="jacobi", precision="f32dc", omega=pm.AUTO,
+                       eps=1e-6) as s:
+            assert s.relaxation() == pair
+            res, _ = s.solve()
+        assert res.sweeps == r.sweeps
+        assert (res.last.wk33, res.last.wmean_velz) == (r.last.wk33, r.last.wmean_velz)
+        with pm.Solver(m, boundary_method=0, voxel_size=1e-5, mode="jacobi", precision="f32dc", omega=pair[0],
+                       omega_p=pair[1], eps=1e-6) as s:
+            res2, _ = s.solve()
+        assert (res2.sweeps, res2.last.wk33) == (res.sweeps, res.last.wk33)

@@ -21,8 +21,9 @@ def say(*a):
 
 
 def solo(raw, bc, dx):
-    with pm.Solver(raw, boundary_method=bc, voxel_size=dx, mode="jacobi", precision="f32dc", omega=pm.OMEGA,
-                   omega_p=pm.OMEGA_P, eps=1e-6) as s:
+    # the serving default: the relaxation pair chosen from the porosity
+    with pm.Solver(raw, boundary_method=bc, voxel_size=dx, mode="jacobi", precision="f32dc", omega=pm.AUTO,
+                   eps=1e-6) as s:
         res, _ = s.solve()
         return res
 
