@@ -266,8 +266,10 @@ def bench_serve(a):
         print(f"[bench] t={t:.1f}s completed {done}/{len(reqs)}, {running} solves running", file=sys.stderr,
               flush=True)
 
-    resp, st = pm.serve(reqs, realtime=True, progress=progress, max_solves=a.max_solves, precision=a.precision,
-                        omega=a.omega, omega_p=a.omega_p, it_eval=a.it_eval, max_iter=a.max_iter)
+    small = pm.SMALL_CELLS if a.small_cells is None else a.small_cells
+    resp, st = pm.serve(reqs, realtime=True, progress=progress, max_solves=a.max_solves, small_cells=small,
+                        precision=a.precision, omega=a.omega, omega_p=a.omega_p, it_eval=a.it_eval,
+                        max_iter=a.max_iter)
     print(f"[bench] served: {json.dumps(st.as_dict())}", file=sys.stderr, flush=True)
     line = {
         "metric": METRIC, "value": round(st.solves_per_hour, 1), "unit": "solves/h", "n_gpus": 1,
@@ -277,7 +279,7 @@ def bench_serve(a):
         "config": {"workload": f"{len(media)} tensor requests, Poisson {a.rate}/s, sides log-uniform 48-320, "
                                f"sphere packs / Gaussian fields, phi U[0.15,0.45], bc U{{0,1,2}}, eps 1e-6, "
                                f"seed 4", "precision": a.precision, "omega": a.omega, "omega_p": a.omega_p,
-                   "it_eval": a.it_eval},
+                   "it_eval": a.it_eval, "small_cells": small},
         "latency_p50_s": round(st.latency_p50_s, 3), "latency_p99_s": round(st.latency_p99_s, 3),
         "latency_max_s": round(st.latency_max_s, 3), "wall_s": round(st.wall_s, 3),
         "completed": st.completed, "max_running_solves": st.max_running, "solve_cap": a.max_solves,
@@ -326,6 +328,8 @@ def main():
     ap.add_argument("--compare", action="store_true", help="also time the one-at-a-time baselines")
     ap.add_argument("--baseline-requests", type=int, default=0)
     ap.add_argument("--max-solves", type=int, default=0, help="serving: most solves on the GPU at once")
+    ap.add_argument("--small-cells", type=int, default=None,
+                    help="serving: requests of at most this many voxels run in their own batch (0: one batch)")
     ap.add_argument("--max-iter", type=int, default=400_000, help="sweep cap per solve (batch, serving)")
     a = ap.parse_args()
     if a.gpus != 1:

@@ -14,7 +14,9 @@
  * batch engine.  Requests arrive over time, each expands into the three
  * rotated solves, is admitted when its device-memory footprint fits the
  * budget (first come, first served), shares the GPU with every other running
- * solve and is evicted when done.
+ * solve and is evicted when done.  Optionally the small requests run in a
+ * second batch that gets about half the GPU time while both have work
+ * (pm_serve_params.small_cells).
  */
 #ifndef POREMAPS_AMD_SERVE_H
 #define POREMAPS_AMD_SERVE_H
@@ -63,6 +65,11 @@ typedef struct {
     int max_solves;          /* most solves on the GPU at once (0: limited by memory only);
                                 a cap trades throughput under overload for shorter waits of
                                 the requests already admitted */
+    int64_t small_cells;     /* requests of at most this many voxels run in a second batch
+                                (the memory budget holds for both together) that is stepped, while both
+                                hold solves, for about as long as each step of the first: a
+                                short request then advances at the pace of the short ones,
+                                not of the largest solve.  0: one batch */
 } pm_serve_params;
 
 typedef struct {

@@ -35,6 +35,8 @@ FIELDS = {"velx": 0, "vely": 1, "velz": 2, "press": 3, "nhood": 4}
 # (0.6, 30) below 0.20 and (2.5, 7) otherwise.
 OMEGA, OMEGA_P = 2.5, 7.0
 AUTO = 0.0
+# serving: requests of at most this many voxels (128^3) run in their own batch
+SMALL_CELLS = 128 ** 3
 
 
 class Params(C.Structure):
@@ -373,7 +375,7 @@ SERVE_PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_
 
 class ServeParams(C.Structure):
     _fields_ = [("batch", BatchParams), ("realtime", C.c_int), ("progress", SERVE_PROGRESS_FN),
-                ("user", C.c_void_p), ("max_solves", C.c_int)]
+                ("user", C.c_void_p), ("max_solves", C.c_int), ("small_cells", C.c_int64)]
 
 
 class ServeStats(C.Structure):
@@ -423,12 +425,14 @@ def tensor(raw: np.ndarray, *, boundary_method=0, voxel_size=1.0, porosity=-1.0,
     return np.array(k[:]).reshape(3, 3), list(res)
 
 
-def serve(requests, *, realtime=True, progress=None, max_solves=0, **batch_kw):
+def serve(requests, *, realtime=True, progress=None, max_solves=0, small_cells=SMALL_CELLS, **batch_kw):
     """Serve permeability-tensor requests: `requests` is a sequence of
     (raw, boundary_method, voxel_size, eps, arrival_s) (eps <= 0: the batch's
     eps).  ``progress(completed, running, t_s)`` is called as requests
-    complete and about once a second; a true return stops serving.  Returns
-    (responses, stats) with one Response per request, in request order."""
+    complete and about once a second; a true return stops serving.
+    Requests of at most ``small_cells`` voxels run in a second batch that
+    shares the GPU time with the first (0: one batch).  Returns (responses,
+    stats) with one Response per request, in request order."""
     raws = [np.ascontiguousarray(r[0], dtype=np.uint8) for r in requests]
     n = len(raws)
     req = (Request * max(n, 1))()
@@ -438,7 +442,7 @@ def serve(requests, *, realtime=True, progress=None, max_solves=0, **batch_kw):
     resp = (Response * max(n, 1))()
     st = ServeStats()
     cb = SERVE_PROGRESS_FN(lambda d, r, t, _u: int(bool(progress(d, r, t)))) if progress else SERVE_PROGRESS_FN(0)
-    sp = ServeParams(_batch_params(**batch_kw), int(realtime), cb, None, int(max_solves))
+    sp = ServeParams(_batch_params(**batch_kw), int(realtime), cb, None, int(max_solves), int(small_cells))
     _check(_serve_lib().pm_serve(C.byref(sp), req, n, resp, C.byref(st)))
     return list(resp)[:n], st
 

@@ -83,13 +83,43 @@ extern "C" int pm_tensor(const pm_batch_params *p, const uint8_t *raw, int nx, i
 extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, pm_response *resp,
                         pm_serve_stats *stats) {
     if (!p || n < 0 || (!req && n) || (!resp && n)) return pm_fail(PM_E_ARG, "null argument");
-    pm_batch *b = nullptr;
-    int rc = pm_batch_create(&p->batch, &b);
-    if (rc != PM_OK) return rc;
+    // Lanes: lane 0 takes every request, or with small_cells > 0 only those
+    // of more voxels; lane 1 the requests of at most small_cells voxels.  Each
+    // lane is a batch (one it_eval interval of all its solves per step); while
+    // both hold solves, lane 1 steps for about as long as one lane-0 step
+    // took, so a short request advances at the pace of the short ones and not
+    // at that of the largest solve on the GPU.  Every solve runs the same
+    // sweeps either way, so its result does not depend on its lane.
     struct Live {
         int req, rot;
     };
-    std::vector<Live> live;                 // slot -> (request, rotation), req = -1 free
+    struct Lane {
+        pm_batch *b = nullptr;
+        std::vector<Live> live;  // slot -> (request, rotation), req = -1 free
+        int running = 0;
+    };
+    Lane lanes[2];
+    const bool two = p->small_cells > 0;
+    int rc = PM_OK;
+    // one memory budget for both lanes, enforced below over their sum (each
+    // lane's batch holds the whole budget as its own limit)
+    pm_batch_params bp = p->batch;
+    if (two && bp.budget_bytes <= 0) {
+        // the default budget, as a batch computes it, fixed for both lanes
+        pm_batch *probe = nullptr;
+        rc = pm_batch_create(&bp, &probe);
+        if (rc != PM_OK) return rc;
+        int64_t u = 0;
+        pm_batch_memory(probe, &u, &bp.budget_bytes);
+        pm_batch_destroy(probe);
+    }
+    rc = pm_batch_create(&bp, &lanes[0].b);
+    if (rc == PM_OK && two) rc = pm_batch_create(&bp, &lanes[1].b);
+    if (rc != PM_OK) {
+        for (Lane &L : lanes)
+            if (L.b) pm_batch_destroy(L.b);
+        return rc;
+    }
     std::vector<int> left(n, 3);
     std::vector<int> order(n);
     for (int i = 0; i < n; i++) {
@@ -108,13 +138,52 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
     int64_t peak = 0;
     double admit_time = 0.0, step_time = 0.0;
     std::vector<uint8_t> rot;
+    auto lane_of = [&](const pm_request &q) {
+        return two && (int64_t)q.nx * q.ny * q.nz <= p->small_cells ? 1 : 0;
+    };
+    // one interval of a lane, then its finished solves evicted and answered
+    auto step_lane = [&](Lane &L) -> int {
+        int still = 0;
+        const double t_step = now();
+        int r = pm_batch_step(L.b, &still);
+        step_time += now() - t_step;
+        intervals++;
+        if (r != PM_OK) return r;
+        bool progressed = still > 0;
+        for (size_t s = 0; s < L.live.size(); s++) {
+            if (L.live[s].req < 0) continue;
+            int st;
+            pm_result rr;
+            if ((r = pm_batch_status(L.b, (int)s, &st, &rr)) != PM_OK) return r;
+            if (st == PM_SAMPLE_RUNNING) continue;
+            const int id = L.live[s].req, ro = L.live[s].rot;
+            scatter(ro, rr.last, resp[id].k);
+            resp[id].sweeps[ro] = rr.sweeps;
+            resp[id].state[ro] = st;
+            pm_batch_remove(L.b, (int)s);
+            L.live[s] = Live{-1, -1};
+            L.running--;
+            running--;
+            progressed = true;
+            if (--left[id] == 0) {
+                resp[id].done_s = now();
+                done++;
+            }
+        }
+        if (!progressed) return pm_fail(PM_E_STATE, "admitted solves cannot progress");
+        return PM_OK;
+    };
     while (rc == PM_OK && done < n) {
         // admission: first come, first served, while the three solves fit
+        // their lane's budget
         while (next < order.size()) {
             const pm_request &q = req[order[next]];
             if (p->realtime && q.arrival_s > now()) break;
-            int64_t used, budget, need = 0;
-            pm_batch_memory(b, &used, &budget);
+            Lane &L = lanes[lane_of(q)];
+            int64_t used = 0, budget = 0, used1 = 0, need = 0;
+            pm_batch_memory(lanes[0].b, &used, &budget);
+            if (lanes[1].b) pm_batch_memory(lanes[1].b, &used1, nullptr);
+            used += used1;
             for (int r = 0; r < 3; r++) {
                 int d[3];
                 const int nn[3] = {q.nx, q.ny, q.nz};
@@ -129,19 +198,21 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
             for (int r = 0; r < 3 && rc == PM_OK; r++) {
                 int d[3], slot = -1;
                 rotate(q.raw, q.nx, q.ny, q.nz, r, rot, d);
-                rc = pm_batch_add(b, rot.data(), d[0], d[1], d[2], q.boundary_method, q.voxel_size, q.porosity,
+                rc = pm_batch_add(L.b, rot.data(), d[0], d[1], d[2], q.boundary_method, q.voxel_size, q.porosity,
                                   &slot);
-                if (rc == PM_OK && q.eps > 0.0) rc = pm_batch_set_eps(b, slot, q.eps);
+                if (rc == PM_OK && q.eps > 0.0) rc = pm_batch_set_eps(L.b, slot, q.eps);
                 if (rc != PM_OK) break;
-                if ((int)live.size() <= slot) live.resize(slot + 1, Live{-1, -1});
-                live[slot] = Live{id, r};
+                if ((int)L.live.size() <= slot) L.live.resize(slot + 1, Live{-1, -1});
+                L.live[slot] = Live{id, r};
+                L.running++;
                 running++;
             }
             next++;
             admit_time += now() - t_admit;
-            int64_t u2;
-            pm_batch_memory(b, &u2, nullptr);
-            peak = std::max(peak, u2);
+            int64_t u2 = 0, u3 = 0;
+            pm_batch_memory(lanes[0].b, &u2, nullptr);
+            if (lanes[1].b) pm_batch_memory(lanes[1].b, &u3, nullptr);
+            peak = std::max(peak, u2 + u3);
         }
         if (rc != PM_OK) break;
         max_running = std::max(max_running, running);
@@ -152,40 +223,29 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
             }
             continue;
         }
-        int still = 0;
-        const double t_step = now();
-        rc = pm_batch_step(b, &still);
-        step_time += now() - t_step;
-        intervals++;
-        if (rc != PM_OK) break;
-        bool progressed = still > 0;
-        for (size_t s = 0; s < live.size(); s++) {
-            if (live[s].req < 0) continue;
-            int st;
-            pm_result rr;
-            if ((rc = pm_batch_status(b, (int)s, &st, &rr)) != PM_OK) break;
-            if (st == PM_SAMPLE_RUNNING) continue;
-            const int id = live[s].req, r = live[s].rot;
-            scatter(r, rr.last, resp[id].k);
-            resp[id].sweeps[r] = rr.sweeps;
-            resp[id].state[r] = st;
-            pm_batch_remove(b, (int)s);
-            live[s] = Live{-1, -1};
-            running--;
-            progressed = true;
-            if (--left[id] == 0) {
-                resp[id].done_s = now();
-                done++;
-            }
+        double t_large = 0.0;
+        if (lanes[0].running > 0) {
+            const double ts = now();
+            rc = step_lane(lanes[0]);
+            t_large = now() - ts;
         }
-        if (rc == PM_OK && !progressed) rc = pm_fail(PM_E_STATE, "admitted solves cannot progress");
+        if (rc == PM_OK && lanes[1].running > 0) {
+            const double ts = now();
+            int k = 0;
+            do {
+                rc = step_lane(lanes[1]);
+                k++;
+            } while (rc == PM_OK && lanes[1].running > 0 && lanes[0].running > 0 && now() - ts < t_large &&
+                     k < 64);
+        }
         if (rc == PM_OK && p->progress && (done > reported || now() - last_report >= 1.0)) {
             reported = done;
             last_report = now();
             if (p->progress(done, running, last_report, p->user)) rc = pm_fail(PM_E_ABORTED, "stopped by the caller");
         }
     }
-    pm_batch_destroy(b);
+    for (Lane &L : lanes)
+        if (L.b) pm_batch_destroy(L.b);
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         std::vector<double> lat;

@@ -86,9 +86,12 @@ def test_serve_matches_requests_solved_alone():
                  for m in media]
     # room for one request at a time: later arrivals queue for memory
     assert 2 * min(needs) > max(needs)
-    for budget, realtime in ((0, False), (max(needs), True)):
-        say("serve", "budget", budget, "realtime", realtime)
-        resp, st = pm.serve(reqs, realtime=realtime, budget_bytes=budget)
+    # small_cells 5000: the 16^3 requests (4,096 voxels) in the second lane,
+    # the others in the first, both lanes busy at once
+    for budget, realtime, small in ((0, False, pm.SMALL_CELLS), (max(needs), True, pm.SMALL_CELLS),
+                                    (0, False, 5000), (0, False, 0)):
+        say("serve", "budget", budget, "realtime", realtime, "small_cells", small)
+        resp, st = pm.serve(reqs, realtime=realtime, budget_bytes=budget, small_cells=small)
         assert st.completed == len(reqs)
         assert st.latency_p50_s <= st.latency_p99_s <= st.latency_max_s
         assert st.solves_per_hour > 0
