@@ -12,7 +12,7 @@
 // environment variables:
 //   --mode=reference|jacobi|sor3        POREMAPS_MODE
 //   --precision=f64|f32dc               POREMAPS_PRECISION
-//   --omega=<w>                         POREMAPS_OMEGA (0: the pair by porosity)
+//   --omega=<w>                         POREMAPS_OMEGA (0, the fast modes' default: the pair by porosity)
 //   --omega-p=<w>                       POREMAPS_OMEGA_P (pressure relaxation; default omega)
 //   --device=<n>                        POREMAPS_DEVICE
 // Deviation: hard errors (missing file, bad .inp, bad geometry, HIP failure)
@@ -96,6 +96,8 @@ int main(int argc, char **argv) {
         else if (v == "f32dc") prm.precision = PM_PREC_F32DC;
         else { std::fprintf(stderr, "unknown precision %s\n", v.c_str()); return 2; }
     }
+    // fast modes without --omega: the relaxation pair chosen from the porosity
+    if (prm.mode != PM_MODE_REFERENCE) prm.omega = 0.0;
     if (knob(flags, "--omega", "POREMAPS_OMEGA", &v)) prm.omega = std::atof(v.c_str());
     if (knob(flags, "--omega-p", "POREMAPS_OMEGA_P", &v)) prm.omega_p = std::atof(v.c_str());
     if (knob(flags, "--device", "POREMAPS_DEVICE", &v)) prm.device = std::atoi(v.c_str());
