@@ -437,3 +437,30 @@ def test_cli_drop_in_channel(tmp_path, channel_raw):
         assert np.array_equal(v, f[2:-2, 2:-2, 2:-2] * 5e-5)
     dd = np.fromfile(tmp_path / ("domain_decomp_" + geom), np.int32)
     assert dd.size == 24 * 14 * 50 and (dd == 0).all()
+
+
+def test_cli_fast_mode_default_relaxation(tmp_path, channel_raw):
+    """--mode=jacobi --precision=f32dc without --omega: the pair chosen from
+    the porosity, the same solve as the library's with omega=AUTO (sweep
+    count, last log row and velocity field)."""
+    say("cli fast mode")
+    (tmp_path / "rect_channel_24_14_50_vs_5e-05.raw").write_bytes(channel_raw.tobytes())
+    inp = open(os.path.join(GOLDEN, "input_rect_channel_24_14_50_vs_5e-05.inp")).read()
+    inp = inp.replace("eps 1e-07", "eps 1e-06").replace("porosity 1.0", "porosity -1.0")
+    (tmp_path / "input.inp").write_text(inp)
+    r = subprocess.run([pm.CLI_PATH, "input.inp", "--mode=jacobi", "--precision=f32dc"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    with pm.Solver(channel_raw, boundary_method=0, voxel_size=5e-5, mode="jacobi", precision="f32dc",
+                   omega=pm.AUTO, eps=1e-6) as s:
+        phi = s.porosity
+        assert s.relaxation() == ((0.6, 30.0) if phi < 0.2 else (2.5, 7.0))
+        res, rows = s.solve()
+        velz = s.field("velz")
+    log = (tmp_path / "permeability_rect_channel_24_14_50_vs_5e-05.log").read_text().splitlines()
+    last = log[-1].split(", ")
+    assert int(last[0]) == rows[-1]["iteration"]
+    assert last[-1] == "%e" % rows[-1]["wk33"]
+    v = np.fromfile(tmp_path / "velz_rect_channel_24_14_50_vs_5e-05.raw", np.float64).reshape(50, 14, 24)
+    assert np.array_equal(v, velz * 5e-5)  # the files hold velocities times the voxel size
+
