@@ -85,19 +85,22 @@ __global__ __launch_bounds__(kB) void k_b_pres(const BSample *__restrict__ S, co
                   q1, (T)s.cp, (T)3.0);
 }
 
-// Interleaved correction state (defect correction): the first (velocity)
-// pass of an interval, ghost fill, fold (over the fluid list: X += d, d = 0;
-// the halo copies of d are refilled with zeros by the following ghost fill)
-// and residual.
-__global__ __launch_bounds__(kB) void k_b_vel4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
-                                               int parity) {
+// Interleaved correction state (defect correction): the passes of an
+// interval, ghost fill, fold (over the fluid list: X += d, d = 0; the halo
+// copies of d are refilled with zeros by the following ghost fill) and
+// residual.
+// The first pass of an interval: every interval starts from a zero
+// correction state (the previous interval ended with the fold, the ghost fill
+// of the zeroed state and the cleared outlet buffers; a new sample starts
+// from zeros), so first_voxel4_zero computes it from the records and the
+// residual, halo copies included.
+__global__ __launch_bounds__(kB) void k_b_first0(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                                 int parity) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    const int64_t c = s.list[t];
-    vel_voxel4(s.g, c, t, s.pc[c], parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA), parity ? s.A4 : s.B4,
-               s.R4, (float)s.cu);
+    first_voxel4_zero(s.g, t, s.rec, parity ? s.A4 : s.B4, s.R4, (float)s.cu);
 }
 
 // Lagged single pass (the sweeps after the first of an interval; from the
@@ -467,14 +470,13 @@ static void launch_interval(pm_batch *b) {
     const dim3 blk(kB);
     // the relaxation factors are per sample (sample descriptors)
     if (b->dc()) {
-        // symplectic Jacobi in lagged form: the first sweep from the settled
-        // state (+ ghost fill), then one lagged pass per sweep (it writes its
-        // cells' halo copies itself), then the settle pass
+        // symplectic Jacobi in lagged form: the first sweep from the zero
+        // state, then one lagged pass per sweep (both write their cells' halo
+        // copies themselves), then the settle pass
         for (int s = 0; s < n; s++) {
             const int par = s & 1;
             if (s == 0) {
-                hipLaunchKernelGGL(k_b_vel4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
-                hipLaunchKernelGGL(k_b_ghost4, dim3(b->nG), blk, 0, st, b->dS, b->dG, 1 - par);
+                hipLaunchKernelGGL(k_b_first0, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
             } else {
                 hipLaunchKernelGGL(k_b_lag4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
             }

@@ -15,11 +15,12 @@ struct Timer {
     hipEvent_t open_ev = nullptr;
 };
 
-enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_N };
+enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_FIRST0, TC_N };
 // JI_RECORDS: the interleaved state's lagged pass reading per-voxel records
 // (codes of the cell and its recompute targets, gathered residuals).
 enum JacobiImpl { JI_PAIR = 0, JI_LIST = 1, JI_STREAM = 2, JI_RECORDS = 3 };
-static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep"};
+static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep",
+                                        "dc_first_pass"};
 
 // One Jacobi kernel configuration.  Every configuration produces the same
 // iterates bit for bit, so the choice only changes the speed.
@@ -84,6 +85,9 @@ struct pm_ctx {
     // pm_params.omega was 0: the relaxation pair is chosen at each geometry
     // load from the porosity
     bool auto_relax = false;
+    // the FP32 correction state is zero everywhere (after a fold or at the
+    // start): the next first pass needs only the records and the residual
+    bool dzero = false;
     int prm_tile_y = 64;
     int prm_tile_x = 64;
     // Jacobi kernels: velocity + pressure pair over the fluid list, one lagged

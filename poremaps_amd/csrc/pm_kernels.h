@@ -80,10 +80,12 @@ void launch_jacobi(const Grid &g, const FluidLists &L, const uint8_t *pc, const 
 // (u^n, p^{n-1}) to set 1 = (u^{n+1}, p^n); JAC_SETTLE completes set 0 in
 // place: p^n and the outlet buffer from set 0's velocities (set 1 pointers:
 // p1 = set 0's p, q1 = set 0's q).
-enum { JAC_FIRST = 0, JAC_LAG = 1, JAC_SETTLE = 2, JAC_PAIR = 3 };
+enum { JAC_FIRST = 0, JAC_LAG = 1, JAC_SETTLE = 2, JAC_PAIR = 3, JAC_FIRST_ZERO = 4 };
 
 // Interleaved FP32 correction state (one float4 {p, u, v, w} per cell) of
 // the defect-correction modes.  Jacobi: stage JAC_FIRST / JAC_LAG / JAC_PAIR
+// (JAC_FIRST_ZERO: JAC_FIRST from a state that is zero everywhere, halos and
+// outlet buffer included, computed from the records and the residual alone)
 // (velocity + pressure kernels, S1 complete) from S0 into S1; JAC_SETTLE
 // completes S0 in place (pass S1 = S0, q1 = its outlet buffer).  3-colour
 // SOR: the three colour launches, in place.

@@ -238,6 +238,8 @@ static int dc_prepare(pm_ctx *c) {
     RC(fill_f32(c, c->qD[0], g.qtotal(), 0.f));
     RC(fill_f32(c, c->qD[1], g.qtotal(), 0.f));
     HIPCHK(hipGetLastError());
+    // every caller has just zeroed the correction state (fold or start)
+    c->dzero = true;
     return PM_OK;
 }
 
@@ -606,14 +608,18 @@ static int one_sweep(pm_ctx *c) {
         const float fcu = (float)cu, fcp = (float)cp;
         if (c->jacobi()) {
             RC(tuner_begin(c));
-            tb(c, TC_SWEEP);
+            // the first pass from the zero state is timed apart: it is not a
+            // full sweep
+            const bool first0 = !c->lagged && c->dzero && c->jac_impl != JI_PAIR &&
+                                pm::jacobi4_fills_halos(pm::JAC_FIRST_ZERO, c->lists);
+            tb(c, first0 ? TC_FIRST0 : TC_SWEEP);
             int stage = pm::JAC_PAIR;
             if (c->jac_impl == JI_PAIR) {
                 RC(settle(c));
                 pm::launch_jacobi4(pm::JAC_PAIR, g, c->lists, c->pc, c->S4[0], c->qD[0], c->S4[1], c->qD[1], c->R4,
                                    fcu, fcp, s);
             } else {
-                stage = c->lagged ? pm::JAC_LAG : pm::JAC_FIRST;
+                stage = c->lagged ? pm::JAC_LAG : c->dzero ? pm::JAC_FIRST_ZERO : pm::JAC_FIRST;
                 pm::launch_jacobi4(stage, g, c->lists, c->pc, c->S4[0], c->qD[0], c->S4[1], c->qD[1], c->R4, fcu,
                                    fcp, s);
                 c->lagged = true;
@@ -626,10 +632,12 @@ static int one_sweep(pm_ctx *c) {
             }
             std::swap(c->S4[0], c->S4[1]);
             std::swap(c->qD[0], c->qD[1]);
+            c->dzero = false;
             RC(tuner_end(c));
         } else {
             tb(c, TC_SWEEP);
             pm::launch_colour4(g, c->lists, c->pc, c->S4[0], c->qD[0], c->R4, fcu, fcp, s);
+            c->dzero = false;
             te(c);
             tb(c, TC_GHOST);
             pm::launch_ghost4(g, c->G, c->S4[0], c->qD[0], c->lists.split, s);
@@ -916,6 +924,8 @@ static double algo_bytes(const pm_ctx *c, int cat) {
         return n * 97.0;
     case TC_RESIDUAL:  // read X (32) + code, write R (16)
         return n * 49.0;
+    case TC_FIRST0:    // first pass from the zero state: read code + R (16), write the state (16)
+        return n * 33.0;
     default:
         return 0.0;
     }

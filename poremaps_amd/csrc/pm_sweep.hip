@@ -150,6 +150,14 @@ __global__ __launch_bounds__(kListBlock) void k_jac4_rec(Grid g, const uint2 *__
     lag_voxel4_rec<IMAGES>(g, t, rec, S0, q0, S1, R, cu, cp, 0.f);
 }
 
+// First pass from a zero correction state (first_voxel4_zero).
+__global__ __launch_bounds__(kListBlock) void k_jac4_first_zero(Grid g, const uint2 *__restrict__ rec, int n,
+                                                                float4 *S1, Res4 R, float cu) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    first_voxel4_zero(g, t, rec, S1, R, cu);
+}
+
 __global__ __launch_bounds__(kListBlock) void k_lag_records(Grid g, const int32_t *__restrict__ list, int n,
                                                             const uint8_t *__restrict__ pc, uint2 *rec) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
@@ -586,6 +594,7 @@ void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const ui
 }
 
 bool jacobi4_fills_halos(int stage, const FluidLists &L) {
+    if (stage == JAC_FIRST_ZERO) return L.rec && L.fill_halos;
     return stage == JAC_LAG && L.select_loads && L.rec && L.use_rec && L.fill_halos;
 }
 
@@ -593,7 +602,10 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s) {
     if (L.n_all == 0) return;
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
-    if (stage == JAC_FIRST) {
+    if (stage == JAC_FIRST_ZERO && !jacobi4_fills_halos(stage, L)) stage = JAC_FIRST;  // the state is zero
+    if (stage == JAC_FIRST_ZERO) {
+        hipLaunchKernelGGL(k_jac4_first_zero, grid, blk, 0, s, g, L.rec, L.n_all, S1, R, cu);
+    } else if (stage == JAC_FIRST) {
         hipLaunchKernelGGL(k_jac4<0>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (jacobi4_fills_halos(stage, L)) {
         hipLaunchKernelGGL(k_jac4_rec<true>, grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);

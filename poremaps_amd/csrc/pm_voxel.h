@@ -424,6 +424,47 @@ __device__ __forceinline__ void lag_voxel4_eager(const Grid &g, int64_t c, int t
     __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c]));
 }
 
+// First pass (vel_voxel4 with p carried) from a correction state that is zero
+// everywhere, halos and outlet buffer included: the state just after a fold
+// or at the start.  Every viscous sum and pressure difference is then +0, so
+// vel_from4 gives u = 0 + (0 + Ru) * cu (0 on a closed face) and p = 0: the same
+// values from the voxel's record and momentum residual alone.  Also writes
+// the cell's periodic halo copies (the ghost fill of the new state; the
+// pressure it pins is 0 already).
+__device__ __forceinline__ void first_voxel4_zero(const Grid &g, int t, const uint2 *__restrict__ rec,
+                                                  float4 *__restrict__ S1, Res4 R, float cu) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const v2u rc = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&rec[t]));
+    const int64_t c = (int64_t)rc.x;
+    const Cases cs = decode((uint8_t)(rc.y & 0xff));
+    const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
+    const float z = 0.f;
+    v4f o;
+    o.x = z;
+    o.y = cs.bx ? z : z + (z + r.x) * cu;
+    o.z = cs.by ? z : z + (z + r.y) * cu;
+    o.w = cs.bz ? z : z + (z + r.z) * cu;
+    __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c]));
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const int64_t sx = g.sx, sy = g.sy;
+    if (g.pz) {
+        const int64_t zs = (int64_t)g.nz * sy;
+        if (i >= g.nz) __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c - zs]));
+        if (i <= 3) __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c + zs]));
+    }
+    if (g.px) {
+        if (k >= g.nx) __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c - g.nx]));
+        if (k <= 3) __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c + g.nx]));
+    }
+    if (g.py) {
+        const int64_t ys = (int64_t)g.ny * sx;
+        if (j >= g.ny) __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c - ys]));
+        if (j <= 3) __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c + ys]));
+    }
+}
+
 // The cells whose pressure the lagged pass at c recomputes: the -x and -y
 // neighbours, or their periodic images when c is on the first interior
 // index of a periodic axis, and the -z neighbour.
