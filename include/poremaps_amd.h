@@ -73,6 +73,10 @@ typedef struct {
                                 point does not depend on it; the Jacobi sweep is stable while
                                 omega * omega_p stays below about 19 (a divergent solve
                                 restarts with omega_p cut by a quarter: pm_result.restarts) */
+    int z_slabs;             /* reference mode: sweep as `mpirun -np P` does with
+                                dom_decomposition 1 1 P (P z-slabs, each against frozen
+                                copies of its neighbours' edge planes; 0 or 1: one domain;
+                                2 <= P <= nz / 3) */
 } pm_params;
 
 /* One log row (src/output.cc:193-230 column order). */

@@ -47,7 +47,7 @@ class Params(C.Structure):
         ("solving_algorithm", C.c_int), ("eps", C.c_double), ("porosity", C.c_double),
         ("dom_interest", C.c_int * 6),
         ("mode", C.c_int), ("precision", C.c_int), ("omega", C.c_double), ("device", C.c_int),
-        ("omega_p", C.c_double),
+        ("omega_p", C.c_double), ("z_slabs", C.c_int),
     ]
 
 
@@ -125,7 +125,7 @@ def _check(rc, io=False):
 def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_000_000,
                 it_eval=100, it_write=100, solving_algorithm=2, eps=1e-6, porosity=-1.0,
                 dom_interest=(0, 0, 0, 0, 0, 0), mode="reference", precision="f64", omega=1.0,
-                device=0, omega_p=0.0) -> Params:
+                device=0, omega_p=0.0, z_slabs=0) -> Params:
     p = Params()
     lib().pm_params_default(C.byref(p))
     p.nx, p.ny, p.nz = nx, ny, nz
@@ -139,6 +139,7 @@ def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_0
     p.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
     p.omega = omega
     p.omega_p = omega_p
+    p.z_slabs = z_slabs
     p.device = device
     return p
 

@@ -85,6 +85,9 @@ struct pm_ctx {
     // pm_params.omega was 0: the relaxation pair is chosen at each geometry
     // load from the porosity
     bool auto_relax = false;
+    // reference mode with pm_params.z_slabs = P > 1: the slabs' working copies
+    // (4 per slab: p, u, v, w)
+    std::vector<double *> slab;
     // the FP32 correction state is zero everywhere (after a fold or at the
     // start): the next first pass needs only the records and the residual
     bool dzero = false;

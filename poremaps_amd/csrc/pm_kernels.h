@@ -35,6 +35,14 @@ void launch_pins(const Grid &g, const uint8_t *G, T *p, T pin_in, T pin_hi, T pi
 // exchange, pins), as a wavefront of LDS tiles (REF_TILES), in one workgroup
 // (REF_ONE_WG) or one launch per hyperplane (REF_PLANES); the same bits.
 enum { REF_TILES = 0, REF_ONE_WG = 1, REF_PLANES = 2 };
+// One exact-order sweep the way `mpirun -np P` runs it with dom_decomposition
+// 1 1 P: P z-slabs, each swept (tiled wavefront) against frozen copies of its
+// neighbours' two edge planes, then the exchange and the pins.  f: p, u, v, w;
+// slab: 4 * P working arrays, slab k's of Grid::base + Grid::sy * (nk + 4)
+// elements (nk = its planes); 2 <= P <= kMaxSlabs.
+constexpr int kMaxSlabs = 256;
+void launch_ref_slab_sweep(const Grid &g, const uint8_t *G, const int32_t *code, double *const f[4],
+                           double *const *slab, int P, int solver, hipStream_t s);
 void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
                        double *v, double *w, int solver, int nsweeps, int impl, hipStream_t s);
 

@@ -557,6 +557,52 @@ void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, dou
     }
 }
 
+void launch_ref_slab_sweep(const Grid &g, const uint8_t *G, const int32_t *code, double *const f[4],
+                           double *const *slab, int P, int solver, hipStream_t s) {
+    // slab k: interior planes [z0, z0 + nk), the first nz % P slabs one plane
+    // more; its working copy holds planes z0 .. z0 + nk + 3 of the halo-2
+    // array (frozen neighbour planes on both sides), at the same pitches, so
+    // its geometry and codes are the global arrays offset by z0 planes
+    int z0[kMaxSlabs], nk[kMaxSlabs];
+    for (int k = 0, z = 0; k < P; k++) {
+        nk[k] = g.nz / P + (k < g.nz % P ? 1 : 0);
+        z0[k] = z;
+        z += nk[k];
+    }
+    auto slab_grid = [&](int k) {
+        Grid gk = g;
+        gk.nz = nk[k];
+        gk.NZ = nk[k] + 4;
+        gk.total = gk.base + gk.sy * gk.NZ;
+        return gk;
+    };
+    for (int k = 0; k < P; k++) {
+        const Grid gk = slab_grid(k);
+        for (int q = 0; q < 4; q++)
+            (void)hipMemcpyAsync(slab[4 * k + q], f[q] + (int64_t)z0[k] * g.sy, sizeof(double) * (size_t)gk.total,
+                                 hipMemcpyDeviceToDevice, s);
+    }
+    for (int k = 0; k < P; k++) {
+        const Grid gk = slab_grid(k);
+        const int64_t off = (int64_t)z0[k] * g.sy;
+        const int ntz = (gk.nz + kRT - 1) / kRT, nty = (gk.ny + kRT - 1) / kRT, ntx = (gk.nx + kRT - 1) / kRT;
+        for (int D = 0; D <= (ntz - 1) + (nty - 1) + (ntx - 1); D++) {
+            const int a_lo = std::max(0, D - (nty - 1) - (ntx - 1)), a_hi = std::min(ntz - 1, D);
+            hipLaunchKernelGGL(k_ref_tile, dim3((unsigned)((a_hi - a_lo + 1) * nty)), dim3(kRThreads), 0, s, gk,
+                               G + off, code + off, slab[4 * k], slab[4 * k + 1], slab[4 * k + 2], slab[4 * k + 3],
+                               solver, D, a_lo, nty, ntx);
+        }
+    }
+    for (int k = 0; k < P; k++) {
+        const int64_t from = g.base + 2 * g.sy, n = (int64_t)nk[k] * g.sy;
+        for (int q = 0; q < 4; q++)
+            (void)hipMemcpyAsync(f[q] + (int64_t)z0[k] * g.sy + from, slab[4 * k + q] + from, sizeof(double) * (size_t)n,
+                                 hipMemcpyDeviceToDevice, s);
+    }
+    launch_exchange_full<double>(g, f[0], f[1], f[2], f[3], s);
+    launch_pins<double>(g, G, f[0], (double)(g.nz + 2), 2.0, 3.0, s);
+}
+
 void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const double *u, const double *v,
                    const double *w, const DomBox &box, bool full, void *scratch, RedSums *out,
                    hipStream_t s) {

@@ -90,6 +90,9 @@ static int check_params(const pm_params *p) {
         return fail(PM_E_ARG, "omega must be in (0, 100), or 0 for the pair by porosity (got %g)", p->omega);
     if (!(p->omega_p >= 0.0 && p->omega_p < 100.0))
         return fail(PM_E_ARG, "omega_p must be in [0, 100) (got %g)", p->omega_p);
+    if (p->z_slabs < 0 || p->z_slabs > pm::kMaxSlabs || (p->z_slabs > 1 && p->z_slabs > p->nz / 3))
+        return fail(PM_E_ARG, "z_slabs must be 0, 1 or 2 .. min(%d, nz / 3) (got %d with nz %d)", pm::kMaxSlabs,
+                    p->z_slabs, p->nz);
     return PM_OK;
 }
 
@@ -110,7 +113,9 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
     // keeps separate colour lists
     if (fast) per += (jac || dc) ? 4 : 8;
-    return per * g.total + 4 * g.qtotal() * 8 + (int64_t)pm::reduce_scratch_bytes() + (1 << 20);
+    // reference mode in z-slabs: the slabs' working copies of p, u, v, w
+    const int64_t slabs = (!fast && p->z_slabs > 1) ? 32 * (g.total + (int64_t)4 * p->z_slabs * g.sy) : 0;
+    return per * g.total + slabs + 4 * g.qtotal() * 8 + (int64_t)pm::reduce_scratch_bytes() + (1 << 20);
 }
 
 static void free_all(pm_ctx *c) {
@@ -122,6 +127,8 @@ static void free_all(pm_ctx *c) {
     f(c->red_scratch); f(c->red_dev);
     f(c->lists.all);
     f(c->lists.rec);
+    for (double *x : c->slab) f(x);
+    c->slab.clear();
     if (!c->lists.col_in_all)
         for (int q = 0; q < 3; q++) f(c->lists.col[q]);
     c->lists = pm::FluidLists();
@@ -309,6 +316,15 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     RC(dalloc(&c->code, g.total));
     for (int q = 0; q < 4; q++) RC(dalloc(&c->X[q], g.total));
     for (int q = 0; q < 2; q++) RC(dalloc(&c->qX[q], g.qtotal()));
+    if (!c->fast() && c->prm.z_slabs > 1) {
+        // working copies of the z-slabs (their planes + 2 frozen planes each side)
+        const int P = c->prm.z_slabs;
+        c->slab.assign((size_t)4 * P, nullptr);
+        for (int k = 0; k < P; k++) {
+            const int nk = g.nz / P + (k < g.nz % P ? 1 : 0);
+            for (int q = 0; q < 4; q++) RC(dalloc(&c->slab[4 * k + q], g.base + g.sy * (nk + 4)));
+        }
+    }
     if (c->fast() && !c->dc() && c->jacobi())
         for (int q = 0; q < 4; q++) RC(dalloc(&c->X2[q], g.total));
     if (c->dc()) {
@@ -578,8 +594,12 @@ static int one_sweep(pm_ctx *c) {
     const double cu = om * pm::kDt, cp = wp * pm::kC2 * pm::kDt;
     if (!c->fast()) {
         tb(c, TC_REF);
-        pm::launch_ref_sweeps(g, c->G, c->code, c->X[0], c->X[1], c->X[2], c->X[3], c->prm.solving_algorithm,
-                              1, c->ref_impl, s);
+        if (!c->slab.empty())
+            pm::launch_ref_slab_sweep(g, c->G, c->code, c->X, c->slab.data(), c->prm.z_slabs,
+                                      c->prm.solving_algorithm, s);
+        else
+            pm::launch_ref_sweeps(g, c->G, c->code, c->X[0], c->X[1], c->X[2], c->X[3], c->prm.solving_algorithm,
+                                  1, c->ref_impl, s);
         te(c);
     } else if (!c->dc()) {
         const double pin_in = (double)(g.nz + 2);

@@ -102,8 +102,15 @@ int main(int argc, char **argv) {
     if (knob(flags, "--omega-p", "POREMAPS_OMEGA_P", &v)) prm.omega_p = std::atof(v.c_str());
     if (knob(flags, "--device", "POREMAPS_DEVICE", &v)) prm.device = std::atoi(v.c_str());
 
-    std::printf("Given domain decomposition nx = %i, ny = %i, nz = %i (one GPU: ignored)\n",
-                extra.dom_decomposition[0], extra.dom_decomposition[1], extra.dom_decomposition[2]);
+    // dom_decomposition 1 1 P (P z-slabs) in the reference mode: the iterates of
+    // `mpirun -np P`, each slab swept against frozen copies of its
+    // neighbours' edge planes; any other decomposition runs as one domain
+    const int *dd = extra.dom_decomposition;
+    const bool slabs = prm.mode == PM_MODE_REFERENCE && dd[0] == 1 && dd[1] == 1 && dd[2] > 1 &&
+                       dd[2] <= prm.nz / 3;
+    if (slabs) prm.z_slabs = dd[2];
+    std::printf("Given domain decomposition nx = %i, ny = %i, nz = %i (%s)\n", dd[0], dd[1], dd[2],
+                slabs ? "reproduced as z-slabs on one GPU" : "one GPU: ignored");
     std::printf("Domain size: domx = %i, domy = %i, domz = %i; boundary_method %i; solver %i\n", prm.nx, prm.ny,
                 prm.nz, prm.boundary_method, prm.solving_algorithm);
 

@@ -148,6 +148,40 @@ def test_reference_sweeps_large_domain_bit_exact(form):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("P", [2, 3, 4])
+@pytest.mark.parametrize("solver,bc", [(2, 0), (1, 2), (3, 4), (2, 1)])
+def test_reference_z_slabs_match_slab_oracle(P, solver, bc):
+    """dom_decomposition 1 1 P on one GPU: P z-slabs, each swept against
+    frozen copies of its neighbours' edge planes, then the exchange and the
+    pins -- bit for bit the oracle's restatement of `mpirun -np P`
+    (orc_sweeps_ref_slabs), with slabs of unequal thickness (nz 13)."""
+    raw = random_geometry((13, 9, 11), 0.35, 11 * P + bc)
+    say("reference z-slabs", P, solver, bc)
+    G, code, st = oracle_state(raw, bc)
+    O.sweeps_ref_slabs(solver, G, code, *st, bc, 5, P)
+    with pm.Solver(raw, boundary_method=bc, solving_algorithm=solver, z_slabs=P) as s:
+        s.step(5)
+        for name, a, b in zip("puvw", s.state_halo(), st):
+            assert np.array_equal(a, b), f"{name} differs"
+
+
+def test_reference_z_slabs_several_tiles_per_slab():
+    """Slabs of 14 / 13 / 13 planes (two tile layers each) on a 40 x 20 x 30
+    field; the decomposition changes the iterates (frozen slab edges), and the
+    GPU follows the oracle's slabs, not the one-domain sweep."""
+    raw = random_geometry((40, 20, 30), 0.35, 77)
+    say("reference z-slabs large")
+    G, code, st = oracle_state(raw, 0)
+    one = [a.copy() for a in st]
+    O.sweeps_ref_slabs(2, G, code, *st, 0, 3, 3)
+    O.sweeps_ref_slabs(2, G, code, *one, 0, 3, 1)
+    assert not np.array_equal(st[1], one[1])
+    with pm.Solver(raw, boundary_method=0, solving_algorithm=2, z_slabs=3) as s:
+        s.step(3)
+        for name, a, b in zip("puvw", s.state_halo(), st):
+            assert np.array_equal(a, b), f"{name} differs"
+
+
 def test_reference_solve_channel_matches_binary_and_oracle(channel_raw, known):
     kn = known["channel_gs"]
     say("reference solve channel")
@@ -463,4 +497,28 @@ def test_cli_fast_mode_default_relaxation(tmp_path, channel_raw):
     assert last[-1] == "%e" % rows[-1]["wk33"]
     v = np.fromfile(tmp_path / "velz_rect_channel_24_14_50_vs_5e-05.raw", np.float64).reshape(50, 14, 24)
     assert np.array_equal(v, velz * 5e-5)  # the files hold velocities times the voxel size
+
+
+def test_cli_dom_decomposition_z_slabs(tmp_path, channel_raw):
+    """dom_decomposition 1 1 3 in the .inp: the CLI sweeps as `mpirun -np 3`
+    does (three z-slabs), the same solve as the library's z_slabs=3."""
+    say("cli z-slabs")
+    (tmp_path / "rect_channel_24_14_50_vs_5e-05.raw").write_bytes(channel_raw.tobytes())
+    inp = open(os.path.join(GOLDEN, "input_rect_channel_24_14_50_vs_5e-05.inp")).read()
+    inp = inp.replace("eps 1e-07", "eps 1e-06").replace("porosity 1.0", "porosity -1.0")
+    inp = inp.replace("dom_decomposition 0 0 0", "dom_decomposition 1 1 3")
+    (tmp_path / "input.inp").write_text(inp)
+    r = subprocess.run([pm.CLI_PATH, "input.inp"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "reproduced as z-slabs" in r.stdout
+    with pm.Solver(channel_raw, boundary_method=0, solving_algorithm=2, eps=1e-6, voxel_size=5e-5,
+                   z_slabs=3) as s:
+        res, rows = s.solve()
+        velz = s.field("velz")
+    log = (tmp_path / "permeability_rect_channel_24_14_50_vs_5e-05.log").read_text().splitlines()
+    last = log[-1].split(", ")
+    assert int(last[0]) == rows[-1]["iteration"]
+    assert last[-1] == "%e" % rows[-1]["wk33"]
+    v = np.fromfile(tmp_path / "velz_rect_channel_24_14_50_vs_5e-05.raw", np.float64).reshape(50, 14, 24)
+    assert np.array_equal(v, velz * 5e-5)
 
