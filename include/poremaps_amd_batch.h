@@ -61,6 +61,10 @@ int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int nz, int bo
 /* Stopping tolerance of one sample (default: the batch's eps), the eps of
  * the reference's stop rule (src/main.cc:181-185). */
 int pm_batch_set_eps(pm_batch *b, int slot, double eps);
+/* The sample's dom_interest box (x0 x1 y0 y1 z0 z1, interior indices,
+ * inclusive; all 0: none): k13, k23, k33 of its log rows and result then
+ * follow src/evaluation.cc:54-211 for that sub-domain. */
+int pm_batch_set_dom_interest(pm_batch *b, int slot, const int box[6]);
 
 /* One it_eval interval for every running sample, then their convergence
  * tests; *n_running = samples still running afterwards. */

@@ -43,6 +43,8 @@ typedef struct {
     double porosity;         /* -1: computed */
     double eps;              /* stopping tolerance; <= 0: the batch's eps */
     double arrival_s;        /* seconds after the start of pm_serve */
+    int dom_interest[6];     /* sub-domain x0 x1 y0 y1 z0 z1 of the physical axes (interior
+                                indices, inclusive; all 0: none), rotated with each solve */
 } pm_request;
 
 typedef struct {
@@ -50,6 +52,9 @@ typedef struct {
     double arrival_s, admit_s, done_s;
     int sweeps[3];
     int state[3];            /* PM_SAMPLE_* of the three solves */
+    double k_dom[9];         /* with a dom_interest box: the sub-domain tensor, k[3*i + j]
+                                from each solve's (k13, k23, k33) (src/evaluation.cc:54-211,
+                                pressure gradient over the box along the flow axis) */
 } pm_response;
 
 /* Called when a request completes and at least once a second while solves

@@ -309,6 +309,13 @@ class Batch:
     def set_eps(self, slot: int, eps: float):
         _check(_batch_lib().pm_batch_set_eps(self._h, slot, eps))
 
+    def set_dom_interest(self, slot: int, box):
+        """x0 x1 y0 y1 z0 z1 (interior indices, inclusive) of the sample's
+        sub-domain permeability (k13, k23, k33)."""
+        L = _batch_lib()
+        L.pm_batch_set_dom_interest.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        _check(L.pm_batch_set_dom_interest(self._h, slot, (C.c_int * 6)(*box)))
+
     def step(self) -> int:
         n = C.c_int()
         _check(_batch_lib().pm_batch_step(self._h, C.byref(n)))
@@ -355,16 +362,22 @@ class Batch:
 class Request(C.Structure):
     _fields_ = [("raw", C.c_void_p), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("boundary_method", C.c_int), ("voxel_size", C.c_double), ("porosity", C.c_double),
-                ("eps", C.c_double), ("arrival_s", C.c_double)]
+                ("eps", C.c_double), ("arrival_s", C.c_double), ("dom_interest", C.c_int * 6)]
 
 
 class Response(C.Structure):
     _fields_ = [("k", C.c_double * 9), ("arrival_s", C.c_double), ("admit_s", C.c_double),
-                ("done_s", C.c_double), ("sweeps", C.c_int * 3), ("state", C.c_int * 3)]
+                ("done_s", C.c_double), ("sweeps", C.c_int * 3), ("state", C.c_int * 3),
+                ("k_dom", C.c_double * 9)]
 
     @property
     def tensor(self) -> np.ndarray:
         return np.array(self.k[:], dtype=np.float64).reshape(3, 3)
+
+    @property
+    def tensor_dom(self) -> np.ndarray:
+        """The dom_interest sub-domain tensor (zeros without a box)."""
+        return np.array(self.k_dom[:], dtype=np.float64).reshape(3, 3)
 
     @property
     def latency_s(self) -> float:
@@ -428,8 +441,9 @@ def tensor(raw: np.ndarray, *, boundary_method=0, voxel_size=1.0, porosity=-1.0,
 
 def serve(requests, *, realtime=True, progress=None, max_solves=0, small_cells=SMALL_CELLS, **batch_kw):
     """Serve permeability-tensor requests: `requests` is a sequence of
-    (raw, boundary_method, voxel_size, eps, arrival_s) (eps <= 0: the batch's
-    eps).  ``progress(completed, running, t_s)`` is called as requests
+    (raw, boundary_method, voxel_size, eps, arrival_s[, dom_interest]) (eps
+    <= 0: the batch's eps; dom_interest: x0 x1 y0 y1 z0 z1 of the physical
+    axes, for Response.tensor_dom).  ``progress(completed, running, t_s)`` is called as requests
     complete and about once a second; a true return stops serving.
     Requests of at most ``small_cells`` voxels run in a second batch that
     shares the GPU time with the first (0: one batch).  Returns (responses,
@@ -437,9 +451,11 @@ def serve(requests, *, realtime=True, progress=None, max_solves=0, small_cells=S
     raws = [np.ascontiguousarray(r[0], dtype=np.uint8) for r in requests]
     n = len(raws)
     req = (Request * max(n, 1))()
-    for i, (raw, (_, bc, dx, eps, t)) in enumerate(zip(raws, requests)):
+    for i, (raw, r) in enumerate(zip(raws, requests)):
+        _, bc, dx, eps, t = r[:5]
+        box = tuple(r[5]) if len(r) > 5 else (0,) * 6
         nz, ny, nx = raw.shape
-        req[i] = Request(raw.ctypes.data, nx, ny, nz, bc, dx, -1.0, eps, t)
+        req[i] = Request(raw.ctypes.data, nx, ny, nz, bc, dx, -1.0, eps, t, (C.c_int * 6)(*box))
     resp = (Response * max(n, 1))()
     st = ServeStats()
     cb = SERVE_PROGRESS_FN(lambda d, r, t, _u: int(bool(progress(d, r, t)))) if progress else SERVE_PROGRESS_FN(0)

@@ -45,6 +45,8 @@ struct BSample {
     const uint2 *rec;
     // relaxation of this sample: omega*dt, omega_p*c2*dt
     double cu, cp;
+    // dom_interest sub-domain of the permeability (interior indices, inclusive)
+    DomBox box;
     int chunk_begin, chunk_end;
 };
 
@@ -229,6 +231,22 @@ __global__ __launch_bounds__(kB) void k_b_reduce(const BSample *__restrict__ S, 
         nadd(a.s[1], v[c]);
         nadd(a.s[2], wc);
         a.wmax = wc > 0.0 ? wc : 0.0;
+        if (s.box.on) {
+            // the sub-domain sums of k_reduce_state<true>
+            int i, j, k;
+            cell_ijk(s.g, c, i, j, k);
+            i -= 2; j -= 2; k -= 2;
+            const DomBox &bx = s.box;
+            if (k >= bx.d[0] && k <= bx.d[1] && j >= bx.d[2] && j <= bx.d[3] && i >= bx.d[4] && i <= bx.d[5]) {
+                const double *p = which == 2 ? s.X[0] : (const double *)(which ? s.B[0] : s.A[0]);
+                a.n[1] = 1;
+                nadd(a.s[3], u[c]);
+                nadd(a.s[4], v[c]);
+                nadd(a.s[5], wc);
+                if (i == bx.d[4]) { a.n[2] = 1; nadd(a.s[6], p[c]); }
+                if (i == bx.d[5]) { a.n[3] = 1; nadd(a.s[7], p[c]); }
+            }
+        }
     }
     block_reduce_store(a, part + blockIdx.x);
 }
@@ -445,6 +463,9 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
         d.B4 = c->S4[1];
         d.R4 = c->R4;
         d.rec = c->lists.rec;
+        const int *dm = c->prm.dom_interest;
+        d.box.on = !(dm[0] == 0 && dm[1] == 0 && dm[2] == 0 && dm[3] == 0 && dm[4] == 0 && dm[5] == 0);
+        for (int q = 0; q < 6; q++) d.box.d[q] = dm[q];
         const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
         d.cu = om * pm::kDt;
         d.cp = wp * pm::kC2 * pm::kDt;
@@ -624,6 +645,21 @@ extern "C" int pm_batch_set_eps(pm_batch *b, int slot, double eps) {
     if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
     if (!(eps > 0.0)) return pm_fail(PM_E_ARG, "eps must be positive");
     b->ctx[slot]->prm.eps = eps;
+    return PM_OK;
+}
+
+extern "C" int pm_batch_set_dom_interest(pm_batch *b, int slot, const int box[6]) {
+    if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    if (!box) return pm_fail(PM_E_ARG, "null box");
+    pm_ctx *c = b->ctx[slot];
+    const int n[3] = {c->prm.nx, c->prm.ny, c->prm.nz};
+    const bool off = box[0] == 0 && box[1] == 0 && box[2] == 0 && box[3] == 0 && box[4] == 0 && box[5] == 0;
+    for (int a = 0; a < 3 && !off; a++)
+        if (box[2 * a] < 0 || box[2 * a] > box[2 * a + 1] || box[2 * a + 1] >= n[a])
+            return pm_fail(PM_E_ARG, "dom_interest %d %d outside axis %d of %d cells", box[2 * a], box[2 * a + 1], a,
+                           n[a]);
+    for (int q = 0; q < 6; q++) c->prm.dom_interest[q] = box[q];
+    b->shape_dirty = true;  // the sample descriptors carry the box
     return PM_OK;
 }
 

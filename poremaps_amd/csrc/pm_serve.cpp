@@ -49,6 +49,22 @@ void scatter(int r, const pm_log_row &row, double k[9]) {
     k[3 * az + az] = row.wk33;
 }
 
+// The sub-domain (k13, k23, k33) of rotation r, scattered the same way.
+void scatter_dom(int r, const pm_log_row &row, double k[9]) {
+    const int ax = kNewAxis[r][0], ay = kNewAxis[r][1], az = kNewAxis[r][2];
+    k[3 * ax + az] = row.k13;
+    k[3 * ay + az] = row.k23;
+    k[3 * az + az] = row.k33;
+}
+
+// A physical-axes box (x0 x1 y0 y1 z0 z1) in the axes of rotation r.
+void rotate_box(const int box[6], int r, int out[6]) {
+    for (int a = 0; a < 3; a++) {
+        out[2 * a] = box[2 * kNewAxis[r][a]];
+        out[2 * a + 1] = box[2 * kNewAxis[r][a] + 1];
+    }
+}
+
 }  // namespace
 
 extern "C" int pm_tensor(const pm_batch_params *p, const uint8_t *raw, int nx, int ny, int nz, int bc, double dx,
@@ -158,6 +174,7 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
             if (st == PM_SAMPLE_RUNNING) continue;
             const int id = L.live[s].req, ro = L.live[s].rot;
             scatter(ro, rr.last, resp[id].k);
+            scatter_dom(ro, rr.last, resp[id].k_dom);
             resp[id].sweeps[ro] = rr.sweeps;
             resp[id].state[ro] = st;
             pm_batch_remove(L.b, (int)s);
@@ -201,6 +218,11 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
                 rc = pm_batch_add(L.b, rot.data(), d[0], d[1], d[2], q.boundary_method, q.voxel_size, q.porosity,
                                   &slot);
                 if (rc == PM_OK && q.eps > 0.0) rc = pm_batch_set_eps(L.b, slot, q.eps);
+                if (rc == PM_OK) {
+                    int box[6];
+                    rotate_box(q.dom_interest, r, box);
+                    rc = pm_batch_set_dom_interest(L.b, slot, box);
+                }
                 if (rc != PM_OK) break;
                 if ((int)L.live.size() <= slot) L.live.resize(slot + 1, Live{-1, -1});
                 L.live[slot] = Live{id, r};

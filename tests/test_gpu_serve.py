@@ -120,3 +120,32 @@ def test_serve_per_request_eps():
     for r, eps in zip(resp, (1e-6, 1e-9)):
         k, _ = pm.tensor(m, boundary_method=0, voxel_size=1e-5, eps=eps)
         assert np.array_equal(r.tensor, k)
+
+
+def test_serve_dom_interest_tensor():
+    """A request with a dom_interest box: each rotated solve reports the
+    sub-domain (k13, k23, k33) of the box in its own axes, and the response's
+    tensor_dom equals those of the rotated arrays solved alone with the
+    rotated boxes (the reference's sub-domain permeability, pressure gradient
+    along each solve's flow axis)."""
+    raw = aniso()  # (nz, ny, nx) = (18, 14, 22)
+    box = (3, 17, 2, 11, 4, 13)  # x0 x1 y0 y1 z0 z1
+    say("serve dom_interest")
+    resp, _ = pm.serve([(raw, 0, 1e-5, 1e-6, 0.0, box)], realtime=False)
+    kd = resp[0].tensor_dom
+    assert np.array_equal(resp[0].tensor, pm.tensor(raw, boundary_method=0, voxel_size=1e-5)[0])
+    # rotation r: array axes (z, y, x) -> transpose perm; box of the rotated
+    # array's (x, y, z) = box of physical axes new_axis[r]
+    perms = ((0, 1, 2), (2, 0, 1), (1, 2, 0))
+    new_axis = ((0, 1, 2), (1, 2, 0), (2, 0, 1))
+    for perm, na in zip(perms, new_axis):
+        a = np.ascontiguousarray(raw.transpose(perm))
+        rb = sum(((box[2 * ax], box[2 * ax + 1]) for ax in na), ())
+        with pm.Solver(a, boundary_method=0, voxel_size=1e-5, mode="jacobi", precision="f32dc", omega=pm.AUTO,
+                       eps=1e-6, dom_interest=rb) as s:
+            _, rows = s.solve()
+        last = rows[-1]
+        ax, ay, az = na
+        assert (kd[ax, az], kd[ay, az], kd[az, az]) == (last["k13"], last["k23"], last["k33"])
+        assert last["k33"] != 0.0
+
