@@ -273,12 +273,19 @@ constexpr int kRN = kRB * kRB * kRB;     // 1728 cells, 63.9 KB of LDS
 
 constexpr int kRThreads = 4 * kRT * kRT;  // 4 waves copy, the first one computes
 
-__global__ __launch_bounds__(kRThreads) void k_ref_tile(Grid g, const uint8_t *__restrict__ G,
-                                                       const int32_t *__restrict__ code, double *p, double *u,
-                                                       double *v, double *w, int solver, int D, int a0, int nty,
-                                                       int ntx) {
-    const int A = a0 + (int)blockIdx.x / nty, B = (int)blockIdx.x % nty, C = D - A - B;
-    if (C < 0 || C >= ntx) return;  // the same for the whole workgroup
+// Slab table of the batched z-slab sweep (kernel argument): slab k's first
+// interior plane, its planes and its working copies p, u, v, w.
+constexpr int kSlabsBatched = 16;
+struct RefSlabs {
+    int z0[kSlabsBatched], nk[kSlabsBatched];
+    double *f[4 * kSlabsBatched];
+};
+
+// Tile (A, B, C) of the grid g (whole workgroup; the caller checked that the
+// tile exists).
+__device__ __forceinline__ void ref_tile(const Grid &g, const uint8_t *__restrict__ G,
+                                         const int32_t *__restrict__ code, double *p, double *u, double *v,
+                                         double *w, int solver, int A, int B, int C) {
     __shared__ double Lp[kRN], Lu[kRN], Lv[kRN], Lw[kRN];
     __shared__ int32_t Lc[kRN];
     __shared__ uint8_t LG[kRN];
@@ -331,6 +338,35 @@ __global__ __launch_bounds__(kRThreads) void k_ref_tile(Grid g, const uint8_t *_
         const int l = ((a2 + 2) * kRB + (b2 + 2)) * kRB + (x2 + 2);
         p[gc] = Lp[l]; u[gc] = Lu[l]; v[gc] = Lv[l]; w[gc] = Lw[l];
     }
+}
+
+__global__ __launch_bounds__(kRThreads) void k_ref_tile(Grid g, const uint8_t *__restrict__ G,
+                                                       const int32_t *__restrict__ code, double *p, double *u,
+                                                       double *v, double *w, int solver, int D, int a0, int nty,
+                                                       int ntx) {
+    const int A = a0 + (int)blockIdx.x / nty, B = (int)blockIdx.x % nty, C = D - A - B;
+    if (C < 0 || C >= ntx) return;  // the same for the whole workgroup
+    ref_tile(g, G, code, p, u, v, w, solver, A, B, C);
+}
+
+// The same for the z-slabs of one tile-diagonal D at once (the slabs are
+// independent within a sweep): block -> (slab, A, B), the slab's grid and
+// working copy from the table.
+__global__ __launch_bounds__(kRThreads) void k_ref_tile_slabs(Grid g, const uint8_t *__restrict__ G,
+                                                             const int32_t *__restrict__ code, RefSlabs tab,
+                                                             int solver, int D, int ntz_max, int nty, int ntx) {
+    const int per = ntz_max * nty;
+    const int k = (int)blockIdx.x / per, A = ((int)blockIdx.x / nty) % ntz_max, B = (int)blockIdx.x % nty;
+    const int C = D - A - B;
+    const int nk = tab.nk[k];
+    if (A >= (nk + kRT - 1) / kRT || C < 0 || C >= ntx) return;  // the same for the whole workgroup
+    Grid gk = g;
+    gk.nz = nk;
+    gk.NZ = nk + 4;
+    gk.total = gk.base + gk.sy * gk.NZ;
+    const int64_t off = (int64_t)tab.z0[k] * g.sy;
+    ref_tile(gk, G + off, code + off, tab.f[4 * k], tab.f[4 * k + 1], tab.f[4 * k + 2], tab.f[4 * k + 3], solver,
+             A, B, C);
 }
 
 // Whole sweeps in one workgroup (small domains): hyperplanes, then the
@@ -582,15 +618,31 @@ void launch_ref_slab_sweep(const Grid &g, const uint8_t *G, const int32_t *code,
             (void)hipMemcpyAsync(slab[4 * k + q], f[q] + (int64_t)z0[k] * g.sy, sizeof(double) * (size_t)gk.total,
                                  hipMemcpyDeviceToDevice, s);
     }
-    for (int k = 0; k < P; k++) {
-        const Grid gk = slab_grid(k);
-        const int64_t off = (int64_t)z0[k] * g.sy;
-        const int ntz = (gk.nz + kRT - 1) / kRT, nty = (gk.ny + kRT - 1) / kRT, ntx = (gk.nx + kRT - 1) / kRT;
-        for (int D = 0; D <= (ntz - 1) + (nty - 1) + (ntx - 1); D++) {
-            const int a_lo = std::max(0, D - (nty - 1) - (ntx - 1)), a_hi = std::min(ntz - 1, D);
-            hipLaunchKernelGGL(k_ref_tile, dim3((unsigned)((a_hi - a_lo + 1) * nty)), dim3(kRThreads), 0, s, gk,
-                               G + off, code + off, slab[4 * k], slab[4 * k + 1], slab[4 * k + 2], slab[4 * k + 3],
-                               solver, D, a_lo, nty, ntx);
+    if (P <= kSlabsBatched) {
+        // every slab's tiles of one diagonal in one launch
+        RefSlabs tab;
+        int ntz_max = 0;
+        for (int k = 0; k < P; k++) {
+            tab.z0[k] = z0[k];
+            tab.nk[k] = nk[k];
+            for (int q = 0; q < 4; q++) tab.f[4 * k + q] = slab[4 * k + q];
+            ntz_max = std::max(ntz_max, (nk[k] + kRT - 1) / kRT);
+        }
+        const int nty = (g.ny + kRT - 1) / kRT, ntx = (g.nx + kRT - 1) / kRT;
+        for (int D = 0; D <= (ntz_max - 1) + (nty - 1) + (ntx - 1); D++)
+            hipLaunchKernelGGL(k_ref_tile_slabs, dim3((unsigned)(P * ntz_max * nty)), dim3(kRThreads), 0, s, g, G,
+                               code, tab, solver, D, ntz_max, nty, ntx);
+    } else {
+        for (int k = 0; k < P; k++) {
+            const Grid gk = slab_grid(k);
+            const int64_t off = (int64_t)z0[k] * g.sy;
+            const int ntz = (gk.nz + kRT - 1) / kRT, nty = (gk.ny + kRT - 1) / kRT, ntx = (gk.nx + kRT - 1) / kRT;
+            for (int D = 0; D <= (ntz - 1) + (nty - 1) + (ntx - 1); D++) {
+                const int a_lo = std::max(0, D - (nty - 1) - (ntx - 1)), a_hi = std::min(ntz - 1, D);
+                hipLaunchKernelGGL(k_ref_tile, dim3((unsigned)((a_hi - a_lo + 1) * nty)), dim3(kRThreads), 0, s,
+                                   gk, G + off, code + off, slab[4 * k], slab[4 * k + 1], slab[4 * k + 2],
+                                   slab[4 * k + 3], solver, D, a_lo, nty, ntx);
+            }
         }
     }
     for (int k = 0; k < P; k++) {
