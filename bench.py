@@ -8,7 +8,8 @@
 
 A step is one it_eval interval of the solver (100 sweeps of the bulk sweep,
 with the defect-correction fold and residual of that interval) over the whole
-synthetic field, fields resident in HBM.  value = GLUP/s = nx*ny*nz*sweeps/s.
+synthetic field, fields resident in HBM.  value = GLUP/s = nx*ny*nz*sweeps/s
+(dense count); fluid_glups counts the fluid voxels only.
 
 Workloads (synthetic, generated from a seed):
   config 0: the reference's own CPU case, a tube 54x54x50 (r 20, bc 0, dx 2e-5);
@@ -388,6 +389,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if a.precision == "f32dc" else "f64",
         "data": "synthetic",
+        "fluid_glups": round(glups * float((g == 0).mean()), 3),
         "config": {"workload": spec["desc"], "nx": g.shape[2], "ny": g.shape[1], "nz": g.shape[0],
                    "mode": a.mode, "precision": a.precision, "omega": a.omega, "omega_p": a.omega_p,
                    "sweeps_per_step": a.it_eval, "porosity": float((g == 0).mean())},
