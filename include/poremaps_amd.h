@@ -73,11 +73,17 @@ typedef struct {
                                 point does not depend on it; the Jacobi sweep is stable while
                                 omega * omega_p stays below about 19 (a divergent solve
                                 restarts with omega_p cut by a quarter: pm_result.restarts) */
-    int z_slabs;             /* reference mode: sweep as `mpirun -np P` does with
-                                dom_decomposition 1 1 P (P z-slabs, each against frozen
-                                copies of its neighbours' edge planes; 0 or 1: one domain;
-                                2 <= P <= nz / 3) */
+    int z_slabs;             /* reference mode: shorthand for dom_decomposition 1 1 P
+                                (0 or 1: not used) */
+    int dom_decomposition[3];/* reference mode: sweep as `mpirun -np P*Q*R` does with
+                                dom_decomposition P Q R (src/main_stokes_solver.cc:161-193,
+                                src/geometry.cc:129-171): every block swept against its halo
+                                copies of the other blocks as the sweep started; 0 0 0 or
+                                1 1 1: one domain.  1 <= P <= nx, Q <= ny, R <= nz, at most
+                                PM_MAX_BLOCKS blocks.  Fast modes ignore it. */
 } pm_params;
+
+#define PM_MAX_BLOCKS 4096
 
 /* One log row (src/output.cc:193-230 column order). */
 typedef struct {

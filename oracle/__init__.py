@@ -77,6 +77,9 @@ def _declare(L):
     L.orc_sweep_ref.argtypes = [_I, _u8p, _i32p, _I, _I, _I, _f64p, _f64p, _f64p, _f64p]
     L.orc_sweeps_ref_slabs.argtypes = [_I, _u8p, _i32p, _I, _I, _I, _I, _f64p, _f64p, _f64p, _f64p, _I, _I]
     L.orc_sweeps_ref_slabs.restype = _I
+    L.orc_sweeps_ref_blocks.argtypes = [_I, _u8p, _i32p, _I, _I, _I, _I, _f64p, _f64p, _f64p, _f64p, _VP, _I,
+                                        C.POINTER(C.c_int)]
+    L.orc_sweeps_ref_blocks.restype = _I
     L.orc_convergence.argtypes = [_u8p, _f64p, _I, _I, _I, C.POINTER(C.c_double)]
     L.orc_convergence.restype = _D
     L.orc_permeability.argtypes = [_u8p, _f64p, _f64p, _f64p, _f64p, _I, _I, _I,
@@ -155,6 +158,21 @@ def sweeps_ref_slabs(solver: int, G, code, p, u, v, w, bc: int, nsweeps: int, th
     dom_decomposition 1 1 P runs them; threads = 1 is the serial sweep."""
     if lib().orc_sweeps_ref_slabs(solver, G, code, *_dims(G), bc, p, u, v, w, nsweeps, threads) != 0:
         raise MemoryError("orc_sweeps_ref_slabs: allocation failed")
+
+
+def sweeps_ref_blocks(solver: int, G, code, p, u, v, w, bc: int, nsweeps: int, dims, outlet=None) -> None:
+    """nsweeps reference sweeps, exchanges and pins as `mpirun -np P*Q*R`
+    with dom_decomposition P Q R runs them (dims = (P, Q, R), x, y, z).
+    outlet: the outlet plane before its last pins ((ny+4, nx+4) float64,
+    updated in place), carried between calls; None for one call from a state
+    that has not been swept since its pins."""
+    d = (C.c_int * 3)(*dims)
+    ptr = None
+    if outlet is not None:
+        assert outlet.dtype == np.float64 and outlet.flags.c_contiguous and outlet.shape == G.shape[1:]
+        ptr = outlet.ctypes.data
+    if lib().orc_sweeps_ref_blocks(solver, G, code, *_dims(G), bc, p, u, v, w, ptr, nsweeps, d) != 0:
+        raise MemoryError("orc_sweeps_ref_blocks: allocation failed")
 
 
 def convergence(G, w, kprev: float):

@@ -342,118 +342,255 @@ void orc_permeability(const uint8_t *G, const double *u, const double *v,
     out[6] = wmax * dx;
 }
 
-/* src/calc_flow.cc:82-165 at one rank. */
-/* The reference's sweep with the interior cut into P z-slabs, the way
- * `mpirun -np P` runs it with dom_decomposition 1 1 P
- * (src/parallelization.cc:193-318, src/boundary_conditions.cc:46-78): every
- * slab sweeps its own planes in lexicographic order (orc_sweep_ref) against
- * its own frozen copies of its neighbours' planes, then copies its x / y
- * periodic halos, then takes its 2-plane z halos from the neighbouring slabs'
- * edge planes (from the last / first slab across the periodic z boundary),
- * then the slabs owning the pinned planes apply the pins.  One OpenMP thread
- * per slab; slab k holds planes [z0, z0 + n) of the interior, the first
- * nz % P slabs one plane more.  Geometry and codes are read in place (a slab's
- * rows of the halo-including arrays); the state is copied into the slabs and
- * back.  P = 1 gives orc_sweep_ref + exchange + pins exactly.  Used to time
- * the CPU baseline on all host cores. */
-int orc_sweeps_ref_slabs(int solver, const uint8_t *G, const int32_t *code, int nx, int ny, int nz, int bc,
-                         double *p, double *u, double *v, double *w, int nsweeps, int P)
+/* The reference's sweep under `mpirun -np P*Q*R` with dom_decomposition
+ * P Q R, one OpenMP thread per block (several blocks per thread when the
+ * runtime grants fewer threads).
+ *
+ * Blocks: src/geometry.cc:129-171 (get_dom_limits): along each axis the first
+ * size % dims blocks hold one cell more.  Rank numbering (MPI_Cart_create
+ * under MPICH, src/main_stokes_solver.cc:182-193): z fastest, then y, then x.
+ * Neighbours: MPI_Cart_shift with the periods of
+ * src/main_stokes_solver.cc:163-165; a block is its own neighbour along a
+ * periodic axis with one block.
+ *
+ * Per sweep (src/calc_flow.cc:105-117): every block sweeps its interior in
+ * lexicographic order against its own halo copies (orc_sweep_ref on the
+ * block's halo-2 array); then communicate_halos (src/parallelization.cc:
+ * 193-318) axis by axis: every block packs its two edge layers of the full
+ * face (halo rows included), then unpacks its neighbours' layers into its
+ * halo (nothing where there is no neighbour); then the blocks holding
+ * global z-plane 1 / nz+1, nz+2 apply the pins
+ * (src/boundary_conditions.cc:46-78).  A block's halo copy of another
+ * block's cells is therefore taken before the pins: on the outlet plane
+ * z = nz+1 it holds the swept value, not 3.0.
+ *
+ * Geometry and codes: the windows of the global halo-2 arrays (the geometry
+ * halo exchange of src/parallelization.cc:67-190 gives every block exactly
+ * these cells; codes depend only on them).  The state starts from the
+ * windows of the global arrays, except that a block's halo cells on the
+ * outlet plane inside the global interior and outside the block take
+ * outlet[] (the swept values of that plane before its last pins, (ny+4) x
+ * (nx+4) like a plane of the global arrays; NULL: the plane of p, right for
+ * a state that has not been swept since its pins).  On return the global
+ * arrays hold every block's interior and the global halo cells next to it
+ * (faces, edges, corners) from its halo; outlet[] (if given) the swept
+ * outlet plane.
+ * 0, or -1 (allocation failure). */
+typedef struct {
+    int lo[3], n[3], N[3];   /* first interior cell (x, y, z), extent, extent with halo */
+    int nb[6];               /* neighbour block along -x, +x, -y, +y, -z, +z; -1 none */
+    uint8_t *G;
+    int32_t *code;
+    double *f[4];
+    double *snd[2];          /* the packed low / high edge layers of the four fields */
+} orc_block;
+
+static inline size_t bix(const orc_block *b, int i, int j, int k)
 {
-    const int NX = HX(nx), NY = HY(ny);
-    const size_t plane = (size_t)NX * NY;
-    int px, py, pz;
-    periodic_axes(bc, &px, &py, &pz);
-    if (P < 1) P = 1;
-    if (P > nz) P = nz;
-    int *z0 = malloc(sizeof(int) * (size_t)P), *nk = malloc(sizeof(int) * (size_t)P);
-    double **S = calloc((size_t)P * 4, sizeof(double *));
-    if (!z0 || !nk || !S) { free(z0); free(nk); free(S); return -1; }
-    int fail = 0;
-    for (int k = 0, z = 0; k < P; k++) {
-        nk[k] = nz / P + (k < nz % P ? 1 : 0);
-        z0[k] = z;
-        z += nk[k];
-        for (int q = 0; q < 4 && !fail; q++) {
-            S[4 * k + q] = malloc(sizeof(double) * plane * (size_t)(nk[k] + 4));
-            if (!S[4 * k + q]) fail = 1;
+    return ((size_t)i * b->N[1] + j) * b->N[0] + k;
+}
+
+/* Edge layers of axis a (0 x, 1 y, 2 z): layer l of the face lies at index l
+ * along a; the face is every cell of the other two axes (halos included). */
+static size_t face_cells(const orc_block *b, int a)
+{
+    return (size_t)b->N[0] * b->N[1] * b->N[2] / (size_t)b->N[a];
+}
+
+static void face_copy(orc_block *b, int a, int l, double *buf, int pack)
+{
+    /* buf: the four fields' layer l, face order (slow, fast) of the other two
+     * axes, field by field */
+    const size_t fc = face_cells(b, a);
+    for (int q = 0; q < 4; q++) {
+        double *f = b->f[q];
+        double *o = buf + fc * (size_t)q;
+        size_t t = 0;
+        if (a == 0) {
+            for (int i = 0; i < b->N[2]; i++)
+                for (int j = 0; j < b->N[1]; j++, t++) {
+                    if (pack) o[t] = f[bix(b, i, j, l)]; else f[bix(b, i, j, l)] = o[t];
+                }
+        } else if (a == 1) {
+            for (int i = 0; i < b->N[2]; i++)
+                for (int k = 0; k < b->N[0]; k++, t++) {
+                    if (pack) o[t] = f[bix(b, i, l, k)]; else f[bix(b, i, l, k)] = o[t];
+                }
+        } else {
+            for (int j = 0; j < b->N[1]; j++)
+                for (int k = 0; k < b->N[0]; k++, t++) {
+                    if (pack) o[t] = f[bix(b, l, j, k)]; else f[bix(b, l, j, k)] = o[t];
+                }
         }
     }
+}
+
+int orc_sweeps_ref_blocks(int solver, const uint8_t *G, const int32_t *code, int nx, int ny, int nz, int bc,
+                          double *p, double *u, double *v, double *w, double *outlet, int nsweeps,
+                          const int dims_in[3])
+{
+    const int NXg = HX(nx), NYg = HY(ny);
+    const int size[3] = {nx, ny, nz};
+    int per[3];
+    periodic_axes(bc, &per[0], &per[1], &per[2]);
+    int dims[3];
+    for (int a = 0; a < 3; a++) {
+        dims[a] = dims_in[a] < 1 ? 1 : dims_in[a];
+        if (dims[a] > size[a]) dims[a] = size[a];
+    }
+    const int nblk = dims[0] * dims[1] * dims[2];
+    orc_block *B = calloc((size_t)nblk, sizeof(orc_block));
+    if (!B) return -1;
+    int fail = 0;
+    for (int cx = 0; cx < dims[0]; cx++)
+        for (int cy = 0; cy < dims[1]; cy++)
+            for (int cz = 0; cz < dims[2]; cz++) {
+                orc_block *b = B + ((size_t)cx * dims[1] + cy) * dims[2] + cz;
+                const int c[3] = {cx, cy, cz};
+                for (int a = 0; a < 3; a++) {
+                    const int base = size[a] / dims[a], extra = size[a] - dims[a] * base;
+                    b->lo[a] = c[a] * base + (c[a] < extra ? c[a] : extra);
+                    b->n[a] = base + (c[a] < extra ? 1 : 0);
+                    b->N[a] = b->n[a] + 4;
+                    int cl = c[a] - 1, cr = c[a] + 1;
+                    if (cl < 0) cl = per[a] ? dims[a] - 1 : -1;
+                    if (cr >= dims[a]) cr = per[a] ? 0 : -1;
+                    int cc[3] = {cx, cy, cz};
+                    cc[a] = cl;
+                    b->nb[2 * a] = cl < 0 ? -1 : (cc[0] * dims[1] + cc[1]) * dims[2] + cc[2];
+                    cc[a] = cr;
+                    b->nb[2 * a + 1] = cr < 0 ? -1 : (cc[0] * dims[1] + cc[1]) * dims[2] + cc[2];
+                }
+                const size_t cells = (size_t)b->N[0] * b->N[1] * b->N[2];
+                size_t fmax = 0;
+                for (int a = 0; a < 3; a++)
+                    if (face_cells(b, a) > fmax) fmax = face_cells(b, a);
+                b->G = malloc(cells);
+                b->code = malloc(cells * sizeof(int32_t));
+                for (int q = 0; q < 4; q++) b->f[q] = malloc(cells * sizeof(double));
+                for (int h = 0; h < 2; h++) b->snd[h] = malloc(fmax * 4 * 2 * sizeof(double));
+                if (!b->G || !b->code || !b->f[0] || !b->f[1] || !b->f[2] || !b->f[3] || !b->snd[0] || !b->snd[1])
+                    fail = 1;
+            }
     if (!fail) {
         double *glob[4] = {p, u, v, w};
-#pragma omp parallel num_threads(P)
+#pragma omp parallel num_threads(nblk)
         {
 #ifdef _OPENMP
             const int tid = omp_get_thread_num(), nth = omp_get_num_threads();
 #else
             const int tid = 0, nth = 1;
 #endif
-            /* every phase: the slabs tid, tid + nth, ... (one each when the
-             * runtime grants P threads) */
-            for (int k = tid; k < P; k += nth)
-                for (int q = 0; q < 4; q++)
-                    memcpy(S[4 * k + q], glob[q] + plane * (size_t)z0[k],
-                           sizeof(double) * plane * (size_t)(nk[k] + 4));
+            for (int bi = tid; bi < nblk; bi += nth) {
+                orc_block *b = B + bi;
+                for (int i = 0; i < b->N[2]; i++)
+                    for (int j = 0; j < b->N[1]; j++)
+                        for (int k = 0; k < b->N[0]; k++) {
+                            const int gi = b->lo[2] + i, gj = b->lo[1] + j, gk = b->lo[0] + k;
+                            const size_t g = ix(NXg, NYg, gi, gj, gk), l = bix(b, i, j, k);
+                            b->G[l] = G[g];
+                            b->code[l] = code[g];
+                            for (int q = 0; q < 4; q++) b->f[q][l] = glob[q][g];
+                            const int inside = i >= 2 && i < b->N[2] - 2 && j >= 2 && j < b->N[1] - 2 && k >= 2 &&
+                                               k < b->N[0] - 2;
+                            if (outlet && gi == nz + 1 && !inside && gj >= 2 && gj < ny + 2 && gk >= 2 &&
+                                gk < nx + 2)
+                                b->f[0][l] = outlet[(size_t)gj * NXg + gk];
+                        }
+            }
 #pragma omp barrier
             for (int it = 0; it < nsweeps; it++) {
-                for (int k = tid; k < P; k += nth) {
-                    double **f = S + 4 * k;
-                    orc_sweep_ref(solver, G + plane * (size_t)z0[k], code + plane * (size_t)z0[k], nx, ny, nk[k],
-                                  f[0], f[1], f[2], f[3]);
-                    for (int q = 0; q < 4; q++) exchange_f64(f[q], nx, ny, nk[k], px, py, 0);
+                for (int bi = tid; bi < nblk; bi += nth) {
+                    orc_block *b = B + bi;
+                    orc_sweep_ref(solver, b->G, b->code, b->n[0], b->n[1], b->n[2], b->f[0], b->f[1], b->f[2],
+                                  b->f[3]);
                 }
+                for (int a = 0; a < 3; a++) {
 #pragma omp barrier
-                /* z halos: planes 0, 1 from the slab below (its planes n, n+1),
-                 * planes n+2, n+3 from the slab above (its planes 2, 3) */
-                for (int k = tid; k < P; k += nth) {
-                    double **f = S + 4 * k;
-                    const int below = k > 0 ? k - 1 : (pz ? P - 1 : -1);
-                    const int above = k < P - 1 ? k + 1 : (pz ? 0 : -1);
-                    for (int q = 0; q < 4; q++) {
-                        if (below >= 0)
-                            memcpy(f[q], S[4 * below + q] + plane * (size_t)nk[below], sizeof(double) * plane * 2);
-                        if (above >= 0)
-                            memcpy(f[q] + plane * (size_t)(nk[k] + 2), S[4 * above + q] + plane * 2,
-                                   sizeof(double) * plane * 2);
+                    for (int bi = tid; bi < nblk; bi += nth) {
+                        orc_block *b = B + bi;
+                        const size_t fc4 = face_cells(b, a) * 4;
+                        /* low edge layers 2, 3; high edge layers N-4, N-3 */
+                        face_copy(b, a, 2, b->snd[0], 1);
+                        face_copy(b, a, 3, b->snd[0] + fc4, 1);
+                        face_copy(b, a, b->N[a] - 4, b->snd[1], 1);
+                        face_copy(b, a, b->N[a] - 3, b->snd[1] + fc4, 1);
+                    }
+#pragma omp barrier
+                    for (int bi = tid; bi < nblk; bi += nth) {
+                        orc_block *b = B + bi;
+                        const size_t fc4 = face_cells(b, a) * 4;
+                        const int r = b->nb[2 * a + 1], l = b->nb[2 * a];
+                        /* from the neighbour above its low layers, from the one
+                         * below its high layers */
+                        if (r >= 0) {
+                            face_copy(b, a, b->N[a] - 2, B[r].snd[0], 0);
+                            face_copy(b, a, b->N[a] - 1, B[r].snd[0] + fc4, 0);
+                        }
+                        if (l >= 0) {
+                            face_copy(b, a, 0, B[l].snd[1], 0);
+                            face_copy(b, a, 1, B[l].snd[1] + fc4, 0);
+                        }
                     }
                 }
 #pragma omp barrier
-                /* pins: global plane 1 (slab 0's plane 1), planes nz+1 and
-                 * nz+2 (the last slab's planes n+1 and n+2) */
-                for (int k = tid; k < P; k += nth) {
-                    const uint8_t *Gk = G + plane * (size_t)z0[k];
-                    double *pk = S[4 * k];
-                    for (int j = 2; j < NY - 2; j++)
-                        for (int x = 2; x < NX - 2; x++) {
-                            const size_t r = (size_t)j * NX + x;
-                            if (k == 0 && Gk[plane + r] == 0) pk[plane + r] = (double)(nz + 2);
-                            if (k == P - 1) {
-                                const size_t hi = plane * (size_t)(nk[k] + 2) + r;
-                                const size_t out = plane * (size_t)(nk[k] + 1) + r;
-                                if (Gk[hi] == 0) pk[hi] = 2.0;
-                                if (Gk[out] == 0) pk[out] = 3.0;
+                for (int bi = tid; bi < nblk; bi += nth) {
+                    orc_block *b = B + bi;
+                    const int cz = (bi % dims[2]);
+                    double *pb = b->f[0];
+                    for (int j = 2; j < b->N[1] - 2; j++)
+                        for (int k = 2; k < b->N[0] - 2; k++) {
+                            if (cz == 0 && b->G[bix(b, 1, j, k)] == 0) pb[bix(b, 1, j, k)] = (double)(nz + 2);
+                            if (cz == dims[2] - 1) {
+                                const size_t hi = bix(b, b->N[2] - 2, j, k), out = bix(b, b->N[2] - 3, j, k);
+                                if (outlet) outlet[(size_t)(b->lo[1] + j) * NXg + (b->lo[0] + k)] = pb[out];
+                                if (b->G[hi] == 0) pb[hi] = 2.0;
+                                if (b->G[out] == 0) pb[out] = 3.0;
                             }
                         }
                 }
-#pragma omp barrier
             }
-            /* back: every slab's planes (x / y halos included), and the
-             * outer z-halo planes of the first and the last slab */
-            for (int k = tid; k < P; k += nth)
-                for (int q = 0; q < 4; q++) {
-                    memcpy(glob[q] + plane * (size_t)(z0[k] + 2), S[4 * k + q] + plane * 2,
-                           sizeof(double) * plane * (size_t)nk[k]);
-                    if (k == 0) memcpy(glob[q], S[q], sizeof(double) * plane * 2);
-                    if (k == P - 1)
-                        memcpy(glob[q] + plane * (size_t)(nz + 2), S[4 * k + q] + plane * (size_t)(nk[k] + 2),
-                               sizeof(double) * plane * 2);
-                }
+#pragma omp barrier
+            /* back: interiors, and the global halo cells of the boundary blocks */
+            for (int bi = tid; bi < nblk; bi += nth) {
+                orc_block *b = B + bi;
+                for (int i = 0; i < b->N[2]; i++)
+                    for (int j = 0; j < b->N[1]; j++)
+                        for (int k = 0; k < b->N[0]; k++) {
+                            const int gi = b->lo[2] + i, gj = b->lo[1] + j, gk = b->lo[0] + k;
+                            /* on every axis: inside the block, or in the
+                             * global halo (the block is on that face) */
+                            const int kz = (i >= 2 && i < b->N[2] - 2) || gi < 2 || gi >= nz + 2;
+                            const int ky = (j >= 2 && j < b->N[1] - 2) || gj < 2 || gj >= ny + 2;
+                            const int kx = (k >= 2 && k < b->N[0] - 2) || gk < 2 || gk >= nx + 2;
+                            const int keep = kz && ky && kx;
+                            if (!keep) continue;
+                            const size_t g = ix(NXg, NYg, gi, gj, gk), l = bix(b, i, j, k);
+                            for (int q = 0; q < 4; q++) glob[q][g] = b->f[q][l];
+                        }
+            }
         }
     }
-    for (int q = 0; q < 4 * P; q++) free(S[q]);
-    free(S); free(z0); free(nk);
+    for (int bi = 0; bi < nblk; bi++) {
+        free(B[bi].G); free(B[bi].code);
+        for (int q = 0; q < 4; q++) free(B[bi].f[q]);
+        free(B[bi].snd[0]); free(B[bi].snd[1]);
+    }
+    free(B);
     return fail ? -1 : 0;
 }
 
+/* dom_decomposition 1 1 P: P z-slabs (orc_sweeps_ref_blocks).  Used to time
+ * the CPU baseline on all host cores; P = 1 is orc_sweep_ref + exchange +
+ * pins exactly. */
+int orc_sweeps_ref_slabs(int solver, const uint8_t *G, const int32_t *code, int nx, int ny, int nz, int bc,
+                         double *p, double *u, double *v, double *w, int nsweeps, int P)
+{
+    const int dims[3] = {1, 1, P};
+    return orc_sweeps_ref_blocks(solver, G, code, nx, ny, nz, bc, p, u, v, w, NULL, nsweeps, dims);
+}
+
+/* src/calc_flow.cc:82-165 at one rank. */
 int orc_calc_flow(const uint8_t *G, const int32_t *code, int nx, int ny, int nz,
                   int bc, const orc_params *prm,
                   double *p, double *u, double *v, double *w,

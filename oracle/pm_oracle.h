@@ -33,10 +33,15 @@ void orc_pins(const uint8_t *G, int nx, int ny, int nz, double *p);
 void orc_sweep_ref(int solver, const uint8_t *G, const int32_t *code,
                    int nx, int ny, int nz,
                    double *p, double *u, double *v, double *w);
-/* nsweeps reference sweeps (+ exchange + pins) with the interior cut into P
- * z-slabs swept by one thread each, as `mpirun -np P` with
- * dom_decomposition 1 1 P runs them; P = 1: the serial sweep.  0 or -1
- * (allocation failure). */
+/* nsweeps reference sweeps (+ halo exchange + pins) as `mpirun -np P*Q*R`
+ * runs them with dom_decomposition P Q R (dims = {P, Q, R}): blocks swept
+ * against frozen halo copies, one OpenMP thread per block.  outlet: the
+ * outlet plane (z = nz+1, (ny+4)*(nx+4)) before its last pins, carried
+ * between calls (NULL: the plane of p).  0 or -1 (allocation failure). */
+int orc_sweeps_ref_blocks(int solver, const uint8_t *G, const int32_t *code, int nx, int ny, int nz, int bc,
+                          double *p, double *u, double *v, double *w, double *outlet, int nsweeps,
+                          const int dims[3]);
+/* The same with dom_decomposition 1 1 P (z-slabs); P = 1: the serial sweep. */
 int orc_sweeps_ref_slabs(int solver, const uint8_t *G, const int32_t *code, int nx, int ny, int nz, int bc,
                          double *p, double *u, double *v, double *w, int nsweeps, int P);
 double orc_convergence(const uint8_t *G, const double *w, int nx, int ny, int nz,

@@ -47,7 +47,7 @@ class Params(C.Structure):
         ("solving_algorithm", C.c_int), ("eps", C.c_double), ("porosity", C.c_double),
         ("dom_interest", C.c_int * 6),
         ("mode", C.c_int), ("precision", C.c_int), ("omega", C.c_double), ("device", C.c_int),
-        ("omega_p", C.c_double), ("z_slabs", C.c_int),
+        ("omega_p", C.c_double), ("z_slabs", C.c_int), ("dom_decomposition", C.c_int * 3),
     ]
 
 
@@ -111,6 +111,7 @@ def lib():
         L.pm_write_raw_f64.argtypes = [C.c_char_p, C.c_void_p, C.c_int64, C.c_double, C.c_int]
         L.pm_write_raw_i32.argtypes = [C.c_char_p, C.c_void_p, C.c_int64]
         L.pm_write_log_row.argtypes = [C.c_char_p, P(LogRow), C.c_int]
+        L.pm_rank_map.argtypes = [C.c_int, C.c_int, C.c_int, P(C.c_int), C.c_void_p]
         _lib = L
     return _lib
 
@@ -125,7 +126,7 @@ def _check(rc, io=False):
 def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_000_000,
                 it_eval=100, it_write=100, solving_algorithm=2, eps=1e-6, porosity=-1.0,
                 dom_interest=(0, 0, 0, 0, 0, 0), mode="reference", precision="f64", omega=1.0,
-                device=0, omega_p=0.0, z_slabs=0) -> Params:
+                device=0, omega_p=0.0, z_slabs=0, dom_decomposition=(0, 0, 0)) -> Params:
     p = Params()
     lib().pm_params_default(C.byref(p))
     p.nx, p.ny, p.nz = nx, ny, nz
@@ -140,8 +141,18 @@ def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_0
     p.omega = omega
     p.omega_p = omega_p
     p.z_slabs = z_slabs
+    p.dom_decomposition = (C.c_int * 3)(*dom_decomposition)
     p.device = device
     return p
+
+
+def rank_map(shape_zyx, dims) -> np.ndarray:
+    """The reference's domain_decomp_ contents: the rank of every voxel under
+    dom_decomposition dims (x, y, z), shape (nz, ny, nx) int32."""
+    nz, ny, nx = shape_zyx
+    out = np.empty((nz, ny, nx), np.int32)
+    _check(lib().pm_rank_map(nx, ny, nz, (C.c_int * 3)(*dims), out.ctypes.data), io=True)
+    return out
 
 
 def device_bytes(params: Params) -> int:

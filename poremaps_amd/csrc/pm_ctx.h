@@ -85,9 +85,14 @@ struct pm_ctx {
     // pm_params.omega was 0: the relaxation pair is chosen at each geometry
     // load from the porosity
     bool auto_relax = false;
-    // reference mode with pm_params.z_slabs = P > 1: the slabs' working copies
-    // (4 per slab: p, u, v, w)
-    std::vector<double *> slab;
+    // reference mode with a domain decomposition: the blocks (device table),
+    // the most tiles of any block per axis, the state as each sweep starts
+    // and the outlet plane as the last sweep left it before its pins
+    pm::RefBox *ref_tab = nullptr;
+    int ref_nblk = 0;
+    int ref_ntile[3] = {0, 0, 0};
+    double *ref_snap[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *ref_outlet = nullptr;
     // the FP32 correction state is zero everywhere (after a fold or at the
     // start): the next first pass needs only the records and the residual
     bool dzero = false;

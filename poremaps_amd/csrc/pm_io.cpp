@@ -187,3 +187,34 @@ extern "C" int pm_read_raw_u8(const char *path, uint8_t *out, int64_t n) {
         return io_fail(PM_E_GEOMETRY, "geometry %s holds %zu bytes, size_x_y_z needs %lld", path, got, (long long)n);
     return PM_OK;
 }
+
+// src/output.cc:156-190 (write_domain_decomposition) with the blocks of
+// src/geometry.cc:129-171 (get_dom_limits: along each axis the first
+// size % dims blocks hold one cell more) and the rank of each block in the
+// Cartesian communicator (src/main_stokes_solver.cc:182-193; MPICH numbers
+// the blocks with z fastest, then y, then x).
+extern "C" int pm_rank_map(int nx, int ny, int nz, const int dims_in[3], int32_t *out) {
+    if (!dims_in || !out || nx < 1 || ny < 1 || nz < 1) return io_fail(PM_E_ARG, "pm_rank_map: bad argument");
+    const int size[3] = {nx, ny, nz};
+    int dims[3];
+    for (int a = 0; a < 3; a++) {
+        dims[a] = dims_in[a] < 1 ? 1 : dims_in[a];
+        if (dims[a] > size[a])
+            return io_fail(PM_E_ARG, "pm_rank_map: %d blocks along an axis of %d cells", dims[a], size[a]);
+    }
+    // block coordinate of every cell along each axis
+    std::vector<int> coord[3];
+    for (int a = 0; a < 3; a++) {
+        const int base = size[a] / dims[a], extra = size[a] - dims[a] * base;
+        coord[a].resize((size_t)size[a]);
+        for (int c = 0, x = 0; c < dims[a]; c++)
+            for (int m = 0; m < base + (c < extra ? 1 : 0); m++) coord[a][(size_t)x++] = c;
+    }
+    for (int i = 0; i < nz; i++)
+        for (int j = 0; j < ny; j++) {
+            int32_t *row = out + ((int64_t)i * ny + j) * nx;
+            const int yz = coord[1][(size_t)j] * dims[2] + coord[2][(size_t)i];
+            for (int k = 0; k < nx; k++) row[k] = (int32_t)(coord[0][(size_t)k] * dims[1] * dims[2] + yz);
+        }
+    return PM_OK;
+}

@@ -3,6 +3,8 @@
 
 #include "pm_grid.h"
 
+#include <vector>
+
 namespace pm {
 
 // dom_interest box in 0-based interior coordinates (x min/max, y, z).
@@ -35,14 +37,28 @@ void launch_pins(const Grid &g, const uint8_t *G, T *p, T pin_in, T pin_hi, T pi
 // exchange, pins), as a wavefront of LDS tiles (REF_TILES), in one workgroup
 // (REF_ONE_WG) or one launch per hyperplane (REF_PLANES); the same bits.
 enum { REF_TILES = 0, REF_ONE_WG = 1, REF_PLANES = 2 };
-// One exact-order sweep the way `mpirun -np P` runs it with dom_decomposition
-// 1 1 P: P z-slabs, each swept (tiled wavefront) against frozen copies of its
-// neighbours' two edge planes, then the exchange and the pins.  f: p, u, v, w;
-// slab: 4 * P working arrays, slab k's of Grid::base + Grid::sy * (nk + 4)
-// elements (nk = its planes); 2 <= P <= kMaxSlabs.
-constexpr int kMaxSlabs = 256;
-void launch_ref_slab_sweep(const Grid &g, const uint8_t *G, const int32_t *code, double *const f[4],
-                           double *const *slab, int P, int solver, hipStream_t s);
+// The four FP64 fields p, u, v, w (kernel argument).
+struct RefFields {
+    double *f[4];
+};
+// A block of a domain decomposition in interior coordinates: first cell and
+// extent along x, y, z.
+struct RefBox {
+    int lo[3], n[3];
+};
+// The blocks of dom_decomposition dims (x, y, z) in the reference's rank
+// order (src/geometry.cc:129-171; z fastest).
+void ref_blocks(const Grid &g, const int dims[3], std::vector<RefBox> *out);
+// One exact-order sweep the way `mpirun -np P*Q*R` runs it with
+// dom_decomposition P Q R: every block swept (tiled wavefront) against the
+// state as the sweep started (snap, 4 arrays of Grid::total, overwritten),
+// whose outlet plane is taken from `outlet` (the plane z = nz+1 as the last
+// sweep left it before its pins, Grid::qtotal elements, updated); then the
+// exchange and the pins.  tab: nblk blocks in device memory; ntile: the most
+// tiles of any block along x, y, z.
+void launch_ref_block_sweep(const Grid &g, const uint8_t *G, const int32_t *code, double *const f[4],
+                            double *const snap[4], double *outlet, const RefBox *tab, int nblk, const int ntile[3],
+                            int solver, hipStream_t s);
 void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
                        double *v, double *w, int solver, int nsweeps, int impl, hipStream_t s);
 

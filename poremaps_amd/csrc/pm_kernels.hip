@@ -273,31 +273,34 @@ constexpr int kRN = kRB * kRB * kRB;     // 1728 cells, 63.9 KB of LDS
 
 constexpr int kRThreads = 4 * kRT * kRT;  // 4 waves copy, the first one computes
 
-// Slab table of the batched z-slab sweep (kernel argument): slab k's first
-// interior plane, its planes and its working copies p, u, v, w.
-constexpr int kSlabsBatched = 16;
-struct RefSlabs {
-    int z0[kSlabsBatched], nk[kSlabsBatched];
-    double *f[4 * kSlabsBatched];
-};
-
-// Tile (A, B, C) of the grid g (whole workgroup; the caller checked that the
-// tile exists).
+// Tile (A, B, C) of the box lo + [0, n) (interior coordinates, x, y, z) of
+// the grid g (whole workgroup; the caller checked that the tile exists).
+// Rim cells inside the box come from the live fields f, those outside from
+// fo: for one domain (the box is the interior, fo = f) they are the halo,
+// frozen during a sweep; for a block of a decomposition they are the other
+// blocks' cells and the halo as they were when the sweep started (the
+// block's own halo copies in the reference's multi-rank run).
 __device__ __forceinline__ void ref_tile(const Grid &g, const uint8_t *__restrict__ G,
-                                         const int32_t *__restrict__ code, double *p, double *u, double *v,
-                                         double *w, int solver, int A, int B, int C) {
+                                         const int32_t *__restrict__ code, const RefFields &f, const RefFields &fo,
+                                         const RefBox &box, int solver, int A, int B, int C) {
     __shared__ double Lp[kRN], Lu[kRN], Lv[kRN], Lw[kRN];
     __shared__ int32_t Lc[kRN];
     __shared__ uint8_t LG[kRN];
     // tile origin in interior coordinates = origin of its rim box in padded ones
-    const int i0 = A * kRT, j0 = B * kRT, k0 = C * kRT;
-    const int tz = min(kRT, g.nz - i0), ty = min(kRT, g.ny - j0), tx = min(kRT, g.nx - k0);
+    const int i0 = box.lo[2] + A * kRT, j0 = box.lo[1] + B * kRT, k0 = box.lo[0] + C * kRT;
+    const int tz = min(kRT, box.n[2] - A * kRT), ty = min(kRT, box.n[1] - B * kRT),
+              tx = min(kRT, box.n[0] - C * kRT);
     const int by = ty + 4, bx = tx + 4, nbox = (tz + 4) * by * bx;
     for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
         const int a = t / (by * bx), r = t - a * by * bx, b = r / bx, x = r - b * bx;
         const int64_t gc = g.at(i0 + a, j0 + b, k0 + x);
         const int l = (a * kRB + b) * kRB + x;
-        Lp[l] = p[gc]; Lu[l] = u[gc]; Lv[l] = v[gc]; Lw[l] = w[gc];
+        // interior coordinates of the cell: i0 + a - 2, ...
+        const bool in = (unsigned)(i0 + a - 2 - box.lo[2]) < (unsigned)box.n[2] &&
+                        (unsigned)(j0 + b - 2 - box.lo[1]) < (unsigned)box.n[1] &&
+                        (unsigned)(k0 + x - 2 - box.lo[0]) < (unsigned)box.n[0];
+        const RefFields &src = in ? f : fo;
+        Lp[l] = src.f[0][gc]; Lu[l] = src.f[1][gc]; Lv[l] = src.f[2][gc]; Lw[l] = src.f[3][gc];
         Lc[l] = code[gc]; LG[l] = G[gc];
     }
     __syncthreads();
@@ -336,37 +339,35 @@ __device__ __forceinline__ void ref_tile(const Grid &g, const uint8_t *__restric
         const int a2 = t / (ty * tx), r = t - a2 * ty * tx, b2 = r / tx, x2 = r - b2 * tx;
         const int64_t gc = g.at(i0 + 2 + a2, j0 + 2 + b2, k0 + 2 + x2);
         const int l = ((a2 + 2) * kRB + (b2 + 2)) * kRB + (x2 + 2);
-        p[gc] = Lp[l]; u[gc] = Lu[l]; v[gc] = Lv[l]; w[gc] = Lw[l];
+        f.f[0][gc] = Lp[l]; f.f[1][gc] = Lu[l]; f.f[2][gc] = Lv[l]; f.f[3][gc] = Lw[l];
     }
 }
 
 __global__ __launch_bounds__(kRThreads) void k_ref_tile(Grid g, const uint8_t *__restrict__ G,
-                                                       const int32_t *__restrict__ code, double *p, double *u,
-                                                       double *v, double *w, int solver, int D, int a0, int nty,
-                                                       int ntx) {
+                                                       const int32_t *__restrict__ code, RefFields f, int solver,
+                                                       int D, int a0, int nty, int ntx) {
     const int A = a0 + (int)blockIdx.x / nty, B = (int)blockIdx.x % nty, C = D - A - B;
     if (C < 0 || C >= ntx) return;  // the same for the whole workgroup
-    ref_tile(g, G, code, p, u, v, w, solver, A, B, C);
+    const RefBox box = {{0, 0, 0}, {g.nx, g.ny, g.nz}};
+    ref_tile(g, G, code, f, f, box, solver, A, B, C);
 }
 
-// The same for the z-slabs of one tile-diagonal D at once (the slabs are
-// independent within a sweep): block -> (slab, A, B), the slab's grid and
-// working copy from the table.
-__global__ __launch_bounds__(kRThreads) void k_ref_tile_slabs(Grid g, const uint8_t *__restrict__ G,
-                                                             const int32_t *__restrict__ code, RefSlabs tab,
-                                                             int solver, int D, int ntz_max, int nty, int ntx) {
-    const int per = ntz_max * nty;
-    const int k = (int)blockIdx.x / per, A = ((int)blockIdx.x / nty) % ntz_max, B = (int)blockIdx.x % nty;
+// The tiles of tile-diagonal D of every block of a decomposition at once (the
+// blocks are independent within a sweep, each swept against the state fo
+// taken when the sweep started): workgroup -> (block, A, B), C = D - A - B,
+// tile counts per axis up to ntz, nty of the largest block.
+__global__ __launch_bounds__(kRThreads) void k_ref_tile_blocks(Grid g, const uint8_t *__restrict__ G,
+                                                              const int32_t *__restrict__ code, RefFields f,
+                                                              RefFields fo, const RefBox *__restrict__ tab,
+                                                              int solver, int D, int ntz, int nty) {
+    const int per = ntz * nty;
+    const int k = (int)blockIdx.x / per, A = ((int)blockIdx.x / nty) % ntz, B = (int)blockIdx.x % nty;
+    const RefBox box = tab[k];
     const int C = D - A - B;
-    const int nk = tab.nk[k];
-    if (A >= (nk + kRT - 1) / kRT || C < 0 || C >= ntx) return;  // the same for the whole workgroup
-    Grid gk = g;
-    gk.nz = nk;
-    gk.NZ = nk + 4;
-    gk.total = gk.base + gk.sy * gk.NZ;
-    const int64_t off = (int64_t)tab.z0[k] * g.sy;
-    ref_tile(gk, G + off, code + off, tab.f[4 * k], tab.f[4 * k + 1], tab.f[4 * k + 2], tab.f[4 * k + 3], solver,
-             A, B, C);
+    if (A >= (box.n[2] + kRT - 1) / kRT || B >= (box.n[1] + kRT - 1) / kRT || C < 0 ||
+        C >= (box.n[0] + kRT - 1) / kRT)
+        return;  // the same for the whole workgroup
+    ref_tile(g, G, code, f, fo, box, solver, A, B, C);
 }
 
 // Whole sweeps in one workgroup (small domains): hyperplanes, then the
@@ -572,7 +573,7 @@ void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, dou
             for (int D = 0; D <= (ntz - 1) + (nty - 1) + (ntx - 1); D++) {
                 const int a_lo = std::max(0, D - (nty - 1) - (ntx - 1)), a_hi = std::min(ntz - 1, D);
                 hipLaunchKernelGGL(k_ref_tile, dim3((unsigned)((a_hi - a_lo + 1) * nty)), dim3(kRThreads), 0, s,
-                                   g, G, code, p, u, v, w, solver, D, a_lo, nty, ntx);
+                                   g, G, code, RefFields{{p, u, v, w}}, solver, D, a_lo, nty, ntx);
             }
             launch_exchange_full<double>(g, p, u, v, w, s);
             launch_pins<double>(g, G, p, (double)(g.nz + 2), 2.0, 3.0, s);
@@ -593,65 +594,43 @@ void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, dou
     }
 }
 
-void launch_ref_slab_sweep(const Grid &g, const uint8_t *G, const int32_t *code, double *const f[4],
-                           double *const *slab, int P, int solver, hipStream_t s) {
-    // slab k: interior planes [z0, z0 + nk), the first nz % P slabs one plane
-    // more; its working copy holds planes z0 .. z0 + nk + 3 of the halo-2
-    // array (frozen neighbour planes on both sides), at the same pitches, so
-    // its geometry and codes are the global arrays offset by z0 planes
-    int z0[kMaxSlabs], nk[kMaxSlabs];
-    for (int k = 0, z = 0; k < P; k++) {
-        nk[k] = g.nz / P + (k < g.nz % P ? 1 : 0);
-        z0[k] = z;
-        z += nk[k];
-    }
-    auto slab_grid = [&](int k) {
-        Grid gk = g;
-        gk.nz = nk[k];
-        gk.NZ = nk[k] + 4;
-        gk.total = gk.base + gk.sy * gk.NZ;
-        return gk;
-    };
-    for (int k = 0; k < P; k++) {
-        const Grid gk = slab_grid(k);
-        for (int q = 0; q < 4; q++)
-            (void)hipMemcpyAsync(slab[4 * k + q], f[q] + (int64_t)z0[k] * g.sy, sizeof(double) * (size_t)gk.total,
-                                 hipMemcpyDeviceToDevice, s);
-    }
-    if (P <= kSlabsBatched) {
-        // every slab's tiles of one diagonal in one launch
-        RefSlabs tab;
-        int ntz_max = 0;
-        for (int k = 0; k < P; k++) {
-            tab.z0[k] = z0[k];
-            tab.nk[k] = nk[k];
-            for (int q = 0; q < 4; q++) tab.f[4 * k + q] = slab[4 * k + q];
-            ntz_max = std::max(ntz_max, (nk[k] + kRT - 1) / kRT);
-        }
-        const int nty = (g.ny + kRT - 1) / kRT, ntx = (g.nx + kRT - 1) / kRT;
-        for (int D = 0; D <= (ntz_max - 1) + (nty - 1) + (ntx - 1); D++)
-            hipLaunchKernelGGL(k_ref_tile_slabs, dim3((unsigned)(P * ntz_max * nty)), dim3(kRThreads), 0, s, g, G,
-                               code, tab, solver, D, ntz_max, nty, ntx);
-    } else {
-        for (int k = 0; k < P; k++) {
-            const Grid gk = slab_grid(k);
-            const int64_t off = (int64_t)z0[k] * g.sy;
-            const int ntz = (gk.nz + kRT - 1) / kRT, nty = (gk.ny + kRT - 1) / kRT, ntx = (gk.nx + kRT - 1) / kRT;
-            for (int D = 0; D <= (ntz - 1) + (nty - 1) + (ntx - 1); D++) {
-                const int a_lo = std::max(0, D - (nty - 1) - (ntx - 1)), a_hi = std::min(ntz - 1, D);
-                hipLaunchKernelGGL(k_ref_tile, dim3((unsigned)((a_hi - a_lo + 1) * nty)), dim3(kRThreads), 0, s,
-                                   gk, G + off, code + off, slab[4 * k], slab[4 * k + 1], slab[4 * k + 2],
-                                   slab[4 * k + 3], solver, D, a_lo, nty, ntx);
+void ref_blocks(const Grid &g, const int dims_in[3], std::vector<RefBox> *out) {
+    const int size[3] = {g.nx, g.ny, g.nz};
+    int dims[3];
+    for (int a = 0; a < 3; a++) dims[a] = std::max(1, std::min(dims_in[a], size[a]));
+    out->clear();
+    // src/geometry.cc:129-171: along each axis the first size % dims blocks
+    // hold one cell more; blocks in rank order (z fastest)
+    for (int cx = 0; cx < dims[0]; cx++)
+        for (int cy = 0; cy < dims[1]; cy++)
+            for (int cz = 0; cz < dims[2]; cz++) {
+                const int c[3] = {cx, cy, cz};
+                RefBox b;
+                for (int a = 0; a < 3; a++) {
+                    const int base = size[a] / dims[a], extra = size[a] - dims[a] * base;
+                    b.lo[a] = c[a] * base + std::min(c[a], extra);
+                    b.n[a] = base + (c[a] < extra ? 1 : 0);
+                }
+                out->push_back(b);
             }
-        }
-    }
-    for (int k = 0; k < P; k++) {
-        const int64_t from = g.base + 2 * g.sy, n = (int64_t)nk[k] * g.sy;
-        for (int q = 0; q < 4; q++)
-            (void)hipMemcpyAsync(f[q] + (int64_t)z0[k] * g.sy + from, slab[4 * k + q] + from, sizeof(double) * (size_t)n,
-                                 hipMemcpyDeviceToDevice, s);
-    }
+}
+
+void launch_ref_block_sweep(const Grid &g, const uint8_t *G, const int32_t *code, double *const f[4],
+                            double *const snap[4], double *outlet, const RefBox *tab, int nblk, const int ntile[3],
+                            int solver, hipStream_t s) {
+    // the state as the sweep starts, with the outlet plane as it was swept
+    // (before its pins): what the blocks' halo copies hold in the reference
+    for (int q = 0; q < 4; q++)
+        (void)hipMemcpyAsync(snap[q], f[q], sizeof(double) * (size_t)g.total, hipMemcpyDeviceToDevice, s);
+    const int64_t plane = g.base + (int64_t)(g.nz + 1) * g.sy;
+    (void)hipMemcpyAsync(snap[0] + plane, outlet + g.base, sizeof(double) * (size_t)g.sy, hipMemcpyDeviceToDevice,
+                         s);
+    const RefFields live{{f[0], f[1], f[2], f[3]}}, old{{snap[0], snap[1], snap[2], snap[3]}};
+    for (int D = 0; D <= (ntile[0] - 1) + (ntile[1] - 1) + (ntile[2] - 1); D++)
+        hipLaunchKernelGGL(k_ref_tile_blocks, dim3((unsigned)(nblk * ntile[2] * ntile[1])), dim3(kRThreads), 0, s,
+                           g, G, code, live, old, tab, solver, D, ntile[2], ntile[1]);
     launch_exchange_full<double>(g, f[0], f[1], f[2], f[3], s);
+    (void)hipMemcpyAsync(outlet + g.base, f[0] + plane, sizeof(double) * (size_t)g.sy, hipMemcpyDeviceToDevice, s);
     launch_pins<double>(g, G, f[0], (double)(g.nz + 2), 2.0, 3.0, s);
 }
 

@@ -64,6 +64,18 @@ extern "C" void pm_params_default(pm_params *p) {
     p->omega_p = 0.0;
 }
 
+// The decomposition of the reference mode (x, y, z blocks; 1 1 1 = one
+// domain): dom_decomposition, or 1 1 z_slabs.
+static void ref_dims(const pm_params *p, int d[3]) {
+    for (int a = 0; a < 3; a++) d[a] = std::max(1, p->dom_decomposition[a]);
+    if (d[0] == 1 && d[1] == 1 && d[2] == 1 && p->z_slabs > 1) d[2] = p->z_slabs;
+}
+static bool ref_decomposed(const pm_params *p) {
+    int d[3];
+    ref_dims(p, d);
+    return p->mode == PM_MODE_REFERENCE && d[0] * d[1] * d[2] > 1;
+}
+
 // porosity below which the pair by porosity (pm_params.omega = 0) takes the
 // large pressure relaxation
 constexpr double kAutoTightPhi = 0.20;
@@ -90,9 +102,19 @@ static int check_params(const pm_params *p) {
         return fail(PM_E_ARG, "omega must be in (0, 100), or 0 for the pair by porosity (got %g)", p->omega);
     if (!(p->omega_p >= 0.0 && p->omega_p < 100.0))
         return fail(PM_E_ARG, "omega_p must be in [0, 100) (got %g)", p->omega_p);
-    if (p->z_slabs < 0 || p->z_slabs > pm::kMaxSlabs || (p->z_slabs > 1 && p->z_slabs > p->nz / 3))
-        return fail(PM_E_ARG, "z_slabs must be 0, 1 or 2 .. min(%d, nz / 3) (got %d with nz %d)", pm::kMaxSlabs,
-                    p->z_slabs, p->nz);
+    const int *dd = p->dom_decomposition;
+    if (p->z_slabs < 0 || p->z_slabs > p->nz)
+        return fail(PM_E_ARG, "z_slabs must be 0 .. nz (got %d with nz %d)", p->z_slabs, p->nz);
+    if (dd[0] < 0 || dd[1] < 0 || dd[2] < 0 || dd[0] > p->nx || dd[1] > p->ny || dd[2] > p->nz)
+        return fail(PM_E_ARG, "dom_decomposition %d %d %d must lie in 0 .. size_x_y_z (%d %d %d)", dd[0], dd[1],
+                    dd[2], p->nx, p->ny, p->nz);
+    if (p->z_slabs > 1 && (dd[0] > 1 || dd[1] > 1 || dd[2] > 1))
+        return fail(PM_E_ARG, "give either z_slabs or dom_decomposition");
+    int d[3];
+    ref_dims(p, d);
+    if ((int64_t)d[0] * d[1] * d[2] > PM_MAX_BLOCKS)
+        return fail(PM_E_ARG, "dom_decomposition %d %d %d has more than %d blocks", d[0], d[1], d[2],
+                    PM_MAX_BLOCKS);
     return PM_OK;
 }
 
@@ -113,9 +135,11 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
     // keeps separate colour lists
     if (fast) per += (jac || dc) ? 4 : 8;
-    // reference mode in z-slabs: the slabs' working copies of p, u, v, w
-    const int64_t slabs = (!fast && p->z_slabs > 1) ? 32 * (g.total + (int64_t)4 * p->z_slabs * g.sy) : 0;
-    return per * g.total + slabs + 4 * g.qtotal() * 8 + (int64_t)pm::reduce_scratch_bytes() + (1 << 20);
+    // reference mode in blocks: the state as a sweep starts, the outlet plane
+    // and the block table
+    const int64_t blocks = ref_decomposed(p) ? 32 * g.total + 8 * g.qtotal() + (int64_t)sizeof(pm::RefBox) *
+                                                   PM_MAX_BLOCKS : 0;
+    return per * g.total + blocks + 4 * g.qtotal() * 8 + (int64_t)pm::reduce_scratch_bytes() + (1 << 20);
 }
 
 static void free_all(pm_ctx *c) {
@@ -127,8 +151,9 @@ static void free_all(pm_ctx *c) {
     f(c->red_scratch); f(c->red_dev);
     f(c->lists.all);
     f(c->lists.rec);
-    for (double *x : c->slab) f(x);
-    c->slab.clear();
+    for (int q = 0; q < 4; q++) { f(c->ref_snap[q]); c->ref_snap[q] = nullptr; }
+    f(c->ref_outlet); f(c->ref_tab);
+    c->ref_outlet = nullptr; c->ref_tab = nullptr; c->ref_nblk = 0;
     if (!c->lists.col_in_all)
         for (int q = 0; q < 3; q++) f(c->lists.col[q]);
     c->lists = pm::FluidLists();
@@ -258,6 +283,10 @@ static int init_state(pm_ctx *c) {
     pm::launch_init_fields(g, c->G, c->X[0], c->X[1], c->X[2], c->X[3], c->stream);
     // src/calc_flow.cc:105-108: the pins are applied before sweep 0
     pm::launch_pins<double>(g, c->G, c->X[0], (double)(g.nz + 2), 2.0, 3.0, c->stream);
+    if (c->ref_outlet)
+        // the pinned initial outlet plane is also its value before the pins
+        HIPCHK(hipMemcpyAsync(c->ref_outlet + g.base, c->X[0] + g.base + (int64_t)(g.nz + 1) * g.sy,
+                              sizeof(double) * (size_t)g.sy, hipMemcpyDeviceToDevice, c->stream));
     if (c->fast()) {
         pm::launch_outlet_plane(g, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], 3.0, c->qX[0], c->stream);
         HIPCHK(hipMemcpyAsync(c->qX[1], c->qX[0], sizeof(double) * (size_t)g.qtotal(),
@@ -316,14 +345,18 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     RC(dalloc(&c->code, g.total));
     for (int q = 0; q < 4; q++) RC(dalloc(&c->X[q], g.total));
     for (int q = 0; q < 2; q++) RC(dalloc(&c->qX[q], g.qtotal()));
-    if (!c->fast() && c->prm.z_slabs > 1) {
-        // working copies of the z-slabs (their planes + 2 frozen planes each side)
-        const int P = c->prm.z_slabs;
-        c->slab.assign((size_t)4 * P, nullptr);
-        for (int k = 0; k < P; k++) {
-            const int nk = g.nz / P + (k < g.nz % P ? 1 : 0);
-            for (int q = 0; q < 4; q++) RC(dalloc(&c->slab[4 * k + q], g.base + g.sy * (nk + 4)));
-        }
+    if (ref_decomposed(&c->prm)) {
+        int d[3];
+        ref_dims(&c->prm, d);
+        std::vector<pm::RefBox> blocks;
+        pm::ref_blocks(g, d, &blocks);
+        c->ref_nblk = (int)blocks.size();
+        for (const pm::RefBox &b : blocks)
+            for (int a = 0; a < 3; a++) c->ref_ntile[a] = std::max(c->ref_ntile[a], (b.n[a] + 7) / 8);
+        for (int q = 0; q < 4; q++) RC(dalloc(&c->ref_snap[q], g.total));
+        RC(dalloc(&c->ref_outlet, g.qtotal()));
+        RC(dalloc(&c->ref_tab, c->ref_nblk));
+        HIPCHK(hipMemcpy(c->ref_tab, blocks.data(), sizeof(pm::RefBox) * blocks.size(), hipMemcpyHostToDevice));
     }
     if (c->fast() && !c->dc() && c->jacobi())
         for (int q = 0; q < 4; q++) RC(dalloc(&c->X2[q], g.total));
@@ -594,9 +627,9 @@ static int one_sweep(pm_ctx *c) {
     const double cu = om * pm::kDt, cp = wp * pm::kC2 * pm::kDt;
     if (!c->fast()) {
         tb(c, TC_REF);
-        if (!c->slab.empty())
-            pm::launch_ref_slab_sweep(g, c->G, c->code, c->X, c->slab.data(), c->prm.z_slabs,
-                                      c->prm.solving_algorithm, s);
+        if (c->ref_nblk > 1)
+            pm::launch_ref_block_sweep(g, c->G, c->code, c->X, c->ref_snap, c->ref_outlet, c->ref_tab, c->ref_nblk,
+                                       c->ref_ntile, c->prm.solving_algorithm, s);
         else
             pm::launch_ref_sweeps(g, c->G, c->code, c->X[0], c->X[1], c->X[2], c->X[3], c->prm.solving_algorithm,
                                   1, c->ref_impl, s);

@@ -102,15 +102,17 @@ int main(int argc, char **argv) {
     if (knob(flags, "--omega-p", "POREMAPS_OMEGA_P", &v)) prm.omega_p = std::atof(v.c_str());
     if (knob(flags, "--device", "POREMAPS_DEVICE", &v)) prm.device = std::atoi(v.c_str());
 
-    // dom_decomposition 1 1 P (P z-slabs) in the reference mode: the iterates of
-    // `mpirun -np P`, each slab swept against frozen copies of its
-    // neighbours' edge planes; any other decomposition runs as one domain
+    // dom_decomposition P Q R in the reference mode: the iterates of
+    // `mpirun -np P*Q*R`, each block swept against its halo copies of the
+    // other blocks as the sweep started; 0 0 0 (MPI_Dims_create over one
+    // rank) is one domain.  The fast modes ignore it.
     const int *dd = extra.dom_decomposition;
-    const bool slabs = prm.mode == PM_MODE_REFERENCE && dd[0] == 1 && dd[1] == 1 && dd[2] > 1 &&
-                       dd[2] <= prm.nz / 3;
-    if (slabs) prm.z_slabs = dd[2];
+    const bool blocks = prm.mode == PM_MODE_REFERENCE && dd[0] > 0 && dd[1] > 0 && dd[2] > 0 &&
+                        dd[0] * dd[1] * dd[2] > 1;
+    if (blocks)
+        for (int a = 0; a < 3; a++) prm.dom_decomposition[a] = dd[a];
     std::printf("Given domain decomposition nx = %i, ny = %i, nz = %i (%s)\n", dd[0], dd[1], dd[2],
-                slabs ? "reproduced as z-slabs on one GPU" : "one GPU: ignored");
+                blocks ? "its blocks reproduced on one GPU" : "one domain on one GPU");
     std::printf("Domain size: domx = %i, domy = %i, domz = %i; boundary_method %i; solver %i\n", prm.nx, prm.ny,
                 prm.nz, prm.boundary_method, prm.solving_algorithm);
 
@@ -169,8 +171,11 @@ int main(int argc, char **argv) {
     }
     if (ok && wo[3] == 1) {
         std::printf("Writing domain decomposition to file\n");
-        std::vector<int32_t> zero((size_t)n, 0);
-        ok = pm_write_raw_i32(("domain_decomp_" + geom).c_str(), zero.data(), n) == PM_OK;
+        // the rank of every voxel, as the run with these blocks writes it
+        std::vector<int32_t> rank((size_t)n, 0);
+        const int ones[3] = {1, 1, 1};
+        ok = pm_rank_map(prm.nx, prm.ny, prm.nz, blocks ? dd : ones, rank.data()) == PM_OK &&
+             pm_write_raw_i32(("domain_decomp_" + geom).c_str(), rank.data(), n) == PM_OK;
     }
     if (!ok) {
         std::fprintf(stderr, "%s %s\n", pm_last_error(), pm_io_last_error());

@@ -34,6 +34,27 @@ def known():
 
 
 @pytest.fixture(scope="session")
+def ref_fields():
+    """Fields, codes and rank maps written by the reference binary
+    (tests/golden/make_reference_fields.py)."""
+    with np.load(os.path.join(GOLDEN, "reference_fields.npz")) as f:
+        return {k: f[k] for k in f.files}
+
+
+@pytest.fixture(scope="session")
+def ref_meta():
+    with open(os.path.join(GOLDEN, "reference_fields.json")) as f:
+        return json.load(f)
+
+
+def ref_run(meta, group, tag):
+    for r in meta[group]["runs"]:
+        if r["tag"] == tag:
+            return r
+    raise KeyError(tag)
+
+
+@pytest.fixture(scope="session")
 def channel_raw():
     return np.fromfile(os.path.join(GOLDEN, "rect_channel_24_14_50_vs_5e-05.raw"),
                        dtype=np.uint8).reshape(50, 14, 24)

@@ -154,7 +154,9 @@ def test_reference_z_slabs_match_slab_oracle(P, solver, bc):
     """dom_decomposition 1 1 P on one GPU: P z-slabs, each swept against
     frozen copies of its neighbours' edge planes, then the exchange and the
     pins -- bit for bit the oracle's restatement of `mpirun -np P`
-    (orc_sweeps_ref_slabs), with slabs of unequal thickness (nz 13)."""
+    (orc_sweeps_ref_slabs, pinned to the reference binary's decomposed runs
+    in tests/test_reference_fields.py), with slabs of unequal thickness
+    (nz 13)."""
     raw = random_geometry((13, 9, 11), 0.35, 11 * P + bc)
     say("reference z-slabs", P, solver, bc)
     G, code, st = oracle_state(raw, bc)
@@ -501,7 +503,8 @@ def test_cli_fast_mode_default_relaxation(tmp_path, channel_raw):
 
 def test_cli_dom_decomposition_z_slabs(tmp_path, channel_raw):
     """dom_decomposition 1 1 3 in the .inp: the CLI sweeps as `mpirun -np 3`
-    does (three z-slabs), the same solve as the library's z_slabs=3."""
+    does (three z-slabs), the same solve as the library's z_slabs=3, and
+    writes each voxel's rank into domain_decomp_."""
     say("cli z-slabs")
     (tmp_path / "rect_channel_24_14_50_vs_5e-05.raw").write_bytes(channel_raw.tobytes())
     inp = open(os.path.join(GOLDEN, "input_rect_channel_24_14_50_vs_5e-05.inp")).read()
@@ -510,7 +513,11 @@ def test_cli_dom_decomposition_z_slabs(tmp_path, channel_raw):
     (tmp_path / "input.inp").write_text(inp)
     r = subprocess.run([pm.CLI_PATH, "input.inp"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
-    assert "reproduced as z-slabs" in r.stdout
+    assert "its blocks reproduced" in r.stdout
+    # each voxel's rank (src/output.cc:156-190): slabs of 17, 17, 16 planes
+    dd = np.fromfile(tmp_path / "domain_decomp_rect_channel_24_14_50_vs_5e-05.raw", np.int32).reshape(50, 14, 24)
+    assert np.array_equal(dd, pm.rank_map((50, 14, 24), (1, 1, 3)))
+    assert (dd[:17] == 0).all() and (dd[17:34] == 1).all() and (dd[34:] == 2).all()
     with pm.Solver(channel_raw, boundary_method=0, solving_algorithm=2, eps=1e-6, voxel_size=5e-5,
                    z_slabs=3) as s:
         res, rows = s.solve()

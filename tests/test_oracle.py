@@ -378,74 +378,24 @@ def test_defect_correction_reproduces_fp64_sweeps(mode, bc):
 
 
 # ---------------------------------------------------------------------------
-# z-slab threaded reference sweeps (the CPU baseline on all host cores)
+# threaded block sweeps (the CPU baseline on all host cores): one block is
+# the serial sweep; decompositions against the reference binary are pinned
+# in tests/test_reference_fields.py
 # ---------------------------------------------------------------------------
-def _slab_sweeps_sequential(G, code, st, bc, nsweeps, P):
-    """The z-slab decomposition of orc_sweeps_ref_slabs written out serially
-    with numpy copies: each slab a separate array with its own halo planes,
-    swept with the serial oracle sweep; then x / y periodic copies, 2-plane
-    z halos from the neighbouring slabs, pins on the owning slabs."""
-    nz = G.shape[0] - 4
-    px = py = bc == 0
-    pz = bc in (0, 1, 2)
-    sizes = [nz // P + (1 if k < nz % P else 0) for k in range(P)]
-    z0 = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(int)
-    slabs = [[f[z0[k]:z0[k] + sizes[k] + 4].copy() for f in st] for k in range(P)]
-    for _ in range(nsweeps):
-        for k in range(P):
-            Gk, ck = G[z0[k]:z0[k] + sizes[k] + 4], code[z0[k]:z0[k] + sizes[k] + 4]
-            O.sweep_ref(2, np.ascontiguousarray(Gk), np.ascontiguousarray(ck), *slabs[k])
-            for f in slabs[k]:
-                if px:
-                    f[:, :, -2:], f[:, :, :2] = f[:, :, 2:4].copy(), f[:, :, -4:-2].copy()
-                if py:
-                    f[:, -2:, :], f[:, :2, :] = f[:, 2:4, :].copy(), f[:, -4:-2, :].copy()
-        new = [[f.copy() for f in s] for s in slabs]
-        for k in range(P):
-            below = k - 1 if k > 0 else (P - 1 if pz else None)
-            above = k + 1 if k < P - 1 else (0 if pz else None)
-            for q in range(4):
-                if below is not None:
-                    new[k][q][:2] = slabs[below][q][sizes[below]:sizes[below] + 2]
-                if above is not None:
-                    new[k][q][-2:] = slabs[above][q][2:4]
-        slabs = new
-        fl = (G[1, 2:-2, 2:-2] == 0)
-        slabs[0][0][1, 2:-2, 2:-2][fl] = nz + 2
-        last, n = slabs[-1], sizes[-1]
-        Gl = G[z0[-1]:z0[-1] + n + 4]
-        for plane, val in ((n + 2, 2.0), (n + 1, 3.0)):
-            m = Gl[plane, 2:-2, 2:-2] == 0
-            last[0][plane, 2:-2, 2:-2][m] = val
-    out = [f.copy() for f in st]
-    for k in range(P):
-        for q in range(4):
-            out[q][z0[k] + 2:z0[k] + 2 + sizes[k]] = slabs[k][q][2:2 + sizes[k]]
-    for q in range(4):
-        out[q][:2] = slabs[0][q][:2]
-        out[q][-2:] = slabs[-1][q][-2:]
-    return out
-
-
-@pytest.mark.parametrize("bc,P", [(0, 1), (0, 3), (2, 4), (4, 2), (1, 5)])
-def test_slab_threaded_reference_sweeps(bc, P):
+@pytest.mark.parametrize("bc", [0, 2, 4])
+def test_one_block_is_the_serial_sweep(bc):
     raw = random_geometry((17, 10, 12), 0.4, 11 + bc)
     G = O.geom_halo(raw, bc)
     code = O.eval_geometry(G)
     st = [f.copy() for f in O.init_fields(G, bc)]
     O.pins(G, st[0])
-    exp = _slab_sweeps_sequential(G, code, st, bc, 6, P)
     got = [f.copy() for f in st]
-    O.sweeps_ref_slabs(2, G, code, *got, bc, 6, P)
-    for a, b in zip(got, exp):
+    O.sweeps_ref_slabs(2, G, code, *got, bc, 6, 1)
+    ser = [f.copy() for f in st]
+    for _ in range(6):
+        O.sweep_ref(2, G, code, *ser)
+        for f in ser:
+            O.exchange(f, bc)
+        O.pins(G, ser[0])
+    for a, b in zip(got, ser):
         assert np.array_equal(a, b)
-    if P == 1:
-        # one slab: the serial sweep, exchange and pins
-        ser = [f.copy() for f in st]
-        for _ in range(6):
-            O.sweep_ref(2, G, code, *ser)
-            for f in ser:
-                O.exchange(f, bc)
-            O.pins(G, ser[0])
-        for a, b in zip(got, ser):
-            assert np.array_equal(a, b)
