@@ -291,6 +291,7 @@ struct pm_batch {
     pm::RedAcc *dPart = nullptr;
     pm::RedSums *dOut = nullptr, *hOut = nullptr;
     int capS = 0, capC = 0, capG = 0, capP = 0;
+    int capPart = 0, capOut = 0;     // reduction partials / outputs (grow-only)
     int nS = 0, nC = 0, nG = 0, nP = 0;
     bool shape_dirty = true;
     hipGraph_t graph = nullptr;
@@ -548,6 +549,10 @@ extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
     if (!b) return pm_fail(PM_E_ARG, "null batch");
     BCHK(hipSetDevice(b->prm.device));
     if (b->shape_dirty) {
+        // the captured interval's nodes refer to the tables, the reduction
+        // buffers and every running sample's arrays: it is destroyed before
+        // any of them is freed, reallocated or rewritten
+        drop_graph(b);
         b->run.clear();
         for (size_t s = 0; s < b->ctx.size(); s++)
             if (b->ctx[s] && b->state[s] == PM_SAMPLE_RUNNING && b->ctx[s]->lists.n_all > 0)
@@ -564,14 +569,25 @@ extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
     if ((rc = grow(&b->dS, &b->capS, (int)S.size())) || (rc = grow(&b->dC, &b->capC, (int)C.size())) ||
         (rc = grow(&b->dG, &b->capG, (int)G.size())) || (rc = grow(&b->dP, &b->capP, (int)P.size())))
         return rc;
-    int capPart = b->capC;
-    if (!b->dPart || b->shape_dirty) {
+    // reduction partials (one per chunk) and outputs (one per sample):
+    // grow-only, reallocated only while no graph refers to them (sizes change
+    // only with the membership, which dropped the graph above)
+    if (b->capPart < b->capC) {
         if (b->dPart) (void)hipFree(b->dPart);
+        b->dPart = nullptr;
+        b->capPart = 0;
+        BCHK(hipMalloc((void **)&b->dPart, sizeof(pm::RedAcc) * (size_t)b->capC));
+        b->capPart = b->capC;
+    }
+    if (b->capOut < b->capS) {
         if (b->dOut) (void)hipFree(b->dOut);
         if (b->hOut) (void)hipHostFree(b->hOut);
-        BCHK(hipMalloc((void **)&b->dPart, sizeof(pm::RedAcc) * (size_t)capPart));
+        b->dOut = nullptr;
+        b->hOut = nullptr;
+        b->capOut = 0;
         BCHK(hipMalloc((void **)&b->dOut, sizeof(pm::RedSums) * (size_t)b->capS));
         BCHK(hipHostMalloc((void **)&b->hOut, sizeof(pm::RedSums) * (size_t)b->capS, hipHostMallocDefault));
+        b->capOut = b->capS;
     }
     BCHK(hipMemcpyAsync(b->dS, S.data(), sizeof(BSample) * S.size(), hipMemcpyHostToDevice, b->stream));
     if (b->shape_dirty) {
@@ -582,7 +598,6 @@ extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
         b->nC = (int)C.size();
         b->nG = (int)G.size();
         b->nP = (int)P.size();
-        drop_graph(b);
     }
     if (b->prm.use_graph) {
         if (!b->exec) {
@@ -688,6 +703,8 @@ extern "C" int pm_batch_get_field(pm_batch *b, int slot, int field, void *out) {
 extern "C" int pm_batch_remove(pm_batch *b, int slot) {
     if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
     BCHK(hipStreamSynchronize(b->stream));
+    // the captured interval refers to this sample's arrays
+    drop_graph(b);
     pm_destroy(b->ctx[slot]);
     b->ctx[slot] = nullptr;
     b->used -= b->bytes[slot];
