@@ -187,6 +187,22 @@ int pm_relaxation(pm_ctx *ctx, double *omega, double *omega_p);
 
 void pm_destroy(pm_ctx *ctx);
 
+/* No counterpart in the reference: pm_permeability (the same reduction
+ * kernels, the same results bit for bit) for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The caller passes its own device arrays,
+ * interior only, x fastest (nx*ny*nz elements): geom 0 = fluid / 1 = solid,
+ * p, u, v, w in FP64; out receives k13 k23 k33 wk13 wk23 wk33 wmax wmean in
+ * host memory; porosity -1.0 computes it from geom; stream: a hipStream_t
+ * (NULL: the null stream), synchronized before return.  The partial sums
+ * live in a fixed 1.38 MB array of the library on each GPU, kept until the
+ * process ends: the call allocates nothing, and calls that overlap on one
+ * GPU race for that array.  At most PM_NOALLOC_MAX_VOXELS voxels. */
+#define PM_NOALLOC_MAX_VOXELS 2147483647LL
+int pm_permeability_noalloc(const uint8_t *geom, const double *p, const double *u, const double *v,
+                            const double *w, int nx, int ny, int nz, const int dom_interest[6],
+                            double voxel_size, double porosity, double out[8], void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,9 @@ struct DomBox {
 };
 
 constexpr int kRedThreads = 256;
+// Most workgroups of a state reduction (one partial sum each): 32 waves per
+// CU, enough loads in flight.
+constexpr int kRedBlocksMax = 8192;
 
 // Results of a state reduction: sum[0..2] fluid u, v, w; sum[3..5] box fluid
 // u, v, w; sum[6], sum[7] p on the box's zmin / zmax planes; count[0..3] the
@@ -147,6 +150,7 @@ void launch_ghost_fast(const Grid &g, const uint8_t *G, double *p, double *u, do
 void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const double *u, const double *v,
                    const double *w, const DomBox &box, bool full, void *scratch, RedSums *out,
                    hipStream_t s);
+// scratch: kRedBlocksMax block partials (reduce_scratch_bytes()).
 size_t reduce_scratch_bytes();
 
 void launch_outlet_plane(const Grid &g, const uint8_t *pc, const double *p, const double *u,

@@ -451,7 +451,6 @@ __global__ __launch_bounds__(1024) void k_ref_sweeps_1wg(Grid g, const uint8_t *
 // reductions (deterministic, compensated FP64)
 // ---------------------------------------------------------------------------
 
-constexpr int kRedBlocks = 8192;  // 32 waves per CU: enough loads in flight
 
 // FULL = false: only the sum of w and the fluid count (compute_convergence).
 template <bool FULL>
@@ -640,7 +639,7 @@ void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const doubl
     RedAcc *part = (RedAcc *)scratch;
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int64_t rows = n / g.nx;
-    const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (rows + 3) / 4));
+    const int nb = (int)std::min<int64_t>(kRedBlocksMax, std::max<int64_t>(1, (rows + 3) / 4));
     if (full)
         hipLaunchKernelGGL(k_reduce_state<true>, dim3(nb), dim3(kRedThreads), 0, s, g, G, p, u, v, w, box, part);
     else
@@ -648,6 +647,6 @@ void launch_reduce(const Grid &g, const uint8_t *G, const double *p, const doubl
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, s, part, nb, out);
 }
 
-size_t reduce_scratch_bytes() { return sizeof(RedAcc) * kRedBlocks; }
+size_t reduce_scratch_bytes() { return sizeof(RedAcc) * kRedBlocksMax; }
 
 }  // namespace pm
