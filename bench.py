@@ -22,16 +22,22 @@ Workloads (synthetic, generated from a seed):
 
 roofline: the dominant kernel group ("sweep") timed with HIP events on the
 library's own stream over the timed region; achieved = algorithmic bytes per
-launch (8*S+1 B per voxel for the state, + 16 B of FP32 residual in the
-defect-correction mode) / average launch duration; peak = 8 TB/s HBM;
-traffic = the measured HBM bytes per launch (rocprofv3 read requests by size
-+ WRITE_SIZE, calibrated on a device copy of known size) from the committed
-counter summary when the run matches it.
-cpu_baseline: the CPU oracle's reference-order sweep (FP64) timed on this
-host on a bounded sample of the same field, in z-slabs on all host cores (at
-most 16), with the one-thread rate beside.
---tte: time-to-eps (eps 1e-6) of the plain FP64 Jacobi solver and of the
-selected mode, reported beside the line.
+launch, 8*S+1 B per voxel of the interior (read and write p, u, v, w in the
+swept precision S = 4 or 8 B, read a 1-byte code), over the average launch
+duration; peak = 8 TB/s HBM.  achieved_with_residual_GBps adds the 16 B of
+FP32 residual the defect-correction sweep also reads.  traffic = the
+measured HBM bytes per launch (rocprofv3 read requests by size + WRITE_SIZE,
+calibrated on a device copy of known size) from the committed counter summary
+named in traffic_source, when the run uses its workload and kernel form.
+time_to_eps (default on; --no-tte skips it): eps 1e-6 from fields resident in
+HBM to the stop rule, for the plain FP64 Jacobi solver, the FP32+FP64-refined
+Jacobi and 3-colour SOR, and the FP64 solves at the same relaxations (k_zz of
+each refined solve relative to its FP64 twin).
+cpu_baseline: the reference POREMAPS itself (oracle/_ref/POREMAPS, built from
+the reference's sources by `make -C oracle ref`) under mpiexec on this
+host's cores (at most 16), on the same field: the rate of the difference of a
+short and a longer run (start-up and geometry read cancel); the CPU oracle's
+threaded block sweep when the binary or mpiexec is missing.
 """
 from __future__ import annotations
 
@@ -130,6 +136,55 @@ def cpu_baseline(g, bc, budget_s=20.0):
             "sample_1core": f"{done} serial sweeps on 1 thread, {el:.1f} s"}
 
 
+def cpu_baseline_reference(g, spec, budget_s=25.0):
+    """The reference binary on this host's cores: `mpiexec -n P POREMAPS`
+    with dom_decomposition 0 0 0 (MPI_Dims_create picks the blocks), solver 2,
+    no output files; two runs of k1 and k2 sweeps, rate = cells * (k2 - k1) /
+    (t2 - t1).  None when the binary or mpiexec is missing."""
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "POREMAPS")
+    mpiexec = os.environ.get("MPIEXEC", "/opt/conda/bin/mpiexec")
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    P = max(1, min(cores, 16))
+    nz, ny, nx = g.shape
+    with tempfile.TemporaryDirectory(prefix="pm_cpu_") as d:
+        np.ascontiguousarray(g, dtype=np.uint8).tofile(os.path.join(d, "geom.raw"))
+
+        def run(k):
+            with open(os.path.join(d, "run.inp"), "w") as f:
+                f.write(f"dom_decomposition 0 0 0\nboundary_method {spec['bc']}\ngeometry_file_name geom.raw\n"
+                        f"size_x_y_z  {nx} {ny} {nz}\nvoxel_size  {spec['dx']!r}\nmax_iter    {k}\n"
+                        f"it_eval    100\nit_write   100\nlog_file_name run.log\nsolving_algorithm 2\n"
+                        f"eps 1e-06\nporosity -1.0\ndom_interest 0 0 0 0 0 0\nwrite_output 0 0 0 0")
+            t0 = time.perf_counter()
+            r = subprocess.run([mpiexec, "-n", str(P), exe, "run.inp"], cwd=d, capture_output=True, text=True,
+                               timeout=600)
+            el = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(f"reference run failed: {r.stderr[-500:]}")
+            print(f"[bench] cpu baseline: reference binary, {P} ranks, {k} sweeps in {el:.1f}s", file=sys.stderr,
+                  flush=True)
+            return el
+
+        t1 = run(2)
+        t2 = run(12)
+        per = max((t2 - t1) / 10, 1e-4)
+        k = int(max(12, min(2000, 2 + budget_s / per)))
+        tk = run(k) if k > 12 else t2
+    rate = g.size * (k - 2) / (tk - t1) / 1e9
+    return {"value": round(rate, 4), "unit": "GLUP/s", "cores": P, "kind": "reference",
+            "sample": f"reference POREMAPS (oracle/_ref, g++ -O3, MPICH) under mpiexec -n {P}, dom_decomposition "
+                      f"0 0 0, solving_algorithm 2, FP64, the same {nx}x{ny}x{nz} field: {k} minus 2 sweeps, "
+                      f"{tk - t1:.1f} s"}
+
+
 def time_kernels(s, n):
     import ctypes as C
     import poremaps_amd as pm
@@ -166,7 +221,43 @@ def measured_traffic(spec, form, precision):
     if not os.path.exists(path):
         return None
     m = re.search(known[form][1], open(path).read())
-    return int(m.group(1)) if m else None
+    return (int(m.group(1)), "profiles/" + known[form][0]) if m else None
+
+
+def default_tte(g, kw):
+    """Time to eps 1e-6 on the field: the plain FP64 Jacobi solver, the
+    FP32+FP64-refined Jacobi (2.5, 7) and 3-colour SOR (1.8, 6), and the FP64
+    solves at the same relaxations; speedups against the plain solver and each
+    refined k_zz relative to its FP64 twin."""
+    import poremaps_amd as pm
+
+    runs = [("fp64_plain_jacobi", dict(mode="jacobi", precision="f64", omega=1.0, omega_p=0.0)),
+            ("f32dc_jacobi", dict(mode="jacobi", precision="f32dc", omega=2.5, omega_p=7.0)),
+            ("f32dc_sor3", dict(mode="sor3", precision="f32dc", omega=1.8, omega_p=6.0)),
+            ("fp64_jacobi_same_relaxation", dict(mode="jacobi", precision="f64", omega=2.5, omega_p=7.0)),
+            ("fp64_sor3_same_relaxation", dict(mode="sor3", precision="f64", omega=1.8, omega_p=6.0))]
+    out = {}
+    for label, mk in runs:
+        s = pm.Solver(g, **dict(kw, **mk, eps=1e-6))
+        last = [time.time()]
+
+        def progress(it, conv, secs, label=label):
+            if time.time() - last[0] > 20:
+                last[0] = time.time()
+                print(f"[bench] tte {label}: sweep {it}, conv {conv:.3e}", file=sys.stderr, flush=True)
+
+        res, _ = s.solve(progress=progress)
+        s.close()
+        out[label] = {"seconds": round(res.seconds, 3), "sweeps": res.sweeps, "wk33": res.last.wk33,
+                      "glups": round(res.glups, 2), "omega": mk["omega"], "omega_p": mk["omega_p"]}
+        print(f"[bench] tte {label}: {out[label]}", file=sys.stderr, flush=True)
+    plain = out["fp64_plain_jacobi"]["seconds"]
+    for lab, twin in (("f32dc_jacobi", "fp64_jacobi_same_relaxation"), ("f32dc_sor3", "fp64_sor3_same_relaxation")):
+        v = out[lab]
+        v["speedup_vs_fp64_plain"] = round(plain / v["seconds"], 2)
+        v["wk33_rel_vs_fp64_same_relaxation"] = (v["wk33"] - out[twin]["wk33"]) / out[twin]["wk33"]
+        v["wk33_rel_vs_fp64_plain"] = (v["wk33"] - out["fp64_plain_jacobi"]["wk33"]) / out["fp64_plain_jacobi"]["wk33"]
+    return out
 
 
 def _solve_alone(raw, bc, dx, **kw):
@@ -318,7 +409,8 @@ def main():
     ap.add_argument("--omega-p", type=float, default=None,
                     help="pressure relaxation of the fast modes (default 7.0 for one field; 0 with omega 0)")
     ap.add_argument("--it-eval", type=int, default=100)
-    ap.add_argument("--tte", action="store_true")
+    ap.add_argument("--tte", action="store_true", help="time-to-eps of the selected mode beside the plain solver")
+    ap.add_argument("--no-tte", action="store_true", help="skip the default time-to-eps set")
     ap.add_argument("--accuracy", action="store_true",
                     help="with --tte: FP64 at the same relaxation and an eps 1e-9 reference solve")
     ap.add_argument("--reference-eps", type=float, default=1e-9,
@@ -373,9 +465,14 @@ def main():
     s.close()
     group = "sweep" if "sweep" in kt else "reference_sweep"
     sw = kt[group]
-    achieved = sw["bytes"] / (sw["avg_ms"] * 1e-3) / 1e9
+    # algorithmic bytes per sweep: read + write p, u, v, w (S bytes each) and
+    # a 1-byte code per interior voxel (the FP32 correction state has S = 4)
+    S = 4 if a.precision == "f32dc" else 8
+    algo = (8 * S + 1) * n
+    achieved = algo / (sw["avg_ms"] * 1e-3) / 1e9
     print(f"[bench] {sweeps} sweeps in {el:.3f}s: {glups:.2f} GLUP/s; kernels {json.dumps(kt)}",
           file=sys.stderr, flush=True)
+    traffic = measured_traffic(spec, choice, a.precision)
     line = {
         "metric": METRIC,
         "value": round(glups, 3),
@@ -395,13 +492,17 @@ def main():
                    "sweeps_per_step": a.it_eval, "porosity": float((g == 0).mean())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic(spec, choice, a.precision),
+                     "traffic": traffic[0] if traffic else None,
+                     "traffic_source": traffic[1] if traffic else None,
                      "kernel": group, "avg_ms": round(sw["avg_ms"], 4),
-                     "bytes_per_launch": sw["bytes"]},
+                     "bytes_per_launch": algo, "bytes_per_voxel": 8 * S + 1,
+                     "achieved_with_residual_GBps": round(sw["bytes"] / (sw["avg_ms"] * 1e-3) / 1e9, 1)},
         "kernels": {k2: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"],
                          "GBps": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k2, v in kt.items()},
         "sweep_kernel": {"form": choice, "autotuner_ms_per_sweep": tuned},
     }
+    if not (a.tte or a.accuracy or a.no_tte) and a.mode != "reference":
+        line["time_to_eps"] = default_tte(g, kw)
     if a.tte or a.accuracy:
         tte = {}
         runs = [("fp64_plain_jacobi", dict(mode="jacobi", precision="f64", omega=1.0, omega_p=0.0)),
@@ -445,7 +546,12 @@ def main():
         line["time_to_eps"] = tte
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(g, spec["bc"])
+        cb = None
+        try:
+            cb = cpu_baseline_reference(g, spec)
+        except Exception as e:  # the binary failed: fall back to the oracle port
+            print(f"[bench] reference binary unavailable ({e}); CPU oracle instead", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cb if cb is not None else cpu_baseline(g, spec["bc"])
     print(json.dumps(line), flush=True)
 
 

@@ -98,6 +98,9 @@ def _declare(L):
     L.orc_outlet_plane.argtypes = [_u8p, _I, _I, _I, _I] + [_f64p] * 4 + [_D, _f64p]
     L.orc_residual.argtypes = [_u8p, _I, _I, _I] + [_f64p] * 5 + [_f32p] * 4
     L.orc_fold.argtypes = [_u8p, _I, _I, _I, _I] + [_f64p] * 4 + [_f32p] * 4
+    L.orc_solve_fast.argtypes = [_I, _I, _u8p, _u8p, _I, _I, _I, _I, _D, _D, _D, _I, _I] + [_f64p] * 4 + [
+        C.POINTER(C.c_double)]
+    L.orc_solve_fast.restype = _I
     L.orc_d2_ref.argtypes = [_I] + [_D] * 5
     L.orc_d2_ref.restype = _D
     L.orc_d2_f64.argtypes = [_I] + [_D] * 5
@@ -268,6 +271,22 @@ def residual(pc, X, q):
     R = tuple(np.zeros(pc.shape, np.float32) for _ in range(4))
     lib().orc_residual(pc, *_dims(pc), *X, q, *R)
     return R
+
+
+def solve_fast(raw, bc, mode="jacobi", precision="f32dc", omega=2.5, omega_p=None, eps=1e-6, it_eval=100,
+               max_iter=10 ** 7):
+    """A whole fast-mode solve of the geometry raw (nz, ny, nx) to the stop
+    rule: returns (sweeps, kprev, (p, u, v, w) halo-2 FP64 state, G)."""
+    G = geom_halo(raw, bc)
+    pc = pack_code(eval_geometry(G))
+    st = [a.copy() for a in init_fields(G, bc)]
+    pins(G, st[0])
+    k = C.c_double(0.0)
+    t = lib().orc_solve_fast(1 if mode == "jacobi" else 2, 1 if precision == "f32dc" else 0, G, pc, *_dims(G), bc,
+                             omega, omega if omega_p is None else omega_p, eps, it_eval, max_iter, *st, C.byref(k))
+    if t < 0:
+        raise MemoryError("orc_solve_fast: allocation failed")
+    return t, k.value, st, G
 
 
 def fold(G, bc, X, d) -> None:

@@ -748,3 +748,95 @@ float orc_d2_f32(int cs, float ll, float l, float c, float r, float rr)
 {
     return d2_f32(cs, ll, l, c, r, rr);
 }
+
+/* A whole fast-mode solve (the build's Jacobi / 3-colour SOR with the
+ * relaxation pair (omega, omega_p), FP64 or FP32 correction sweeps with FP64
+ * defect correction) from the initial state in p, u, v, w (after the pins) to
+ * the reference's stop rule (src/calc_flow.cc:101,120-128): the convergence
+ * value of the FP64 state every it_eval sweeps past sweep 1000.  In the
+ * defect-correction form the FP32 correction d obeys the same sweep with the
+ * FP32 residual R of the FP64 state added to its forces, pins and outlet
+ * buffer 0; every it_eval sweeps (sweep index t % it_eval == 0, before the
+ * convergence test of that sweep) d is folded into the state and R and the
+ * outlet buffer recomputed.  Returns the sweeps done (the final state in
+ * p, u, v, w; *kprev the last convergence test's permeability in voxel
+ * units), or -1 on allocation failure. */
+int orc_solve_fast(int mode, int dc, const uint8_t *G, const uint8_t *pc, int nx, int ny, int nz, int bc,
+                   double omega, double omega_p, double eps, int it_eval, int max_iter,
+                   double *p, double *u, double *v, double *w, double *kprev_out)
+{
+    const size_t n = (size_t)HX(nx) * HY(ny) * HZ(nz), nq = (size_t)HX(nx) * HY(ny);
+    const double pin_in = (double)(nz + 2);
+    double *X[4] = {p, u, v, w};
+    double *B[4] = {NULL, NULL, NULL, NULL}, *q = calloc(nq, sizeof(double)), *qb = NULL;
+    float *d[2][4] = {{NULL}}, *qd[2] = {NULL, NULL}, *R[4] = {NULL, NULL, NULL, NULL};
+    int fail = !q, t = 0;
+    double kprev = 0.0, conv = 1.0 + eps;
+    if (!dc && mode == 1) {
+        qb = calloc(nq, sizeof(double));
+        fail |= !qb;
+        for (int f = 0; f < 4; f++) {
+            B[f] = malloc(n * sizeof(double));
+            fail |= !B[f];
+            if (B[f]) memcpy(B[f], X[f], n * sizeof(double));
+        }
+    }
+    if (dc) {
+        for (int f = 0; f < 4; f++) {
+            for (int h = 0; h < 2; h++) { d[h][f] = calloc(n, sizeof(float)); fail |= !d[h][f]; }
+            R[f] = calloc(n, sizeof(float));
+            fail |= !R[f];
+        }
+        for (int h = 0; h < 2; h++) { qd[h] = calloc(nq, sizeof(float)); fail |= !qd[h]; }
+    }
+    if (!fail) {
+        orc_outlet_plane(pc, nx, ny, nz, bc, p, u, v, w, 3.0, q);
+        if (qb) memcpy(qb, q, nq * sizeof(double));
+        if (dc) orc_residual(pc, nx, ny, nz, p, u, v, w, q, R[0], R[1], R[2], R[3]);
+        double **A = X, **Bn = B, *qa = q, *qn = qb;
+        while (t < max_iter && conv >= eps) {
+            if (!dc && mode == 1) {
+                orc_sweep_jacobi_f64(pc, nx, ny, nz, bc, omega, omega_p, A[0], A[1], A[2], A[3], Bn[0], Bn[1],
+                                     Bn[2], Bn[3], qa, qn, 3.0, NULL, NULL, NULL, NULL);
+                orc_ghost_f64(G, nx, ny, nz, bc, Bn[0], Bn[1], Bn[2], Bn[3], qn, pin_in, 2.0, 3.0);
+                double **tmp = A; A = Bn; Bn = tmp;
+                double *tq = qa; qa = qn; qn = tq;
+            } else if (!dc) {
+                orc_sweep_colour_f64(pc, nx, ny, nz, omega, omega_p, A[0], A[1], A[2], A[3], qa, 3.0, NULL, NULL,
+                                     NULL, NULL);
+                orc_ghost_f64(G, nx, ny, nz, bc, A[0], A[1], A[2], A[3], qa, pin_in, 2.0, 3.0);
+            } else {
+                if (mode == 1) {
+                    orc_sweep_jacobi_f32(pc, nx, ny, nz, bc, omega, omega_p, d[0][0], d[0][1], d[0][2], d[0][3],
+                                         d[1][0], d[1][1], d[1][2], d[1][3], qd[0], qd[1], 0.f, R[0], R[1], R[2],
+                                         R[3]);
+                    orc_ghost_f32(G, nx, ny, nz, bc, d[1][0], d[1][1], d[1][2], d[1][3], qd[1], 0.f, 0.f, 0.f);
+                    for (int f = 0; f < 4; f++) { float *tmp = d[0][f]; d[0][f] = d[1][f]; d[1][f] = tmp; }
+                    float *tq = qd[0]; qd[0] = qd[1]; qd[1] = tq;
+                } else {
+                    orc_sweep_colour_f32(pc, nx, ny, nz, omega, omega_p, d[0][0], d[0][1], d[0][2], d[0][3], qd[0],
+                                         0.f, R[0], R[1], R[2], R[3]);
+                    orc_ghost_f32(G, nx, ny, nz, bc, d[0][0], d[0][1], d[0][2], d[0][3], qd[0], 0.f, 0.f, 0.f);
+                }
+                if (t % it_eval == 0) {
+                    orc_fold(G, nx, ny, nz, bc, p, u, v, w, d[0][0], d[0][1], d[0][2], d[0][3]);
+                    orc_outlet_plane(pc, nx, ny, nz, bc, p, u, v, w, 3.0, q);
+                    orc_residual(pc, nx, ny, nz, p, u, v, w, q, R[0], R[1], R[2], R[3]);
+                    memset(qd[0], 0, nq * sizeof(float));
+                    memset(qd[1], 0, nq * sizeof(float));
+                }
+            }
+            if (t % it_eval == 0 && t > 1000) conv = orc_convergence(G, dc ? w : A[3], nx, ny, nz, &kprev);
+            t++;
+        }
+        if (!dc && A != X)
+            for (int f = 0; f < 4; f++) memcpy(X[f], A[f], n * sizeof(double));
+    }
+    for (int f = 0; f < 4; f++) {
+        free(B[f]); free(R[f]);
+        for (int h = 0; h < 2; h++) free(d[h][f]);
+    }
+    free(q); free(qb); free(qd[0]); free(qd[1]);
+    if (kprev_out) *kprev_out = kprev;
+    return fail ? -1 : t;
+}

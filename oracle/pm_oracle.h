@@ -124,6 +124,15 @@ void orc_fold(const uint8_t *G, int nx, int ny, int nz, int bc,
               double *Xp, double *Xu, double *Xv, double *Xw,
               float *dp, float *du, float *dv, float *dw);
 
+/* A whole fast-mode solve to the reference's stop rule: mode 1 Jacobi, 2
+ * 3-colour SOR; dc 0 FP64, 1 FP32 correction sweeps + FP64 defect correction
+ * (fold every it_eval sweeps).  From the initial state in p, u, v, w (pins
+ * applied); returns the sweeps (final state in p, u, v, w, *kprev the last
+ * convergence test's mean(w)/Re) or -1. */
+int orc_solve_fast(int mode, int dc, const uint8_t *G, const uint8_t *pc, int nx, int ny, int nz, int bc,
+                   double omega, double omega_p, double eps, int it_eval, int max_iter,
+                   double *p, double *u, double *v, double *w, double *kprev);
+
 /* get_2nd_derivation (reference form) and the fast-mode forms. */
 double orc_d2_ref(int cs, double ll, double l, double c, double r, double rr);
 double orc_d2_f64(int cs, double ll, double l, double c, double r, double rr);
