@@ -32,8 +32,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["pm_kernels.hip", "pm_sweep.hip", "pm_solver.hip", "pm_batch.hip", "pm_noalloc.hip", "pm_io.cpp",
-           "pm_serve.cpp"]
+SOURCES = ["pm_kernels.hip", "pm_sweep.hip", "pm_march.hip", "pm_solver.hip", "pm_batch.hip", "pm_noalloc.hip",
+           "pm_io.cpp", "pm_serve.cpp"]
 
 
 def _run(cmd):

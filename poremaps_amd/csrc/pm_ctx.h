@@ -18,7 +18,9 @@ struct Timer {
 enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_FIRST0, TC_N };
 // JI_RECORDS: the interleaved state's lagged pass reading per-voxel records
 // (codes of the cell and its recompute targets, gathered residuals).
-enum JacobiImpl { JI_PAIR = 0, JI_LIST = 1, JI_STREAM = 2, JI_RECORDS = 3 };
+// JI_MARCH: the interleaved state's lagged pass as a z-march of tiles through
+// LDS (pm_march.hip).
+enum JacobiImpl { JI_PAIR = 0, JI_LIST = 1, JI_STREAM = 2, JI_RECORDS = 3, JI_MARCH = 4 };
 static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep",
                                         "dc_first_pass"};
 
@@ -27,14 +29,15 @@ static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_re
 struct SweepVariant {
     int impl;           // JacobiImpl
     bool select_loads;  // fluid-list kernels: all neighbours + select, or case-predicated loads
-    int zs;             // dense kernel: planes per z segment
+    int zs;             // dense kernel / z-march: planes per z segment
+    int ty = 4;         // z-march: tile rows (4 or 8)
 };
 
 // Autotuner: the first sweeps of a solve are run with each candidate in turn
 // (windows of `window` sweeps timed with events on the solver's stream), then
 // the fastest is kept for the rest of the context's life.
 struct Tuner {
-    static constexpr int kMax = 8;
+    static constexpr int kMax = 12;
     bool active = false, done = false;
     int n = 0, idx = 0, in_window = 0, window = 6, first_scored = 0;
     SweepVariant cand[kMax];
@@ -103,6 +106,8 @@ struct pm_ctx {
     // stream_zs planes)
     int jac_impl = JI_PAIR;
     int stream_zs = 64;
+    // z-march form: tile rows (4 or 8) and planes per z segment (0: all)
+    int march_ty = 4, march_zseg = 128;
     Tuner tuner;
     // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence
     bool lagged = false;

@@ -95,6 +95,14 @@ struct FluidLists {
     // (Grid::at_s) and rec holding the colour passes' records (colour-split
     // offset + packed code, block, outlet flag)
     bool split = false;
+    // FP32+DC Jacobi: the list position of the first fluid cell of every row
+    // segment (z, y, x column of tile_x cells), nz * ny * ceil(nx / tile_x)
+    // entries, for the z-march form of the lagged pass (tile_x 64 only);
+    // use_march selects that form, with tiles of 64 x march_ty cells and z
+    // segments of march_zseg planes (0: all)
+    int32_t *rowstart = nullptr;
+    bool use_march = false;
+    int march_ty = 4, march_zseg = 128;
 };
 
 // FP64 fast modes (per-field state p, u, v, w): the velocity + pressure
@@ -123,6 +131,11 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
 bool jacobi4_fills_halos(int stage, const FluidLists &L);
 void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
                     float cp, hipStream_t s);
+// The lagged pass (JAC_LAG, halo copies included) as a z-march of 64 x ty
+// tiles through LDS (pm_march.hip); needs L.rowstart, L.rec and R.q.
+bool march_supported(const Grid &g, const FluidLists &L);
+void launch_jacobi4_march(const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0, const float *q0,
+                          float4 *S1, Res4 R, float cu, float cp, int ty, int zseg, hipStream_t s);
 // The per-voxel records L.rec of the list: the lagged pass's, or the colour
 // passes' when L.split.
 void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, hipStream_t s);

@@ -132,6 +132,8 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     if (dc) per += (jac ? 32 : 16) + 4 + 16 + (jac ? 16 : 0);
     // Jacobi and FP32+DC SOR: per-voxel records (8 B per fluid cell)
     if (fast && (jac || dc)) per += 8;
+    // FP32+DC Jacobi: the z-march form's row-segment starts (4 B per 64 cells)
+    if (dc && jac) per += 1;
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
     // keeps separate colour lists
     if (fast) per += (jac || dc) ? 4 : 8;
@@ -151,6 +153,7 @@ static void free_all(pm_ctx *c) {
     f(c->red_scratch); f(c->red_dev);
     f(c->lists.all);
     f(c->lists.rec);
+    f(c->lists.rowstart);
     for (int q = 0; q < 4; q++) { f(c->ref_snap[q]); c->ref_snap[q] = nullptr; }
     f(c->ref_outlet); f(c->ref_tab);
     c->ref_outlet = nullptr; c->ref_tab = nullptr; c->ref_nblk = 0;
@@ -197,8 +200,13 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
         else if (std::strcmp(jm, "list") == 0) c->jac_impl = JI_LIST;
         else if (std::strcmp(jm, "stream") == 0) c->jac_impl = JI_STREAM;
         else if (std::strcmp(jm, "records") == 0) c->jac_impl = JI_RECORDS;
+        else if (std::strcmp(jm, "march") == 0) c->jac_impl = JI_MARCH;
     }
     if (const char *zs = std::getenv("POREMAPS_STREAM_ZS")) c->stream_zs = std::max(1, std::atoi(zs));
+    // z-march form (POREMAPS_JACOBI=march): tile rows 4 or 8, planes per z
+    // segment (0: all)
+    if (const char *mt = std::getenv("POREMAPS_MARCH_TY")) c->march_ty = std::atoi(mt) == 8 ? 8 : 4;
+    if (const char *mz = std::getenv("POREMAPS_MARCH_ZSEG")) c->march_zseg = std::max(0, std::atoi(mz));
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -385,6 +393,12 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         const int tx = c->prm_tile_x > 0 ? std::min(c->prm_tile_x, g.nx) : g.nx;
         std::vector<int32_t> all;
         std::vector<int32_t> col[3];
+        // FP32+DC Jacobi with columns 64 cells wide: the list position of
+        // every row segment's first fluid cell (the z-march form's index)
+        // (one column as wide as the domain is the same for nx <= 64)
+        const bool rows = c->dc() && c->jacobi() && (tx == 64 || (tx == g.nx && g.nx <= 64));
+        const int ntx = (g.nx + 63) / 64;
+        std::vector<int32_t> rowstart(rows ? (size_t)g.nz * g.ny * ntx : 0);
         all.reserve((size_t)fluid);
         for (int q = 0; q < 3; q++) col[q].reserve((size_t)fluid / 3 + 1);
         for (int j0 = 0; j0 < g.ny; j0 += ty)
@@ -392,6 +406,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
             for (int i = 0; i < g.nz; i++)
                 for (int j = j0; j < std::min(j0 + ty, g.ny); j++) {
                     const uint8_t *row = raw + ((int64_t)i * g.ny + j) * g.nx;
+                    if (rows) rowstart[((size_t)i * g.ny + j) * ntx + k0 / 64] = (int32_t)all.size();
                     for (int k = k0; k < std::min(k0 + tx, g.nx); k++) {
                         if (row[k] != 0) continue;
                         const int32_t e = (int32_t)g.at(i + 2, j + 2, k + 2);
@@ -406,6 +421,13 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         const char *ld = std::getenv("POREMAPS_SWEEP_LOADS");
         c->lists.select_loads = !(ld && std::strcmp(ld, "branch") == 0);
         c->lists.use_rec = c->jac_impl == JI_RECORDS;
+        if (rows) {
+            RC(dalloc(&c->lists.rowstart, (int64_t)rowstart.size()));
+            HIPCHK(hipMemcpy(c->lists.rowstart, rowstart.data(), sizeof(int32_t) * rowstart.size(),
+                             hipMemcpyHostToDevice));
+        }
+        c->lists.march_ty = c->march_ty;
+        c->lists.march_zseg = c->march_zseg;
         const bool sor = c->prm.mode == PM_MODE_SOR3;
         if (sor && c->dc()) {
             // the colour lists back to back: the residual's momentum part,
@@ -425,6 +447,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         RC(dalloc(&c->lists.all, std::max<int64_t>(1, c->lists.n_all)));
         if (c->dc()) RC(dalloc(&c->R4.m, std::max<int64_t>(1, c->lists.n_all)));
         if (c->jacobi() || c->lists.split) RC(dalloc(&c->lists.rec, std::max<int64_t>(1, c->lists.n_all)));
+        c->lists.use_march = c->jac_impl == JI_MARCH && pm::march_supported(g, c->lists);
         if (c->dc() && c->jacobi()) RC(dalloc(&c->R4.q, std::max<int64_t>(1, c->lists.n_all)));
         HIPCHK(hipMemcpy(c->lists.all, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
         if (sor && c->dc()) {
@@ -539,10 +562,16 @@ static void apply_variant(pm_ctx *c, const SweepVariant &v) {
     c->lists.select_loads = v.select_loads;
     c->stream_zs = v.zs;
     c->lists.use_rec = v.impl == JI_RECORDS;
+    c->lists.use_march = v.impl == JI_MARCH && pm::march_supported(c->g, c->lists);
+    if (v.impl == JI_MARCH) {
+        c->lists.march_ty = c->march_ty = v.ty;
+        c->lists.march_zseg = c->march_zseg = v.zs;
+    }
 }
 
 static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *buf, size_t len) {
     if (c->dc() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
+    else if (c->dc() && v.impl == JI_MARCH) std::snprintf(buf, len, "interleaved-zmarch-ty%d-zs%d", v.ty, v.zs);
     else if (c->dc())
         std::snprintf(buf, len, "interleaved-lagged-%s",
                       v.impl == JI_RECORDS ? "records" : v.select_loads ? "eager" : "branch");
@@ -569,8 +598,15 @@ static void tuner_setup(pm_ctx *c) {
                                 {JI_STREAM, true, 16}};
     const SweepVariant aos[] = {{JI_RECORDS, true, 64}, {JI_PAIR, true, 64}, {JI_LIST, false, 64},
                                 {JI_LIST, true, 64}, {JI_RECORDS, true, 64}};
-    if (c->dc())
+    // the z-march form: 64 x 4 tiles, z segments of 128 planes (64 x 8 tiles
+    // and 64-plane segments measured slower on the 512^3 field: 4.09 and 2.40
+    // against 2.37 ms per sweep)
+    const SweepVariant march[] = {{JI_MARCH, true, 128, 4}};
+    if (c->dc()) {
         for (const auto &v : aos) tn.cand[tn.n++] = v;
+        if (pm::march_supported(c->g, c->lists))
+            for (const auto &v : march) tn.cand[tn.n++] = v;
+    }
     else
         for (const auto &v : soa) tn.cand[tn.n++] = v;
     tn.first_scored = 1;
@@ -994,7 +1030,8 @@ extern "C" int pm_relaxation(pm_ctx *c, double *omega, double *omega_p) {
 extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int tried_len) {
     if (!c || !name || len <= 0) return fail(PM_E_ARG, "null argument");
     char buf[64];
-    const SweepVariant cur = {c->jac_impl, c->lists.select_loads, c->stream_zs};
+    const SweepVariant cur = {c->jac_impl, c->lists.select_loads,
+                              c->jac_impl == JI_MARCH ? c->march_zseg : c->stream_zs, c->march_ty};
     const char *sor = !c->fast() ? "n/a" : !c->dc() ? "list-colour" : c->lists.split ? "interleaved-colour-split"
                                                                                     : "interleaved-colour";
     std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(c, cur, buf, sizeof buf) : sor);

@@ -595,6 +595,7 @@ void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const ui
 
 bool jacobi4_fills_halos(int stage, const FluidLists &L) {
     if (stage == JAC_FIRST_ZERO) return L.rec && L.fill_halos;
+    if (stage == JAC_LAG && L.use_march && L.rowstart) return true;
     return stage == JAC_LAG && L.select_loads && L.rec && L.use_rec && L.fill_halos;
 }
 
@@ -605,6 +606,8 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
     if (stage == JAC_FIRST_ZERO && !jacobi4_fills_halos(stage, L)) stage = JAC_FIRST;  // the state is zero
     if (stage == JAC_FIRST_ZERO) {
         hipLaunchKernelGGL(k_jac4_first_zero, grid, blk, 0, s, g, L.rec, L.n_all, S1, R, cu);
+    } else if (stage == JAC_LAG && L.use_march && L.rowstart) {
+        launch_jacobi4_march(g, L, pc, S0, q0, S1, R, cu, cp, L.march_ty, L.march_zseg, s);
     } else if (stage == JAC_FIRST) {
         hipLaunchKernelGGL(k_jac4<0>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (jacobi4_fills_halos(stage, L)) {

@@ -23,9 +23,14 @@ def say(*a):
     print("[case]", *a, flush=True)
 
 
-def solve(raw, bc, precision, form=None, zs=None, loads=None):
-    old = {k: os.environ.get(k) for k in ("POREMAPS_JACOBI", "POREMAPS_STREAM_ZS", "POREMAPS_SWEEP_LOADS")}
+def solve(raw, bc, precision, form=None, zs=None, loads=None, march=None):
+    old = {k: os.environ.get(k) for k in ("POREMAPS_JACOBI", "POREMAPS_STREAM_ZS", "POREMAPS_SWEEP_LOADS",
+                                          "POREMAPS_MARCH_TY", "POREMAPS_MARCH_ZSEG")}
     try:
+        for k in ("POREMAPS_MARCH_TY", "POREMAPS_MARCH_ZSEG"):
+            os.environ.pop(k, None)
+        if march:
+            os.environ["POREMAPS_MARCH_TY"], os.environ["POREMAPS_MARCH_ZSEG"] = str(march[0]), str(march[1])
         if loads:
             os.environ["POREMAPS_SWEEP_LOADS"] = loads
         else:
@@ -58,17 +63,22 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
     raw = random_geometry((36, 20, 70), 0.4, 71 + bc)
     say("kernel forms", precision, bc)
     ref, ref_fields, _ = solve(raw, bc, precision, "pair")
-    for form, zs, loads in (("list", None, None), ("list", None, "branch"), ("records", None, None),
-                            ("stream", 16, None), ("stream", 64, None), (None, None, None)):
-        res, fields, (name, tried) = solve(raw, bc, precision, form, zs, loads)
-        say(form, zs, loads, name, res.sweeps)
+    forms = [("list", None, None, None), ("list", None, "branch", None), ("records", None, None, None),
+             ("stream", 16, None, None), ("stream", 64, None, None)]
+    if precision == "f32dc":
+        # the z-march of the interleaved lagged pass: tiles of 64 x 4 / 64 x 8,
+        # z segments of 5 planes (several per solve), 128 and the whole depth
+        forms += [("march", None, None, (4, 5)), ("march", None, None, (8, 128)), ("march", None, None, (4, 0))]
+    for form, zs, loads, march in forms + [(None, None, None, None)]:
+        res, fields, (name, tried) = solve(raw, bc, precision, form, zs, loads, march)
+        say(form, zs, loads, march, name, res.sweeps)
         assert res.sweeps == ref.sweeps and res.last.wk33 == ref.last.wk33, (form, zs)
         for a, b, n in zip(fields, ref_fields, ("velx", "vely", "velz", "press")):
             assert np.array_equal(a, b), (form, zs, n)
         if form is None:
-            # FP32+DC runs on the interleaved state: pair and the three lagged
-            # forms (eager, branch, records)
-            assert len(tried) == (4 if precision == "f32dc" else 6) and name in tried
+            # FP32+DC runs on the interleaved state: pair, the three lagged
+            # list forms (eager, branch, records) and the z-march
+            assert len(tried) == (5 if precision == "f32dc" else 6) and name in tried
 
 
 def test_divergent_relaxation_restarts_slower():
