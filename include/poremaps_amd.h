@@ -81,6 +81,14 @@ typedef struct {
                                 copies of the other blocks as the sweep started; 0 0 0 or
                                 1 1 1: one domain.  1 <= P <= nx, Q <= ny, R <= nz, at most
                                 PM_MAX_BLOCKS blocks.  Fast modes ignore it. */
+    int walls;               /* 0: the reference's domain faces -- the halo of a non-periodic
+                                face is fluid, with u = v = w = 0 and p = -1 frozen
+                                (src/geometry.cc:98-106, src/parallelization.cc:193-318);
+                                1: no-slip walls on the non-periodic x and y faces
+                                (boundary_method 1-4): their halo layers are solid, so the
+                                first cells use the one-sided wall stencils of
+                                src/solver.cc:57-91.  Every mode.  No reference
+                                counterpart (opt-in physical model). */
 } pm_params;
 
 #define PM_MAX_BLOCKS 4096

@@ -68,6 +68,7 @@ class OrcParams(C.Structure):
 
 def _declare(L):
     L.orc_geom_halo.argtypes = [_u8p, _I, _I, _I, _I, _u8p]
+    L.orc_geom_halo_walls.argtypes = [_u8p, _I, _I, _I, _I, _I, _u8p]
     L.orc_eval_geometry.argtypes = [_u8p, _I, _I, _I, _i32p]
     L.orc_porosity.argtypes = [_u8p, _I, _I, _I]
     L.orc_porosity.restype = _D
@@ -118,12 +119,13 @@ def _dims(a):
 # reference-faithful pieces
 # ----------------------------------------------------------------------------
 
-def geom_halo(raw: np.ndarray, bc: int) -> np.ndarray:
-    """raw (nz, ny, nx) uint8 0/1 -> haloed geometry (nz+4, ny+4, nx+4)."""
+def geom_halo(raw: np.ndarray, bc: int, walls: int = 0) -> np.ndarray:
+    """raw (nz, ny, nx) uint8 0/1 -> haloed geometry (nz+4, ny+4, nx+4);
+    walls = 1: no-slip walls on the non-periodic x / y faces (solid halos)."""
     raw = np.ascontiguousarray(raw, dtype=np.uint8)
     nz, ny, nx = raw.shape
     G = np.empty((nz + 4, ny + 4, nx + 4), np.uint8)
-    lib().orc_geom_halo(raw, nx, ny, nz, bc, G)
+    lib().orc_geom_halo_walls(raw, nx, ny, nz, bc, walls, G)
     return G
 
 
@@ -274,10 +276,10 @@ def residual(pc, X, q):
 
 
 def solve_fast(raw, bc, mode="jacobi", precision="f32dc", omega=2.5, omega_p=None, eps=1e-6, it_eval=100,
-               max_iter=10 ** 7):
+               max_iter=10 ** 7, walls=0):
     """A whole fast-mode solve of the geometry raw (nz, ny, nx) to the stop
     rule: returns (sweeps, kprev, (p, u, v, w) halo-2 FP64 state, G)."""
-    G = geom_halo(raw, bc)
+    G = geom_halo(raw, bc, walls)
     pc = pack_code(eval_geometry(G))
     st = [a.copy() for a in init_fields(G, bc)]
     pins(G, st[0])

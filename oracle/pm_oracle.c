@@ -92,15 +92,32 @@ DEFINE_EXCHANGE(exchange_f32, float)
  * cell 0 (fluid), then the geometry halo exchange. */
 void orc_geom_halo(const uint8_t *raw, int nx, int ny, int nz, int bc, uint8_t *G)
 {
+    orc_geom_halo_walls(raw, nx, ny, nz, bc, 0, G);
+}
+
+/* The build's opt-in physical walls (no reference code): walls = 1 makes the
+ * halo layers of the non-periodic x and y faces solid, so the domain faces
+ * are no-slip walls handled by the same one-sided stencils as interior solid
+ * voxels (the reference leaves them fluid, src/geometry.cc:98-106).  walls = 0
+ * is orc_geom_halo. */
+void orc_geom_halo_walls(const uint8_t *raw, int nx, int ny, int nz, int bc, int walls, uint8_t *G)
+{
     const int NX = HX(nx), NY = HY(ny), NZ = HZ(nz);
     int px, py, pz;
+    periodic_axes(bc, &px, &py, &pz);
     memset(G, 0, (size_t)NX * NY * NZ);
+    if (walls)
+        for (int i = 0; i < NZ; i++)
+            for (int j = 0; j < NY; j++)
+                for (int k = 0; k < NX; k++) {
+                    const int hx = k < 2 || k >= NX - 2, hy = j < 2 || j >= NY - 2;
+                    if ((hx && !px) || (hy && !py)) G[ix(NX, NY, i, j, k)] = 1;
+                }
     size_t c = 0;
     for (int i = 0; i < nz; i++)
         for (int j = 0; j < ny; j++)
             for (int k = 0; k < nx; k++)
                 G[ix(NX, NY, i + 2, j + 2, k + 2)] = raw[c++];
-    periodic_axes(bc, &px, &py, &pz);
     exchange_u8(G, nx, ny, nz, px, py, pz);
 }
 

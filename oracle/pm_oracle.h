@@ -24,6 +24,9 @@ extern "C" {
 
 /* --- reference-faithful pieces (FP64) ---------------------------------- */
 void orc_geom_halo(const uint8_t *raw, int nx, int ny, int nz, int bc, uint8_t *G);
+/* walls = 1: the halo layers of the non-periodic x / y faces solid (no-slip
+ * domain walls; build-side option, the reference leaves them fluid). */
+void orc_geom_halo_walls(const uint8_t *raw, int nx, int ny, int nz, int bc, int walls, uint8_t *G);
 void orc_eval_geometry(const uint8_t *G, int nx, int ny, int nz, int32_t *code);
 double orc_porosity(const uint8_t *G, int nx, int ny, int nz);
 void orc_init_fields(const uint8_t *G, int nx, int ny, int nz, int bc,

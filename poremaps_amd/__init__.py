@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("dom_interest", C.c_int * 6),
         ("mode", C.c_int), ("precision", C.c_int), ("omega", C.c_double), ("device", C.c_int),
         ("omega_p", C.c_double), ("z_slabs", C.c_int), ("dom_decomposition", C.c_int * 3),
+        ("walls", C.c_int),
     ]
 
 
@@ -126,7 +127,7 @@ def _check(rc, io=False):
 def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_000_000,
                 it_eval=100, it_write=100, solving_algorithm=2, eps=1e-6, porosity=-1.0,
                 dom_interest=(0, 0, 0, 0, 0, 0), mode="reference", precision="f64", omega=1.0,
-                device=0, omega_p=0.0, z_slabs=0, dom_decomposition=(0, 0, 0)) -> Params:
+                device=0, omega_p=0.0, z_slabs=0, dom_decomposition=(0, 0, 0), walls=0) -> Params:
     p = Params()
     lib().pm_params_default(C.byref(p))
     p.nx, p.ny, p.nz = nx, ny, nz
@@ -142,6 +143,7 @@ def make_params(nx, ny, nz, *, boundary_method=0, voxel_size=1.0, max_iter=100_0
     p.omega_p = omega_p
     p.z_slabs = z_slabs
     p.dom_decomposition = (C.c_int * 3)(*dom_decomposition)
+    p.walls = walls
     p.device = device
     return p
 

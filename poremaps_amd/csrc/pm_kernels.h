@@ -27,7 +27,7 @@ struct RedSums {
     double wmax;
 };
 
-void launch_geom_fill(const Grid &g, const uint8_t *raw, uint8_t *G, hipStream_t s);
+void launch_geom_fill(const Grid &g, const uint8_t *raw, uint8_t *G, int walls, hipStream_t s);
 template <typename T>
 void launch_exchange_full(const Grid &g, T *f0, T *f1, T *f2, T *f3, hipStream_t s);
 void launch_eval_geometry(const Grid &g, const uint8_t *G, int32_t *code, uint8_t *pc, hipStream_t s);

@@ -22,15 +22,19 @@ namespace pm {
 // geometry and initial state
 // ---------------------------------------------------------------------------
 
-// src/geometry.cc:98-115: interior bytes shifted by the halo, everything else 0.
-__global__ void k_geom_fill(Grid g, const uint8_t *__restrict__ raw, uint8_t *__restrict__ G) {
+// src/geometry.cc:98-115: interior bytes shifted by the halo, everything else 0
+// (fluid).  walls: the halo layers of the non-periodic x / y faces are solid
+// instead (the build's opt-in no-slip domain walls; periodic halos are then
+// copied by the exchange).
+__global__ void k_geom_fill(Grid g, const uint8_t *__restrict__ raw, uint8_t *__restrict__ G, int walls) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
     if (k >= g.NX) return;
     const bool in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
-    G[g.at(i, j, k)] =
-        in ? raw[((int64_t)(i - 2) * g.ny + (j - 2)) * g.nx + (k - 2)] : (uint8_t)0;
+    const bool hx = k < 2 || k >= g.nx + 2, hy = j < 2 || j >= g.ny + 2;
+    const uint8_t halo = (walls && ((hx && !g.px) || (hy && !g.py))) ? 1 : 0;
+    G[g.at(i, j, k)] = in ? raw[((int64_t)(i - 2) * g.ny + (j - 2)) * g.nx + (k - 2)] : halo;
 }
 
 // Two-layer periodic copy along one axis over the full extent of the other
@@ -520,8 +524,8 @@ static inline dim3 row_grid(const Grid &g) {
 }
 static inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_geom_fill(const Grid &g, const uint8_t *raw, uint8_t *G, hipStream_t s) {
-    hipLaunchKernelGGL(k_geom_fill, row_grid(g), dim3(64), 0, s, g, raw, G);
+void launch_geom_fill(const Grid &g, const uint8_t *raw, uint8_t *G, int walls, hipStream_t s) {
+    hipLaunchKernelGGL(k_geom_fill, row_grid(g), dim3(64), 0, s, g, raw, G, walls);
 }
 
 template <typename T>

@@ -110,6 +110,7 @@ static int check_params(const pm_params *p) {
                     dd[2], p->nx, p->ny, p->nz);
     if (p->z_slabs > 1 && (dd[0] > 1 || dd[1] > 1 || dd[2] > 1))
         return fail(PM_E_ARG, "give either z_slabs or dom_decomposition");
+    if (p->walls < 0 || p->walls > 1) return fail(PM_E_ARG, "walls must be 0 or 1 (got %d)", p->walls);
     int d[3];
     ref_dims(p, d);
     if ((int64_t)d[0] * d[1] * d[2] > PM_MAX_BLOCKS)
@@ -470,7 +471,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         (void)hipFree(raw_dev);
         return fail(PM_E_HIP, "geometry upload: %s", hipGetErrorString(e));
     }
-    pm::launch_geom_fill(g, raw_dev, c->G, c->stream);
+    pm::launch_geom_fill(g, raw_dev, c->G, c->prm.walls, c->stream);
     pm::launch_exchange_full<uint8_t>(g, c->G, nullptr, nullptr, nullptr, c->stream);
     HIPCHK(hipMemsetAsync(c->pc, 0xff, (size_t)g.total, c->stream));
     HIPCHK(hipMemsetAsync(c->code, 0, sizeof(int32_t) * (size_t)g.total, c->stream));

@@ -15,6 +15,8 @@
 //   --omega=<w>                         POREMAPS_OMEGA (0, the fast modes' default: the pair by porosity)
 //   --omega-p=<w>                       POREMAPS_OMEGA_P (pressure relaxation; default omega)
 //   --device=<n>                        POREMAPS_DEVICE
+//   --walls=reference|noslip            POREMAPS_WALLS (noslip: the non-periodic x / y faces
+//                                       are no-slip walls; the reference leaves them open)
 // Deviation: hard errors (missing file, bad .inp, bad geometry, HIP failure)
 // exit non-zero instead of continuing.
 #include "../../include/poremaps_amd.h"
@@ -101,6 +103,11 @@ int main(int argc, char **argv) {
     if (knob(flags, "--omega", "POREMAPS_OMEGA", &v)) prm.omega = std::atof(v.c_str());
     if (knob(flags, "--omega-p", "POREMAPS_OMEGA_P", &v)) prm.omega_p = std::atof(v.c_str());
     if (knob(flags, "--device", "POREMAPS_DEVICE", &v)) prm.device = std::atoi(v.c_str());
+    if (knob(flags, "--walls", "POREMAPS_WALLS", &v)) {
+        if (v == "reference") prm.walls = 0;
+        else if (v == "noslip") prm.walls = 1;
+        else { std::fprintf(stderr, "unknown walls %s\n", v.c_str()); return 2; }
+    }
 
     // dom_decomposition P Q R in the reference mode: the iterates of
     // `mpirun -np P*Q*R`, each block swept against its halo copies of the

@@ -37,6 +37,9 @@ def test_library_loads_and_reports_errors_without_gpu_work():
     bad = pm.make_params(8, 8, 8, boundary_method=7)
     assert pm.device_bytes(bad) == -1
     assert b"boundary_method" in L.pm_last_error()
+    assert pm.device_bytes(pm.make_params(8, 8, 8, walls=1)) == pm.device_bytes(p)
+    assert pm.device_bytes(pm.make_params(8, 8, 8, walls=2)) == -1
+    assert b"walls" in L.pm_last_error()
 
 
 def test_read_inp_reference_files():
