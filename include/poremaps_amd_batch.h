@@ -45,6 +45,9 @@ typedef struct {
 #define PM_SAMPLE_CONVERGED 1
 #define PM_SAMPLE_MAX_ITER 2
 #define PM_SAMPLE_DIVERGED 3
+#define PM_SAMPLE_REJECTED 4   /* pm_serve responses only: the request was not admitted (its three
+                                  solves exceed the whole memory budget, or pm_batch_add refused
+                                  its geometry or parameters); its k is zero */
 
 void pm_batch_params_default(pm_batch_params *p);
 int pm_batch_create(const pm_batch_params *p, pm_batch **out);

@@ -87,6 +87,7 @@ typedef struct {
     int intervals;           /* batch steps */
     double admit_s;          /* host time spent admitting requests (rotation, upload, first sweep) */
     double step_s;           /* host time spent in batch steps (GPU intervals) */
+    int rejected;            /* requests answered PM_SAMPLE_REJECTED (not in the latency figures) */
 } pm_serve_stats;
 
 int pm_serve(const pm_serve_params *p, const pm_request *req, int n, pm_response *resp, pm_serve_stats *stats);

@@ -268,7 +268,8 @@ class BatchParams(C.Structure):
                 ("budget_bytes", C.c_int64), ("omega_p", C.c_double)]
 
 
-SAMPLE_STATES = {0: "running", 1: "converged", 2: "max_iter", 3: "diverged"}
+SAMPLE_STATES = {0: "running", 1: "converged", 2: "max_iter", 3: "diverged", 4: "rejected"}
+SAMPLE_REJECTED = 4
 
 
 def _batch_lib():
@@ -409,7 +410,7 @@ class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("completed", C.c_int), ("latency_p50_s", C.c_double),
                 ("latency_p99_s", C.c_double), ("latency_mean_s", C.c_double), ("latency_max_s", C.c_double),
                 ("solves_per_hour", C.c_double), ("max_running", C.c_int), ("peak_bytes", C.c_int64),
-                ("intervals", C.c_int), ("admit_s", C.c_double), ("step_s", C.c_double)]
+                ("intervals", C.c_int), ("admit_s", C.c_double), ("step_s", C.c_double), ("rejected", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

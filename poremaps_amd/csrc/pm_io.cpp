@@ -8,6 +8,7 @@
 #include <unistd.h>
 
 #include <cerrno>
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -51,7 +52,7 @@ bool to_int(const std::string &s, int *v) {
     char *end = nullptr;
     errno = 0;
     long x = std::strtol(s.c_str(), &end, 10);
-    if (end == s.c_str() || *end != '\0' || errno) return false;
+    if (end == s.c_str() || *end != '\0' || errno || x < INT_MIN || x > INT_MAX) return false;
     *v = (int)x;
     return true;
 }
@@ -151,7 +152,7 @@ static int write_at_zero(const char *path, const void *data, size_t bytes) {
 }
 
 extern "C" int pm_write_raw_f64(const char *path, const double *v, int64_t n, double scale, int rescale) {
-    if (!path || (!v && n)) return io_fail(PM_E_ARG, "null argument");
+    if (!path || (!v && n) || n < 0) return io_fail(PM_E_ARG, "null argument or negative size");
     if (!rescale) return write_at_zero(path, v, sizeof(double) * (size_t)n);
     std::vector<double> t((size_t)n);
     for (int64_t q = 0; q < n; q++) t[(size_t)q] = v[q] * scale;
@@ -159,7 +160,7 @@ extern "C" int pm_write_raw_f64(const char *path, const double *v, int64_t n, do
 }
 
 extern "C" int pm_write_raw_i32(const char *path, const int32_t *v, int64_t n) {
-    if (!path || (!v && n)) return io_fail(PM_E_ARG, "null argument");
+    if (!path || (!v && n) || n < 0) return io_fail(PM_E_ARG, "null argument or negative size");
     return write_at_zero(path, v, sizeof(int32_t) * (size_t)n);
 }
 
@@ -178,7 +179,7 @@ extern "C" int pm_write_log_row(const char *path, const pm_log_row *r, int first
 }
 
 extern "C" int pm_read_raw_u8(const char *path, uint8_t *out, int64_t n) {
-    if (!path || !out) return io_fail(PM_E_ARG, "null argument");
+    if (!path || !out || n < 0) return io_fail(PM_E_ARG, "null argument or negative size");
     FILE *f = std::fopen(path, "rb");
     if (!f) return io_fail(PM_E_ARG, "cannot open geometry %s: %s", path, std::strerror(errno));
     size_t got = std::fread(out, 1, (size_t)n, f);

@@ -157,6 +157,14 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
     auto lane_of = [&](const pm_request &q) {
         return two && (int64_t)q.nx * q.ny * q.nz <= p->small_cells ? 1 : 0;
     };
+    int rejected = 0;
+    auto reject = [&](int id) {
+        for (int r = 0; r < 3; r++) resp[id].state[r] = PM_SAMPLE_REJECTED;
+        resp[id].admit_s = resp[id].done_s = now();
+        left[id] = 0;
+        rejected++;
+        done++;
+    };
     // one interval of a lane, then its finished solves evicted and answered
     auto step_lane = [&](Lane &L) -> int {
         int still = 0;
@@ -207,9 +215,15 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
                 for (int a = 0; a < 3; a++) d[a] = nn[kNewAxis[r][a]];
                 need += pm_batch_sample_bytes(&p->batch, d[0], d[1], d[2]);
             }
+            const int id = order[next];
+            if (need > budget) {
+                // can never fit, even alone: answered as rejected
+                reject(id);
+                next++;
+                continue;
+            }
             if (used + need > budget && running > 0) break;  // back-pressure
             if (p->max_solves > 0 && running > 0 && running + 3 > p->max_solves) break;
-            const int id = order[next];
             resp[id].admit_s = now();
             const double t_admit = now();
             for (int r = 0; r < 3 && rc == PM_OK; r++) {
@@ -223,11 +237,27 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
                     rotate_box(q.dom_interest, r, box);
                     rc = pm_batch_set_dom_interest(L.b, slot, box);
                 }
-                if (rc != PM_OK) break;
+                if (rc != PM_OK) {
+                    if (slot >= 0) pm_batch_remove(L.b, slot);
+                    break;
+                }
                 if ((int)L.live.size() <= slot) L.live.resize(slot + 1, Live{-1, -1});
                 L.live[slot] = Live{id, r};
                 L.running++;
                 running++;
+            }
+            if (rc == PM_E_MEMORY || rc == PM_E_ARG || rc == PM_E_GEOMETRY) {
+                // this request alone is refused: its admitted rotations leave
+                // the batch, the others keep being served
+                for (size_t s = 0; s < L.live.size(); s++)
+                    if (L.live[s].req == id) {
+                        pm_batch_remove(L.b, (int)s);
+                        L.live[s] = Live{-1, -1};
+                        L.running--;
+                        running--;
+                    }
+                reject(id);
+                rc = PM_OK;
             }
             next++;
             admit_time += now() - t_admit;
@@ -273,7 +303,7 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
         std::vector<double> lat;
         double first = 1e300, last = 0.0;
         for (int i = 0; i < n; i++) {
-            if (resp[i].done_s <= 0.0) continue;
+            if (resp[i].done_s <= 0.0 || resp[i].state[0] == PM_SAMPLE_REJECTED) continue;
             lat.push_back(resp[i].done_s - resp[i].arrival_s);
             first = std::min(first, resp[i].arrival_s);
             last = std::max(last, resp[i].done_s);
@@ -300,6 +330,7 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
         stats->intervals = intervals;
         stats->admit_s = admit_time;
         stats->step_s = step_time;
+        stats->rejected = rejected;
     }
     return rc;
 }

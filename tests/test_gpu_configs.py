@@ -12,7 +12,8 @@
 * Full size: the 256^3 sphere pack and the 512^3 Gaussian field, FP32 +
   FP64-refined k_zz against the FP64 solve at the same relaxation (bar 1e-5).
 * The reference-order Gauss-Seidel (solving_algorithm 2) to eps 1e-6 against
-  the fast modes at eps 1e-6 (bar 1e-4 on k_zz).
+  the fast modes at eps 1e-6 (bar 1e-4 on k_zz), on small media of both kinds
+  and on the 256^3 sphere pack itself.
 
 Reductions are compensated parallel sums on the GPU and serial sums in the
 oracle, so k is compared to 1e-12 relative; fields and sweep counts exactly.
@@ -250,6 +251,24 @@ def test_fast_modes_within_1e4_of_reference_gs(medium):
     say("  reference GS", gs.sweeps, gs.last.wk33)
     for mode, pair in (("jacobi", pair_by_porosity(raw)), ("sor3", (1.8, 6.0))):
         with pm.Solver(raw, boundary_method=bc, voxel_size=1e-5, mode=mode, precision="f32dc", omega=pair[0],
+                       omega_p=pair[1], eps=1e-6) as s:
+            res, _ = s.solve()
+        say(" ", mode, pair, res.sweeps, res.last.wk33)
+        assert abs(res.last.wk33 / gs.last.wk33 - 1) < 1e-4, (mode, res.last.wk33, gs.last.wk33)
+
+
+def test_sphere_pack_256_fast_modes_within_1e4_of_reference_gs():
+    """The sphere-pack workload at full size (256^3, phi 0.35, bc 2): the
+    reference's lexicographic Gauss-Seidel (solving_algorithm 2, exact order,
+    about a minute) to eps 1e-6, and the FP32 + FP64-refined Jacobi and
+    3-colour SOR within 1e-4 of its k_zz."""
+    raw = geo.sphere_pack(256, 0.35, 6, 12, seed=1)
+    say("sphere pack 256^3: reference GS to eps")
+    with pm.Solver(raw, boundary_method=2, voxel_size=1e-5, solving_algorithm=2, eps=1e-6) as s:
+        gs, _ = s.solve()
+    say("  reference GS", gs.sweeps, gs.last.wk33, f"{gs.seconds:.1f}s")
+    for mode, pair in (("jacobi", (2.5, 7.0)), ("sor3", (1.8, 6.0))):
+        with pm.Solver(raw, boundary_method=2, voxel_size=1e-5, mode=mode, precision="f32dc", omega=pair[0],
                        omega_p=pair[1], eps=1e-6) as s:
             res, _ = s.solve()
         say(" ", mode, pair, res.sweeps, res.last.wk33)

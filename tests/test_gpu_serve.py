@@ -149,3 +149,28 @@ def test_serve_dom_interest_tensor():
         assert (kd[ax, az], kd[ay, az], kd[az, az]) == (last["k13"], last["k23"], last["k33"])
         assert last["k33"] != 0.0
 
+
+
+def test_serve_rejects_only_the_requests_it_cannot_admit():
+    """A request larger than the whole memory budget, and one whose geometry
+    holds a byte other than 0 / 1, are answered PM_SAMPLE_REJECTED with zero k;
+    every other request in the stream is still served, equal to its tensor
+    solved alone, and the rejected ones stay out of the latency figures."""
+    small = [geo.sphere_pack((16, 16, 16), 0.5, 2, 3, seed=s, axes=(0, 1, 2)) for s in (21, 22)]
+    big = geo.sphere_pack((40, 40, 40), 0.5, 2, 3, seed=23, axes=(0, 1, 2))
+    bad = small[0].copy()
+    bad[3, 4, 5] = 2
+    with pm.Batch() as b:
+        needs = [sum(b.sample_bytes(np.transpose(m, t).shape) for t in ((0, 1, 2), (2, 0, 1), (1, 2, 0)))
+                 for m in (small[0], big)]
+    assert needs[1] > needs[0]
+    reqs = [(small[0], 0, 1e-5, 1e-6, 0.0), (big, 0, 1e-5, 1e-6, 0.01), (bad, 0, 1e-5, 1e-6, 0.02),
+            (small[1], 0, 1e-5, 1e-6, 0.03)]
+    say("serve rejects", needs)
+    resp, st = pm.serve(reqs, realtime=False, budget_bytes=needs[0])
+    assert st.rejected == 2 and st.completed == 2
+    for i in (1, 2):
+        assert list(resp[i].state) == [pm.SAMPLE_REJECTED] * 3 and not resp[i].tensor.any()
+    for i in (0, 3):
+        k, res = pm.tensor(reqs[i][0], boundary_method=0, voxel_size=1e-5)
+        assert list(resp[i].state) == [1, 1, 1] and np.array_equal(resp[i].tensor, k)
