@@ -108,6 +108,8 @@ struct pm_ctx {
     int stream_zs = 64;
     // z-march form: tile rows (4 or 8) and planes per z segment (0: all)
     int march_ty = 4, march_zseg = 128;
+    int sor_chunk = 0;          // FluidLists::sor_chunk (POREMAPS_SOR_CHUNK)
+    bool sor_fused = true;      // FluidLists::sor_fused (POREMAPS_SOR_STAGE=seq: false)
     Tuner tuner;
     // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence
     bool lagged = false;

@@ -103,6 +103,16 @@ struct FluidLists {
     int32_t *rowstart = nullptr;
     bool use_march = false;
     int march_ty = 4, march_zseg = 128;
+    // FP32+DC 3-colour SOR in z-chunks of sor_chunk planes (0: three whole
+    // passes): colour 1 trails colour 0 by 3 planes and colour 2 trails it by
+    // 6, so a chunk is re-read by the next colour while it is still cached.
+    // The colour lists are then in (z, y, x) order and plane_off[q * (nz + 1)
+    // + i] is the position, within colour list q, of its first voxel on plane
+    // i.  sor_fused: the three colours of a stage (chunk k, k - 1, k - 2) in
+    // one launch; else one launch per colour and chunk.
+    int sor_chunk = 0;
+    bool sor_fused = true;
+    std::vector<int32_t> plane_off;
 };
 
 // FP64 fast modes (per-field state p, u, v, w): the velocity + pressure

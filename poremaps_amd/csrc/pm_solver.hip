@@ -189,6 +189,10 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     // 256 x whole 1.467, plain (z, y, x) order 1.496, 64 x 192 1.507, 64 x 64
     // 1.558
     if (p->mode == PM_MODE_SOR3 && p->precision == PM_PREC_F32DC) c->prm_tile_x = 0;
+    if (const char *sc = std::getenv("POREMAPS_SOR_CHUNK")) c->sor_chunk = std::max(0, std::atoi(sc));
+    if (const char *ss = std::getenv("POREMAPS_SOR_STAGE")) c->sor_fused = std::strcmp(ss, "seq") != 0;
+    // the z-chunked SOR walks the colour lists plane by plane
+    if (p->mode == PM_MODE_SOR3 && p->precision == PM_PREC_F32DC && c->sor_chunk > 0) c->prm_tile_y = 0;
     if (const char *ty = std::getenv("POREMAPS_LIST_TILE")) c->prm_tile_y = std::max(0, std::atoi(ty));
     if (const char *tx = std::getenv("POREMAPS_LIST_TILE_X")) c->prm_tile_x = std::max(0, std::atoi(tx));
     if (const char *ri = std::getenv("POREMAPS_REF")) {
@@ -442,6 +446,25 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
                 c->lists.off_col[q] = (int)all.size();
                 c->lists.n_col[q] = (int)col[q].size();
                 all.insert(all.end(), col[q].begin(), col[q].end());
+            }
+            if (c->lists.split && c->sor_chunk > 0 && ty == g.ny && tx == g.nx) {
+                // plane offsets of the colour lists, which are in (z, y, x)
+                // order here
+                const int np = g.nz + 1;
+                c->lists.plane_off.assign((size_t)3 * np, 0);
+                for (int q = 0; q < 3; q++) {
+                    int32_t *po = c->lists.plane_off.data() + (size_t)q * np;
+                    size_t t = 0;
+                    for (int i = 0; i < g.nz; i++) {
+                        po[i] = (int32_t)t;
+                        while (t < col[q].size() && (col[q][t] - g.base) / g.sy == i + 2) t++;
+                    }
+                    po[g.nz] = (int32_t)t;
+                    if (t != col[q].size()) c->lists.plane_off.clear();  // not in plane order
+                    if (c->lists.plane_off.empty()) break;
+                }
+                c->lists.sor_chunk = c->lists.plane_off.empty() ? 0 : c->sor_chunk;
+                c->lists.sor_fused = c->sor_fused;
             }
         }
         c->lists.n_all = (int)all.size();
@@ -1035,6 +1058,11 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
                               c->jac_impl == JI_MARCH ? c->march_zseg : c->stream_zs, c->march_ty};
     const char *sor = !c->fast() ? "n/a" : !c->dc() ? "list-colour" : c->lists.split ? "interleaved-colour-split"
                                                                                     : "interleaved-colour";
+    if (c->fast() && c->dc() && c->lists.split && c->lists.sor_chunk > 0) {
+        std::snprintf(buf, sizeof buf, "interleaved-colour-split-z%d%s", c->lists.sor_chunk,
+                      c->lists.sor_fused ? "" : "-seq");
+        sor = buf;
+    }
     std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(c, cur, buf, sizeof buf) : sor);
     if (tried && tried_len > 0) {
         std::string t;

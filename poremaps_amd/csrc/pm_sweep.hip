@@ -119,6 +119,23 @@ __global__ __launch_bounds__(kListBlock) void k_colour4_split(Grid g, const uint
     colour_voxel4_split(g, t0 + t, rec, S, q, R, cu, cp, 0.f);
 }
 
+// Up to three colours' list ranges of one stage of the z-chunked SOR in one
+// launch: blocks [b[q], b[q + 1]) take colour q's positions t0[q] ..
+// t0[q] + n[q] - 1.  The ranges lie on planes far enough apart that no cell
+// of one is read or written by another (launch_colour4).
+struct ColourStage {
+    int t0[3], n[3], b[3];
+};
+
+__global__ __launch_bounds__(kListBlock) void k_colour4_stage(Grid g, const uint2 *__restrict__ rec, ColourStage st,
+                                                              float4 *S, float *q, Res4 R, float cu, float cp) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int col = b >= st.b[2] ? 2 : (b >= st.b[1] ? 1 : 0);
+    const int t = (b - st.b[col]) * kListBlock + threadIdx.x;
+    if (t >= st.n[col]) return;
+    colour_voxel4_split(g, st.t0[col] + t, rec, S, q, R, cu, cp, 0.f);
+}
+
 __global__ __launch_bounds__(kListBlock) void k_colour_records(Grid g, const int32_t *__restrict__ list, int n,
                                                                const uint8_t *__restrict__ pc, uint2 *rec) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
@@ -626,8 +643,65 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
     }
 }
 
+// Colour q's list range on planes [z0, z1) (clipped to the interior).
+static void colour_planes(const Grid &g, const FluidLists &L, int q, int z0, int z1, int &t0, int &n) {
+    z0 = std::max(z0, 0);
+    z1 = std::min(z1, g.nz);
+    t0 = L.off_col[q];
+    n = 0;
+    if (z1 <= z0) return;
+    const int32_t *po = L.plane_off.data() + (size_t)q * (g.nz + 1);
+    t0 += po[z0];
+    n = po[z1] - po[z0];
+}
+
+// The three colour passes of one SOR sweep in z-chunks of Z planes.  A cell
+// of colour q reads the cells within 2 planes of its own, of the two other
+// colours, as the three whole passes leave them: colour 1 on plane i needs
+// colour 0 finished on planes <= i + 2, and colour 0 on plane i + 2 must
+// still read colour 1's old values on plane i.  Colour 1 therefore trails
+// colour 0 by 3 planes and colour 2 trails colour 1 by 3: stage k updates
+// colour 0 on [kZ, kZ + Z), colour 1 on [(k-1)Z - 3, kZ - 3) and colour 2 on
+// [(k-2)Z - 6, (k-1)Z - 6) (fused: one launch; their reads and writes are
+// at least one plane apart), or, in sequence, colour q on [kZ - 3q,
+// kZ + Z - 3q) for q = 0, 1, 2.  Both give the three whole passes' values
+// bit for bit; halos are frozen during a sweep as there.
+static void launch_colour4_chunked(const Grid &g, const FluidLists &L, float4 *S, float *q, Res4 R, float cu,
+                                   float cp, hipStream_t s) {
+    const int Z = L.sor_chunk;
+    for (int k = 0;; k++) {
+        ColourStage st;
+        int lo[3];
+        for (int col = 0; col < 3; col++) lo[col] = L.sor_fused ? (k - col) * Z - 3 * col : k * Z - 3 * col;
+        if (lo[2] >= g.nz) break;
+        if (L.sor_fused) {
+            int nb = 0;
+            for (int col = 0; col < 3; col++) {
+                colour_planes(g, L, col, lo[col], lo[col] + Z, st.t0[col], st.n[col]);
+                st.b[col] = nb;
+                nb += (int)lblocks(st.n[col]);
+            }
+            if (nb > 0)
+                hipLaunchKernelGGL(k_colour4_stage, dim3((unsigned)nb), dim3(kListBlock), 0, s, g, L.rec, st, S, q,
+                                   R, cu, cp);
+        } else {
+            for (int col = 0; col < 3; col++) {
+                int t0, n;
+                colour_planes(g, L, col, lo[col], lo[col] + Z, t0, n);
+                if (n > 0)
+                    hipLaunchKernelGGL(k_colour4_split, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.rec, n, t0,
+                                       S, q, R, cu, cp);
+            }
+        }
+    }
+}
+
 void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
                     float cp, hipStream_t s) {
+    if (L.split && L.sor_chunk > 0 && !L.plane_off.empty()) {
+        launch_colour4_chunked(g, L, S, q, R, cu, cp, s);
+        return;
+    }
     for (int col = 0; col < 3; col++) {
         const int n = L.n_col[col];
         if (n == 0) continue;

@@ -329,6 +329,28 @@ def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
             assert np.array_equal(a[m], b[m]), f"{name} differs"
 
 
+@pytest.mark.parametrize("stage", ["fused", "seq"])
+@pytest.mark.parametrize("chunk", [1, 3, 16])
+@pytest.mark.parametrize("shape,bc", [((13, 9, 11), 0), ((40, 6, 7), 1), ((27, 5, 9), 2), ((7, 8, 6), 3),
+                                      ((3, 6, 7), 4), ((64, 5, 4), 0)])
+def test_sor_f32dc_z_chunks_bit_exact(stage, chunk, shape, bc):
+    """FP32+DC 3-colour SOR in z-chunks (colour 1 trailing colour 0 by 3
+    planes, colour 2 by 6), the three colours of a stage in one launch or one
+    launch each: the oracle's three whole passes bit for bit, for chunks of
+    one plane, of a few planes and of more planes than the domain holds."""
+    raw = random_geometry(shape, 0.3, sum(shape) + bc + chunk)
+    say("sor f32dc z-chunks", stage, chunk, shape, bc)
+    ref = oracle_dc(raw, bc, "sor3", 1.8, 16, 5, 6.0)
+    with env("POREMAPS_SOR_CHUNK", str(chunk)), env("POREMAPS_SOR_STAGE", stage), \
+            pm.Solver(raw, boundary_method=bc, mode="sor3", omega=1.8, omega_p=6.0, precision="f32dc",
+                      it_eval=5) as s:
+        assert s.kernel_choice()[0] == f"interleaved-colour-split-z{chunk}" + ("-seq" if stage == "seq" else "")
+        s.step(16)
+        m = nonc(ref[0])
+        for name, a, b in zip("puvw", s.state_halo(), ref):
+            assert np.array_equal(a[m], b[m]), f"{name} differs"
+
+
 @pytest.mark.parametrize("halos", ["fused", "separate", "march4", "march8"])
 @pytest.mark.parametrize("shape,bc", [((2, 5, 6), 0), ((3, 3, 7), 1), ((6, 2, 3), 0), ((5, 6, 7), 2),
                                       ((4, 5, 6), 4), ((7, 6, 9), 3), ((9, 8, 40), 0), ((11, 13, 70), 0),
