@@ -112,6 +112,9 @@ struct FluidLists {
     // one launch; else one launch per colour and chunk.
     int sor_chunk = 0;
     bool sor_fused = true;
+    // FP32+DC record pass: waves per SIMD its registers must allow (0, 6, 8;
+    // 0: the compiler's allocation)
+    int rec_occ = 0;
     std::vector<int32_t> plane_off;
 };
 

@@ -158,10 +158,13 @@ __global__ __launch_bounds__(kListBlock) void k_jac_rec(Grid g, const uint2 *__r
 
 // Lagged pass from the per-voxel records (lag_voxel4_rec); IMAGES: the pass
 // also writes the ghost fill's periodic copies and halo pins of its cells.
-template <bool IMAGES>
-__global__ __launch_bounds__(kListBlock) void k_jac4_rec(Grid g, const uint2 *__restrict__ rec, int n,
-                                                         const float4 *__restrict__ S0, const float *__restrict__ q0,
-                                                         float4 *S1, Res4 R, float cu, float cp) {
+// OCC: waves per SIMD the register allocation must allow (0: the compiler's
+// choice, 81 VGPRs = 5 waves); the same code at higher occupancy, for the
+// autotuner.
+template <bool IMAGES, int OCC>
+__global__ __launch_bounds__(kListBlock, OCC > 0 ? OCC : 1) void k_jac4_rec(
+    Grid g, const uint2 *__restrict__ rec, int n, const float4 *__restrict__ S0, const float *__restrict__ q0,
+    float4 *S1, Res4 R, float cu, float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     lag_voxel4_rec<IMAGES>(g, t, rec, S0, q0, S1, R, cu, cp, 0.f);
@@ -628,9 +631,14 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
     } else if (stage == JAC_FIRST) {
         hipLaunchKernelGGL(k_jac4<0>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (jacobi4_fills_halos(stage, L)) {
-        hipLaunchKernelGGL(k_jac4_rec<true>, grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
+        if (L.rec_occ == 8)
+            hipLaunchKernelGGL((k_jac4_rec<true, 8>), grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
+        else if (L.rec_occ == 6)
+            hipLaunchKernelGGL((k_jac4_rec<true, 6>), grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
+        else
+            hipLaunchKernelGGL((k_jac4_rec<true, 0>), grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
     } else if (stage == JAC_LAG && L.select_loads && L.rec && L.use_rec) {
-        hipLaunchKernelGGL(k_jac4_rec<false>, grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
+        hipLaunchKernelGGL((k_jac4_rec<false, 0>), grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu, cp);
     } else if (stage == JAC_LAG && L.select_loads) {
         hipLaunchKernelGGL(k_jac4<4>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (stage == JAC_LAG) {

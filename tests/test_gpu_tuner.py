@@ -77,8 +77,9 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
             assert np.array_equal(a, b), (form, zs, n)
         if form is None:
             # FP32+DC runs on the interleaved state: pair, the three lagged
-            # list forms (eager, branch, records) and the z-march
-            assert len(tried) == (5 if precision == "f32dc" else 6) and name in tried
+            # list forms (eager, branch, records; records also at 6 and 8
+            # waves per SIMD) and the z-march
+            assert len(tried) == (7 if precision == "f32dc" else 6) and name in tried
 
 
 def test_divergent_relaxation_restarts_slower():
