@@ -334,6 +334,18 @@ def bench_batch(a):
                 print(f"[bench] {label} {i}: {dt:.2f}s", file=sys.stderr, flush=True)
             line[label] = {"seconds": round(tot, 3), "solves_per_hour": round(len(media) / tot * 3600.0, 1),
                            "batch_speedup": round(tot / el, 2)}
+    if a.accuracy:
+        # each sample against its FP64 solve with the same relaxation pair
+        # (omega 0: the library picks the same pair from the porosity), alone
+        errs = []
+        for i, (m, (_, r)) in enumerate(zip(media, st)):
+            _, ref = _solve_alone(m["raw"], 0, 1e-5, it_eval=a.it_eval, mode="jacobi", precision="f64",
+                                  omega=a.omega, omega_p=a.omega_p)
+            errs.append(abs(r.last.wk33 / ref.last.wk33 - 1.0))
+            print(f"[bench] fp64 same relaxation {i}: k rel err {errs[-1]:.2e}, sweeps {r.sweeps} / {ref.sweeps}",
+                  file=sys.stderr, flush=True)
+        line["k_rel_err_vs_fp64_same_relaxation"] = {"max": max(errs), "median": float(np.median(errs)),
+                                                     "samples": len(errs)}
     print(json.dumps(line), flush=True)
 
 
@@ -392,6 +404,22 @@ def bench_serve(a):
         line["one_at_a_time_fp64_plain"] = {"requests": n_base, "seconds": round(tot, 3),
                                             "solves_per_hour": round(sph, 1)}
         line["speedup_vs_plain"] = round(st.solves_per_hour / sph, 2)
+    if a.accuracy:
+        # responses against the FP64 tensor with the same relaxation pairs
+        n_acc = min(a.baseline_requests or 8, len(reqs))
+        diag, off = [], []
+        for i in range(n_acc):
+            raw, bc = reqs[i][0], reqs[i][1]
+            k64, _ = pm.tensor(raw, boundary_method=bc, voxel_size=1e-5, precision="f64", omega=a.omega,
+                               omega_p=a.omega_p, it_eval=a.it_eval, max_iter=a.max_iter)
+            k = resp[i].tensor
+            d = np.diag(k64)
+            diag.append(float(np.max(np.abs(np.diag(k) / d - 1.0))))
+            off.append(float(np.max(np.abs((k - k64) - np.diag(np.diag(k - k64)))) / np.max(np.abs(d))))
+            print(f"[bench] fp64 tensor {i}: diagonal rel err {diag[-1]:.2e}, off-diagonal {off[-1]:.2e}",
+                  file=sys.stderr, flush=True)
+        line["k_err_vs_fp64_same_relaxation"] = {"requests": n_acc, "diagonal_rel_max": max(diag),
+                                                 "off_diagonal_rel_to_diag_max": max(off)}
     print(json.dumps(line), flush=True)
 
 
@@ -412,7 +440,9 @@ def main():
     ap.add_argument("--tte", action="store_true", help="time-to-eps of the selected mode beside the plain solver")
     ap.add_argument("--no-tte", action="store_true", help="skip the default time-to-eps set")
     ap.add_argument("--accuracy", action="store_true",
-                    help="with --tte: FP64 at the same relaxation and an eps 1e-9 reference solve")
+                    help="with --tte: FP64 at the same relaxation and an eps 1e-9 reference solve; configs 3 "
+                         "and 4: every batch sample / the first --baseline-requests (8) responses against the "
+                         "FP64 solves with the same relaxation")
     ap.add_argument("--reference-eps", type=float, default=1e-9,
                     help="--accuracy: eps of the tightly converged reference solve (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
