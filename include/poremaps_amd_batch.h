@@ -73,6 +73,17 @@ int pm_batch_set_dom_interest(pm_batch *b, int slot, const int box[6]);
  * tests; *n_running = samples still running afterwards. */
 int pm_batch_step(pm_batch *b, int *n_running);
 
+/* pm_batch_step in two halves, for callers that overlap host work or another
+ * batch (on its own stream) with the interval: pm_batch_launch enqueues the
+ * interval and the read-back of its sums on the batch's stream and returns;
+ * pm_batch_ready gives 1 once that work has finished on the GPU, else 0;
+ * pm_batch_finish waits for it and runs the convergence tests.  Between
+ * launch and finish the batch takes no other call (PM_E_STATE).  No
+ * counterpart in the reference. */
+int pm_batch_launch(pm_batch *b);
+int pm_batch_ready(pm_batch *b);
+int pm_batch_finish(pm_batch *b, int *n_running);
+
 /* State and result of a sample (result.last: permeability at the last
  * interval end). */
 int pm_batch_status(pm_batch *b, int slot, int *state, pm_result *res);

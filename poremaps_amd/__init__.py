@@ -283,6 +283,9 @@ def _batch_lib():
         L.pm_batch_add.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                    C.c_double, P(C.c_int)]
         L.pm_batch_step.argtypes = [C.c_void_p, P(C.c_int)]
+        L.pm_batch_launch.argtypes = [C.c_void_p]
+        L.pm_batch_ready.argtypes = [C.c_void_p]
+        L.pm_batch_finish.argtypes = [C.c_void_p, P(C.c_int)]
         L.pm_batch_status.argtypes = [C.c_void_p, C.c_int, P(C.c_int), P(Result)]
         L.pm_batch_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         L.pm_batch_remove.argtypes = [C.c_void_p, C.c_int]
@@ -333,6 +336,19 @@ class Batch:
     def step(self) -> int:
         n = C.c_int()
         _check(_batch_lib().pm_batch_step(self._h, C.byref(n)))
+        return n.value
+
+    def launch(self):
+        """Enqueue one interval on the batch's stream and return (finish()
+        completes it; ready() tells whether the GPU is done with it)."""
+        _check(_batch_lib().pm_batch_launch(self._h))
+
+    def ready(self) -> bool:
+        return bool(_batch_lib().pm_batch_ready(self._h))
+
+    def finish(self) -> int:
+        n = C.c_int()
+        _check(_batch_lib().pm_batch_finish(self._h, C.byref(n)))
         return n.value
 
     def status(self, slot: int):

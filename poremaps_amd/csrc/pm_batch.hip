@@ -296,6 +296,8 @@ struct pm_batch {
     bool shape_dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool in_flight = false;          // launched, not yet finished
+    bool launched_empty = false;     // the launched step had no running sample
     bool dc() const { return prm.precision == PM_PREC_F32DC; }
 };
 
@@ -389,6 +391,7 @@ static int prologue(pm_ctx *c) {
 extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int nz, int bc, double dx, double phi,
                             int *slot) {
     if (!b || !raw || !slot) return pm_fail(PM_E_ARG, "null argument");
+    if (b->in_flight) return pm_fail(PM_E_STATE, "batch interval launched and not finished");
     pm_params q = sample_params(&b->prm, nx, ny, nz, bc, dx, phi);
     const int64_t need = pm_device_bytes(&q);
     if (need < 0) return PM_E_ARG;
@@ -545,8 +548,24 @@ static void swap_current(pm_batch *b, pm_ctx *c) {
     }
 }
 
+#define NOT_IN_FLIGHT(b)                                                                         \
+    do {                                                                                         \
+        if ((b)->in_flight) return pm_fail(PM_E_STATE, "batch interval launched and not finished"); \
+    } while (0)
+
 extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
+    int rc = pm_batch_launch(b);
+    return rc == PM_OK ? pm_batch_finish(b, n_running) : rc;
+}
+
+extern "C" int pm_batch_ready(pm_batch *b) {
+    if (!b || !b->in_flight || b->launched_empty) return 1;
+    return hipStreamQuery(b->stream) == hipErrorNotReady ? 0 : 1;
+}
+
+extern "C" int pm_batch_launch(pm_batch *b) {
     if (!b) return pm_fail(PM_E_ARG, "null batch");
+    NOT_IN_FLIGHT(b);
     BCHK(hipSetDevice(b->prm.device));
     if (b->shape_dirty) {
         // the captured interval's nodes refer to the tables, the reduction
@@ -559,7 +578,7 @@ extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
                 b->run.push_back((int)s);
     }
     if (b->run.empty()) {
-        if (n_running) *n_running = 0;
+        b->in_flight = b->launched_empty = true;
         return PM_OK;
     }
     std::vector<BSample> S;
@@ -613,8 +632,22 @@ extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
     }
     BCHK(hipGetLastError());
     BCHK(hipMemcpyAsync(b->hOut, b->dOut, sizeof(pm::RedSums) * (size_t)b->nS, hipMemcpyDeviceToHost, b->stream));
-    BCHK(hipStreamSynchronize(b->stream));
     b->shape_dirty = false;
+    b->in_flight = true;
+    b->launched_empty = false;
+    return PM_OK;
+}
+
+extern "C" int pm_batch_finish(pm_batch *b, int *n_running) {
+    if (!b) return pm_fail(PM_E_ARG, "null batch");
+    if (!b->in_flight) return pm_fail(PM_E_STATE, "pm_batch_finish without pm_batch_launch");
+    b->in_flight = false;
+    if (b->launched_empty) {
+        if (n_running) *n_running = 0;
+        return PM_OK;
+    }
+    BCHK(hipSetDevice(b->prm.device));
+    BCHK(hipStreamSynchronize(b->stream));
     int running = 0;
     for (size_t i = 0; i < b->run.size(); i++) {
         const int slot = b->run[i];
@@ -658,6 +691,7 @@ extern "C" int pm_batch_step(pm_batch *b, int *n_running) {
 
 extern "C" int pm_batch_set_eps(pm_batch *b, int slot, double eps) {
     if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    NOT_IN_FLIGHT(b);
     if (!(eps > 0.0)) return pm_fail(PM_E_ARG, "eps must be positive");
     b->ctx[slot]->prm.eps = eps;
     return PM_OK;
@@ -665,6 +699,7 @@ extern "C" int pm_batch_set_eps(pm_batch *b, int slot, double eps) {
 
 extern "C" int pm_batch_set_dom_interest(pm_batch *b, int slot, const int box[6]) {
     if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    NOT_IN_FLIGHT(b);
     if (!box) return pm_fail(PM_E_ARG, "null box");
     pm_ctx *c = b->ctx[slot];
     const int n[3] = {c->prm.nx, c->prm.ny, c->prm.nz};
@@ -680,6 +715,7 @@ extern "C" int pm_batch_set_dom_interest(pm_batch *b, int slot, const int box[6]
 
 extern "C" int pm_batch_status(pm_batch *b, int slot, int *state, pm_result *res) {
     if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    NOT_IN_FLIGHT(b);
     pm_ctx *c = b->ctx[slot];
     if (state) *state = b->state[slot];
     if (res) {
@@ -696,12 +732,14 @@ extern "C" int pm_batch_status(pm_batch *b, int slot, int *state, pm_result *res
 
 extern "C" int pm_batch_get_field(pm_batch *b, int slot, int field, void *out) {
     if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    NOT_IN_FLIGHT(b);
     BCHK(hipStreamSynchronize(b->stream));
     return pm_get_field(b->ctx[slot], field, out);
 }
 
 extern "C" int pm_batch_remove(pm_batch *b, int slot) {
     if (!b || slot < 0 || slot >= (int)b->ctx.size() || !b->ctx[slot]) return pm_fail(PM_E_ARG, "no sample in slot %d", slot);
+    NOT_IN_FLIGHT(b);
     BCHK(hipStreamSynchronize(b->stream));
     // the captured interval refers to this sample's arrays
     drop_graph(b);

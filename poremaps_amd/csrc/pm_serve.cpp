@@ -165,11 +165,12 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
         rejected++;
         done++;
     };
-    // one interval of a lane, then its finished solves evicted and answered
-    auto step_lane = [&](Lane &L) -> int {
+    // the end of a launched interval of a lane: its finished solves evicted
+    // and answered
+    auto finish_lane = [&](Lane &L) -> int {
         int still = 0;
         const double t_step = now();
-        int r = pm_batch_step(L.b, &still);
+        int r = pm_batch_finish(L.b, &still);
         step_time += now() - t_step;
         intervals++;
         if (r != PM_OK) return r;
@@ -197,6 +198,12 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
         }
         if (!progressed) return pm_fail(PM_E_STATE, "admitted solves cannot progress");
         return PM_OK;
+    };
+    auto step_lane = [&](Lane &L) -> int {
+        const double t_step = now();
+        int r = pm_batch_launch(L.b);
+        step_time += now() - t_step;
+        return r == PM_OK ? finish_lane(L) : r;
     };
     while (rc == PM_OK && done < n) {
         // admission: first come, first served, while the three solves fit
@@ -275,6 +282,11 @@ extern "C" int pm_serve(const pm_serve_params *p, const pm_request *req, int n, 
             }
             continue;
         }
+        // time slices: one lane-0 interval, then lane 1 for about as long.
+        // (Stepping lane 1 on its stream while a lane-0 interval runs leaves it
+        // only the CUs the large interval frees as its workgroups retire: on
+        // the serving workload the same throughput with p50 11.4 s instead of
+        // 1.4 s.)
         double t_large = 0.0;
         if (lanes[0].running > 0) {
             const double ts = now();

@@ -121,3 +121,39 @@ def test_relaxation_by_porosity_and_batch_equals_solo():
                        omega_p=pair[1], eps=1e-6) as s:
             res2, _ = s.solve()
         assert (res2.sweeps, res2.last.wk33) == (res.sweeps, res.last.wk33)
+
+
+def test_two_batches_overlapped_on_their_streams(channel_raw):
+    """Batch A's interval launched, batch B stepped on its own stream while it
+    runs, then A finished (the serving loop's two lanes): every sample of both
+    ends as its solve run alone, bit for bit.  While A's interval is in flight
+    A takes no other call."""
+    c = cases(channel_raw)
+    say("overlapped batches")
+    with pm.Batch(precision="f32dc", omega=pm.AUTO) as a, pm.Batch(precision="f32dc", omega=pm.AUTO) as b:
+        sa = [a.add(raw, boundary_method=bc, voxel_size=dx) for raw, bc, dx in c[:2]]
+        sb = [b.add(raw, boundary_method=bc, voxel_size=dx) for raw, bc, dx in c[2:]]
+        with pytest.raises(pm.PoremapsError) as e:
+            a.finish()
+        assert e.value.code == -3
+        na = nb = 1
+        for _ in range(100000):
+            if na:
+                a.launch()
+                with pytest.raises(pm.PoremapsError) as e:
+                    a.add(c[0][0], boundary_method=0, voxel_size=1e-5)
+                assert e.value.code == -3
+            k = 0
+            while nb and k < 8 and not (na and a.ready()):
+                nb = b.step()
+                k += 1
+            if na:
+                na = a.finish()
+            if not na and not nb:
+                break
+        done = [(a, s, case) for s, case in zip(sa, c[:2])] + [(b, s, case) for s, case in zip(sb, c[2:])]
+        for batch, slot, (raw, bc, dx) in done:
+            state, res = batch.status(slot)
+            ref, velz, _ = alone(raw, bc, dx, "f32dc", pm.AUTO)
+            assert state == "converged" and res.sweeps == ref.sweeps and res.last.wk33 == ref.last.wk33
+            assert np.array_equal(batch.field(slot, "velz"), velz)
