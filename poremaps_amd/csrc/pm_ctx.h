@@ -32,6 +32,7 @@ struct SweepVariant {
     int zs;             // dense kernel / z-march: planes per z segment
     int ty = 4;         // z-march: tile rows (4 or 8)
     int occ = 0;        // FP32+DC record pass: occupancy target (0, 6, 8 waves per SIMD)
+    bool skip2 = false; // FP32+DC record pass: +-2 loads only where the case reads them
 };
 
 // Autotuner: the first sweeps of a solve are run with each candidate in turn
@@ -111,6 +112,7 @@ struct pm_ctx {
     int march_ty = 4, march_zseg = 128;
     int sor_chunk = 0;          // FluidLists::sor_chunk (POREMAPS_SOR_CHUNK)
     int rec_occ = 0;            // FluidLists::rec_occ (POREMAPS_REC_OCC)
+    bool rec_skip2 = false;     // FluidLists::rec_skip2 (POREMAPS_REC_SKIP2=1)
     bool sor_fused = true;      // FluidLists::sor_fused (POREMAPS_SOR_STAGE=seq: false)
     Tuner tuner;
     // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence

@@ -115,6 +115,9 @@ struct FluidLists {
     // FP32+DC record pass: waves per SIMD its registers must allow (0, 6, 8;
     // 0: the compiler's allocation)
     int rec_occ = 0;
+    // FP32+DC record pass: the +-2 stencil loads only where the cell's case
+    // reads them (rec_skip2_supported: the state below 2^32 bytes)
+    bool rec_skip2 = false;
     std::vector<int32_t> plane_off;
 };
 
@@ -142,6 +145,7 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
 // Whether launch_jacobi4 of this stage leaves S1's halos filled (the record
 // pass with L.fill_halos): the caller then skips launch_ghost4.
 bool jacobi4_fills_halos(int stage, const FluidLists &L);
+bool rec_skip2_supported(const Grid &g);
 void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
                     float cp, hipStream_t s);
 // The lagged pass (JAC_LAG, halo copies included) as a z-march of 64 x ty

@@ -211,6 +211,7 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     // z-march form (POREMAPS_JACOBI=march): tile rows 4 or 8, planes per z
     // segment (0: all)
     if (const char *mt = std::getenv("POREMAPS_MARCH_TY")) c->march_ty = std::atoi(mt) == 8 ? 8 : 4;
+    if (const char *rk = std::getenv("POREMAPS_REC_SKIP2")) c->rec_skip2 = std::atoi(rk) == 1;
     if (const char *ro = std::getenv("POREMAPS_REC_OCC")) {
         const int o = std::atoi(ro);
         c->rec_occ = o == 6 || o == 8 ? o : 0;
@@ -438,6 +439,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         c->lists.march_ty = c->march_ty;
         c->lists.march_zseg = c->march_zseg;
         c->lists.rec_occ = c->rec_occ;
+        c->lists.rec_skip2 = c->rec_skip2 && pm::rec_skip2_supported(g);
         const bool sor = c->prm.mode == PM_MODE_SOR3;
         if (sor && c->dc()) {
             // the colour lists back to back: the residual's momentum part,
@@ -597,11 +599,15 @@ static void apply_variant(pm_ctx *c, const SweepVariant &v) {
         c->lists.march_zseg = c->march_zseg = v.zs;
     }
     c->lists.rec_occ = c->rec_occ = v.impl == JI_RECORDS ? v.occ : 0;
+    c->rec_skip2 = v.impl == JI_RECORDS && v.skip2;
+    c->lists.rec_skip2 = c->rec_skip2 && pm::rec_skip2_supported(c->g);
 }
 
 static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *buf, size_t len) {
     if (c->dc() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
     else if (c->dc() && v.impl == JI_MARCH) std::snprintf(buf, len, "interleaved-zmarch-ty%d-zs%d", v.ty, v.zs);
+    else if (c->dc() && v.impl == JI_RECORDS && v.skip2)
+        std::snprintf(buf, len, "interleaved-lagged-records-skip2");
     else if (c->dc() && v.impl == JI_RECORDS && v.occ > 0)
         std::snprintf(buf, len, "interleaved-lagged-records-occ%d", v.occ);
     else if (c->dc())
@@ -629,16 +635,18 @@ static void tuner_setup(pm_ctx *c) {
                                 {JI_LIST, true, 64}, {JI_RECORDS, true, 64}, {JI_STREAM, true, 64},
                                 {JI_STREAM, true, 16}};
     // the record pass also at 6 and 8 waves per SIMD (its registers capped at
-    // 80 and 64, a few spilled)
+    // 80 and 64, a few spilled), and with its +-2 stencil loads made only by
+    // the lanes whose case reads them
     const SweepVariant aos[] = {{JI_RECORDS, true, 64}, {JI_PAIR, true, 64}, {JI_LIST, false, 64},
                                 {JI_LIST, true, 64}, {JI_RECORDS, true, 64, 4, 6}, {JI_RECORDS, true, 64, 4, 8},
-                                {JI_RECORDS, true, 64}};
+                                {JI_RECORDS, true, 64, 4, 0, true}, {JI_RECORDS, true, 64}};
     // the z-march form: 64 x 4 tiles, z segments of 128 planes (64 x 8 tiles
     // and 64-plane segments measured slower on the 512^3 field: 4.09 and 2.40
     // against 2.37 ms per sweep)
     const SweepVariant march[] = {{JI_MARCH, true, 128, 4}};
     if (c->dc()) {
-        for (const auto &v : aos) tn.cand[tn.n++] = v;
+        for (const auto &v : aos)
+            if (!v.skip2 || pm::rec_skip2_supported(c->g)) tn.cand[tn.n++] = v;
         if (pm::march_supported(c->g, c->lists))
             for (const auto &v : march) tn.cand[tn.n++] = v;
     }
@@ -1067,7 +1075,8 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
     char buf[64];
     const SweepVariant cur = {c->jac_impl, c->lists.select_loads,
                               c->jac_impl == JI_MARCH ? c->march_zseg : c->stream_zs, c->march_ty,
-                              c->jac_impl == JI_RECORDS ? c->lists.rec_occ : 0};
+                              c->jac_impl == JI_RECORDS ? c->lists.rec_occ : 0,
+                              c->jac_impl == JI_RECORDS && c->lists.rec_skip2};
     const char *sor = !c->fast() ? "n/a" : !c->dc() ? "list-colour" : c->lists.split ? "interleaved-colour-split"
                                                                                     : "interleaved-colour";
     if (c->fast() && c->dc() && c->lists.split && c->lists.sor_chunk > 0) {
