@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -48,6 +49,9 @@ struct BSample {
     // dom_interest sub-domain of the permeability (interior indices, inclusive)
     DomBox box;
     int chunk_begin, chunk_end;
+    // the lagged pass reads its +-2 stencil neighbours only where the case
+    // uses them (rec_skip2_supported)
+    int skip2;
 };
 
 struct BChunk {
@@ -114,8 +118,12 @@ __global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, co
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    lag_voxel4_rec<true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-                         parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+    if (s.skip2)
+        lag_voxel4_rec<true, true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
+                                   parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+    else
+        lag_voxel4_rec<true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
+                             parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
 }
 
 __global__ __launch_bounds__(kB) void k_b_settle4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
@@ -297,6 +305,7 @@ struct pm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool in_flight = false;          // launched, not yet finished
+    bool skip2 = true;               // BSample::skip2 where supported (POREMAPS_BATCH_SKIP2=0: off)
     bool launched_empty = false;     // the launched step had no running sample
     bool dc() const { return prm.precision == PM_PREC_F32DC; }
 };
@@ -353,6 +362,7 @@ extern "C" int pm_batch_create(const pm_batch_params *p, pm_batch **out) {
     pm_batch *b = new (std::nothrow) pm_batch;
     if (!b) return pm_fail(PM_E_MEMORY, "host allocation failed");
     b->prm = *p;
+    if (const char *sk = std::getenv("POREMAPS_BATCH_SKIP2")) b->skip2 = std::atoi(sk) != 0;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete b;
@@ -476,6 +486,7 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
         d.qA = b->dc() ? (void *)c->qD[0] : (void *)c->qX[0];
         d.qB = b->dc() ? (void *)c->qD[1] : (void *)c->qX[1];
         d.qX = c->qX[0];
+        d.skip2 = b->skip2 && pm::rec_skip2_supported(d.g) ? 1 : 0;
         const int s = (int)S.size();
         d.chunk_begin = (int)C.size();
         for (int t = 0; t < d.n; t += pm::kB) C.push_back({s, t});

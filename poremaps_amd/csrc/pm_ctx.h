@@ -113,6 +113,7 @@ struct pm_ctx {
     int sor_chunk = 0;          // FluidLists::sor_chunk (POREMAPS_SOR_CHUNK)
     int rec_occ = 0;            // FluidLists::rec_occ (POREMAPS_REC_OCC)
     bool rec_skip2 = false;     // FluidLists::rec_skip2 (POREMAPS_REC_SKIP2=1)
+    bool sor_skip2 = true;      // FluidLists::sor_skip2 (POREMAPS_SOR_SKIP2=0: off)
     bool sor_fused = true;      // FluidLists::sor_fused (POREMAPS_SOR_STAGE=seq: false)
     Tuner tuner;
     // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence

@@ -212,6 +212,7 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     // segment (0: all)
     if (const char *mt = std::getenv("POREMAPS_MARCH_TY")) c->march_ty = std::atoi(mt) == 8 ? 8 : 4;
     if (const char *rk = std::getenv("POREMAPS_REC_SKIP2")) c->rec_skip2 = std::atoi(rk) == 1;
+    if (const char *sk = std::getenv("POREMAPS_SOR_SKIP2")) c->sor_skip2 = std::atoi(sk) != 0;
     if (const char *ro = std::getenv("POREMAPS_REC_OCC")) {
         const int o = std::atoi(ro);
         c->rec_occ = o == 6 || o == 8 ? o : 0;
@@ -440,6 +441,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         c->lists.march_zseg = c->march_zseg;
         c->lists.rec_occ = c->rec_occ;
         c->lists.rec_skip2 = c->rec_skip2 && pm::rec_skip2_supported(g);
+        c->lists.sor_skip2 = c->sor_skip2;
         const bool sor = c->prm.mode == PM_MODE_SOR3;
         if (sor && c->dc()) {
             // the colour lists back to back: the residual's momentum part,
@@ -1079,9 +1081,12 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
                               c->jac_impl == JI_RECORDS && c->lists.rec_skip2};
     const char *sor = !c->fast() ? "n/a" : !c->dc() ? "list-colour" : c->lists.split ? "interleaved-colour-split"
                                                                                     : "interleaved-colour";
-    if (c->fast() && c->dc() && c->lists.split && c->lists.sor_chunk > 0) {
-        std::snprintf(buf, sizeof buf, "interleaved-colour-split-z%d%s", c->lists.sor_chunk,
-                      c->lists.sor_fused ? "" : "-seq");
+    if (c->fast() && c->dc() && c->lists.split) {
+        char z[32] = "";
+        if (c->lists.sor_chunk > 0)
+            std::snprintf(z, sizeof z, "-z%d%s", c->lists.sor_chunk, c->lists.sor_fused ? "" : "-seq");
+        std::snprintf(buf, sizeof buf, "interleaved-colour-split%s%s", z,
+                      c->lists.sor_skip2 && pm::rec_skip2_supported(c->g) ? "-skip2" : "");
         sor = buf;
     }
     std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(c, cur, buf, sizeof buf) : sor);

@@ -112,11 +112,12 @@ __global__ __launch_bounds__(kListBlock) void k_colour4(Grid g, const int32_t *_
 
 // The same on the colour-split state, from the voxels' records
 // (colour_voxel4_split).
+template <bool SKIP2>
 __global__ __launch_bounds__(kListBlock) void k_colour4_split(Grid g, const uint2 *__restrict__ rec, int n, int t0,
                                                               float4 *S, float *q, Res4 R, float cu, float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
-    colour_voxel4_split(g, t0 + t, rec, S, q, R, cu, cp, 0.f);
+    colour_voxel4_split<SKIP2>(g, t0 + t, rec, S, q, R, cu, cp, 0.f);
 }
 
 // Up to three colours' list ranges of one stage of the z-chunked SOR in one
@@ -127,13 +128,14 @@ struct ColourStage {
     int t0[3], n[3], b[3];
 };
 
+template <bool SKIP2>
 __global__ __launch_bounds__(kListBlock) void k_colour4_stage(Grid g, const uint2 *__restrict__ rec, ColourStage st,
                                                               float4 *S, float *q, Res4 R, float cu, float cp) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int col = b >= st.b[2] ? 2 : (b >= st.b[1] ? 1 : 0);
     const int t = (b - st.b[col]) * kListBlock + threadIdx.x;
     if (t >= st.n[col]) return;
-    colour_voxel4_split(g, st.t0[col] + t, rec, S, q, R, cu, cp, 0.f);
+    colour_voxel4_split<SKIP2>(g, st.t0[col] + t, rec, S, q, R, cu, cp, 0.f);
 }
 
 __global__ __launch_bounds__(kListBlock) void k_colour_records(Grid g, const int32_t *__restrict__ list, int n,
@@ -685,6 +687,7 @@ static void colour_planes(const Grid &g, const FluidLists &L, int q, int z0, int
 static void launch_colour4_chunked(const Grid &g, const FluidLists &L, float4 *S, float *q, Res4 R, float cu,
                                    float cp, hipStream_t s) {
     const int Z = L.sor_chunk;
+    const bool skip2 = L.sor_skip2 && rec_skip2_supported(g);
     for (int k = 0;; k++) {
         ColourStage st;
         int lo[3];
@@ -697,16 +700,22 @@ static void launch_colour4_chunked(const Grid &g, const FluidLists &L, float4 *S
                 st.b[col] = nb;
                 nb += (int)lblocks(st.n[col]);
             }
-            if (nb > 0)
-                hipLaunchKernelGGL(k_colour4_stage, dim3((unsigned)nb), dim3(kListBlock), 0, s, g, L.rec, st, S, q,
-                                   R, cu, cp);
+            if (nb > 0 && skip2)
+                hipLaunchKernelGGL(k_colour4_stage<true>, dim3((unsigned)nb), dim3(kListBlock), 0, s, g, L.rec, st, S,
+                                   q, R, cu, cp);
+            else if (nb > 0)
+                hipLaunchKernelGGL(k_colour4_stage<false>, dim3((unsigned)nb), dim3(kListBlock), 0, s, g, L.rec, st,
+                                   S, q, R, cu, cp);
         } else {
             for (int col = 0; col < 3; col++) {
                 int t0, n;
                 colour_planes(g, L, col, lo[col], lo[col] + Z, t0, n);
-                if (n > 0)
-                    hipLaunchKernelGGL(k_colour4_split, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.rec, n, t0,
-                                       S, q, R, cu, cp);
+                if (n > 0 && skip2)
+                    hipLaunchKernelGGL(k_colour4_split<true>, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.rec, n,
+                                       t0, S, q, R, cu, cp);
+                else if (n > 0)
+                    hipLaunchKernelGGL(k_colour4_split<false>, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.rec, n,
+                                       t0, S, q, R, cu, cp);
             }
         }
     }
@@ -721,8 +730,11 @@ void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float
     for (int col = 0; col < 3; col++) {
         const int n = L.n_col[col];
         if (n == 0) continue;
-        if (L.split)
-            hipLaunchKernelGGL(k_colour4_split, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.rec, n,
+        if (L.split && L.sor_skip2 && rec_skip2_supported(g))
+            hipLaunchKernelGGL(k_colour4_split<true>, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.rec, n,
+                               L.off_col[col], S, q, R, cu, cp);
+        else if (L.split)
+            hipLaunchKernelGGL(k_colour4_split<false>, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.rec, n,
                                L.off_col[col], S, q, R, cu, cp);
         else
             hipLaunchKernelGGL(k_colour4, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.all, n, L.off_col[col], pc,

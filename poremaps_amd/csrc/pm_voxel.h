@@ -760,6 +760,9 @@ __device__ __forceinline__ uint2 colour_record(const Grid &g, int64_t c, const u
 // lie in the other blocks of the row: k-1 is block b-1 (slot m-1 from block
 // 0), k+1 block b+1 (slot m+1 from block 2), k-2 and k+2 likewise.  The
 // continuity residual is read from the w lane of the momentum residual.
+// SKIP2: the +-2 neighbours only for the lanes whose case reads them
+// (buf_load4; the state below 2^32 bytes).
+template <bool SKIP2 = false>
 __device__ __forceinline__ void colour_voxel4_split(const Grid &g, int t, const uint2 *__restrict__ rec,
                                                     float4 *S, float *q, Res4 R, float cu, float cp,
                                                     float pin_out) {
@@ -775,9 +778,21 @@ __device__ __forceinline__ void colour_voxel4_split(const Grid &g, int t, const 
     const int64_t oM2 = b == 2 ? -2 * cb : cb - 1, oP2 = b == 0 ? 2 * cb : 1 - cb;
     const float4 C = S[c];
     Star4 s;
-    s.XM = S[c + oM1]; s.XP = S[c + oP1]; s.XM2 = S[c + oM2]; s.XP2 = S[c + oP2];
-    s.YM = S[c - sj]; s.YP = S[c + sj]; s.YM2 = S[c - 2 * sj]; s.YP2 = S[c + 2 * sj];
-    s.ZM = S[c - si]; s.ZP = S[c + si]; s.ZM2 = S[c - 2 * si]; s.ZP2 = S[c + 2 * si];
+    s.XM = S[c + oM1]; s.XP = S[c + oP1];
+    s.YM = S[c - sj]; s.YP = S[c + sj];
+    s.ZM = S[c - si]; s.ZP = S[c + si];
+    if (SKIP2) {
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)S, (short)0, (int)(uint32_t)(g.total * 16), 0x00020000);
+        const Cases c2 = decode((uint8_t)(f & 0xffu));
+        s.XM2 = buf_load4(rs, c + oM2, c2.cx == 5); s.XP2 = buf_load4(rs, c + oP2, c2.cx == 4);
+        s.YM2 = buf_load4(rs, c - 2 * sj, c2.cy == 5); s.YP2 = buf_load4(rs, c + 2 * sj, c2.cy == 4);
+        s.ZM2 = buf_load4(rs, c - 2 * si, c2.cz == 5); s.ZP2 = buf_load4(rs, c + 2 * si, c2.cz == 4);
+    } else {
+        s.XM2 = S[c + oM2]; s.XP2 = S[c + oP2];
+        s.YM2 = S[c - 2 * sj]; s.YP2 = S[c + 2 * sj];
+        s.ZM2 = S[c - 2 * si]; s.ZP2 = S[c + 2 * si];
+    }
     const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
     const float4 Rm = make_float4(r.x, r.y, r.z, 0.f);
     const float rp = r.w;

@@ -1,6 +1,8 @@
 """Batch engine on the GPU: every sample of a batch (any mix of sizes and
 boundary methods, samples joining and leaving between intervals) ends with the
 sweep count, k and fields of the same solve run alone -- bit for bit."""
+import os
+
 import numpy as np
 import pytest
 
@@ -38,11 +40,22 @@ def run_batch(b, slots):
     return {s: b.status(s) for s in slots}
 
 
-@pytest.mark.parametrize("precision,graph", [("f32dc", True), ("f32dc", False), ("f64", True)])
-def test_batch_equals_single_solves(channel_raw, precision, graph):
-    say("batch vs single", precision, graph)
+@pytest.mark.parametrize("precision,graph,skip2", [("f32dc", True, "1"), ("f32dc", False, "1"), ("f32dc", True, "0"),
+                                                  ("f64", True, "1")])
+def test_batch_equals_single_solves(channel_raw, precision, graph, skip2):
+    """skip2 "0": the lagged pass with every stencil load (POREMAPS_BATCH_SKIP2=0)."""
+    say("batch vs single", precision, graph, skip2)
     cs = cases(channel_raw)
-    with pm.Batch(precision=precision, omega=2.0, omega_p=pm.OMEGA_P, use_graph=graph) as b:
+    old = os.environ.get("POREMAPS_BATCH_SKIP2")
+    os.environ["POREMAPS_BATCH_SKIP2"] = skip2
+    try:
+        b = pm.Batch(precision=precision, omega=2.0, omega_p=pm.OMEGA_P, use_graph=graph)
+    finally:
+        if old is None:
+            os.environ.pop("POREMAPS_BATCH_SKIP2", None)
+        else:
+            os.environ["POREMAPS_BATCH_SKIP2"] = old
+    with b:
         slots = [b.add(raw, boundary_method=bc, voxel_size=dx) for raw, bc, dx in cs]
         st = run_batch(b, slots)
         for slot, (raw, bc, dx) in zip(slots, cs):
