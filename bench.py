@@ -211,8 +211,12 @@ def measured_traffic(spec, form, precision):
     rocprofv3 counter summary (read requests by size + WRITE_SIZE, calibrated
     on a device copy of known size), when the run uses the workload and kernel
     form it was measured on; else None."""
-    known = {"interleaved-lagged-records": ("jacobi_f32dc_grf512_hbm_traffic.md",
+    known = {"interleaved-lagged-records-skip2": ("jacobi_f32dc_grf512_hbm_traffic.md",
+                                                  r"traffic_bytes_per_launch k_jac4_rec_skip2: (\d+)"),
+             "interleaved-lagged-records": ("jacobi_f32dc_grf512_hbm_traffic.md",
                                             r"traffic_bytes_per_launch k_jac4_rec: (\d+)"),
+             "interleaved-colour-split-skip2": ("sor3_f32dc_grf512_hbm_traffic.md",
+                                                r"traffic_bytes_per_sweep k_colour4_split_skip2 x3: (\d+)"),
              "interleaved-colour-split": ("sor3_f32dc_grf512_hbm_traffic.md",
                                           r"traffic_bytes_per_sweep k_colour4_split x3: (\d+)")}
     if spec["name"] != "grf_512" or form not in known or precision != "f32dc":
