@@ -33,6 +33,7 @@ struct SweepVariant {
     int ty = 4;         // z-march: tile rows (4 or 8)
     int occ = 0;        // FP32+DC record pass: occupancy target (0, 6, 8 waves per SIMD)
     bool skip2 = false; // FP32+DC record pass: +-2 loads only where the case reads them
+    bool skip1 = false; // with skip2: every other stencil load too, only where used
 };
 
 // Autotuner: the first sweeps of a solve are run with each candidate in turn
@@ -113,6 +114,7 @@ struct pm_ctx {
     int sor_chunk = 0;          // FluidLists::sor_chunk (POREMAPS_SOR_CHUNK)
     int rec_occ = 0;            // FluidLists::rec_occ (POREMAPS_REC_OCC)
     bool rec_skip2 = false;     // FluidLists::rec_skip2 (POREMAPS_REC_SKIP2=1)
+    bool rec_skip1 = false;     // FluidLists::rec_skip1 (POREMAPS_REC_SKIP2=2: both)
     bool sor_skip2 = true;      // FluidLists::sor_skip2 (POREMAPS_SOR_SKIP2=0: off)
     bool sor_fused = true;      // FluidLists::sor_fused (POREMAPS_SOR_STAGE=seq: false)
     Tuner tuner;

@@ -118,6 +118,9 @@ struct FluidLists {
     // FP32+DC record pass: the +-2 stencil loads only where the cell's case
     // reads them (rec_skip2_supported: the state below 2^32 bytes)
     bool rec_skip2 = false;
+    // with rec_skip2: the +-1 loads and the recompute targets' loads too,
+    // each only where it is used
+    bool rec_skip1 = false;
     // FP32+DC SOR (colour-split rows): the same for the colour passes
     bool sor_skip2 = true;
     std::vector<int32_t> plane_off;

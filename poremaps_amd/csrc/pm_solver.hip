@@ -211,7 +211,10 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     // z-march form (POREMAPS_JACOBI=march): tile rows 4 or 8, planes per z
     // segment (0: all)
     if (const char *mt = std::getenv("POREMAPS_MARCH_TY")) c->march_ty = std::atoi(mt) == 8 ? 8 : 4;
-    if (const char *rk = std::getenv("POREMAPS_REC_SKIP2")) c->rec_skip2 = std::atoi(rk) == 1;
+    if (const char *rk = std::getenv("POREMAPS_REC_SKIP2")) {
+        c->rec_skip2 = std::atoi(rk) >= 1;
+        c->rec_skip1 = std::atoi(rk) == 2;
+    }
     if (const char *sk = std::getenv("POREMAPS_SOR_SKIP2")) c->sor_skip2 = std::atoi(sk) != 0;
     if (const char *ro = std::getenv("POREMAPS_REC_OCC")) {
         const int o = std::atoi(ro);
@@ -441,6 +444,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         c->lists.march_zseg = c->march_zseg;
         c->lists.rec_occ = c->rec_occ;
         c->lists.rec_skip2 = c->rec_skip2 && pm::rec_skip2_supported(g);
+        c->lists.rec_skip1 = c->rec_skip1;
         c->lists.sor_skip2 = c->sor_skip2;
         const bool sor = c->prm.mode == PM_MODE_SOR3;
         if (sor && c->dc()) {
@@ -602,14 +606,16 @@ static void apply_variant(pm_ctx *c, const SweepVariant &v) {
     }
     c->lists.rec_occ = c->rec_occ = v.impl == JI_RECORDS ? v.occ : 0;
     c->rec_skip2 = v.impl == JI_RECORDS && v.skip2;
+    c->rec_skip1 = c->rec_skip2 && v.skip1;
     c->lists.rec_skip2 = c->rec_skip2 && pm::rec_skip2_supported(c->g);
+    c->lists.rec_skip1 = c->rec_skip1;
 }
 
 static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *buf, size_t len) {
     if (c->dc() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
     else if (c->dc() && v.impl == JI_MARCH) std::snprintf(buf, len, "interleaved-zmarch-ty%d-zs%d", v.ty, v.zs);
     else if (c->dc() && v.impl == JI_RECORDS && v.skip2)
-        std::snprintf(buf, len, "interleaved-lagged-records-skip2");
+        std::snprintf(buf, len, "interleaved-lagged-records-%s", v.skip1 ? "skipall" : "skip2");
     else if (c->dc() && v.impl == JI_RECORDS && v.occ > 0)
         std::snprintf(buf, len, "interleaved-lagged-records-occ%d", v.occ);
     else if (c->dc())
@@ -641,7 +647,8 @@ static void tuner_setup(pm_ctx *c) {
     // the lanes whose case reads them
     const SweepVariant aos[] = {{JI_RECORDS, true, 64}, {JI_PAIR, true, 64}, {JI_LIST, false, 64},
                                 {JI_LIST, true, 64}, {JI_RECORDS, true, 64, 4, 6}, {JI_RECORDS, true, 64, 4, 8},
-                                {JI_RECORDS, true, 64, 4, 0, true}, {JI_RECORDS, true, 64}};
+                                {JI_RECORDS, true, 64, 4, 0, true}, {JI_RECORDS, true, 64, 4, 0, true, true},
+                                {JI_RECORDS, true, 64}};
     // the z-march form: 64 x 4 tiles, z segments of 128 planes (64 x 8 tiles
     // and 64-plane segments measured slower on the 512^3 field: 4.09 and 2.40
     // against 2.37 ms per sweep)
@@ -1078,7 +1085,8 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
     const SweepVariant cur = {c->jac_impl, c->lists.select_loads,
                               c->jac_impl == JI_MARCH ? c->march_zseg : c->stream_zs, c->march_ty,
                               c->jac_impl == JI_RECORDS ? c->lists.rec_occ : 0,
-                              c->jac_impl == JI_RECORDS && c->lists.rec_skip2};
+                              c->jac_impl == JI_RECORDS && c->lists.rec_skip2,
+                              c->jac_impl == JI_RECORDS && c->lists.rec_skip2 && c->lists.rec_skip1};
     const char *sor = !c->fast() ? "n/a" : !c->dc() ? "list-colour" : c->lists.split ? "interleaved-colour-split"
                                                                                     : "interleaved-colour";
     if (c->fast() && c->dc() && c->lists.split) {

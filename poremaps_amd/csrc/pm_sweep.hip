@@ -165,13 +165,13 @@ __global__ __launch_bounds__(kListBlock) void k_jac_rec(Grid g, const uint2 *__r
 // autotuner.
 // SKIP2: the +-2 stencil loads only for the lanes whose case reads them
 // (load_star4_masked; the state below 2^32 bytes).
-template <bool IMAGES, int OCC, bool SKIP2 = false>
+template <bool IMAGES, int OCC, bool SKIP2 = false, bool SKIP1 = false>
 __global__ __launch_bounds__(kListBlock, OCC > 0 ? OCC : 1) void k_jac4_rec(
     Grid g, const uint2 *__restrict__ rec, int n, const float4 *__restrict__ S0, const float *__restrict__ q0,
     float4 *S1, Res4 R, float cu, float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
-    lag_voxel4_rec<IMAGES, SKIP2>(g, t, rec, S0, q0, S1, R, cu, cp, 0.f);
+    lag_voxel4_rec<IMAGES, SKIP2, SKIP1>(g, t, rec, S0, q0, S1, R, cu, cp, 0.f);
 }
 
 // The masked stencil loads address the state with 32-bit byte offsets.
@@ -638,7 +638,10 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
     } else if (stage == JAC_FIRST) {
         hipLaunchKernelGGL(k_jac4<0>, grid, blk, 0, s, g, L.all, L.n_all, pc, S0, q0, S1, q1, R, cu, cp);
     } else if (jacobi4_fills_halos(stage, L)) {
-        if (L.rec_skip2 && rec_skip2_supported(g))
+        if (L.rec_skip2 && L.rec_skip1 && rec_skip2_supported(g))
+            hipLaunchKernelGGL((k_jac4_rec<true, 5, true, true>), grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R,
+                               cu, cp);
+        else if (L.rec_skip2 && rec_skip2_supported(g))
             hipLaunchKernelGGL((k_jac4_rec<true, 0, true>), grid, blk, 0, s, g, L.rec, L.n_all, S0, q0, S1, R, cu,
                                cp);
         else if (L.rec_occ == 8)

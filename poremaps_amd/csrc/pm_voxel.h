@@ -224,12 +224,30 @@ __device__ __forceinline__ float4 buf_load4(__amdgpu_buffer_rsrc_t rs, int64_t e
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// One float of the state (component q of element e) the same way.
+__device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t rs, int64_t e, int q, bool need) {
+    const uint32_t off = need ? (uint32_t)(e * 16 + 4 * q) : 0xFFFFFFF0u;
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
+
+// ALL1: the +-1 neighbours likewise, each read unless its side is solid
+// (a -1 neighbour is read only when the case is not backward-solid on that
+// axis, a +1 neighbour only when it is not forward-solid: every use of it,
+// the second derivatives, the divergence and the pressure difference, drops
+// out otherwise).
+template <bool ALL1 = false>
 __device__ __forceinline__ Star4 load_star4_masked(const float4 *__restrict__ S, __amdgpu_buffer_rsrc_t rs,
                                                    int64_t c, int64_t sj, int64_t si, const Cases &cs) {
     Star4 s;
-    s.XM = S[c - 1]; s.XP = S[c + 1];
-    s.YM = S[c - sj]; s.YP = S[c + sj];
-    s.ZM = S[c - si]; s.ZP = S[c + si];
+    if (ALL1) {
+        s.XM = buf_load4(rs, c - 1, !cs.bx); s.XP = buf_load4(rs, c + 1, !fwd_solid(cs.cx));
+        s.YM = buf_load4(rs, c - sj, !cs.by); s.YP = buf_load4(rs, c + sj, !fwd_solid(cs.cy));
+        s.ZM = buf_load4(rs, c - si, !cs.bz); s.ZP = buf_load4(rs, c + si, !fwd_solid(cs.cz));
+    } else {
+        s.XM = S[c - 1]; s.XP = S[c + 1];
+        s.YM = S[c - sj]; s.YP = S[c + sj];
+        s.ZM = S[c - si]; s.ZP = S[c + si];
+    }
     s.XM2 = buf_load4(rs, c - 2, cs.cx == 5); s.XP2 = buf_load4(rs, c + 2, cs.cx == 4);
     s.YM2 = buf_load4(rs, c - 2 * sj, cs.cy == 5); s.YP2 = buf_load4(rs, c + 2 * sj, cs.cy == 4);
     s.ZM2 = buf_load4(rs, c - 2 * si, cs.cz == 5); s.ZP2 = buf_load4(rs, c + 2 * si, cs.cz == 4);
@@ -518,8 +536,11 @@ __device__ __forceinline__ uint2 lag_record(const Grid &g, int64_t c, const uint
 // continuity residuals from R.q: two streamed loads instead of eight gathers,
 // the same values.
 // SKIP2: the +-2 neighbours through load_star4_masked (the state below 2^32
-// bytes).
-template <bool IMAGES = false, bool SKIP2 = false>
+// bytes); SKIP1 (with SKIP2): the +-1 neighbours and the six values of the
+// recompute targets too, each read only where it is used (the targets of a
+// solid side are never used; a target's forward velocity only where that
+// face of the target is open).
+template <bool IMAGES = false, bool SKIP2 = false, bool SKIP1 = false>
 __device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2 *__restrict__ rec,
                                                const float4 *__restrict__ S0, const float *__restrict__ q0,
                                                float4 *__restrict__ S1, Res4 R, float cu, float cp, float pin_out) {
@@ -540,16 +561,31 @@ __device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2
     const float rpc = rq.x, rpx = rq.y, rpy = rq.z, rpz = rq.w;
     const float4 C = S0[c];
     Star4 s;
+    float vyx, wzx, uxy, wzy, uxz, vyz;
     if (SKIP2) {
         const __amdgpu_buffer_rsrc_t rs =
             __builtin_amdgcn_make_buffer_rsrc((void *)S0, (short)0, (int)(uint32_t)(g.total * 16), 0x00020000);
-        s = load_star4_masked(S0, rs, c, sx, sy, decode(pcv));
+        const Cases cc = decode(pcv);
+        s = load_star4_masked<SKIP1>(S0, rs, c, sx, sy, cc);
+        if (SKIP1) {
+            const Cases tx = decode(pcx), ty = decode(pcy), tz = decode(pcz);
+            vyx = buf_load1(rs, nxn + sx, 2, !cc.bx && !fwd_solid(tx.cy));
+            wzx = buf_load1(rs, nxn + sy, 3, !cc.bx && !fwd_solid(tx.cz));
+            uxy = buf_load1(rs, nyn + 1, 1, !cc.by && !fwd_solid(ty.cx));
+            wzy = buf_load1(rs, nyn + sy, 3, !cc.by && !fwd_solid(ty.cz));
+            uxz = buf_load1(rs, nzn + 1, 1, !cc.bz && !fwd_solid(tz.cx));
+            vyz = buf_load1(rs, nzn + sx, 2, !cc.bz && !fwd_solid(tz.cy));
+        } else {
+            vyx = S0[nxn + sx].z; wzx = S0[nxn + sy].w;
+            uxy = S0[nyn + 1].y; wzy = S0[nyn + sy].w;
+            uxz = S0[nzn + 1].y; vyz = S0[nzn + sx].z;
+        }
     } else {
         s = load_star4(S0, c, sx, sy);
+        vyx = S0[nxn + sx].z; wzx = S0[nxn + sy].w;
+        uxy = S0[nyn + 1].y; wzy = S0[nyn + sy].w;
+        uxz = S0[nzn + 1].y; vyz = S0[nzn + sx].z;
     }
-    const float vyx = S0[nxn + sx].z, wzx = S0[nxn + sy].w;
-    const float uxy = S0[nyn + 1].y, wzy = S0[nyn + sy].w;
-    const float uxz = S0[nzn + 1].y, vyz = S0[nzn + sx].z;
     const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
     const float4 Rm = make_float4(r.x, r.y, r.z, r.w);
     float4 NX = s.XM, NY = s.YM;

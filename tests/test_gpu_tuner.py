@@ -80,7 +80,7 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
             # list forms (eager, branch, records; records also at 6 and 8
             # waves per SIMD and with the +-2 loads skipped where unused) and
             # the z-march
-            assert len(tried) == (8 if precision == "f32dc" else 6) and name in tried
+            assert len(tried) == (9 if precision == "f32dc" else 6) and name in tried
 
 
 def test_divergent_relaxation_restarts_slower():
