@@ -38,15 +38,36 @@ __device__ __forceinline__ void acc_merge(RedAcc &a, const RedAcc &b) {
 }
 
 
-__device__ void block_reduce_store(RedAcc &a, RedAcc *out) {
-    __shared__ RedAcc sh[kRedThreads];
-    sh[threadIdx.x] = a;
-    __syncthreads();
-    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) acc_merge(sh[threadIdx.x], sh[threadIdx.x + off]);
-        __syncthreads();
+// Block of kRedThreads threads: a butterfly of lane shuffles inside each
+// 64-lane wave (no LDS, no barrier), then the waves' partials through LDS,
+// merged in wave order by thread 0.  The merge order is fixed, so the result
+// is the same on every run; only lane 0's butterfly value is used.
+__device__ __forceinline__ RedAcc acc_shfl_xor(const RedAcc &a, int m) {
+    RedAcc b;
+    for (int q = 0; q < 8; q++) {
+        b.s[q].s = __shfl_xor(a.s[q].s, m, 64);
+        b.s[q].c = __shfl_xor(a.s[q].c, m, 64);
     }
-    if (threadIdx.x == 0) *out = sh[0];
+    for (int q = 0; q < 4; q++) b.n[q] = __shfl_xor(a.n[q], m, 64);
+    b.wmax = __shfl_xor(a.wmax, m, 64);
+    return b;
+}
+
+__device__ void block_reduce_store(RedAcc &a, RedAcc *out) {
+    static_assert(kRedThreads % 64 == 0, "whole waves");
+    constexpr int kWaves = kRedThreads / 64;
+    __shared__ RedAcc sh[kWaves];
+    for (int m = 32; m > 0; m >>= 1) {
+        const RedAcc b = acc_shfl_xor(a, m);
+        acc_merge(a, b);
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        RedAcc r = sh[0];
+        for (int w = 1; w < kWaves; w++) acc_merge(r, sh[w]);
+        *out = r;
+    }
 }
 
 
