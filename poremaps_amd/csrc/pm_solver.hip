@@ -622,7 +622,7 @@ static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *bu
         std::snprintf(buf, len, "interleaved-lagged-%s",
                       v.impl == JI_RECORDS ? "records" : v.select_loads ? "eager" : "branch");
     else if (v.impl == JI_STREAM) std::snprintf(buf, len, "dense-2.5d-zs%d", v.zs);
-    else if (v.impl == JI_RECORDS) std::snprintf(buf, len, "list-lagged-records");
+    else if (v.impl == JI_RECORDS) std::snprintf(buf, len, "list-lagged-records%s", v.skip2 ? "-skip2" : "");
     else std::snprintf(buf, len, "%s-%s", v.impl == JI_PAIR ? "list-pair" : "list-lagged",
                        v.select_loads ? "select" : "branch");
     return buf;
@@ -640,8 +640,8 @@ static void tuner_setup(pm_ctx *c) {
     // the first window (caches and clocks still cold after the load) is run
     // with the last candidate and not scored
     const SweepVariant soa[] = {{JI_STREAM, true, 16}, {JI_PAIR, true, 64}, {JI_PAIR, false, 64},
-                                {JI_LIST, true, 64}, {JI_RECORDS, true, 64}, {JI_STREAM, true, 64},
-                                {JI_STREAM, true, 16}};
+                                {JI_LIST, true, 64}, {JI_RECORDS, true, 64}, {JI_RECORDS, true, 64, 4, 0, true},
+                                {JI_STREAM, true, 64}, {JI_STREAM, true, 16}};
     // the record pass also at 6 and 8 waves per SIMD (its registers capped at
     // 80 and 64, a few spilled), and with its +-2 stencil loads made only by
     // the lanes whose case reads them
@@ -660,7 +660,8 @@ static void tuner_setup(pm_ctx *c) {
             for (const auto &v : march) tn.cand[tn.n++] = v;
     }
     else
-        for (const auto &v : soa) tn.cand[tn.n++] = v;
+        for (const auto &v : soa)
+            if (!v.skip2 || pm::rec_skip2_supported(c->g)) tn.cand[tn.n++] = v;
     tn.first_scored = 1;
     tn.active = true;
 }

@@ -145,8 +145,9 @@ __global__ __launch_bounds__(kListBlock) void k_colour_records(Grid g, const int
     rec[t] = colour_record(g, list[t], pc);
 }
 
-// FP64 per-field lagged pass from the per-voxel records (lag_voxel_rec).
-template <typename T>
+// FP64 per-field lagged pass from the per-voxel records (lag_voxel_rec);
+// SKIP2: its +-2 loads only where the case reads them.
+template <typename T, bool SKIP2 = false>
 __global__ __launch_bounds__(kListBlock) void k_jac_rec(Grid g, const uint2 *__restrict__ rec, int n,
                                                         const T *__restrict__ p0, const T *__restrict__ u0,
                                                         const T *__restrict__ v0, const T *__restrict__ w0,
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(kListBlock) void k_jac_rec(Grid g, const uint2 *__r
                                                         T cu, T cp, T pin_out) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
-    lag_voxel_rec<T>(g, t, rec, p0, u0, v0, w0, q0, p1, u1, v1, w1, cu, cp, pin_out);
+    lag_voxel_rec<T, SKIP2>(g, t, rec, p0, u0, v0, w0, q0, p1, u1, v1, w1, cu, cp, pin_out);
 }
 
 // Lagged pass from the per-voxel records (lag_voxel4_rec); IMAGES: the pass
@@ -598,6 +599,9 @@ static void jacobi_stage_t(int stage, const Grid &g, const FluidLists &L, const 
     if (stage == JAC_FIRST)
         hipLaunchKernelGGL((k_jac_first<double, SEL>), grid, blk, 0, s, g, L.all, L.n_all, pc, p0, u0, v0, w0, q0,
                            p1, u1, v1, w1, cu);
+    else if (stage == JAC_LAG && L.rec && L.use_rec && L.rec_skip2 && rec_skip2_supported(g))
+        hipLaunchKernelGGL((k_jac_rec<double, true>), grid, blk, 0, s, g, L.rec, L.n_all, p0, u0, v0, w0, q0, p1, u1,
+                           v1, w1, cu, cp, pin_out);
     else if (stage == JAC_LAG && L.rec && L.use_rec)
         hipLaunchKernelGGL(k_jac_rec<double>, grid, blk, 0, s, g, L.rec, L.n_all, p0, u0, v0, w0, q0, p1, u1, v1, w1,
                            cu, cp, pin_out);

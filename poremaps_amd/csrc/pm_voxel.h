@@ -650,7 +650,21 @@ __device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2
 // neighbours per field, the pressures of the three recompute targets and the
 // six diagonal velocities their divergences need.  The same expressions in the
 // same order as lag_voxel, so the same values.
+// An element of a per-field array the same way as buf_load4 (element bytes
+// sizeof(T); the array below 2^32 bytes).
 template <typename T>
+__device__ __forceinline__ T buf_load_t(__amdgpu_buffer_rsrc_t rs, int64_t e, bool need) {
+    const uint32_t off = need ? (uint32_t)(e * (int64_t)sizeof(T)) : 0xFFFFFFF0u;
+    if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+    }
+}
+
+// SKIP2: the twelve +-2 loads only for the lanes whose case reads them
+// (buf_load_t; each field below 2^32 bytes).
+template <typename T, bool SKIP2 = false>
 __device__ __forceinline__ void lag_voxel_rec(const Grid &g, int t, const uint2 *__restrict__ rec,
                                               const T *__restrict__ p0, const T *__restrict__ u0,
                                               const T *__restrict__ v0, const T *__restrict__ w0,
@@ -671,14 +685,32 @@ __device__ __forceinline__ void lag_voxel_rec(const Grid &g, int t, const uint2 
     // the cell and its stencil
     const T pc_ = p0[c], uc = u0[c], vc = v0[c], wc = w0[c];
     const T uxm = u0[c - 1], uxp = u0[c + 1];
-    const T uym2 = u0[c - 2 * sx], uym = u0[c - sx], uyp = u0[c + sx], uyp2 = u0[c + 2 * sx];
-    const T uzm2 = u0[c - 2 * sy], uzm = u0[c - sy], uzp = u0[c + sy], uzp2 = u0[c + 2 * sy];
+    const T uym = u0[c - sx], uyp = u0[c + sx];
+    const T uzm = u0[c - sy], uzp = u0[c + sy];
     const T vym = v0[c - sx], vyp = v0[c + sx];
-    const T vxm2 = v0[c - 2], vxm = v0[c - 1], vxp = v0[c + 1], vxp2 = v0[c + 2];
-    const T vzm2 = v0[c - 2 * sy], vzm = v0[c - sy], vzp = v0[c + sy], vzp2 = v0[c + 2 * sy];
+    const T vxm = v0[c - 1], vxp = v0[c + 1];
+    const T vzm = v0[c - sy], vzp = v0[c + sy];
     const T wzm = w0[c - sy], wzp = w0[c + sy];
-    const T wxm2 = w0[c - 2], wxm = w0[c - 1], wxp = w0[c + 1], wxp2 = w0[c + 2];
-    const T wym2 = w0[c - 2 * sx], wym = w0[c - sx], wyp = w0[c + sx], wyp2 = w0[c + 2 * sx];
+    const T wxm = w0[c - 1], wxp = w0[c + 1];
+    const T wym = w0[c - sx], wyp = w0[c + sx];
+    T uym2, uyp2, uzm2, uzp2, vxm2, vxp2, vzm2, vzp2, wxm2, wxp2, wym2, wyp2;
+    if (SKIP2) {
+        const int nb = (int)(uint32_t)(g.total * (int64_t)sizeof(T));
+        const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc((void *)u0, (short)0, nb, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void *)v0, (short)0, nb, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc((void *)w0, (short)0, nb, 0x00020000);
+        const Cases c2 = decode(pcv);
+        uym2 = buf_load_t<T>(ru, c - 2 * sx, c2.cy == 5); uyp2 = buf_load_t<T>(ru, c + 2 * sx, c2.cy == 4);
+        uzm2 = buf_load_t<T>(ru, c - 2 * sy, c2.cz == 5); uzp2 = buf_load_t<T>(ru, c + 2 * sy, c2.cz == 4);
+        vxm2 = buf_load_t<T>(rv, c - 2, c2.cx == 5); vxp2 = buf_load_t<T>(rv, c + 2, c2.cx == 4);
+        vzm2 = buf_load_t<T>(rv, c - 2 * sy, c2.cz == 5); vzp2 = buf_load_t<T>(rv, c + 2 * sy, c2.cz == 4);
+        wxm2 = buf_load_t<T>(rw, c - 2, c2.cx == 5); wxp2 = buf_load_t<T>(rw, c + 2, c2.cx == 4);
+        wym2 = buf_load_t<T>(rw, c - 2 * sx, c2.cy == 5); wyp2 = buf_load_t<T>(rw, c + 2 * sx, c2.cy == 4);
+    } else {
+        uym2 = u0[c - 2 * sx]; uyp2 = u0[c + 2 * sx]; uzm2 = u0[c - 2 * sy]; uzp2 = u0[c + 2 * sy];
+        vxm2 = v0[c - 2]; vxp2 = v0[c + 2]; vzm2 = v0[c - 2 * sy]; vzp2 = v0[c + 2 * sy];
+        wxm2 = w0[c - 2]; wxp2 = w0[c + 2]; wym2 = w0[c - 2 * sx]; wyp2 = w0[c + 2 * sx];
+    }
     // recompute targets: pressures and the +1 velocities their divergences need
     const T pxt = p0[nxn], pyt = p0[nyn], pzt = p0[nzn];
     const T vx_d = v0[nxn + sx], wx_d = w0[nxn + sy];

@@ -25,7 +25,7 @@ def say(*a):
 
 def solve(raw, bc, precision, form=None, zs=None, loads=None, march=None):
     old = {k: os.environ.get(k) for k in ("POREMAPS_JACOBI", "POREMAPS_STREAM_ZS", "POREMAPS_SWEEP_LOADS",
-                                          "POREMAPS_MARCH_TY", "POREMAPS_MARCH_ZSEG")}
+                                          "POREMAPS_MARCH_TY", "POREMAPS_MARCH_ZSEG", "POREMAPS_REC_SKIP2")}
     try:
         for k in ("POREMAPS_MARCH_TY", "POREMAPS_MARCH_ZSEG"):
             os.environ.pop(k, None)
@@ -35,6 +35,12 @@ def solve(raw, bc, precision, form=None, zs=None, loads=None, march=None):
             os.environ["POREMAPS_SWEEP_LOADS"] = loads
         else:
             os.environ.pop("POREMAPS_SWEEP_LOADS", None)
+        os.environ.pop("POREMAPS_REC_SKIP2", None)
+        if form and form.startswith("records-"):
+            # the record pass skipping its unused +-2 loads (skip2) or every
+            # unused load (skipall)
+            os.environ["POREMAPS_REC_SKIP2"] = "2" if form.endswith("skipall") else "1"
+            form = "records"
         if form:
             os.environ["POREMAPS_JACOBI"] = form
         else:
@@ -64,11 +70,12 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
     say("kernel forms", precision, bc)
     ref, ref_fields, _ = solve(raw, bc, precision, "pair")
     forms = [("list", None, None, None), ("list", None, "branch", None), ("records", None, None, None),
-             ("stream", 16, None, None), ("stream", 64, None, None)]
+             ("records-skip2", None, None, None), ("stream", 16, None, None), ("stream", 64, None, None)]
     if precision == "f32dc":
         # the z-march of the interleaved lagged pass: tiles of 64 x 4 / 64 x 8,
         # z segments of 5 planes (several per solve), 128 and the whole depth
-        forms += [("march", None, None, (4, 5)), ("march", None, None, (8, 128)), ("march", None, None, (4, 0))]
+        forms += [("march", None, None, (4, 5)), ("march", None, None, (8, 128)), ("march", None, None, (4, 0)),
+                  ("records-skipall", None, None, None)]
     for form, zs, loads, march in forms + [(None, None, None, None)]:
         res, fields, (name, tried) = solve(raw, bc, precision, form, zs, loads, march)
         say(form, zs, loads, march, name, res.sweeps)
@@ -80,7 +87,7 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
             # list forms (eager, branch, records; records also at 6 and 8
             # waves per SIMD and with the +-2 loads skipped where unused) and
             # the z-march
-            assert len(tried) == (9 if precision == "f32dc" else 6) and name in tried
+            assert len(tried) == (9 if precision == "f32dc" else 7) and name in tried
 
 
 def test_divergent_relaxation_restarts_slower():
