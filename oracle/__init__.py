@@ -17,6 +17,7 @@ Arrays use the reference's halo-2 layout: shape (nz+4, ny+4, nx+4), x fastest.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -102,6 +103,7 @@ def _declare(L):
     L.orc_solve_fast.argtypes = [_I, _I, _u8p, _u8p, _I, _I, _I, _I, _D, _D, _D, _I, _I] + [_f64p] * 4 + [
         C.POINTER(C.c_double)]
     L.orc_solve_fast.restype = _I
+    L.orc_set_slip.argtypes = [_I, _I]
     L.orc_d2_ref.argtypes = [_I] + [_D] * 5
     L.orc_d2_ref.restype = _D
     L.orc_d2_f64.argtypes = [_I] + [_D] * 5
@@ -275,17 +277,31 @@ def residual(pc, X, q):
     return R
 
 
+@contextlib.contextmanager
+def slip(bc, on=True):
+    """The build's slip walls (walls = 2) for the fast-mode sweeps and ghost
+    fills run inside: the non-periodic x / y faces of boundary method bc."""
+    lib().orc_set_slip(1 if on else 0, bc)
+    try:
+        yield
+    finally:
+        lib().orc_set_slip(0, bc)
+
+
 def solve_fast(raw, bc, mode="jacobi", precision="f32dc", omega=2.5, omega_p=None, eps=1e-6, it_eval=100,
                max_iter=10 ** 7, walls=0):
     """A whole fast-mode solve of the geometry raw (nz, ny, nx) to the stop
-    rule: returns (sweeps, kprev, (p, u, v, w) halo-2 FP64 state, G)."""
+    rule: returns (sweeps, kprev, (p, u, v, w) halo-2 FP64 state, G).
+    walls: 0 the reference's open faces, 1 no-slip, 2 slip walls."""
     G = geom_halo(raw, bc, walls)
     pc = pack_code(eval_geometry(G))
     st = [a.copy() for a in init_fields(G, bc)]
     pins(G, st[0])
     k = C.c_double(0.0)
-    t = lib().orc_solve_fast(1 if mode == "jacobi" else 2, 1 if precision == "f32dc" else 0, G, pc, *_dims(G), bc,
-                             omega, omega if omega_p is None else omega_p, eps, it_eval, max_iter, *st, C.byref(k))
+    with slip(bc, walls == 2):
+        t = lib().orc_solve_fast(1 if mode == "jacobi" else 2, 1 if precision == "f32dc" else 0, G, pc, *_dims(G),
+                                 bc, omega, omega if omega_p is None else omega_p, eps, it_eval, max_iter, *st,
+                                 C.byref(k))
     if t < 0:
         raise MemoryError("orc_solve_fast: allocation failed")
     return t, k.value, st, G

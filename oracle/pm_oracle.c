@@ -99,14 +99,16 @@ void orc_geom_halo(const uint8_t *raw, int nx, int ny, int nz, int bc, uint8_t *
  * halo layers of the non-periodic x and y faces solid, so the domain faces
  * are no-slip walls handled by the same one-sided stencils as interior solid
  * voxels (the reference leaves them fluid, src/geometry.cc:98-106).  walls = 0
- * is orc_geom_halo. */
+ * is orc_geom_halo; walls = 2 (slip walls, orc_set_slip: symmetry planes)
+ * fills those halo layers with the mirror image of the first two interior
+ * layers (x first, over every row and plane, then y). */
 void orc_geom_halo_walls(const uint8_t *raw, int nx, int ny, int nz, int bc, int walls, uint8_t *G)
 {
     const int NX = HX(nx), NY = HY(ny), NZ = HZ(nz);
     int px, py, pz;
     periodic_axes(bc, &px, &py, &pz);
     memset(G, 0, (size_t)NX * NY * NZ);
-    if (walls)
+    if (walls == 1)
         for (int i = 0; i < NZ; i++)
             for (int j = 0; j < NY; j++)
                 for (int k = 0; k < NX; k++) {
@@ -118,6 +120,22 @@ void orc_geom_halo_walls(const uint8_t *raw, int nx, int ny, int nz, int bc, int
         for (int j = 0; j < ny; j++)
             for (int k = 0; k < nx; k++)
                 G[ix(NX, NY, i + 2, j + 2, k + 2)] = raw[c++];
+    if (walls == 2) {
+        if (!px)
+            for (int i = 0; i < NZ; i++)
+                for (int j = 0; j < NY; j++)
+                    for (int e = 0; e < 2; e++) {
+                        G[ix(NX, NY, i, j, 1 - e)] = G[ix(NX, NY, i, j, 2 + e)];
+                        G[ix(NX, NY, i, j, NX - 2 + e)] = G[ix(NX, NY, i, j, NX - 3 - e)];
+                    }
+        if (!py)
+            for (int i = 0; i < NZ; i++)
+                for (int k = 0; k < NX; k++)
+                    for (int e = 0; e < 2; e++) {
+                        G[ix(NX, NY, i, 1 - e, k)] = G[ix(NX, NY, i, 2 + e, k)];
+                        G[ix(NX, NY, i, NY - 2 + e, k)] = G[ix(NX, NY, i, NY - 3 - e, k)];
+                    }
+    }
     exchange_u8(G, nx, ny, nz, px, py, pz);
 }
 
@@ -663,6 +681,25 @@ void orc_pack_code(const int32_t *code, int nx, int ny, int nz, uint8_t *pc)
                 int cz = num % 10, cy = (num / 10) % 10, cx = (num / 100) % 10;
                 pc[c] = (uint8_t)((cx - 1) * 36 + (cy - 1) * 6 + (cz - 1));
             }
+}
+
+/* The build's opt-in slip walls (walls = 2; no reference code): with slip on
+ * a non-periodic x (y) axis the cells of the first interior column k = 2
+ * (row j = 2) keep u (v) at 0 -- their -x (-y) face is the wall -- and after
+ * every sweep the halo layers of those faces hold the mirror images of the
+ * tangential velocities (v, w on x faces; u, w on y faces) and 0 for the
+ * normal one; pressure halos are left as they are.  The halos stay fluid,
+ * so the second derivatives see the mirror values: zero shear at the wall.
+ * Set for the fast-mode sweeps and ghost fills that follow (test setting,
+ * not thread-local). */
+static int g_slip_x = 0, g_slip_y = 0;
+
+void orc_set_slip(int slip, int bc)
+{
+    int px, py, pz;
+    periodic_axes(bc, &px, &py, &pz);
+    g_slip_x = slip && !px;
+    g_slip_y = slip && !py;
 }
 
 #define REAL double

@@ -132,6 +132,10 @@ void orc_fold(const uint8_t *G, int nx, int ny, int nz, int bc,
  * (fold every it_eval sweeps).  From the initial state in p, u, v, w (pins
  * applied); returns the sweeps (final state in p, u, v, w, *kprev the last
  * convergence test's mean(w)/Re) or -1. */
+/* The build's slip walls for the fast-mode sweeps and ghost fills that
+ * follow: slip = 1 on the non-periodic x / y faces of boundary method bc, 0
+ * off (see pm_oracle.c). */
+void orc_set_slip(int slip, int bc);
 int orc_solve_fast(int mode, int dc, const uint8_t *G, const uint8_t *pc, int nx, int ny, int nz, int bc,
                    double omega, double omega_p, double eps, int it_eval, int max_iter,
                    double *p, double *u, double *v, double *w, double *kprev);
