@@ -87,8 +87,13 @@ typedef struct {
                                 1: no-slip walls on the non-periodic x and y faces
                                 (boundary_method 1-4): their halo layers are solid, so the
                                 first cells use the one-sided wall stencils of
-                                src/solver.cc:57-91.  Every mode.  No reference
-                                counterpart (opt-in physical model). */
+                                src/solver.cc:57-91.  Every mode.
+                                2: slip (symmetry) walls on those faces: the halo mirrors
+                                the geometry and the tangential velocities, the normal
+                                velocity is held at 0 on the wall face.  Fast modes only
+                                (PM_E_ARG in reference mode), not in a batch.
+                                1 and 2 have no reference counterpart (opt-in physical
+                                models). */
 } pm_params;
 
 #define PM_MAX_BLOCKS 4096

@@ -29,6 +29,7 @@ struct Grid {
     int64_t total;    // elements to allocate per field
     int px, py, pz;   // periodic axes
     int cb;           // block length of a colour-split row, ceil(NX / 3)
+    int slip;         // slip walls on the non-periodic x / y faces (pm_params.walls = 2)
 
     __host__ __device__ int64_t at(int i, int j, int k) const {
         return base + (int64_t)i * sy + (int64_t)j * sx + k;
@@ -79,6 +80,7 @@ inline Grid make_grid(int nx, int ny, int nz, int bc) {
     g.px = (bc == 0);
     g.py = (bc == 0);
     g.pz = (bc == 0 || bc == 1 || bc == 2);
+    g.slip = 0;
     return g;
 }
 

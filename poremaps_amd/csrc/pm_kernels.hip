@@ -23,18 +23,25 @@ namespace pm {
 // ---------------------------------------------------------------------------
 
 // src/geometry.cc:98-115: interior bytes shifted by the halo, everything else 0
-// (fluid).  walls: the halo layers of the non-periodic x / y faces are solid
-// instead (the build's opt-in no-slip domain walls; periodic halos are then
-// copied by the exchange).
+// (fluid).  The build's opt-in walls on the non-periodic x / y faces (periodic
+// halos are copied by the exchange afterwards): walls = 1, no-slip, makes
+// those halo layers solid; walls = 2, slip (symmetry planes), fills them with
+// the mirror image of the first two interior layers (k = 1, 0 from 2, 3;
+// nx + 2, nx + 3 from nx + 1, nx; rows likewise).
 __global__ void k_geom_fill(Grid g, const uint8_t *__restrict__ raw, uint8_t *__restrict__ G, int walls) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
     if (k >= g.NX) return;
-    const bool in = i >= 2 && i < g.nz + 2 && j >= 2 && j < g.ny + 2 && k >= 2 && k < g.nx + 2;
     const bool hx = k < 2 || k >= g.nx + 2, hy = j < 2 || j >= g.ny + 2;
-    const uint8_t halo = (walls && ((hx && !g.px) || (hy && !g.py))) ? 1 : 0;
-    G[g.at(i, j, k)] = in ? raw[((int64_t)(i - 2) * g.ny + (j - 2)) * g.nx + (k - 2)] : halo;
+    int ks = k, js = j;
+    if (walls == 2) {
+        if (!g.px && hx) ks = k < 2 ? 3 - k : 2 * g.nx + 3 - k;
+        if (!g.py && hy) js = j < 2 ? 3 - j : 2 * g.ny + 3 - j;
+    }
+    const bool in = i >= 2 && i < g.nz + 2 && js >= 2 && js < g.ny + 2 && ks >= 2 && ks < g.nx + 2;
+    const uint8_t halo = (walls == 1 && ((hx && !g.px) || (hy && !g.py))) ? 1 : 0;
+    G[g.at(i, j, k)] = in ? raw[((int64_t)(i - 2) * g.ny + (js - 2)) * g.nx + (ks - 2)] : halo;
 }
 
 // Two-layer periodic copy along one axis over the full extent of the other

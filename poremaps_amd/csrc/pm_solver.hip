@@ -110,7 +110,9 @@ static int check_params(const pm_params *p) {
                     dd[2], p->nx, p->ny, p->nz);
     if (p->z_slabs > 1 && (dd[0] > 1 || dd[1] > 1 || dd[2] > 1))
         return fail(PM_E_ARG, "give either z_slabs or dom_decomposition");
-    if (p->walls < 0 || p->walls > 1) return fail(PM_E_ARG, "walls must be 0 or 1 (got %d)", p->walls);
+    if (p->walls < 0 || p->walls > 2) return fail(PM_E_ARG, "walls must be 0, 1 or 2 (got %d)", p->walls);
+    if (p->walls == 2 && p->mode == PM_MODE_REFERENCE)
+        return fail(PM_E_ARG, "slip walls (walls 2) run in the fast modes (mode jacobi or sor3)");
     int d[3];
     ref_dims(p, d);
     if ((int64_t)d[0] * d[1] * d[2] > PM_MAX_BLOCKS)
@@ -183,6 +185,7 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     c->prm = *p;
     c->auto_relax = p->mode != PM_MODE_REFERENCE && p->omega == 0.0;
     c->g = pm::make_grid(p->nx, p->ny, p->nz, p->boundary_method);
+    c->g.slip = p->walls == 2 ? 1 : 0;
     // FP32+DC 3-colour SOR on colour-split rows: columns of whole rows, so
     // that a wavefront's runs in each block are as long as the row allows.
     // 512^3 field, ms per sweep: 64 rows x whole rows 1.456, 128 x whole 1.455,
@@ -208,12 +211,13 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
         else if (std::strcmp(jm, "march") == 0) c->jac_impl = JI_MARCH;
     }
     if (const char *zs = std::getenv("POREMAPS_STREAM_ZS")) c->stream_zs = std::max(1, std::atoi(zs));
+    if (c->g.slip && (c->jac_impl == JI_STREAM || c->jac_impl == JI_MARCH)) c->jac_impl = JI_RECORDS;
     // z-march form (POREMAPS_JACOBI=march): tile rows 4 or 8, planes per z
     // segment (0: all)
     if (const char *mt = std::getenv("POREMAPS_MARCH_TY")) c->march_ty = std::atoi(mt) == 8 ? 8 : 4;
     if (const char *rk = std::getenv("POREMAPS_REC_SKIP2")) {
         c->rec_skip2 = std::atoi(rk) >= 1;
-        c->rec_skip1 = std::atoi(rk) == 2;
+        c->rec_skip1 = std::atoi(rk) == 2 && !c->g.slip;
     }
     if (const char *sk = std::getenv("POREMAPS_SOR_SKIP2")) c->sor_skip2 = std::atoi(sk) != 0;
     if (const char *ro = std::getenv("POREMAPS_REC_OCC")) {
@@ -653,15 +657,27 @@ static void tuner_setup(pm_ctx *c) {
     // and 64-plane segments measured slower on the 512^3 field: 4.09 and 2.40
     // against 2.37 ms per sweep)
     const SweepVariant march[] = {{JI_MARCH, true, 128, 4}};
+    // slip walls are held and mirrored by the list forms only (not by the
+    // dense 2.5-D form or the z-march, nor by the pass that skips every unused
+    // load, which reads the -1 neighbours by the code's own solid flags)
+    const auto usable = [&](const SweepVariant &v) {
+        if (v.skip2 && !pm::rec_skip2_supported(c->g)) return false;
+        return !(c->g.slip && (v.impl == JI_STREAM || v.impl == JI_MARCH || v.skip1));
+    };
     if (c->dc()) {
         for (const auto &v : aos)
-            if (!v.skip2 || pm::rec_skip2_supported(c->g)) tn.cand[tn.n++] = v;
+            if (usable(v)) tn.cand[tn.n++] = v;
         if (pm::march_supported(c->g, c->lists))
-            for (const auto &v : march) tn.cand[tn.n++] = v;
+            for (const auto &v : march)
+                if (usable(v)) tn.cand[tn.n++] = v;
     }
-    else
+    else {
         for (const auto &v : soa)
-            if (!v.skip2 || pm::rec_skip2_supported(c->g)) tn.cand[tn.n++] = v;
+            if (usable(v)) tn.cand[tn.n++] = v;
+        // the unscored first window's form, scored again at the end (the
+        // dense form is not there under slip)
+        if (c->g.slip) tn.cand[tn.n++] = tn.cand[0];
+    }
     tn.first_scored = 1;
     tn.active = true;
 }

@@ -19,6 +19,52 @@ __device__ __forceinline__ void cell_ijk(const Grid &g, int64_t c, int &i, int &
     k = (int)(r - (uint32_t)j * sx);
 }
 
+// Slip walls (Grid::slip): a cell of the first interior column (row) next to
+// a slip x (y) face has the wall on its -x (-y) face, where u (v) is held at
+// 0 -- the same as a solid -1 neighbour does to that component and its
+// pressure difference.  Its second derivatives keep their own cases: they see
+// the mirrored halos.  j, k: the cell's indices with the halo (k == 2: the
+// first interior column).
+__device__ __forceinline__ void slip_hold(const Grid &g, int j, int k, Cases &cs) {
+    if (!g.slip) return;
+    if (!g.px && k == 2) cs.bx = true;
+    if (!g.py && j == 2) cs.by = true;
+}
+__device__ __forceinline__ void slip_hold_at(const Grid &g, int64_t c, Cases &cs) {
+    if (!g.slip) return;
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    slip_hold(g, j, k, cs);
+}
+
+// The cell's mirror images in the slip-wall halos (Grid::slip), written by
+// the passes that write their cells' halo copies themselves: columns k = 2, 3
+// mirror into k = 1, 0 and k = nx + 1, nx into nx + 2, nx + 3 with u = 0, rows
+// likewise with v = 0; p = 0 (the halo pressure of a correction state).
+typedef float slip_v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void slip_images4(const Grid &g, int64_t c, int j, int k, slip_v4f o, float4 *S1) {
+    if (!g.slip) return;
+    if (!g.px) {
+        slip_v4f m = o;
+        m.x = 0.f;
+        m.y = 0.f;
+        if (k == 2) __builtin_nontemporal_store(m, reinterpret_cast<slip_v4f *>(&S1[c - 1]));
+        if (k == 3) __builtin_nontemporal_store(m, reinterpret_cast<slip_v4f *>(&S1[c - 3]));
+        if (k == g.nx + 1) __builtin_nontemporal_store(m, reinterpret_cast<slip_v4f *>(&S1[c + 1]));
+        if (k == g.nx) __builtin_nontemporal_store(m, reinterpret_cast<slip_v4f *>(&S1[c + 3]));
+    }
+    if (!g.py) {
+        slip_v4f m = o;
+        m.x = 0.f;
+        m.z = 0.f;
+        const int64_t sx = g.sx;
+        if (j == 2) __builtin_nontemporal_store(m, reinterpret_cast<slip_v4f *>(&S1[c - sx]));
+        if (j == 3) __builtin_nontemporal_store(m, reinterpret_cast<slip_v4f *>(&S1[c - 3 * sx]));
+        if (j == g.ny + 1) __builtin_nontemporal_store(m, reinterpret_cast<slip_v4f *>(&S1[c + sx]));
+        if (j == g.ny) __builtin_nontemporal_store(m, reinterpret_cast<slip_v4f *>(&S1[c + 3 * sx]));
+    }
+}
+
 __device__ __forceinline__ bool on_outlet(const Grid &g, int64_t c) {
     const int64_t o0 = g.at(g.nz + 1, 0, 0);
     return c >= o0 && c < o0 + g.sy;
@@ -30,7 +76,8 @@ __device__ __forceinline__ void vel_voxel(const Grid &g, int64_t c, uint8_t pcv,
                                           const T *__restrict__ u0, const T *__restrict__ v0,
                                           const T *__restrict__ w0, const T *__restrict__ q0,
                                           T *__restrict__ u1, T *__restrict__ v1, T *__restrict__ w1, T cu) {
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold_at(g, c, cs);
     const bool out = on_outlet(g, c);
     const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
     T fu, fv, fw;
@@ -163,7 +210,8 @@ __device__ __forceinline__ void lag_voxel(const Grid &g, int64_t c, uint8_t pcv,
                                           const T *__restrict__ v0, const T *__restrict__ w0,
                                           const T *__restrict__ q0, T *__restrict__ p1, T *__restrict__ u1,
                                           T *__restrict__ v1, T *__restrict__ w1, T cu, T cp, T pin_out) {
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold_at(g, c, cs);
     int i, j, k;
     cell_ijk(g, c, i, j, k);
     const bool out = i == g.nz + 1;
@@ -305,7 +353,8 @@ __device__ __forceinline__ float4 vel_from4(const Cases &cs, const float4 &C, fl
 __device__ __forceinline__ void vel_voxel4(const Grid &g, int64_t c, int t, uint8_t pcv,
                                            const float4 *__restrict__ S0, const float *__restrict__ q0,
                                            float4 *__restrict__ S1, Res4 R, float cu) {
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold_at(g, c, cs);
     const bool out = on_outlet(g, c);
     const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
     const float4 C = S0[c];
@@ -372,7 +421,8 @@ __device__ __forceinline__ void lag_voxel4(const Grid &g, int64_t c, int t, uint
                                            const uint8_t *__restrict__ pc, const float4 *__restrict__ S0,
                                            const float *__restrict__ q0, float4 *__restrict__ S1, Res4 R, float cu,
                                            float cp, float pin_out) {
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold_at(g, c, cs);
     int i, j, k;
     cell_ijk(g, c, i, j, k);
     const bool out = i == g.nz + 1;
@@ -445,7 +495,8 @@ __device__ __forceinline__ void lag_voxel4_eager(const Grid &g, int64_t c, int t
     if (wx) { NX = S0[nxn]; uxx = S0[nxn + 1].y; }
     if (wy) { NY = S0[nyn]; vyy = S0[nyn + sx].z; }
 
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold_at(g, c, cs);
     const float Pc = out ? pin_out : C.x - (div_from4(pcv, C, s.XP.y, s.YP.z, s.ZP.w) + rpc) * cp;
     const float c2dt = (float)(kC2 * kDt);
     const float dx0 = div_from4(pcx, NX, uxx, vyx, wzx);
@@ -480,7 +531,8 @@ __device__ __forceinline__ void first_voxel4_zero(const Grid &g, int t, const ui
     typedef unsigned int v2u __attribute__((ext_vector_type(2)));
     const v2u rc = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&rec[t]));
     const int64_t c = (int64_t)rc.x;
-    const Cases cs = decode((uint8_t)(rc.y & 0xff));
+    Cases cs = decode((uint8_t)(rc.y & 0xff));
+    slip_hold_at(g, c, cs);
     const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
     const float z = 0.f;
     v4f o;
@@ -506,6 +558,7 @@ __device__ __forceinline__ void first_voxel4_zero(const Grid &g, int t, const ui
         if (j >= g.ny) __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c - ys]));
         if (j <= 3) __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[c + ys]));
     }
+    slip_images4(g, c, j, k, o, S1);
 }
 
 // The cells whose pressure the lagged pass at c recomputes: the -x and -y
@@ -593,7 +646,8 @@ __device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2
     if (wx) { NX = S0[nxn]; uxx = S0[nxn + 1].y; }
     if (wy) { NY = S0[nyn]; vyy = S0[nyn + sx].z; }
 
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold(g, j, k, cs);
     const float Pc = out ? pin_out : C.x - (div_from4(pcv, C, s.XP.y, s.YP.z, s.ZP.w) + rpc) * cp;
     const float c2dt = (float)(kC2 * kDt);
     const float dx0 = div_from4(pcx, NX, uxx, vyx, wzx);
@@ -642,6 +696,7 @@ __device__ __forceinline__ void lag_voxel4_rec(const Grid &g, int t, const uint2
             if (j >= g.ny) __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c - ys]));
             if (j <= 3) __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[c + ys]));
         }
+        slip_images4(g, c, j, k, ov, S1);
     }
 }
 
@@ -723,7 +778,8 @@ __device__ __forceinline__ void lag_voxel_rec(const Grid &g, int t, const uint2 
     if (wx) { uX = u0[nxn]; vX = v0[nxn]; wX = w0[nxn]; uXp = u0[nxn + 1]; }
     if (wy) { uY = u0[nyn]; vY = v0[nyn]; wY = w0[nyn]; vYp = v0[nyn + sx]; }
 
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold(g, j, k, cs);
     // lag_div0 at the cell and at the three targets
     const auto div0 = [](uint8_t code, T u, T v, T w, T uxp1, T vyp1, T wzp1) {
         const Cases d = decode(code);
@@ -796,7 +852,8 @@ __device__ __forceinline__ void colour_voxel4(const Grid &g, int64_t c, int t, c
     const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
     float pxm = s.XM.x, pym = s.YM.x;
     if (out) { pxm = q[qc - 1]; pym = q[qc - g.sx]; }
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold_at(g, c, cs);
     const float dpx = cs.bx ? 0.f : C.x - pxm;
     const float dpy = cs.by ? 0.f : C.x - pym;
     const float dpz = cs.bz ? 0.f : C.x - s.ZM.x;
@@ -867,7 +924,14 @@ __device__ __forceinline__ void colour_voxel4_split(const Grid &g, int t, const 
     const int64_t qc = c - (int64_t)(g.nz + 1) * si;
     float pxm = s.XM.x, pym = s.YM.x;
     if (out) { pxm = q[qc + oM1]; pym = q[qc - sj]; }
-    const Cases cs = decode((uint8_t)(f & 0xffu));
+    Cases cs = decode((uint8_t)(f & 0xffu));
+    if (g.slip) {
+        // c is a colour-split offset: the first interior column (k = 2) is
+        // position 0 of block 2 of its row
+        int i2, j2, r2;
+        cell_ijk(g, c, i2, j2, r2);
+        slip_hold(g, j2, r2 == 2 * g.cb ? 2 : -1, cs);
+    }
     const float dpx = cs.bx ? 0.f : C.x - pxm;
     const float dpy = cs.by ? 0.f : C.x - pym;
     const float dpz = cs.bz ? 0.f : C.x - s.ZM.x;
@@ -913,8 +977,18 @@ __device__ __forceinline__ void ghost_line4(const Grid &g, int64_t t, const uint
     auto sat = [&](int i, int j, int k) { return SPLIT ? g.at_s(i, j, k) : g.at(i, j, k); };
     auto sqat = [&](int j, int k) { return SPLIT ? g.qat_s(j, k) : g.qat(j, k); };
     if (t < nX) {
-        if (!g.px) return;
         const int i = (int)(t / g.ny) + 2, j = (int)(t % g.ny) + 2;
+        if (!g.px) {
+            // slip walls: v, w mirrored, u and p 0
+            if (g.slip)
+                for (int e = 0; e < 2; e++) {
+                    float4 m = S[sat(i, j, 2 + e)], h = S[sat(i, j, g.nx + 1 - e)];
+                    m.x = 0.f; m.y = 0.f; h.x = 0.f; h.y = 0.f;
+                    S[sat(i, j, 1 - e)] = m;
+                    S[sat(i, j, g.nx + 2 + e)] = h;
+                }
+            return;
+        }
         const int src[4] = {g.nx, g.nx + 1, 2, 3};
         const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
         for (int e = 0; e < 4; e++) {
@@ -924,9 +998,19 @@ __device__ __forceinline__ void ghost_line4(const Grid &g, int64_t t, const uint
             if (q && i == iout) q[sqat(j, dst[e])] = q[sqat(j, src[e])];
         }
     } else if (t < nX + nY) {
-        if (!g.py) return;
         const int64_t tt = t - nX;
         const int i = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
+        if (!g.py) {
+            // slip walls: u, w mirrored, v and p 0
+            if (g.slip)
+                for (int e = 0; e < 2; e++) {
+                    float4 m = S[sat(i, 2 + e, k)], h = S[sat(i, g.ny + 1 - e, k)];
+                    m.x = 0.f; m.z = 0.f; h.x = 0.f; h.z = 0.f;
+                    S[sat(i, 1 - e, k)] = m;
+                    S[sat(i, g.ny + 2 + e, k)] = h;
+                }
+            return;
+        }
         const int src[4] = {g.ny, g.ny + 1, 2, 3};
         const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
         for (int e = 0; e < 4; e++) {
@@ -982,7 +1066,8 @@ __device__ __forceinline__ void fold_voxel4_split(int64_t c, int64_t cs, double 
 template <typename T, bool SEL>
 __device__ __forceinline__ void colour_voxel(const Grid &g, int64_t c, uint8_t pcv, T *p, T *u, T *v, T *w,
                                              T *q, T cu, T cp, T pin_out) {
-    const Cases cs = decode(pcv);
+    Cases cs = decode(pcv);
+    slip_hold_at(g, c, cs);
     const bool out = on_outlet(g, c);
     const int64_t qc = c - (int64_t)(g.nz + 1) * g.sy;
     T fu, fv, fw;
@@ -1016,9 +1101,18 @@ __device__ __forceinline__ void ghost_line(const Grid &g, int64_t t, const uint8
     const int64_t nX = (int64_t)g.nz * g.ny, nY = (int64_t)g.nz * g.nx, nZ = (int64_t)g.ny * g.nx;
     const int iout = g.nz + 1;
     if (t < nX) {
-        if (!g.px) return;
         const int i = (int)(t / g.ny) + 2, j = (int)(t % g.ny) + 2;
         const int64_t r = g.at(i, j, 0);
+        if (!g.px) {
+            // slip walls: v, w mirrored, u 0, p left as it is
+            if (g.slip)
+                for (int e = 0; e < 2; e++) {
+                    const int64_t lo = r + 1 - e, lm = r + 2 + e, hi = r + g.nx + 2 + e, hm = r + g.nx + 1 - e;
+                    u[lo] = (T)0; v[lo] = v[lm]; w[lo] = w[lm];
+                    u[hi] = (T)0; v[hi] = v[hm]; w[hi] = w[hm];
+                }
+            return;
+        }
         const int src[4] = {g.nx, g.nx + 1, 2, 3};
         const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
         for (int e = 0; e < 4; e++) {
@@ -1028,9 +1122,19 @@ __device__ __forceinline__ void ghost_line(const Grid &g, int64_t t, const uint8
             if (q && i == iout) q[g.qat(j, dst[e])] = q[g.qat(j, src[e])];
         }
     } else if (t < nX + nY) {
-        if (!g.py) return;
         const int64_t tt = t - nX;
         const int i = (int)(tt / g.nx) + 2, k = (int)(tt % g.nx) + 2;
+        if (!g.py) {
+            // slip walls: u, w mirrored, v 0, p left as it is
+            if (g.slip)
+                for (int e = 0; e < 2; e++) {
+                    const int64_t lo = g.at(i, 1 - e, k), lm = g.at(i, 2 + e, k);
+                    const int64_t hi = g.at(i, g.ny + 2 + e, k), hm = g.at(i, g.ny + 1 - e, k);
+                    v[lo] = (T)0; u[lo] = u[lm]; w[lo] = w[lm];
+                    v[hi] = (T)0; u[hi] = u[hm]; w[hi] = w[hm];
+                }
+            return;
+        }
         const int src[4] = {g.ny, g.ny + 1, 2, 3};
         const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
         for (int e = 0; e < 4; e++) {

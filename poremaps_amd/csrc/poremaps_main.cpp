@@ -15,8 +15,9 @@
 //   --omega=<w>                         POREMAPS_OMEGA (0, the fast modes' default: the pair by porosity)
 //   --omega-p=<w>                       POREMAPS_OMEGA_P (pressure relaxation; default omega)
 //   --device=<n>                        POREMAPS_DEVICE
-//   --walls=reference|noslip            POREMAPS_WALLS (noslip: the non-periodic x / y faces
-//                                       are no-slip walls; the reference leaves them open)
+//   --walls=reference|noslip|slip       POREMAPS_WALLS (noslip / slip: the non-periodic x / y
+//                                       faces are no-slip / slip walls; the reference leaves them
+//                                       open; slip: fast modes only)
 // Deviation: hard errors (missing file, bad .inp, bad geometry, HIP failure)
 // exit non-zero instead of continuing.
 #include "../../include/poremaps_amd.h"
@@ -106,6 +107,7 @@ int main(int argc, char **argv) {
     if (knob(flags, "--walls", "POREMAPS_WALLS", &v)) {
         if (v == "reference") prm.walls = 0;
         else if (v == "noslip") prm.walls = 1;
+        else if (v == "slip") prm.walls = 2;
         else { std::fprintf(stderr, "unknown walls %s\n", v.c_str()); return 2; }
     }
 
