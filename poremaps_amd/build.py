@@ -76,6 +76,12 @@ def build_cli() -> str:
 
 def build_oracle() -> str:
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    # The reference binary (CPU baseline and fixture source) is compiled from
+    # the reference's own sources where they lie; where they are absent (the
+    # machine that runs the GPU work) the copy built beside them travels with
+    # the tree.
+    if os.path.isdir("/root/reference/src") and os.path.exists("/opt/conda/include/mpi.h"):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     return os.path.join(ROOT, "oracle", "libpm_oracle.so")
 
 

@@ -65,6 +65,17 @@ void launch_ref_block_sweep(const Grid &g, const uint8_t *G, const int32_t *code
 void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
                        double *v, double *w, int solver, int nsweeps, int impl, hipStream_t s);
 
+// An entry of the compact SOR state: the entries of its -x, +x, -y, +y, -z,
+// +z neighbours (the zero entry where the neighbour is solid, a wall halo or
+// beyond the halo), the cell's colour-split offset (its outlet-buffer
+// position) and its colour pass record flags (packed code, block k mod 3 in
+// bits 8-9, outlet flag in bit 10; halo and zero entries: kSolid).
+struct CEnt {
+    int32_t xm, xp, ym, yp, zm, zp;
+    uint32_t off, f;
+};
+static_assert(sizeof(CEnt) == 32, "CEnt is two 16-byte loads");
+
 // Device lists of the interior fluid cells' element offsets in column order
 // (columns of tile_y rows x tile_x cells, each marching along z; within a
 // plane of a column ascending y, x): all of them, and split by colour
@@ -124,6 +135,20 @@ struct FluidLists {
     // FP32+DC SOR (colour-split rows): the same for the colour passes
     bool sor_skip2 = true;
     std::vector<int32_t> plane_off;
+    // FP32+DC 3-colour SOR on the compact state (CEnt): the correction state
+    // holds one float4 per fluid cell in list order (entry t = list position
+    // t), then one per fluid cell of a periodic face halo (a copy of its
+    // image), then one zero entry that stands for every solid, wall-halo or
+    // otherwise unused neighbour.  cent: the entries' links and records;
+    // chalo: (halo entry, image entry | bit 31 when the copy's p is 0);
+    // cout: the entries of the outlet plane (p pinned to 0 after a sweep).
+    bool compact = false;
+    CEnt *cent = nullptr;
+    int n_centries = 0;
+    int2 *chalo = nullptr;
+    int n_chalo = 0;
+    int32_t *cout = nullptr;
+    int n_cout = 0;
 };
 
 // FP64 fast modes (per-field state p, u, v, w): the velocity + pressure
@@ -165,6 +190,13 @@ void launch_gather_rq(const Grid &g, const FluidLists &L, Res4 R, hipStream_t s)
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
                       const double *v, const double *w, const double *q, Res4 R, hipStream_t s);
 void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, bool split, hipStream_t s);
+// The ghost fill of the correction state in the lists' layout: the compact
+// state's halo copies and outlet pins (and the outlet buffer's periodic
+// copies), else launch_ghost4 (natural or colour-split rows).
+void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, float4 *S, float *q, hipStream_t s);
+// Whether the compact SOR state can hold this medium: no slip walls, and
+// every entry's byte offset below 2^32 (its masked loads are buffer loads).
+bool compact_supported(const Grid &g, int64_t n_entries);
 // X += d, d = 0 over the fluid list; d's halo copies need a ghost fill after.
 void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s);
 void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p0,

@@ -157,6 +157,7 @@ static void free_all(pm_ctx *c) {
     f(c->lists.all);
     f(c->lists.rec);
     f(c->lists.rowstart);
+    f(c->lists.cent); f(c->lists.chalo); f(c->lists.cout);
     for (int q = 0; q < 4; q++) { f(c->ref_snap[q]); c->ref_snap[q] = nullptr; }
     f(c->ref_outlet); f(c->ref_tab);
     c->ref_outlet = nullptr; c->ref_tab = nullptr; c->ref_nblk = 0;
@@ -352,6 +353,96 @@ int pm_restart_slower(pm_ctx *c) {
     return PM_OK;
 }
 
+// The compact SOR state's tables (FluidLists::compact) from the interior
+// geometry raw (x fastest, 0 fluid) and the fluid list `all` (natural
+// offsets, colour lists back to back): entry t is list position t; then one
+// entry per fluid cell of a periodic face halo, with the image it copies
+// (the dense ghost fill's sources and pins); then the zero entry.  Links by
+// coordinates, so that nothing beyond the halo box is ever an entry.  The
+// records' part of the interior entries is filled on the device from the
+// colour passes' records (launch_lag_records).  Falls back to the
+// colour-split rows where the medium does not fit the compact form.
+static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_t> &all) {
+    const pm::Grid &g = c->g;
+    const int64_t n_all = (int64_t)all.size();
+    std::vector<int32_t> cidx((size_t)g.total, -1);
+    for (int64_t t = 0; t < n_all; t++) cidx[(size_t)all[(size_t)t]] = (int32_t)t;
+    auto fluid_at = [&](int i, int j, int k) {  // halo coordinates of an interior cell
+        return raw[((int64_t)(i - 2) * g.ny + (j - 2)) * g.nx + (k - 2)] == 0;
+    };
+    std::vector<int64_t> hoff;   // natural offsets of the halo entries
+    std::vector<int2> halo;      // (entry, image entry | bit 31: p = 0)
+    int64_t next = n_all;
+    auto add = [&](int i, int j, int k, int si, int sj, int sk, bool zero_p) {
+        if (!fluid_at(si, sj, sk)) return;
+        const int32_t e = cidx[(size_t)g.at(si, sj, sk)];
+        cidx[(size_t)g.at(i, j, k)] = (int32_t)next;
+        hoff.push_back(g.at(i, j, k));
+        halo.push_back(make_int2((int)next, zero_p ? (int)((uint32_t)e | 0x80000000u) : (int)e));
+        next++;
+    };
+    const int iout = g.nz + 1;
+    if (g.px)
+        for (int i = 2; i < g.nz + 2; i++)
+            for (int j = 2; j < g.ny + 2; j++) {
+                const int src[4] = {g.nx, g.nx + 1, 2, 3}, dst[4] = {0, 1, g.nx + 2, g.nx + 3};
+                for (int e = 0; e < 4; e++) add(i, j, dst[e], i, j, src[e], i == iout);
+            }
+    if (g.py)
+        for (int i = 2; i < g.nz + 2; i++)
+            for (int k = 2; k < g.nx + 2; k++) {
+                const int src[4] = {g.ny, g.ny + 1, 2, 3}, dst[4] = {0, 1, g.ny + 2, g.ny + 3};
+                for (int e = 0; e < 4; e++) add(i, dst[e], k, i, src[e], k, i == iout);
+            }
+    if (g.pz)
+        for (int j = 2; j < g.ny + 2; j++)
+            for (int k = 2; k < g.nx + 2; k++) {
+                // the copies of the outlet plane and the pinned planes 1 and
+                // nz + 2 carry p = 0
+                const int src[4] = {g.nz, g.nz + 1, 2, 3}, dst[4] = {0, 1, g.nz + 2, g.nz + 3};
+                for (int e = 0; e < 4; e++) add(dst[e], j, k, src[e], j, k, dst[e] == 1 || dst[e] == g.nz + 2);
+            }
+    const int64_t zero = next, n_ent = next + 1;
+    if (!pm::compact_supported(g, n_ent)) return PM_OK;
+    std::vector<pm::CEnt> ent((size_t)n_ent);
+    std::vector<int32_t> outl;
+    auto link = [&](int64_t off, pm::CEnt &E) {
+        const int64_t r = off - g.base;
+        const int i = (int)(r / g.sy), j = (int)((r % g.sy) / g.sx), k = (int)(r % g.sx);
+        auto at = [&](int ii, int jj, int kk) -> int32_t {
+            if (ii < 0 || ii >= g.NZ || jj < 0 || jj >= g.NY || kk < 0 || kk >= g.NX) return (int32_t)zero;
+            const int32_t e = cidx[(size_t)g.at(ii, jj, kk)];
+            return e < 0 ? (int32_t)zero : e;
+        };
+        E.xm = at(i, j, k - 1); E.xp = at(i, j, k + 1);
+        E.ym = at(i, j - 1, k); E.yp = at(i, j + 1, k);
+        E.zm = at(i - 1, j, k); E.zp = at(i + 1, j, k);
+        E.off = 0;
+        E.f = pm::kSolid;
+        return i;
+    };
+    for (int64_t t = 0; t < n_all; t++)
+        if (link(all[(size_t)t], ent[(size_t)t]) == iout) outl.push_back((int32_t)t);
+    for (size_t h = 0; h < hoff.size(); h++) link(hoff[h], ent[(size_t)(n_all + (int64_t)h)]);
+    pm::CEnt &Z = ent[(size_t)zero];
+    Z.xm = Z.xp = Z.ym = Z.yp = Z.zm = Z.zp = (int32_t)zero;
+    Z.off = 0;
+    Z.f = pm::kSolid;
+    RC(dalloc(&c->lists.cent, n_ent));
+    HIPCHK(hipMemcpy(c->lists.cent, ent.data(), sizeof(pm::CEnt) * ent.size(), hipMemcpyHostToDevice));
+    RC(dalloc(&c->lists.chalo, std::max<int64_t>(1, (int64_t)halo.size())));
+    if (!halo.empty())
+        HIPCHK(hipMemcpy(c->lists.chalo, halo.data(), sizeof(int2) * halo.size(), hipMemcpyHostToDevice));
+    RC(dalloc(&c->lists.cout, std::max<int64_t>(1, (int64_t)outl.size())));
+    if (!outl.empty())
+        HIPCHK(hipMemcpy(c->lists.cout, outl.data(), sizeof(int32_t) * outl.size(), hipMemcpyHostToDevice));
+    c->lists.n_centries = (int)n_ent;
+    c->lists.n_chalo = (int)halo.size();
+    c->lists.n_cout = (int)outl.size();
+    c->lists.compact = true;
+    return PM_OK;
+}
+
 extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     if (!c || !raw) return fail(PM_E_ARG, "null argument");
     HIPCHK(hipSetDevice(c->prm.device));
@@ -458,6 +549,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
             // POREMAPS_SOR=natural.
             const char *so = std::getenv("POREMAPS_SOR");
             c->lists.split = !(so && std::strcmp(so, "natural") == 0);
+            if (so && std::strcmp(so, "compact") == 0) c->sor_compact = true;
             all.clear();
             for (int q = 0; q < 3; q++) {
                 c->lists.off_col[q] = (int)all.size();
@@ -494,6 +586,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         if (sor && c->dc()) {
             c->lists.col_in_all = true;
             for (int q = 0; q < 3; q++) c->lists.col[q] = c->lists.all + c->lists.off_col[q];
+            if (c->lists.split && c->sor_compact && c->lists.sor_chunk == 0) RC(build_compact(c, raw, all));
         } else if (sor)
             for (int q = 0; q < 3; q++) {
                 c->lists.n_col[q] = (int)col[q].size();
@@ -557,7 +650,7 @@ static int settle(pm_ctx *c) {
     if (c->dc()) {
         pm::launch_jacobi4(pm::JAC_SETTLE, g, c->lists, c->pc, c->S4[0], c->qD[0], c->S4[0], c->qD[0], c->R4,
                            (float)cu, (float)cp, s);
-        pm::launch_ghost4(g, c->G, c->S4[0], c->qD[0], c->lists.split, s);
+        pm::launch_ghost4_lists(g, c->lists, c->G, c->S4[0], c->qD[0], s);
     } else {
         pm::launch_jacobi_stage(pm::JAC_SETTLE, g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0],
                                 c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], cu, cp, 3.0, s);
@@ -595,7 +688,7 @@ static void jacobi_pass(pm_ctx *c, double *const *A, double *qa, double *const *
 static void fold_now(pm_ctx *c) {
     const pm::Grid &g = c->g;
     pm::launch_fold4(c->lists, c->X[0], c->X[1], c->X[2], c->X[3], c->S4[0], c->stream);
-    pm::launch_ghost4(g, c->G, c->S4[0], c->qD[0], c->lists.split, c->stream);
+    pm::launch_ghost4_lists(g, c->lists, c->G, c->S4[0], c->qD[0], c->stream);
 }
 
 static void apply_variant(pm_ctx *c, const SweepVariant &v) {
@@ -798,7 +891,7 @@ static int one_sweep(pm_ctx *c) {
             c->dzero = false;
             te(c);
             tb(c, TC_GHOST);
-            pm::launch_ghost4(g, c->G, c->S4[0], c->qD[0], c->lists.split, s);
+            pm::launch_ghost4_lists(g, c->lists, c->G, c->S4[0], c->qD[0], s);
             te(c);
         }
         if (c->t % c->prm.it_eval == 0) {
@@ -1113,6 +1206,7 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
         std::snprintf(buf, sizeof buf, "interleaved-colour-split%s%s", z,
                       c->lists.sor_skip2 && pm::rec_skip2_supported(c->g) ? "-skip2" : "");
         sor = buf;
+        if (c->lists.compact) sor = "interleaved-colour-compact";
     }
     std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(c, cur, buf, sizeof buf) : sor);
     if (tried && tried_len > 0) {

@@ -138,6 +138,146 @@ __global__ __launch_bounds__(kListBlock) void k_colour4_stage(Grid g, const uint
     colour_voxel4_split<SKIP2>(g, st.t0[col] + t, rec, S, q, R, cu, cp, 0.f);
 }
 
+// ---------------------------------------------------------------------------
+// 3-colour SOR on the compact correction state (FluidLists::compact).  The
+// colour-split rows keep every cell of the domain, so at porosity 0.2 a wave's
+// 64 fluid cells and their neighbours lie on about 40 lines of 128 B per load
+// and a colour pass reads nearly every line of the state (1.6 GB of a 512^3
+// field, three times per sweep).  Here the state holds the fluid cells only,
+// in list order, so the cells of one colour pass and their neighbours (the
+// other two colours' cells of the same rows, in the same column order) lie on
+// about a third of those lines; each entry carries the entries of its six
+// +-1 neighbours, and a +-2 neighbour is the +-1 neighbour's own link on the
+// same axis (read, like the value behind it, only by the lanes whose case
+// uses it).  Periodic halo cells have entries of their own that the ghost
+// fill refreshes after each sweep, as the dense halo copies are, so every
+// value read is the one colour_voxel4_split reads, and the same expressions
+// in the same order give the same bits.
+// ---------------------------------------------------------------------------
+
+// One 32-bit word of the entry table (word w of entry e), only for lanes with
+// need set (buffer load; an out-of-range offset makes no request).
+__device__ __forceinline__ int32_t buf_load_link(__amdgpu_buffer_rsrc_t rs, int32_t e, int w, bool need) {
+    const uint32_t off = need ? (uint32_t)e * 32u + 4u * (uint32_t)w : 0xFFFFFFF0u;
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+}
+
+__device__ __forceinline__ void colour_voxel4_compact(const Grid &g, int t, const CEnt *__restrict__ E,
+                                                      __amdgpu_buffer_rsrc_t rsE, __amdgpu_buffer_rsrc_t rsS,
+                                                      float4 *S, float *q, Res4 R, float cu, float cp,
+                                                      float pin_out) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4i e0 = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(&E[t]));
+    const v4i e1 = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(&E[t]) + 1);
+    const uint32_t f = (uint32_t)e1.w;
+    const int b = (int)((f >> 8) & 3u);
+    const bool out = (f >> 10) & 1u;
+    const Cases c2 = decode((uint8_t)(f & 0xffu));
+    const float4 C = S[t];
+    Star4 s;
+    s.XM = S[e0.x]; s.XP = S[e0.y];
+    s.YM = S[e0.z]; s.YP = S[e0.w];
+    s.ZM = S[e1.x]; s.ZP = S[e1.y];
+    const bool nxm = c2.cx == 5, nxp = c2.cx == 4, nym = c2.cy == 5, nyp = c2.cy == 4;
+    const bool nzm = c2.cz == 5, nzp = c2.cz == 4;
+    const int32_t xm2 = buf_load_link(rsE, e0.x, 0, nxm), xp2 = buf_load_link(rsE, e0.y, 1, nxp);
+    const int32_t ym2 = buf_load_link(rsE, e0.z, 2, nym), yp2 = buf_load_link(rsE, e0.w, 3, nyp);
+    const int32_t zm2 = buf_load_link(rsE, e1.x, 4, nzm), zp2 = buf_load_link(rsE, e1.y, 5, nzp);
+    s.XM2 = buf_load4(rsS, xm2, nxm); s.XP2 = buf_load4(rsS, xp2, nxp);
+    s.YM2 = buf_load4(rsS, ym2, nym); s.YP2 = buf_load4(rsS, yp2, nyp);
+    s.ZM2 = buf_load4(rsS, zm2, nzm); s.ZP2 = buf_load4(rsS, zp2, nzp);
+    const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
+    const float4 Rm = make_float4(r.x, r.y, r.z, 0.f);
+    const float rp = r.w;
+    const int64_t cb = g.cb, sj = g.sx;
+    const int64_t oM1 = b == 0 ? 2 * cb - 1 : -cb;
+    const int64_t qc = (int64_t)e1.z - (int64_t)(g.nz + 1) * g.sy;
+    float pxm = s.XM.x, pym = s.YM.x;
+    if (out) { pxm = q[qc + oM1]; pym = q[qc - sj]; }
+    const Cases cs = c2;
+    const float dpx = cs.bx ? 0.f : C.x - pxm;
+    const float dpy = cs.by ? 0.f : C.x - pym;
+    const float dpz = cs.bz ? 0.f : C.x - s.ZM.x;
+    float su, sv, sw;
+    visc_from4(cs, s, C, su, sv, sw);
+    float4 N = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rm, C.x, cu);
+    const float wz = fwd_solid(cs.cz) ? 0.f : s.ZP.w;
+    const float vy = fwd_solid(cs.cy) ? 0.f : s.YP.z;
+    const float ux = fwd_solid(cs.cx) ? 0.f : s.XP.y;
+    const float div0 = ((((wz - N.w) + vy) - N.z) + ux) - N.y;
+    const float div = div0 + rp;
+    N.x = C.x - div * cp;
+    S[t] = N;
+    if (out) q[qc] = pin_out - div0 * (float)(kC2 * kDt);
+}
+
+// One colour's entries t0 .. t0 + n - 1 (its list positions).  nE: entries
+// of the table (the buffers' extents).
+__global__ __launch_bounds__(kListBlock) void k_colour4_compact(Grid g, const CEnt *__restrict__ E, int nE, int n,
+                                                                int t0, float4 *S, float *q, Res4 R, float cu,
+                                                                float cp) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const __amdgpu_buffer_rsrc_t rsE =
+        __builtin_amdgcn_make_buffer_rsrc((void *)E, (short)0, (int)((uint32_t)nE * 32u), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsS =
+        __builtin_amdgcn_make_buffer_rsrc((void *)S, (short)0, (int)((uint32_t)nE * 16u), 0x00020000);
+    colour_voxel4_compact(g, t0 + t, E, rsE, rsS, S, q, R, cu, cp, 0.f);
+}
+
+// The compact state's ghost fill (ghost_line4 on the entries): halo entries
+// copied from their images (p 0 where the dense fill pins it: copies of the
+// outlet plane and the z-halo planes 1 and nz + 2), the outlet plane's p
+// pinned to 0, then the outlet buffer's periodic x / y copies in colour-split
+// rows.  A copy whose image is an outlet entry zeroes p itself, so it does not
+// depend on the order of the two writes.
+__global__ void k_ghost4_compact(Grid g, const int2 *__restrict__ halo, int nh, const int32_t *__restrict__ outl,
+                                 int no, float4 *S, float *q) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nh) {
+        const int2 h = halo[t];
+        float4 v = S[h.y & 0x7fffffff];
+        if (h.y < 0) v.x = 0.f;
+        S[h.x] = v;
+        return;
+    }
+    int64_t u = t - nh;
+    if (u < no) {
+        S[outl[u]].x = 0.f;
+        return;
+    }
+    u -= no;
+    const int iout = g.nz + 1;
+    (void)iout;
+    if (u < g.ny) {
+        if (!g.px) return;
+        const int j = (int)u + 2;
+        const int src[4] = {g.nx, g.nx + 1, 2, 3};
+        const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
+        for (int e = 0; e < 4; e++) q[g.qat_s(j, dst[e])] = q[g.qat_s(j, src[e])];
+        return;
+    }
+    u -= g.ny;
+    if (u < g.nx) {
+        if (!g.py) return;
+        const int k = (int)u + 2;
+        const int src[4] = {g.ny, g.ny + 1, 2, 3};
+        const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
+        for (int e = 0; e < 4; e++) q[g.qat_s(dst[e], k)] = q[g.qat_s(src[e], k)];
+    }
+}
+
+// The records' part of the compact entries (offset and flags of the colour
+// passes' records), list positions 0 .. n - 1.
+__global__ __launch_bounds__(kListBlock) void k_compact_records(const uint2 *__restrict__ rec, int n, CEnt *E) {
+    const int t = blockIdx.x * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint2 r = rec[t];
+    E[t].off = r.x;
+    E[t].f = r.y;
+}
+
 __global__ __launch_bounds__(kListBlock) void k_colour_records(Grid g, const int32_t *__restrict__ list, int n,
                                                                const uint8_t *__restrict__ pc, uint2 *rec) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
@@ -259,6 +399,24 @@ __global__ __launch_bounds__(kListBlock) void k_fold4(const int32_t *__restrict_
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     fold_voxel4(list[t], Xp, Xu, Xv, Xw, d);
+}
+
+// X += d, d = 0 at the fluid cells of the compact state (entry t = list
+// position t, natural offset list[t]), the colour lists walked in space order
+// as k_fold4_split walks them; the halo entries are refilled by the ghost fill
+// after.
+__global__ __launch_bounds__(kListBlock) void k_fold4_compact(const int32_t *__restrict__ list, int n, ListWalk W,
+                                                              double *Xp, double *Xu, double *Xv, double *Xw,
+                                                              float4 *d) {
+    const int t = walk_pos(W, xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x, n);
+    if (t < 0) return;
+    const int64_t c = list[t];
+    const float4 v = d[t];
+    Xp[c] = Xp[c] + (double)v.x;
+    Xu[c] = Xu[c] + (double)v.y;
+    Xv[c] = Xv[c] + (double)v.z;
+    Xw[c] = Xw[c] + (double)v.w;
+    d[t] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 __global__ __launch_bounds__(kListBlock) void k_fold4_split(const int32_t *__restrict__ list,
@@ -728,8 +886,21 @@ static void launch_colour4_chunked(const Grid &g, const FluidLists &L, float4 *S
     }
 }
 
+bool compact_supported(const Grid &g, int64_t n_entries) {
+    return !g.slip && n_entries > 0 && n_entries * 32 < ((int64_t)1 << 31);
+}
+
 void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
                     float cp, hipStream_t s) {
+    if (L.compact) {
+        for (int col = 0; col < 3; col++) {
+            const int n = L.n_col[col];
+            if (n == 0) continue;
+            hipLaunchKernelGGL(k_colour4_compact, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.cent,
+                               L.n_centries, n, L.off_col[col], S, q, R, cu, cp);
+        }
+        return;
+    }
     if (L.split && L.sor_chunk > 0 && !L.plane_off.empty()) {
         launch_colour4_chunked(g, L, S, q, R, cu, cp, s);
         return;
@@ -751,9 +922,13 @@ void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float
 
 void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, hipStream_t s) {
     if (L.n_all == 0 || !L.rec) return;
-    if (L.split)
+    if (L.split) {
         hipLaunchKernelGGL(k_colour_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc,
                            L.rec);
+        if (L.compact)
+            hipLaunchKernelGGL(k_compact_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, L.rec, L.n_all,
+                               L.cent);
+    }
     else
         hipLaunchKernelGGL(k_lag_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc,
                            L.rec);
@@ -780,7 +955,25 @@ void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, bool sp
         hipLaunchKernelGGL(k_ghost4<false>, dim3(nblocks(ghost_lines(g), 256)), dim3(256), 0, s, g, G, S, q);
 }
 
+void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, float4 *S, float *q, hipStream_t s) {
+    if (!L.compact) {
+        launch_ghost4(g, G, S, q, L.split, s);
+        return;
+    }
+    const int64_t n = (int64_t)L.n_chalo + L.n_cout + g.ny + g.nx;
+    hipLaunchKernelGGL(k_ghost4_compact, dim3(nblocks(n, 256)), dim3(256), 0, s, g, L.chalo, L.n_chalo, L.cout,
+                       L.n_cout, S, q);
+}
+
 void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s) {
+    if (L.compact) {
+        if (L.n_all == 0) return;
+        int threads = 0;
+        const ListWalk W = list_walk(L, threads);
+        hipLaunchKernelGGL(k_fold4_compact, dim3(lblocks(threads)), dim3(kListBlock), 0, s, L.all, L.n_all, W, Xp,
+                           Xu, Xv, Xw, d);
+        return;
+    }
     if (L.n_all == 0) return;
     if (L.split) {
         int threads = 0;

@@ -122,6 +122,27 @@ def test_workload_media_solve_matches_oracle_solve(medium, mode, precision):
     assert close([res.last.wk13, res.last.wk23, res.last.wk33], k[3:6])
 
 
+@pytest.mark.parametrize("medium", sorted(MEDIA))
+def test_workload_media_sor_compact_solve_matches_oracle_solve(medium, monkeypatch):
+    """FP32+DC 3-colour SOR on the compact state (fluid cells and periodic
+    halo copies only): the whole solve equals the oracle's, sweep count,
+    fields and k."""
+    make, bc = MEDIA[medium]
+    raw = make(20, 7)
+    pair = (1.8, 6.0)
+    say("workload medium solve, compact SOR", medium)
+    t, k, st = oracle_k(raw, bc, 1e-5, "sor3", "f32dc", pair)
+    monkeypatch.setenv("POREMAPS_SOR", "compact")
+    with pm.Solver(raw, boundary_method=bc, voxel_size=1e-5, mode="sor3", precision="f32dc", omega=pair[0],
+                   omega_p=pair[1], eps=1e-6) as s:
+        assert s.kernel_choice()[0] == "interleaved-colour-compact"
+        res, _ = s.solve()
+        velz, press = s.field("velz"), s.field("press")
+    assert res.sweeps == t
+    assert np.array_equal(velz, st[3][INTERIOR]) and np.array_equal(press, st[0][INTERIOR])
+    assert close([res.last.wk13, res.last.wk23, res.last.wk33], k[3:6])
+
+
 # ---------------------------------------------------------------------------
 # batch, tensor and serving against the oracle
 # ---------------------------------------------------------------------------

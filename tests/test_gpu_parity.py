@@ -308,7 +308,7 @@ def test_fast_f32dc_sweeps_bit_exact(mode, bc, omega, omega_p):
             assert np.array_equal(a[m], b[m]), f"{name} differs"
 
 
-@pytest.mark.parametrize("layout", ["split", "split-all-loads", "natural"])
+@pytest.mark.parametrize("layout", ["split", "split-all-loads", "natural", "compact"])
 @pytest.mark.parametrize("shape,bc", [((6, 5, 28), 0), ((5, 7, 27), 1), ((4, 6, 2), 2), ((7, 4, 60), 0),
                                       ((5, 6, 13), 3), ((9, 12, 130), 1), ((10, 9, 97), 0), ((6, 7, 20), 4)])
 def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
@@ -317,17 +317,18 @@ def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
     nx 27, 28 and 60 are widths whose split rows need a wider row pitch than
     the natural ones; nx 2 gives blocks of two cells; nx 97 and 130 rows of
     several lines per block.  Split rows with the +-2 loads made only where
-    the case reads them (default) and with every load."""
+    the case reads them (default) and with every load; the compact state
+    (fluid cells and periodic halo copies only, linked entries)."""
     raw = random_geometry(shape, 0.3, sum(shape) + bc)
     say("sor f32dc layout", layout, shape, bc)
     ref = oracle_dc(raw, bc, "sor3", 1.8, 16, 5, 6.0)
     split = layout.startswith("split")
-    with env("POREMAPS_SOR", "split" if split else "natural"), \
+    with env("POREMAPS_SOR", "split" if split else layout), \
             env("POREMAPS_SOR_SKIP2", "0" if layout == "split-all-loads" else "1"), \
             pm.Solver(raw, boundary_method=bc, mode="sor3", omega=1.8, omega_p=6.0, precision="f32dc",
                       it_eval=5) as s:
         want = {"split": "interleaved-colour-split-skip2", "split-all-loads": "interleaved-colour-split",
-                "natural": "interleaved-colour"}[layout]
+                "natural": "interleaved-colour", "compact": "interleaved-colour-compact"}[layout]
         assert s.kernel_choice()[0] == want
         s.step(16)
         m = nonc(ref[0])

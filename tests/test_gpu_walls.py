@@ -163,6 +163,8 @@ SLIP_FORMS = {
         "split": ({"POREMAPS_SOR": "split", "POREMAPS_SOR_SKIP2": "0"}, "interleaved-colour-split"),
         "split-z3": ({"POREMAPS_SOR": "split", "POREMAPS_SOR_CHUNK": "3"}, "interleaved-colour-split-z3-skip2"),
         "natural": ({"POREMAPS_SOR": "natural"}, "interleaved-colour"),
+        # the compact state does not hold slip walls: colour-split rows
+        "compact->split": ({"POREMAPS_SOR": "compact"}, "interleaved-colour-split-skip2"),
     },
     ("sor3", "f64"): {"default": ({}, None)},
 }
