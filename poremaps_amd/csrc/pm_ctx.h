@@ -117,7 +117,7 @@ struct pm_ctx {
     bool rec_skip1 = false;     // FluidLists::rec_skip1 (POREMAPS_REC_SKIP2=2: both)
     bool sor_skip2 = true;      // FluidLists::sor_skip2 (POREMAPS_SOR_SKIP2=0: off)
     bool sor_fused = true;      // FluidLists::sor_fused (POREMAPS_SOR_STAGE=seq: false)
-    bool sor_compact = false;   // FluidLists::compact (POREMAPS_SOR=compact)
+    bool sor_compact = true;    // FluidLists::compact (POREMAPS_SOR=split / natural: off)
     Tuner tuner;
     // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence
     bool lagged = false;

@@ -65,16 +65,14 @@ void launch_ref_block_sweep(const Grid &g, const uint8_t *G, const int32_t *code
 void launch_ref_sweeps(const Grid &g, const uint8_t *G, const int32_t *code, double *p, double *u,
                        double *v, double *w, int solver, int nsweeps, int impl, hipStream_t s);
 
-// An entry of the compact SOR state: the entries of its -x, +x, -y, +y, -z,
-// +z neighbours (the zero entry where the neighbour is solid, a wall halo or
-// beyond the halo), the cell's colour-split offset (its outlet-buffer
-// position) and its colour pass record flags (packed code, block k mod 3 in
-// bits 8-9, outlet flag in bit 10; halo and zero entries: kSolid).
-struct CEnt {
-    int32_t xm, xp, ym, yp, zm, zp;
-    uint32_t off, f;
-};
-static_assert(sizeof(CEnt) == 32, "CEnt is two 16-byte loads");
+// Links of the compact SOR state (FluidLists::compact): three arrays of
+// n_centries uint2 back to back, (-x, +x), (-y, +y), (-z, +z) entry by
+// entry: the entries of the six +-1 neighbours (the zero entry where the
+// neighbour is solid, a wall halo or beyond the halo) in the low 26 bits.
+// The top 6 bits of the -x and +x links carry the interior entries' colour
+// pass record: packed code (8 bits), block k mod 3 (2 bits), outlet flag.
+constexpr uint32_t kLinkMask = 0x03FFFFFFu;
+constexpr int kLinkBits = 26;
 
 // Device lists of the interior fluid cells' element offsets in column order
 // (columns of tile_y rows x tile_x cells, each marching along z; within a
@@ -135,15 +133,17 @@ struct FluidLists {
     // FP32+DC SOR (colour-split rows): the same for the colour passes
     bool sor_skip2 = true;
     std::vector<int32_t> plane_off;
-    // FP32+DC 3-colour SOR on the compact state (CEnt): the correction state
-    // holds one float4 per fluid cell in list order (entry t = list position
-    // t), then one per fluid cell of a periodic face halo (a copy of its
-    // image), then one zero entry that stands for every solid, wall-halo or
-    // otherwise unused neighbour.  cent: the entries' links and records;
-    // chalo: (halo entry, image entry | bit 31 when the copy's p is 0);
-    // cout: the entries of the outlet plane (p pinned to 0 after a sweep).
+    // FP32+DC 3-colour SOR on the compact state: the correction state holds
+    // one float4 per fluid cell in list order (entry t = list position t),
+    // then one per fluid cell of a periodic face halo (a copy of its image),
+    // then one zero entry that stands for every solid, wall-halo or otherwise
+    // unused neighbour; the outlet buffer likewise holds one float per entry.
+    // clink: the entries' links and records (kLinkMask); chalo: (halo entry,
+    // image entry | bit 31 when the copy's p is 0 and its outlet buffer value
+    // is copied too); cout: the entries of the outlet plane (p pinned to 0
+    // after a sweep).
     bool compact = false;
-    CEnt *cent = nullptr;
+    uint2 *clink = nullptr;
     int n_centries = 0;
     int2 *chalo = nullptr;
     int n_chalo = 0;

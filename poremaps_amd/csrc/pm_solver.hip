@@ -135,6 +135,9 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     if (dc) per += (jac ? 32 : 16) + 4 + 16 + (jac ? 16 : 0);
     // Jacobi and FP32+DC SOR: per-voxel records (8 B per fluid cell)
     if (fast && (jac || dc)) per += 8;
+    // FP32+DC SOR on the compact state: the links (24 B per entry) and the
+    // outlet buffer by entry (4 B)
+    if (dc && !jac) per += 28;
     // FP32+DC Jacobi: the z-march form's row-segment starts (4 B per 64 cells)
     if (dc && jac) per += 1;
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
@@ -157,7 +160,7 @@ static void free_all(pm_ctx *c) {
     f(c->lists.all);
     f(c->lists.rec);
     f(c->lists.rowstart);
-    f(c->lists.cent); f(c->lists.chalo); f(c->lists.cout);
+    f(c->lists.clink); f(c->lists.chalo); f(c->lists.cout);
     for (int q = 0; q < 4; q++) { f(c->ref_snap[q]); c->ref_snap[q] = nullptr; }
     f(c->ref_outlet); f(c->ref_tab);
     c->ref_outlet = nullptr; c->ref_tab = nullptr; c->ref_nblk = 0;
@@ -192,8 +195,15 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     // 512^3 field, ms per sweep: 64 rows x whole rows 1.456, 128 x whole 1.455,
     // 256 x whole 1.467, plain (z, y, x) order 1.496, 64 x 192 1.507, 64 x 64
     // 1.558
-    if (p->mode == PM_MODE_SOR3 && p->precision == PM_PREC_F32DC) c->prm_tile_x = 0;
+    // The compact state (the default; POREMAPS_SOR=split or natural keeps
+    // every cell) takes its entries in 64 x 64 columns: 0.745 ms per sweep
+    // against 0.762 ms in columns of 64 whole rows.
     if (const char *sc = std::getenv("POREMAPS_SOR_CHUNK")) c->sor_chunk = std::max(0, std::atoi(sc));
+    if (const char *so = std::getenv("POREMAPS_SOR"))
+        c->sor_compact = std::strcmp(so, "split") != 0 && std::strcmp(so, "natural") != 0;
+    // slip walls and the z-chunked passes: colour-split rows
+    if (p->walls == 2 || c->sor_chunk > 0) c->sor_compact = false;
+    if (p->mode == PM_MODE_SOR3 && p->precision == PM_PREC_F32DC && !c->sor_compact) c->prm_tile_x = 0;
     if (const char *ss = std::getenv("POREMAPS_SOR_STAGE")) c->sor_fused = std::strcmp(ss, "seq") != 0;
     // the z-chunked SOR walks the colour lists plane by plane
     if (p->mode == PM_MODE_SOR3 && p->precision == PM_PREC_F32DC && c->sor_chunk > 0) c->prm_tile_y = 0;
@@ -294,7 +304,9 @@ static int dc_prepare(pm_ctx *c) {
     pm::launch_residual4(g, c->lists, c->pc, c->X[0], c->X[1], c->X[2], c->X[3], c->qX[0], c->R4, c->stream);
     pm::launch_gather_rq(g, c->lists, c->R4, c->stream);
     te(c);
-    RC(fill_f32(c, c->qD[0], g.qtotal(), 0.f));
+    // the compact SOR state's outlet buffer holds one value per entry
+    RC(fill_f32(c, c->qD[0], c->lists.compact ? std::max<int64_t>(g.qtotal(), c->lists.n_centries) : g.qtotal(),
+                0.f));
     RC(fill_f32(c, c->qD[1], g.qtotal(), 0.f));
     HIPCHK(hipGetLastError());
     // every caller has just zeroed the correction state (fold or start)
@@ -373,12 +385,14 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
     std::vector<int64_t> hoff;   // natural offsets of the halo entries
     std::vector<int2> halo;      // (entry, image entry | bit 31: p = 0)
     int64_t next = n_all;
-    auto add = [&](int i, int j, int k, int si, int sj, int sk, bool zero_p) {
+    // zero_p: the copy's p is 0; with_q: its outlet buffer value is copied
+    auto add = [&](int i, int j, int k, int si, int sj, int sk, bool zero_p, bool with_q) {
         if (!fluid_at(si, sj, sk)) return;
         const int32_t e = cidx[(size_t)g.at(si, sj, sk)];
         cidx[(size_t)g.at(i, j, k)] = (int32_t)next;
         hoff.push_back(g.at(i, j, k));
-        halo.push_back(make_int2((int)next, zero_p ? (int)((uint32_t)e | 0x80000000u) : (int)e));
+        halo.push_back(make_int2(with_q ? (int)((uint32_t)next | 0x80000000u) : (int)next,
+                                 zero_p ? (int)((uint32_t)e | 0x80000000u) : (int)e));
         next++;
     };
     const int iout = g.nz + 1;
@@ -386,13 +400,13 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
         for (int i = 2; i < g.nz + 2; i++)
             for (int j = 2; j < g.ny + 2; j++) {
                 const int src[4] = {g.nx, g.nx + 1, 2, 3}, dst[4] = {0, 1, g.nx + 2, g.nx + 3};
-                for (int e = 0; e < 4; e++) add(i, j, dst[e], i, j, src[e], i == iout);
+                for (int e = 0; e < 4; e++) add(i, j, dst[e], i, j, src[e], i == iout, i == iout);
             }
     if (g.py)
         for (int i = 2; i < g.nz + 2; i++)
             for (int k = 2; k < g.nx + 2; k++) {
                 const int src[4] = {g.ny, g.ny + 1, 2, 3}, dst[4] = {0, 1, g.ny + 2, g.ny + 3};
-                for (int e = 0; e < 4; e++) add(i, dst[e], k, i, src[e], k, i == iout);
+                for (int e = 0; e < 4; e++) add(i, dst[e], k, i, src[e], k, i == iout, i == iout);
             }
     if (g.pz)
         for (int j = 2; j < g.ny + 2; j++)
@@ -400,42 +414,45 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
                 // the copies of the outlet plane and the pinned planes 1 and
                 // nz + 2 carry p = 0
                 const int src[4] = {g.nz, g.nz + 1, 2, 3}, dst[4] = {0, 1, g.nz + 2, g.nz + 3};
-                for (int e = 0; e < 4; e++) add(dst[e], j, k, src[e], j, k, dst[e] == 1 || dst[e] == g.nz + 2);
+                for (int e = 0; e < 4; e++)
+                    add(dst[e], j, k, src[e], j, k, dst[e] == 1 || dst[e] == g.nz + 2, false);
             }
     const int64_t zero = next, n_ent = next + 1;
     if (!pm::compact_supported(g, n_ent)) return PM_OK;
-    std::vector<pm::CEnt> ent((size_t)n_ent);
+    // links: (-x, +x), (-y, +y), (-z, +z) arrays of n_ent entries
+    std::vector<uint2> lk((size_t)(3 * n_ent));
     std::vector<int32_t> outl;
-    auto link = [&](int64_t off, pm::CEnt &E) {
+    auto link = [&](int64_t off, int64_t e) {
         const int64_t r = off - g.base;
         const int i = (int)(r / g.sy), j = (int)((r % g.sy) / g.sx), k = (int)(r % g.sx);
-        auto at = [&](int ii, int jj, int kk) -> int32_t {
-            if (ii < 0 || ii >= g.NZ || jj < 0 || jj >= g.NY || kk < 0 || kk >= g.NX) return (int32_t)zero;
-            const int32_t e = cidx[(size_t)g.at(ii, jj, kk)];
-            return e < 0 ? (int32_t)zero : e;
+        auto at = [&](int ii, int jj, int kk) -> uint32_t {
+            if (ii < 0 || ii >= g.NZ || jj < 0 || jj >= g.NY || kk < 0 || kk >= g.NX) return (uint32_t)zero;
+            const int32_t m = cidx[(size_t)g.at(ii, jj, kk)];
+            return (uint32_t)(m < 0 ? zero : m);
         };
-        E.xm = at(i, j, k - 1); E.xp = at(i, j, k + 1);
-        E.ym = at(i, j - 1, k); E.yp = at(i, j + 1, k);
-        E.zm = at(i - 1, j, k); E.zp = at(i + 1, j, k);
-        E.off = 0;
-        E.f = pm::kSolid;
+        lk[(size_t)e] = make_uint2(at(i, j, k - 1), at(i, j, k + 1));
+        lk[(size_t)(n_ent + e)] = make_uint2(at(i, j - 1, k), at(i, j + 1, k));
+        lk[(size_t)(2 * n_ent + e)] = make_uint2(at(i - 1, j, k), at(i + 1, j, k));
         return i;
     };
     for (int64_t t = 0; t < n_all; t++)
-        if (link(all[(size_t)t], ent[(size_t)t]) == iout) outl.push_back((int32_t)t);
-    for (size_t h = 0; h < hoff.size(); h++) link(hoff[h], ent[(size_t)(n_all + (int64_t)h)]);
-    pm::CEnt &Z = ent[(size_t)zero];
-    Z.xm = Z.xp = Z.ym = Z.yp = Z.zm = Z.zp = (int32_t)zero;
-    Z.off = 0;
-    Z.f = pm::kSolid;
-    RC(dalloc(&c->lists.cent, n_ent));
-    HIPCHK(hipMemcpy(c->lists.cent, ent.data(), sizeof(pm::CEnt) * ent.size(), hipMemcpyHostToDevice));
+        if (link(all[(size_t)t], t) == iout) outl.push_back((int32_t)t);
+    for (size_t h = 0; h < hoff.size(); h++) link(hoff[h], n_all + (int64_t)h);
+    for (int a = 0; a < 3; a++) lk[(size_t)(a * n_ent + zero)] = make_uint2((uint32_t)zero, (uint32_t)zero);
+    RC(dalloc(&c->lists.clink, 3 * n_ent));
+    HIPCHK(hipMemcpy(c->lists.clink, lk.data(), sizeof(uint2) * lk.size(), hipMemcpyHostToDevice));
     RC(dalloc(&c->lists.chalo, std::max<int64_t>(1, (int64_t)halo.size())));
     if (!halo.empty())
         HIPCHK(hipMemcpy(c->lists.chalo, halo.data(), sizeof(int2) * halo.size(), hipMemcpyHostToDevice));
     RC(dalloc(&c->lists.cout, std::max<int64_t>(1, (int64_t)outl.size())));
     if (!outl.empty())
         HIPCHK(hipMemcpy(c->lists.cout, outl.data(), sizeof(int32_t) * outl.size(), hipMemcpyHostToDevice));
+    // the outlet buffer by entry
+    if (n_ent > g.qtotal()) {
+        (void)hipFree(c->qD[0]);
+        c->qD[0] = nullptr;
+        RC(dalloc(&c->qD[0], n_ent));
+    }
     c->lists.n_centries = (int)n_ent;
     c->lists.n_chalo = (int)halo.size();
     c->lists.n_cout = (int)outl.size();
@@ -549,7 +566,6 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
             // POREMAPS_SOR=natural.
             const char *so = std::getenv("POREMAPS_SOR");
             c->lists.split = !(so && std::strcmp(so, "natural") == 0);
-            if (so && std::strcmp(so, "compact") == 0) c->sor_compact = true;
             all.clear();
             for (int q = 0; q < 3; q++) {
                 c->lists.off_col[q] = (int)all.size();

@@ -155,46 +155,50 @@ __global__ __launch_bounds__(kListBlock) void k_colour4_stage(Grid g, const uint
 // in the same order give the same bits.
 // ---------------------------------------------------------------------------
 
-// One 32-bit word of the entry table (word w of entry e), only for lanes with
-// need set (buffer load; an out-of-range offset makes no request).
-__device__ __forceinline__ int32_t buf_load_link(__amdgpu_buffer_rsrc_t rs, int32_t e, int w, bool need) {
-    const uint32_t off = need ? (uint32_t)e * 32u + 4u * (uint32_t)w : 0xFFFFFFF0u;
-    return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+// Link word w (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z) of entry e, only for lanes
+// with need set (buffer load over the three link arrays of nE entries; an
+// out-of-range offset makes no request).
+__device__ __forceinline__ int32_t buf_load_link(__amdgpu_buffer_rsrc_t rs, int32_t e, int w, int nE, bool need) {
+    const uint32_t off = need ? ((uint32_t)(w >> 1) * (uint32_t)nE + (uint32_t)e) * 8u + 4u * (uint32_t)(w & 1)
+                              : 0xFFFFFFF0u;
+    return (int32_t)((uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0) & kLinkMask);
 }
 
-__device__ __forceinline__ void colour_voxel4_compact(const Grid &g, int t, const CEnt *__restrict__ E,
-                                                      __amdgpu_buffer_rsrc_t rsE, __amdgpu_buffer_rsrc_t rsS,
+__device__ __forceinline__ void colour_voxel4_compact(int t, const uint2 *__restrict__ L, int nE,
+                                                      __amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS,
                                                       float4 *S, float *q, Res4 R, float cu, float cp,
                                                       float pin_out) {
-    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
     typedef float v4f __attribute__((ext_vector_type(4)));
-    const v4i e0 = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(&E[t]));
-    const v4i e1 = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(&E[t]) + 1);
-    const uint32_t f = (uint32_t)e1.w;
-    const int b = (int)((f >> 8) & 3u);
+    const v2u lx = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[t]));
+    const v2u ly = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[nE + t]));
+    const v2u lz = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[2 * nE + t]));
+    const uint32_t f = (lx.x >> kLinkBits) | ((lx.y >> kLinkBits) << 6);
+    const int32_t xm = (int32_t)(lx.x & kLinkMask), xp = (int32_t)(lx.y & kLinkMask);
+    const int32_t ym = (int32_t)ly.x, yp = (int32_t)ly.y, zm = (int32_t)lz.x, zp = (int32_t)lz.y;
     const bool out = (f >> 10) & 1u;
     const Cases c2 = decode((uint8_t)(f & 0xffu));
     const float4 C = S[t];
     Star4 s;
-    s.XM = S[e0.x]; s.XP = S[e0.y];
-    s.YM = S[e0.z]; s.YP = S[e0.w];
-    s.ZM = S[e1.x]; s.ZP = S[e1.y];
+    s.XM = S[xm]; s.XP = S[xp];
+    s.YM = S[ym]; s.YP = S[yp];
+    s.ZM = S[zm]; s.ZP = S[zp];
     const bool nxm = c2.cx == 5, nxp = c2.cx == 4, nym = c2.cy == 5, nyp = c2.cy == 4;
     const bool nzm = c2.cz == 5, nzp = c2.cz == 4;
-    const int32_t xm2 = buf_load_link(rsE, e0.x, 0, nxm), xp2 = buf_load_link(rsE, e0.y, 1, nxp);
-    const int32_t ym2 = buf_load_link(rsE, e0.z, 2, nym), yp2 = buf_load_link(rsE, e0.w, 3, nyp);
-    const int32_t zm2 = buf_load_link(rsE, e1.x, 4, nzm), zp2 = buf_load_link(rsE, e1.y, 5, nzp);
+    const int32_t xm2 = buf_load_link(rsL, xm, 0, nE, nxm), xp2 = buf_load_link(rsL, xp, 1, nE, nxp);
+    const int32_t ym2 = buf_load_link(rsL, ym, 2, nE, nym), yp2 = buf_load_link(rsL, yp, 3, nE, nyp);
+    const int32_t zm2 = buf_load_link(rsL, zm, 4, nE, nzm), zp2 = buf_load_link(rsL, zp, 5, nE, nzp);
     s.XM2 = buf_load4(rsS, xm2, nxm); s.XP2 = buf_load4(rsS, xp2, nxp);
     s.YM2 = buf_load4(rsS, ym2, nym); s.YP2 = buf_load4(rsS, yp2, nyp);
     s.ZM2 = buf_load4(rsS, zm2, nzm); s.ZP2 = buf_load4(rsS, zp2, nzp);
     const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
     const float4 Rm = make_float4(r.x, r.y, r.z, 0.f);
     const float rp = r.w;
-    const int64_t cb = g.cb, sj = g.sx;
-    const int64_t oM1 = b == 0 ? 2 * cb - 1 : -cb;
-    const int64_t qc = (int64_t)e1.z - (int64_t)(g.nz + 1) * g.sy;
+    // the outlet buffer is indexed by entry: the -x / -y neighbours' values
+    // are those of the neighbours' entries (the zero entry's is 0, as a wall
+    // halo's outlet value is)
     float pxm = s.XM.x, pym = s.YM.x;
-    if (out) { pxm = q[qc + oM1]; pym = q[qc - sj]; }
+    if (out) { pxm = q[xm]; pym = q[ym]; }
     const Cases cs = c2;
     const float dpx = cs.bx ? 0.f : C.x - pxm;
     const float dpy = cs.by ? 0.f : C.x - pym;
@@ -209,73 +213,56 @@ __device__ __forceinline__ void colour_voxel4_compact(const Grid &g, int t, cons
     const float div = div0 + rp;
     N.x = C.x - div * cp;
     S[t] = N;
-    if (out) q[qc] = pin_out - div0 * (float)(kC2 * kDt);
+    if (out) q[t] = pin_out - div0 * (float)(kC2 * kDt);
 }
 
 // One colour's entries t0 .. t0 + n - 1 (its list positions).  nE: entries
-// of the table (the buffers' extents).
-__global__ __launch_bounds__(kListBlock) void k_colour4_compact(Grid g, const CEnt *__restrict__ E, int nE, int n,
-                                                                int t0, float4 *S, float *q, Res4 R, float cu,
-                                                                float cp) {
+// of the state (the buffers' extents).
+__global__ __launch_bounds__(kListBlock) void k_colour4_compact(const uint2 *__restrict__ L, int nE, int n, int t0,
+                                                                float4 *S, float *q, Res4 R, float cu, float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
-    const __amdgpu_buffer_rsrc_t rsE =
-        __builtin_amdgcn_make_buffer_rsrc((void *)E, (short)0, (int)((uint32_t)nE * 32u), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsL =
+        __builtin_amdgcn_make_buffer_rsrc((void *)L, (short)0, (int)((uint32_t)nE * 24u), 0x00020000);
     const __amdgpu_buffer_rsrc_t rsS =
         __builtin_amdgcn_make_buffer_rsrc((void *)S, (short)0, (int)((uint32_t)nE * 16u), 0x00020000);
-    colour_voxel4_compact(g, t0 + t, E, rsE, rsS, S, q, R, cu, cp, 0.f);
+    colour_voxel4_compact(t0 + t, L, nE, rsL, rsS, S, q, R, cu, cp, 0.f);
 }
 
 // The compact state's ghost fill (ghost_line4 on the entries): halo entries
-// copied from their images (p 0 where the dense fill pins it: copies of the
-// outlet plane and the z-halo planes 1 and nz + 2), the outlet plane's p
-// pinned to 0, then the outlet buffer's periodic x / y copies in colour-split
-// rows.  A copy whose image is an outlet entry zeroes p itself, so it does not
-// depend on the order of the two writes.
-__global__ void k_ghost4_compact(Grid g, const int2 *__restrict__ halo, int nh, const int32_t *__restrict__ outl,
-                                 int no, float4 *S, float *q) {
+// copied from their images, with p 0 where the dense fill pins it (bit 31 of
+// the image: copies of the outlet plane and the z-halo planes 1 and nz + 2),
+// the x / y halo copies of the outlet plane with their outlet buffer value
+// (bit 31 of the halo entry), then the outlet plane's p pinned to 0.  A copy
+// whose image is an outlet entry zeroes p itself, so it does not depend on
+// the order of the two writes.
+__global__ void k_ghost4_compact(const int2 *__restrict__ halo, int nh, const int32_t *__restrict__ outl, int no,
+                                 float4 *S, float *q) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nh) {
         const int2 h = halo[t];
-        float4 v = S[h.y & 0x7fffffff];
+        const int32_t dst = h.x & 0x7fffffff, src = h.y & 0x7fffffff;
+        float4 v = S[src];
         if (h.y < 0) v.x = 0.f;
-        S[h.x] = v;
+        if (h.x < 0) q[dst] = q[src];
+        S[dst] = v;
         return;
     }
-    int64_t u = t - nh;
-    if (u < no) {
-        S[outl[u]].x = 0.f;
-        return;
-    }
-    u -= no;
-    const int iout = g.nz + 1;
-    (void)iout;
-    if (u < g.ny) {
-        if (!g.px) return;
-        const int j = (int)u + 2;
-        const int src[4] = {g.nx, g.nx + 1, 2, 3};
-        const int dst[4] = {0, 1, g.nx + 2, g.nx + 3};
-        for (int e = 0; e < 4; e++) q[g.qat_s(j, dst[e])] = q[g.qat_s(j, src[e])];
-        return;
-    }
-    u -= g.ny;
-    if (u < g.nx) {
-        if (!g.py) return;
-        const int k = (int)u + 2;
-        const int src[4] = {g.ny, g.ny + 1, 2, 3};
-        const int dst[4] = {0, 1, g.ny + 2, g.ny + 3};
-        for (int e = 0; e < 4; e++) q[g.qat_s(dst[e], k)] = q[g.qat_s(src[e], k)];
-    }
+    const int64_t u = t - nh;
+    if (u < no) S[outl[u]].x = 0.f;
 }
 
-// The records' part of the compact entries (offset and flags of the colour
-// passes' records), list positions 0 .. n - 1.
-__global__ __launch_bounds__(kListBlock) void k_compact_records(const uint2 *__restrict__ rec, int n, CEnt *E) {
+// The colour passes' record flags (packed code, block, outlet flag: bits
+// 0-10 of rec.y) into the top bits of the interior entries' -x / +x links,
+// list positions 0 .. n - 1.
+__global__ __launch_bounds__(kListBlock) void k_compact_records(const uint2 *__restrict__ rec, int n, uint2 *L) {
     const int t = blockIdx.x * kListBlock + threadIdx.x;
     if (t >= n) return;
-    const uint2 r = rec[t];
-    E[t].off = r.x;
-    E[t].f = r.y;
+    const uint32_t f = rec[t].y;
+    uint2 l = L[t];
+    l.x = (l.x & kLinkMask) | ((f & 63u) << kLinkBits);
+    l.y = (l.y & kLinkMask) | (((f >> 6) & 31u) << kLinkBits);
+    L[t] = l;
 }
 
 __global__ __launch_bounds__(kListBlock) void k_colour_records(Grid g, const int32_t *__restrict__ list, int n,
@@ -887,7 +874,7 @@ static void launch_colour4_chunked(const Grid &g, const FluidLists &L, float4 *S
 }
 
 bool compact_supported(const Grid &g, int64_t n_entries) {
-    return !g.slip && n_entries > 0 && n_entries * 32 < ((int64_t)1 << 31);
+    return !g.slip && n_entries > 0 && n_entries <= (int64_t)kLinkMask;
 }
 
 void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float4 *S, float *q, Res4 R, float cu,
@@ -896,7 +883,7 @@ void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float
         for (int col = 0; col < 3; col++) {
             const int n = L.n_col[col];
             if (n == 0) continue;
-            hipLaunchKernelGGL(k_colour4_compact, dim3(lblocks(n)), dim3(kListBlock), 0, s, g, L.cent,
+            hipLaunchKernelGGL(k_colour4_compact, dim3(lblocks(n)), dim3(kListBlock), 0, s, L.clink,
                                L.n_centries, n, L.off_col[col], S, q, R, cu, cp);
         }
         return;
@@ -927,7 +914,7 @@ void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, h
                            L.rec);
         if (L.compact)
             hipLaunchKernelGGL(k_compact_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, L.rec, L.n_all,
-                               L.cent);
+                               L.clink);
     }
     else
         hipLaunchKernelGGL(k_lag_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc,
@@ -960,9 +947,10 @@ void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, f
         launch_ghost4(g, G, S, q, L.split, s);
         return;
     }
-    const int64_t n = (int64_t)L.n_chalo + L.n_cout + g.ny + g.nx;
-    hipLaunchKernelGGL(k_ghost4_compact, dim3(nblocks(n, 256)), dim3(256), 0, s, g, L.chalo, L.n_chalo, L.cout,
-                       L.n_cout, S, q);
+    const int64_t n = (int64_t)L.n_chalo + L.n_cout;
+    if (n > 0)
+        hipLaunchKernelGGL(k_ghost4_compact, dim3(nblocks(n, 256)), dim3(256), 0, s, L.chalo, L.n_chalo, L.cout,
+                           L.n_cout, S, q);
 }
 
 void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s) {
