@@ -40,21 +40,26 @@ def run_batch(b, slots):
     return {s: b.status(s) for s in slots}
 
 
-@pytest.mark.parametrize("precision,graph,skip2", [("f32dc", True, "1"), ("f32dc", False, "1"), ("f32dc", True, "0"),
-                                                  ("f64", True, "1")])
-def test_batch_equals_single_solves(channel_raw, precision, graph, skip2):
-    """skip2 "0": the lagged pass with every stencil load (POREMAPS_BATCH_SKIP2=0)."""
-    say("batch vs single", precision, graph, skip2)
+@pytest.mark.parametrize("precision,graph,skip2,queue", [("f32dc", True, "1", "1"), ("f32dc", False, "1", "1"),
+                                                        ("f32dc", True, "0", "1"), ("f32dc", True, "1", "0"),
+                                                        ("f32dc", False, "1", "0"), ("f64", True, "1", "1")])
+def test_batch_equals_single_solves(channel_raw, precision, graph, skip2, queue):
+    """skip2 "0": the lagged pass with every stencil load (POREMAPS_BATCH_SKIP2=0);
+    queue "1": the interval's passes as persistent kernels over the tile queues
+    (POREMAPS_BATCH_QUEUE=1), "0": one workgroup per tile (the default)."""
+    say("batch vs single", precision, graph, skip2, queue)
     cs = cases(channel_raw)
-    old = os.environ.get("POREMAPS_BATCH_SKIP2")
+    saved = {k: os.environ.get(k) for k in ("POREMAPS_BATCH_SKIP2", "POREMAPS_BATCH_QUEUE")}
     os.environ["POREMAPS_BATCH_SKIP2"] = skip2
+    os.environ["POREMAPS_BATCH_QUEUE"] = queue
     try:
         b = pm.Batch(precision=precision, omega=2.0, omega_p=pm.OMEGA_P, use_graph=graph)
     finally:
-        if old is None:
-            os.environ.pop("POREMAPS_BATCH_SKIP2", None)
-        else:
-            os.environ["POREMAPS_BATCH_SKIP2"] = old
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     with b:
         slots = [b.add(raw, boundary_method=bc, voxel_size=dx) for raw, bc, dx in cs]
         st = run_batch(b, slots)
