@@ -494,6 +494,9 @@ extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int
     pm_ctx *c = nullptr;
     int rc = pm_ctx_create(&q, &c);
     if (rc != PM_OK) return rc;
+    // the batch's kernels read the dense layout
+    c->jac_compact = false;
+    c->sor_compact = false;
     rc = pm_load_geometry(c, raw);
     if (rc == PM_OK) rc = prologue(c);
     if (rc != PM_OK) {

@@ -20,7 +20,9 @@ enum TimerCat { TC_SWEEP = 0, TC_GHOST, TC_FOLD, TC_RESIDUAL, TC_REF, TC_FIRST0,
 // (codes of the cell and its recompute targets, gathered residuals).
 // JI_MARCH: the interleaved state's lagged pass as a z-march of tiles through
 // LDS (pm_march.hip).
-enum JacobiImpl { JI_PAIR = 0, JI_LIST = 1, JI_STREAM = 2, JI_RECORDS = 3, JI_MARCH = 4 };
+// JI_COMPACT: the lagged pass on the compact state (fluid cells only, linked
+// entries; FluidLists::compact_jac).
+enum JacobiImpl { JI_PAIR = 0, JI_LIST = 1, JI_STREAM = 2, JI_RECORDS = 3, JI_MARCH = 4, JI_COMPACT = 5 };
 static const char *kTimerNames[TC_N] = {"sweep", "ghost_fill", "dc_fold", "dc_residual", "reference_sweep",
                                         "dc_first_pass"};
 
@@ -118,6 +120,8 @@ struct pm_ctx {
     bool sor_skip2 = true;      // FluidLists::sor_skip2 (POREMAPS_SOR_SKIP2=0: off)
     bool sor_fused = true;      // FluidLists::sor_fused (POREMAPS_SOR_STAGE=seq: false)
     bool sor_compact = true;    // FluidLists::compact (POREMAPS_SOR=split / natural: off)
+    bool jac_compact = true;    // FluidLists::compact_jac (POREMAPS_JACOBI naming a dense form: off)
+    float layout_ms[2] = {0.f, 0.f};  // FP32+DC Jacobi: ms per sweep, compact / dense layout (0: not timed)
     Tuner tuner;
     // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence
     bool lagged = false;

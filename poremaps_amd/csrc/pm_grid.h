@@ -61,10 +61,14 @@ struct Grid {
 // and at the three cells whose pressure its lagged pass recomputes (-x, -y
 // neighbour or their periodic images, -z neighbour), gathered once per
 // residual so that a sweep reads them with one streamed load.
+// pc (FP32+DC Jacobi on the compact state): the continuity part per entry of
+// the compact state (entry order, 0 at the zero entry), read at the cell's
+// -x, -y, -z neighbours through their links.
 struct Res4 {
     float *p;
     float4 *m;
     float4 *q;
+    float *pc = nullptr;
 };
 
 inline Grid make_grid(int nx, int ny, int nz, int bc) {

@@ -143,6 +143,11 @@ struct FluidLists {
     // is copied too); cout: the entries of the outlet plane (p pinned to 0
     // after a sweep).
     bool compact = false;
+    // FP32+DC Jacobi on the compact state: one entry per fluid cell in list
+    // order and the zero entry, links across periodic faces straight to the
+    // image (no halo entries; the flags hold the codes of the cell and of its
+    // recompute targets, the outlet and first-plane flags)
+    bool compact_jac = false;
     uint2 *clink = nullptr;
     int n_centries = 0;
     int2 *chalo = nullptr;

@@ -135,9 +135,9 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     if (dc) per += (jac ? 32 : 16) + 4 + 16 + (jac ? 16 : 0);
     // Jacobi and FP32+DC SOR: per-voxel records (8 B per fluid cell)
     if (fast && (jac || dc)) per += 8;
-    // FP32+DC SOR on the compact state: the links (24 B per entry) and the
-    // outlet buffer by entry (4 B)
-    if (dc && !jac) per += 28;
+    // FP32+DC on the compact state: the links (24 B per entry) and the outlet
+    // buffer(s) by entry (4 B each; two for Jacobi)
+    if (dc) per += jac ? 32 : 28;
     // FP32+DC Jacobi: the z-march form's row-segment starts (4 B per 64 cells)
     if (dc && jac) per += 1;
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
@@ -155,7 +155,7 @@ static void free_all(pm_ctx *c) {
     f(c->G); f(c->pc); f(c->code);
     for (int q = 0; q < 4; q++) { f(c->X[q]); f(c->X2[q]); }
     for (int q = 0; q < 2; q++) { f(c->qX[q]); f(c->qD[q]); f(c->S4[q]); c->S4[q] = nullptr; }
-    f(c->R4.p); f(c->R4.m); f(c->R4.q); c->R4 = pm::Res4{nullptr, nullptr, nullptr};
+    f(c->R4.p); f(c->R4.m); f(c->R4.q); f(c->R4.pc); c->R4 = pm::Res4{nullptr, nullptr, nullptr, nullptr};
     f(c->red_scratch); f(c->red_dev);
     f(c->lists.all);
     f(c->lists.rec);
@@ -220,7 +220,13 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
         else if (std::strcmp(jm, "stream") == 0) c->jac_impl = JI_STREAM;
         else if (std::strcmp(jm, "records") == 0) c->jac_impl = JI_RECORDS;
         else if (std::strcmp(jm, "march") == 0) c->jac_impl = JI_MARCH;
+        // a named dense form keeps the dense layout; "compact" fixes the
+        // compact pass (no tuning)
+        if (*jm && std::strcmp(jm, "compact") != 0) c->jac_compact = false;
     }
+    // FP32+DC Jacobi on the compact state: not with slip walls (their
+    // mirrored halos are not copies of images)
+    if (p->walls == 2) c->jac_compact = false;
     if (const char *zs = std::getenv("POREMAPS_STREAM_ZS")) c->stream_zs = std::max(1, std::atoi(zs));
     if (c->g.slip && (c->jac_impl == JI_STREAM || c->jac_impl == JI_MARCH)) c->jac_impl = JI_RECORDS;
     // z-march form (POREMAPS_JACOBI=march): tile rows 4 or 8, planes per z
@@ -305,9 +311,9 @@ static int dc_prepare(pm_ctx *c) {
     pm::launch_gather_rq(g, c->lists, c->R4, c->stream);
     te(c);
     // the compact SOR state's outlet buffer holds one value per entry
-    RC(fill_f32(c, c->qD[0], c->lists.compact ? std::max<int64_t>(g.qtotal(), c->lists.n_centries) : g.qtotal(),
-                0.f));
-    RC(fill_f32(c, c->qD[1], g.qtotal(), 0.f));
+    const int64_t nq = std::max<int64_t>(g.qtotal(), c->lists.n_centries);
+    RC(fill_f32(c, c->qD[0], c->lists.compact || c->lists.compact_jac ? nq : g.qtotal(), 0.f));
+    RC(fill_f32(c, c->qD[1], c->lists.compact_jac ? nq : g.qtotal(), 0.f));
     HIPCHK(hipGetLastError());
     // every caller has just zeroed the correction state (fold or start)
     c->dzero = true;
@@ -374,7 +380,9 @@ int pm_restart_slower(pm_ctx *c) {
 // records' part of the interior entries is filled on the device from the
 // colour passes' records (launch_lag_records).  Falls back to the
 // colour-split rows where the medium does not fit the compact form.
-static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_t> &all) {
+// torus (FP32+DC Jacobi): no halo entries; a link across a periodic face goes
+// to the image cell itself.
+static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_t> &all, bool torus) {
     const pm::Grid &g = c->g;
     const int64_t n_all = (int64_t)all.size();
     std::vector<int32_t> cidx((size_t)g.total, -1);
@@ -396,19 +404,19 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
         next++;
     };
     const int iout = g.nz + 1;
-    if (g.px)
+    if (g.px && !torus)
         for (int i = 2; i < g.nz + 2; i++)
             for (int j = 2; j < g.ny + 2; j++) {
                 const int src[4] = {g.nx, g.nx + 1, 2, 3}, dst[4] = {0, 1, g.nx + 2, g.nx + 3};
                 for (int e = 0; e < 4; e++) add(i, j, dst[e], i, j, src[e], i == iout, i == iout);
             }
-    if (g.py)
+    if (g.py && !torus)
         for (int i = 2; i < g.nz + 2; i++)
             for (int k = 2; k < g.nx + 2; k++) {
                 const int src[4] = {g.ny, g.ny + 1, 2, 3}, dst[4] = {0, 1, g.ny + 2, g.ny + 3};
                 for (int e = 0; e < 4; e++) add(i, dst[e], k, i, src[e], k, i == iout, i == iout);
             }
-    if (g.pz)
+    if (g.pz && !torus)
         for (int j = 2; j < g.ny + 2; j++)
             for (int k = 2; k < g.nx + 2; k++) {
                 // the copies of the outlet plane and the pinned planes 1 and
@@ -426,6 +434,12 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
         const int64_t r = off - g.base;
         const int i = (int)(r / g.sy), j = (int)((r % g.sy) / g.sx), k = (int)(r % g.sx);
         auto at = [&](int ii, int jj, int kk) -> uint32_t {
+            if (torus) {
+                // one step beyond a periodic face: the image (interior) cell
+                if (g.px) kk = kk < 2 ? kk + g.nx : kk >= g.nx + 2 ? kk - g.nx : kk;
+                if (g.py) jj = jj < 2 ? jj + g.ny : jj >= g.ny + 2 ? jj - g.ny : jj;
+                if (g.pz) ii = ii < 2 ? ii + g.nz : ii >= g.nz + 2 ? ii - g.nz : ii;
+            }
             if (ii < 0 || ii >= g.NZ || jj < 0 || jj >= g.NY || kk < 0 || kk >= g.NX) return (uint32_t)zero;
             const int32_t m = cidx[(size_t)g.at(ii, jj, kk)];
             return (uint32_t)(m < 0 ? zero : m);
@@ -447,16 +461,102 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
     RC(dalloc(&c->lists.cout, std::max<int64_t>(1, (int64_t)outl.size())));
     if (!outl.empty())
         HIPCHK(hipMemcpy(c->lists.cout, outl.data(), sizeof(int32_t) * outl.size(), hipMemcpyHostToDevice));
-    // the outlet buffer by entry
-    if (n_ent > g.qtotal()) {
-        (void)hipFree(c->qD[0]);
-        c->qD[0] = nullptr;
-        RC(dalloc(&c->qD[0], n_ent));
+    // the outlet buffer(s) by entry
+    if (n_ent > g.qtotal())
+        for (int q = 0; q < (torus ? 2 : 1); q++) {
+            (void)hipFree(c->qD[q]);
+            c->qD[q] = nullptr;
+            RC(dalloc(&c->qD[q], n_ent));
+        }
+    if (torus) {
+        // the neighbours' continuity residuals come from the entries'
+        // residual array through the links, not gathered per cell
+        RC(dalloc(&c->R4.pc, n_ent));
+        RC(fill_f32(c, c->R4.pc, n_ent, 0.f));
     }
     c->lists.n_centries = (int)n_ent;
     c->lists.n_chalo = (int)halo.size();
     c->lists.n_cout = (int)outl.size();
-    c->lists.compact = true;
+    if (torus) c->lists.compact_jac = true;
+    else c->lists.compact = true;
+    return PM_OK;
+}
+
+// The layout of the FP32+DC Jacobi correction state is chosen like a kernel
+// form: the compact pass (each load only where used) and the dense record
+// pass (its unused +-2 loads skipped where the state allows) each run a few
+// sweeps from the zero correction state on the solve's own residual, timed
+// with HIP events on the solver's stream (after a warm-up of 3 sweeps); then
+// both correction sets and outlet buffers are zeroed again -- the FP64 state is
+// never touched -- so the solve starts from exactly the state it would have
+// started from, and the faster layout is kept (the other's buffers are
+// freed).  On the 512^3 Gaussian field the compact pass is the faster, on
+// denser media the dense one can be.  POREMAPS_JACOBI (a fixed form) or
+// POREMAPS_TUNE=0 keeps the compact layout untimed.
+static int choose_jacobi_layout(pm_ctx *c) {
+    if (!c->dc() || !c->jacobi() || !c->lists.compact_jac || c->lists.n_all == 0) return PM_OK;
+    const char *fixed = std::getenv("POREMAPS_JACOBI");
+    const char *off = std::getenv("POREMAPS_TUNE");
+    if ((fixed && *fixed) || (off && std::strcmp(off, "0") == 0)) return PM_OK;
+    const pm::Grid &g = c->g;
+    hipStream_t s = c->stream;
+    const double om = c->prm.omega, wp = c->prm.omega_p > 0.0 ? c->prm.omega_p : om;
+    const float fcu = (float)(om * pm::kDt), fcp = (float)(wp * pm::kC2 * pm::kDt);
+    const int64_t nq = std::max<int64_t>(g.qtotal(), c->lists.n_centries);
+    constexpr int kWarm = 3, kWin = 6;
+    float ms[2] = {0.f, 0.f};
+    for (int lay = 0; lay < 2; lay++) {
+        pm::FluidLists &L = c->lists;
+        L.compact_jac = lay == 0;
+        L.use_rec = lay == 1;
+        L.select_loads = true;
+        L.rec_occ = 0;
+        L.rec_skip2 = lay == 1 && pm::rec_skip2_supported(g);
+        L.rec_skip1 = lay == 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        float4 *A = c->S4[0], *B = c->S4[1];
+        float *qa = c->qD[0], *qb = c->qD[1];
+        for (int it = 0; it < kWarm + kWin; it++) {
+            if (it == kWarm) HIPCHK(hipEventRecord(e0, s));
+            pm::launch_jacobi4(it == 0 ? pm::JAC_FIRST_ZERO : pm::JAC_LAG, g, L, c->pc, A, qa, B, qb, c->R4, fcu, fcp,
+                               s);
+            std::swap(A, B);
+            std::swap(qa, qb);
+        }
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms[lay], e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        for (int q = 0; q < 2; q++) {
+            RC(fill_f32(c, (float *)c->S4[q], 4 * g.total, 0.f));
+            RC(fill_f32(c, c->qD[q], nq, 0.f));
+        }
+    }
+    c->layout_ms[0] = ms[0] / kWin;
+    c->layout_ms[1] = ms[1] / kWin;
+    const bool compact = ms[0] <= ms[1];
+    pm::FluidLists &L = c->lists;
+    L.compact_jac = compact;
+    if (compact) {
+        (void)hipFree(c->R4.q);
+        c->R4.q = nullptr;
+        c->jac_impl = JI_COMPACT;
+    } else {
+        (void)hipFree(L.clink);
+        (void)hipFree(c->R4.pc);
+        L.clink = nullptr;
+        c->R4.pc = nullptr;
+        L.n_centries = 0;
+        c->jac_impl = JI_RECORDS;
+    }
+    L.use_rec = c->jac_impl == JI_RECORDS;
+    L.rec_occ = c->rec_occ;
+    L.rec_skip2 = c->rec_skip2 && pm::rec_skip2_supported(g);
+    L.rec_skip1 = c->rec_skip1;
+    HIPCHK(hipStreamSynchronize(s));
     return PM_OK;
 }
 
@@ -602,7 +702,10 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         if (sor && c->dc()) {
             c->lists.col_in_all = true;
             for (int q = 0; q < 3; q++) c->lists.col[q] = c->lists.all + c->lists.off_col[q];
-            if (c->lists.split && c->sor_compact && c->lists.sor_chunk == 0) RC(build_compact(c, raw, all));
+            if (c->lists.split && c->sor_compact && c->lists.sor_chunk == 0) RC(build_compact(c, raw, all, false));
+        } else if (c->dc() && c->jacobi() && c->jac_compact) {
+            RC(build_compact(c, raw, all, true));
+            if (c->lists.compact_jac) c->jac_impl = JI_COMPACT;
         } else if (sor)
             for (int q = 0; q < 3; q++) {
                 c->lists.n_col[q] = (int)col[q].size();
@@ -649,6 +752,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     c->kprev = 0.0;
     c->loaded = true;
     c->lagged = false;
+    RC(choose_jacobi_layout(c));
     tuner_setup(c);
     return PM_OK;
 }
@@ -708,6 +812,14 @@ static void fold_now(pm_ctx *c) {
 }
 
 static void apply_variant(pm_ctx *c, const SweepVariant &v) {
+    if (v.impl == JI_COMPACT) {
+        // the compact pass: every load, or each only where it is used (also
+        // at 6 and 8 waves per SIMD)
+        c->jac_impl = JI_COMPACT;
+        c->lists.rec_skip1 = c->rec_skip1 = v.skip1;
+        c->lists.rec_occ = c->rec_occ = v.skip1 ? v.occ : 0;
+        return;
+    }
     c->jac_impl = v.impl;
     c->lists.select_loads = v.select_loads;
     c->stream_zs = v.zs;
@@ -725,7 +837,10 @@ static void apply_variant(pm_ctx *c, const SweepVariant &v) {
 }
 
 static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *buf, size_t len) {
-    if (c->dc() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
+    if (v.impl == JI_COMPACT && v.skip1 && v.occ > 0)
+        std::snprintf(buf, len, "interleaved-compact-skipall-occ%d", v.occ);
+    else if (v.impl == JI_COMPACT) std::snprintf(buf, len, "interleaved-compact%s", v.skip1 ? "-skipall" : "");
+    else if (c->dc() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
     else if (c->dc() && v.impl == JI_MARCH) std::snprintf(buf, len, "interleaved-zmarch-ty%d-zs%d", v.ty, v.zs);
     else if (c->dc() && v.impl == JI_RECORDS && v.skip2)
         std::snprintf(buf, len, "interleaved-lagged-records-%s", v.skip1 ? "skipall" : "skip2");
@@ -773,7 +888,14 @@ static void tuner_setup(pm_ctx *c) {
         if (v.skip2 && !pm::rec_skip2_supported(c->g)) return false;
         return !(c->g.slip && (v.impl == JI_STREAM || v.impl == JI_MARCH || v.skip1));
     };
-    if (c->dc()) {
+    if (c->lists.compact_jac) {
+        // the compact state: its pass with every load and with each load only
+        // where it is used (the dense forms read another layout)
+        const SweepVariant cmp[] = {{JI_COMPACT, true, 64}, {JI_COMPACT, true, 64, 4, 0, true, true},
+                                    {JI_COMPACT, true, 64, 4, 6, true, true}, {JI_COMPACT, true, 64, 4, 8, true, true},
+                                    {JI_COMPACT, true, 64}};
+        for (const auto &v : cmp) tn.cand[tn.n++] = v;
+    } else if (c->dc()) {
         for (const auto &v : aos)
             if (usable(v)) tn.cand[tn.n++] = v;
         if (pm::march_supported(c->g, c->lists))
@@ -1210,9 +1332,10 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
     char buf[64];
     const SweepVariant cur = {c->jac_impl, c->lists.select_loads,
                               c->jac_impl == JI_MARCH ? c->march_zseg : c->stream_zs, c->march_ty,
-                              c->jac_impl == JI_RECORDS ? c->lists.rec_occ : 0,
+                              c->jac_impl == JI_RECORDS || c->jac_impl == JI_COMPACT ? c->lists.rec_occ : 0,
                               c->jac_impl == JI_RECORDS && c->lists.rec_skip2,
-                              c->jac_impl == JI_RECORDS && c->lists.rec_skip2 && c->lists.rec_skip1};
+                              (c->jac_impl == JI_RECORDS && c->lists.rec_skip2 && c->lists.rec_skip1) ||
+                                  (c->jac_impl == JI_COMPACT && c->lists.rec_skip1)};
     const char *sor = !c->fast() ? "n/a" : !c->dc() ? "list-colour" : c->lists.split ? "interleaved-colour-split"
                                                                                     : "interleaved-colour";
     if (c->fast() && c->dc() && c->lists.split) {
@@ -1227,11 +1350,17 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
     std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(c, cur, buf, sizeof buf) : sor);
     if (tried && tried_len > 0) {
         std::string t;
+        if (c->layout_ms[0] > 0.f) {
+            // the FP32+DC Jacobi layout choice (choose_jacobi_layout)
+            char e[96];
+            std::snprintf(e, sizeof e, "layout-compact=%.4f;layout-dense=%.4f", c->layout_ms[0], c->layout_ms[1]);
+            t += e;
+        }
         const Tuner &tn = c->tuner;
         if (tn.done)
             for (int q = tn.first_scored; q < tn.n; q++) {
                 char e[96];
-                std::snprintf(e, sizeof e, "%s%s=%.4f", q > tn.first_scored ? ";" : "",
+                std::snprintf(e, sizeof e, "%s%s=%.4f", q > tn.first_scored || !t.empty() ? ";" : "",
                               variant_name(c, tn.cand[q], buf, sizeof buf), tn.ms[q]);
                 t += e;
             }

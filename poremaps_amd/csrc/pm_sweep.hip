@@ -229,6 +229,242 @@ __global__ __launch_bounds__(kListBlock) void k_colour4_compact(const uint2 *__r
     colour_voxel4_compact(t0 + t, L, nE, rsL, rsS, S, q, R, cu, cp, 0.f);
 }
 
+// ---------------------------------------------------------------------------
+// FP32+DC Jacobi on the compact state (FluidLists::compact_jac): one entry per
+// fluid cell in list order and the zero entry, links as for the SOR, but a
+// link across a periodic face goes straight to the image cell.  The record
+// pass and the passes before it write their cells' periodic halo copies
+// themselves, so in the dense layout a halo cell always holds its image's
+// values, except the pressure of the z-halo planes 1 and nz + 2, which the
+// fill pins to 0: plane 1 copies the outlet plane, whose correction pressure
+// is 0 too, and no pass reads the pressure of a +z neighbour.  Reading the
+// image therefore gives the same values, and the compact state needs no
+// halo entries and no ghost fill.  The lagged pass recomputes the pressure at
+// its -x, -y, -z neighbours exactly as lag_voxel4_rec does (the wrapped
+// neighbour is the same periodic image); their +1 neighbours are second hops
+// through the neighbours' links.  Flags (36 bits over the links' top bits):
+// the packed codes of the cell and of its -x, -y, -z neighbours (8 bits
+// each), the outlet flag (bit 32) and the first-plane flag (bit 33).
+// ---------------------------------------------------------------------------
+struct JLinks {
+    int32_t xm, xp, ym, yp, zm, zp;
+    uint64_t f;
+};
+
+__device__ __forceinline__ JLinks jac_links(const uint2 *__restrict__ L, int nE, int t) {
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const v2u lx = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[t]));
+    const v2u ly = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[nE + t]));
+    const v2u lz = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[2 * nE + t]));
+    JLinks r;
+    r.xm = (int32_t)(lx.x & kLinkMask); r.xp = (int32_t)(lx.y & kLinkMask);
+    r.ym = (int32_t)(ly.x & kLinkMask); r.yp = (int32_t)(ly.y & kLinkMask);
+    r.zm = (int32_t)(lz.x & kLinkMask); r.zp = (int32_t)(lz.y & kLinkMask);
+    r.f = (uint64_t)(lx.x >> kLinkBits) | ((uint64_t)(lx.y >> kLinkBits) << 6) |
+          ((uint64_t)(ly.x >> kLinkBits) << 12) | ((uint64_t)(ly.y >> kLinkBits) << 18) |
+          ((uint64_t)(lz.x >> kLinkBits) << 24) | ((uint64_t)(lz.y >> kLinkBits) << 30);
+    return r;
+}
+
+// The cell's stencil: the +-1 neighbours (each only where its side is open,
+// with ALL1) and the +-2 neighbours as second hops, only where the case reads
+// them.
+template <bool ALL1>
+__device__ __forceinline__ Star4 star_compact(const JLinks &l, const Cases &cc, const float4 *__restrict__ S,
+                                              __amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS, int nE) {
+    Star4 s;
+    if (ALL1) {
+        s.XM = buf_load4(rsS, l.xm, !cc.bx); s.XP = buf_load4(rsS, l.xp, !fwd_solid(cc.cx));
+        s.YM = buf_load4(rsS, l.ym, !cc.by); s.YP = buf_load4(rsS, l.yp, !fwd_solid(cc.cy));
+        s.ZM = buf_load4(rsS, l.zm, !cc.bz); s.ZP = buf_load4(rsS, l.zp, !fwd_solid(cc.cz));
+    } else {
+        s.XM = S[l.xm]; s.XP = S[l.xp];
+        s.YM = S[l.ym]; s.YP = S[l.yp];
+        s.ZM = S[l.zm]; s.ZP = S[l.zp];
+    }
+    const bool nxm = cc.cx == 5, nxp = cc.cx == 4, nym = cc.cy == 5, nyp = cc.cy == 4;
+    const bool nzm = cc.cz == 5, nzp = cc.cz == 4;
+    const int32_t xm2 = buf_load_link(rsL, l.xm, 0, nE, nxm), xp2 = buf_load_link(rsL, l.xp, 1, nE, nxp);
+    const int32_t ym2 = buf_load_link(rsL, l.ym, 2, nE, nym), yp2 = buf_load_link(rsL, l.yp, 3, nE, nyp);
+    const int32_t zm2 = buf_load_link(rsL, l.zm, 4, nE, nzm), zp2 = buf_load_link(rsL, l.zp, 5, nE, nzp);
+    s.XM2 = buf_load4(rsS, xm2, nxm); s.XP2 = buf_load4(rsS, xp2, nxp);
+    s.YM2 = buf_load4(rsS, ym2, nym); s.YP2 = buf_load4(rsS, yp2, nyp);
+    s.ZM2 = buf_load4(rsS, zm2, nzm); s.ZP2 = buf_load4(rsS, zp2, nzp);
+    return s;
+}
+
+// Component q of the state at the entry behind link word w of entry e (a
+// second hop), only where need is set.
+__device__ __forceinline__ float hop1(__amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS, int32_t e, int w,
+                                      int q, int nE, bool need) {
+    const int32_t n = buf_load_link(rsL, e, w, nE, need);
+    return buf_load1(rsS, n, q, need);
+}
+
+// lag_voxel4_rec on the compact state.  ALL: every load, else each +-1
+// neighbour and each recompute-target value only where it is used.
+template <bool ALL>
+__device__ __forceinline__ void lag_voxel4_compact(int t, const uint2 *__restrict__ L, int nE,
+                                                   __amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS,
+                                                   const float4 *__restrict__ S0, float4 *__restrict__ S1, Res4 R,
+                                                   float cu, float cp, float pin_out) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const JLinks l = jac_links(L, nE, t);
+    const uint8_t pcv = (uint8_t)(l.f & 0xff), pcx = (uint8_t)((l.f >> 8) & 0xff),
+                  pcy = (uint8_t)((l.f >> 16) & 0xff), pcz = (uint8_t)((l.f >> 24) & 0xff);
+    const bool out = (l.f >> 32) & 1u, first = (l.f >> 33) & 1u;
+    const int32_t zero = nE - 1;
+    const float rpx = R.pc[l.xm], rpy = R.pc[l.ym], rpz = R.pc[l.zm];
+    const float4 C = S0[t];
+    const Cases cc = decode(pcv);
+    const Star4 s = star_compact<ALL>(l, cc, S0, rsL, rsS, nE);
+    float vyx, wzx, uxy, wzy, uxz, vyz;
+    if (ALL) {
+        vyx = hop1(rsL, rsS, l.xm, 3, 2, nE, true); wzx = hop1(rsL, rsS, l.xm, 5, 3, nE, true);
+        uxy = hop1(rsL, rsS, l.ym, 1, 1, nE, true); wzy = hop1(rsL, rsS, l.ym, 5, 3, nE, true);
+        uxz = hop1(rsL, rsS, l.zm, 1, 1, nE, true); vyz = hop1(rsL, rsS, l.zm, 3, 2, nE, true);
+    } else {
+        const Cases tx = decode(pcx), ty = decode(pcy), tz = decode(pcz);
+        vyx = hop1(rsL, rsS, l.xm, 3, 2, nE, !cc.bx && !fwd_solid(tx.cy));
+        wzx = hop1(rsL, rsS, l.xm, 5, 3, nE, !cc.bx && !fwd_solid(tx.cz));
+        uxy = hop1(rsL, rsS, l.ym, 1, 1, nE, !cc.by && !fwd_solid(ty.cx));
+        wzy = hop1(rsL, rsS, l.ym, 5, 3, nE, !cc.by && !fwd_solid(ty.cz));
+        uxz = hop1(rsL, rsS, l.zm, 1, 1, nE, !cc.bz && !fwd_solid(tz.cx));
+        vyz = hop1(rsL, rsS, l.zm, 3, 2, nE, !cc.bz && !fwd_solid(tz.cy));
+    }
+    const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
+    const float4 Rm = make_float4(r.x, r.y, r.z, r.w);
+    const float rpc = r.w;  // the cell's continuity residual
+    // the -x / -y neighbour is the periodic image where the dense pass wraps,
+    // and the image's +x / +y neighbour is this cell
+    const float4 NX = s.XM, NY = s.YM;
+    const float uxx = C.y, vyy = C.z;
+    const Cases cs = cc;
+    const float Pc = out ? pin_out : C.x - (div_from4(pcv, C, s.XP.y, s.YP.z, s.ZP.w) + rpc) * cp;
+    const float c2dt = (float)(kC2 * kDt);
+    const float dx0 = div_from4(pcx, NX, uxx, vyx, wzx);
+    const float dy0 = div_from4(pcy, NY, uxy, vyy, wzy);
+    const float dz0 = div_from4(pcz, s.ZM, uxz, vyz, C.w);
+    float px = out ? pin_out - dx0 * c2dt : NX.x - (dx0 + rpx) * cp;
+    float py = out ? pin_out - dy0 * c2dt : NY.x - (dy0 + rpy) * cp;
+    const float pz = first ? s.ZM.x : s.ZM.x - (dz0 + rpz) * cp;
+    // a wall halo keeps its stored value, 0 (as does its outlet buffer
+    // value); a solid neighbour's value is never used
+    if (l.xm == zero) px = s.XM.x;
+    if (l.ym == zero) py = s.YM.x;
+    const float dpx = cs.bx ? 0.f : Pc - px;
+    const float dpy = cs.by ? 0.f : Pc - py;
+    const float dpz = cs.bz ? 0.f : Pc - pz;
+    float su, sv, sw;
+    visc_from4(cs, s, C, su, sv, sw);
+    const float4 o = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, Rm, Pc, cu);
+    const v4f ov = {o.x, o.y, o.z, o.w};
+    __builtin_nontemporal_store(ov, reinterpret_cast<v4f *>(&S1[t]));
+}
+
+// OCC: waves per SIMD the registers must allow (0: the compiler's choice)
+template <bool ALL, int OCC = 0>
+__global__ __launch_bounds__(kListBlock, OCC > 0 ? OCC : 1) void k_jac4c_lag(const uint2 *__restrict__ L, int nE, int n,
+                                                          const float4 *__restrict__ S0, float4 *__restrict__ S1,
+                                                          Res4 R, float cu, float cp) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const __amdgpu_buffer_rsrc_t rsL =
+        __builtin_amdgcn_make_buffer_rsrc((void *)L, (short)0, (int)((uint32_t)nE * 24u), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsS =
+        __builtin_amdgcn_make_buffer_rsrc((void *)S0, (short)0, (int)((uint32_t)nE * 16u), 0x00020000);
+    lag_voxel4_compact<ALL>(t, L, nE, rsL, rsS, S0, S1, R, cu, cp, 0.f);
+}
+
+// first_voxel4_zero on the compact state (no halo copies to write).
+__global__ __launch_bounds__(kListBlock) void k_jac4c_first_zero(const uint2 *__restrict__ L, int nE, int n,
+                                                                 float4 *__restrict__ S1, Res4 R, float cu) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const v2u lx = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[t]));
+    const uint32_t code = (lx.x >> kLinkBits) | (((lx.y >> kLinkBits) & 3u) << 6);
+    const Cases cs = decode((uint8_t)code);
+    const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
+    const float z = 0.f;
+    v4f o;
+    o.x = z;
+    o.y = cs.bx ? z : z + (z + r.x) * cu;
+    o.z = cs.by ? z : z + (z + r.y) * cu;
+    o.w = cs.bz ? z : z + (z + r.z) * cu;
+    __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(&S1[t]));
+}
+
+// vel_voxel4 (the first pass from a settled state, p carried) on the compact
+// state; the outlet buffer by entry.
+__global__ __launch_bounds__(kListBlock) void k_jac4c_first(const uint2 *__restrict__ L, int nE, int n,
+                                                            const float4 *__restrict__ S0,
+                                                            const float *__restrict__ q0, float4 *__restrict__ S1,
+                                                            Res4 R, float cu) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const __amdgpu_buffer_rsrc_t rsL =
+        __builtin_amdgcn_make_buffer_rsrc((void *)L, (short)0, (int)((uint32_t)nE * 24u), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsS =
+        __builtin_amdgcn_make_buffer_rsrc((void *)S0, (short)0, (int)((uint32_t)nE * 16u), 0x00020000);
+    const JLinks l = jac_links(L, nE, t);
+    const Cases cs = decode((uint8_t)(l.f & 0xff));
+    const bool out = (l.f >> 32) & 1u;
+    const float4 C = S0[t];
+    const Star4 s = star_compact<false>(l, cs, S0, rsL, rsS, nE);
+    const float pxm = out ? q0[l.xm] : s.XM.x;
+    const float pym = out ? q0[l.ym] : s.YM.x;
+    const float pzm = s.ZM.x;
+    const float dpx = cs.bx ? 0.f : C.x - pxm;
+    const float dpy = cs.by ? 0.f : C.x - pym;
+    const float dpz = cs.bz ? 0.f : C.x - pzm;
+    float su, sv, sw;
+    visc_from4(cs, s, C, su, sv, sw);
+    S1[t] = vel_from4(cs, C, su, sv, sw, dpx, dpy, dpz, R.m[t], C.x, cu);
+}
+
+// pres_voxel4 in place (the settle pass) on the compact state, with the pin
+// the dense ghost fill then applies: an outlet cell's correction pressure is
+// 0 and its outlet buffer value pin_out - c2 dt div.  The continuity residual
+// of the cell is the first lane of its gathered residuals.
+__global__ __launch_bounds__(kListBlock) void k_jac4c_settle(const uint2 *__restrict__ L, int nE, int n, float4 *S,
+                                                             float *__restrict__ q1, Res4 R, float cp) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const JLinks l = jac_links(L, nE, t);
+    const Cases cs = decode((uint8_t)(l.f & 0xff));
+    const bool out = (l.f >> 32) & 1u;
+    const float4 N = S[t];
+    const float wz = fwd_solid(cs.cz) ? 0.f : S[l.zp].w;
+    const float vy = fwd_solid(cs.cy) ? 0.f : S[l.yp].z;
+    const float ux = fwd_solid(cs.cx) ? 0.f : S[l.xp].y;
+    const float div0 = ((((wz - N.w) + vy) - N.z) + ux) - N.y;
+    const float div = div0 + R.m[t].w;
+    float p = N.x - div * cp;
+    if (out) {
+        q1[t] = 0.f - div0 * (float)(kC2 * kDt);
+        p = 0.f;
+    }
+    S[t].x = p;
+}
+
+// The compact Jacobi flags into the links' top bits, from the lagged pass's
+// per-voxel records (natural offset; codes of the cell and of its three
+// recompute targets) and the cell's plane.
+__global__ __launch_bounds__(kListBlock) void k_compact_jac_flags(Grid g, const uint2 *__restrict__ rec, int n,
+                                                                  int nE, uint2 *L) {
+    const int t = blockIdx.x * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint2 r = rec[t];
+    int i, j, k;
+    cell_ijk(g, (int64_t)r.x, i, j, k);
+    const uint64_t f = (uint64_t)r.y | ((uint64_t)(i == g.nz + 1) << 32) | ((uint64_t)(i == 2) << 33);
+    uint32_t *w[6] = {&L[t].x, &L[t].y, &L[nE + t].x, &L[nE + t].y, &L[2 * nE + t].x, &L[2 * nE + t].y};
+    for (int q = 0; q < 6; q++)
+        *w[q] = (*w[q] & kLinkMask) | ((uint32_t)((f >> (6 * q)) & 63u) << kLinkBits);
+}
+
 // The compact state's ghost fill (ghost_line4 on the entries): halo entries
 // copied from their images, with p 0 where the dense fill pins it (bit 31 of
 // the image: copies of the outlet plane and the z-halo planes 1 and nz + 2),
@@ -770,6 +1006,7 @@ void launch_jacobi_stage(int stage, const Grid &g, const FluidLists &L, const ui
 }
 
 bool jacobi4_fills_halos(int stage, const FluidLists &L) {
+    if (L.compact_jac) return true;  // no halo entries: nothing to fill
     if (stage == JAC_FIRST_ZERO) return L.rec && L.fill_halos;
     if (stage == JAC_LAG && L.use_march && L.rowstart) return true;
     return stage == JAC_LAG && L.select_loads && L.rec && L.use_rec && L.fill_halos;
@@ -779,6 +1016,24 @@ void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s) {
     if (L.n_all == 0) return;
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
+    if (L.compact_jac) {
+        const int nE = L.n_centries;
+        if (stage == JAC_FIRST_ZERO)
+            hipLaunchKernelGGL(k_jac4c_first_zero, grid, blk, 0, s, L.clink, nE, L.n_all, S1, R, cu);
+        else if (stage == JAC_FIRST)
+            hipLaunchKernelGGL(k_jac4c_first, grid, blk, 0, s, L.clink, nE, L.n_all, S0, q0, S1, R, cu);
+        else if (stage == JAC_LAG && L.rec_skip1 && L.rec_occ == 8)
+            hipLaunchKernelGGL((k_jac4c_lag<false, 8>), grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
+        else if (stage == JAC_LAG && L.rec_skip1 && L.rec_occ == 6)
+            hipLaunchKernelGGL((k_jac4c_lag<false, 6>), grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
+        else if (stage == JAC_LAG && L.rec_skip1)
+            hipLaunchKernelGGL(k_jac4c_lag<false>, grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
+        else if (stage == JAC_LAG)
+            hipLaunchKernelGGL(k_jac4c_lag<true>, grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
+        else  // JAC_SETTLE (in place, S1 == S0); the pair has no compact form
+            hipLaunchKernelGGL(k_jac4c_settle, grid, blk, 0, s, L.clink, nE, L.n_all, S1, q1, R, cp);
+        return;
+    }
     if (stage == JAC_FIRST_ZERO && !jacobi4_fills_halos(stage, L)) stage = JAC_FIRST;  // the state is zero
     if (stage == JAC_FIRST_ZERO) {
         hipLaunchKernelGGL(k_jac4_first_zero, grid, blk, 0, s, g, L.rec, L.n_all, S1, R, cu);
@@ -916,13 +1171,30 @@ void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, h
             hipLaunchKernelGGL(k_compact_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, L.rec, L.n_all,
                                L.clink);
     }
-    else
+    else {
         hipLaunchKernelGGL(k_lag_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc,
                            L.rec);
+        if (L.compact_jac)
+            hipLaunchKernelGGL(k_compact_jac_flags, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.rec,
+                               L.n_all, L.n_centries, L.clink);
+    }
+}
+
+// The compact state's continuity residual per entry (k_gather_rq's
+// counterpart): entry t is list position t.
+__global__ __launch_bounds__(kListBlock) void k_gather_rp_compact(const int32_t *__restrict__ list, int n, Res4 R) {
+    const int t = blockIdx.x * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    R.pc[t] = R.p[list[t]];
 }
 
 void launch_gather_rq(const Grid &g, const FluidLists &L, Res4 R, hipStream_t s) {
-    if (L.n_all == 0 || !R.q) return;
+    if (L.n_all == 0) return;
+    // the compact state's residual per entry (also while its layout is being
+    // timed against the dense one, which needs the gathered one)
+    if (R.pc)
+        hipLaunchKernelGGL(k_gather_rp_compact, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, L.all, L.n_all, R);
+    if (!R.q) return;
     hipLaunchKernelGGL(k_gather_rq, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, R);
 }
 
@@ -943,6 +1215,7 @@ void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, bool sp
 }
 
 void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, float4 *S, float *q, hipStream_t s) {
+    if (L.compact_jac) return;  // no halo entries; the outlet pins are the settle pass's
     if (!L.compact) {
         launch_ghost4(g, G, S, q, L.split, s);
         return;
@@ -954,7 +1227,7 @@ void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, f
 }
 
 void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s) {
-    if (L.compact) {
+    if (L.compact || L.compact_jac) {
         if (L.n_all == 0) return;
         int threads = 0;
         const ListWalk W = list_walk(L, threads);

@@ -4,7 +4,7 @@ The velocity + pressure pair, the lagged single pass over the fluid list
 (all loads up front, case-predicated, or from per-voxel records) and the
 lagged dense 2.5-D pass (two
 z-segment lengths) -- in FP32+DC the pair and lagged passes over the
-interleaved state -- must produce the same
+interleaved state, and the lagged pass on the compact state -- must produce the same
 iterates bit for bit, and a solve whose first sweeps the autotuner spreads
 over all of them must end exactly where a fixed-form solve ends."""
 import os
@@ -36,11 +36,12 @@ def solve(raw, bc, precision, form=None, zs=None, loads=None, march=None):
         else:
             os.environ.pop("POREMAPS_SWEEP_LOADS", None)
         os.environ.pop("POREMAPS_REC_SKIP2", None)
-        if form and form.startswith("records-"):
+        if form and (form.startswith("records-") or form.startswith("compact-")):
             # the record pass skipping its unused +-2 loads (skip2) or every
-            # unused load (skipall)
+            # unused load (skipall); the compact pass skipping every unused
+            # load
             os.environ["POREMAPS_REC_SKIP2"] = "2" if form.endswith("skipall") else "1"
-            form = "records"
+            form = form.split("-")[0]
         if form:
             os.environ["POREMAPS_JACOBI"] = form
         else:
@@ -75,7 +76,8 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
         # the z-march of the interleaved lagged pass: tiles of 64 x 4 / 64 x 8,
         # z segments of 5 planes (several per solve), 128 and the whole depth
         forms += [("march", None, None, (4, 5)), ("march", None, None, (8, 128)), ("march", None, None, (4, 0)),
-                  ("records-skipall", None, None, None)]
+                  ("records-skipall", None, None, None), ("compact", None, None, None),
+                  ("compact-skipall", None, None, None)]
     for form, zs, loads, march in forms + [(None, None, None, None)]:
         res, fields, (name, tried) = solve(raw, bc, precision, form, zs, loads, march)
         say(form, zs, loads, march, name, res.sweeps)
@@ -83,11 +85,20 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
         for a, b, n in zip(fields, ref_fields, ("velx", "vely", "velz", "press")):
             assert np.array_equal(a, b), (form, zs, n)
         if form is None:
-            # FP32+DC runs on the interleaved state: pair, the three lagged
-            # list forms (eager, branch, records; records also at 6 and 8
-            # waves per SIMD and with the +-2 loads skipped where unused) and
-            # the z-march
-            assert len(tried) == (9 if precision == "f32dc" else 7) and name in tried
+            # FP32+DC: the layout (compact or dense) timed first, then the
+            # forms of that layout: the compact pass with every load and with
+            # each load only where used (also at 6 and 8 waves per SIMD), or
+            # the dense interleaved forms; FP64: pair, lagged list, records
+            # and the dense 2.5-D forms
+            forms_tried = {k: v for k, v in tried.items() if not k.startswith("layout-")}
+            assert name in forms_tried
+            if precision == "f32dc":
+                assert set(tried) >= {"layout-compact", "layout-dense"}
+                compact = tried["layout-compact"] <= tried["layout-dense"]
+                assert name.startswith("interleaved-compact") == compact
+                assert len(forms_tried) == (4 if compact else 9)
+            else:
+                assert len(forms_tried) == 7
 
 
 def test_divergent_relaxation_restarts_slower():
