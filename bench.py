@@ -220,7 +220,9 @@ def measured_traffic(spec, form, precision):
              "interleaved-colour-split": ("sor3_f32dc_grf512_hbm_traffic.md",
                                           r"traffic_bytes_per_sweep k_colour4_split x3: (\d+)"),
              "interleaved-colour-compact": ("sor3_compact_f32dc_grf512_hbm_traffic.md",
-                                            r"traffic_bytes_per_sweep k_colour4_compact x3: (\d+)")}
+                                            r"traffic_bytes_per_sweep k_colour4_compact x3: (\d+)"),
+             "interleaved-compact-skipall": ("jacobi_compact_f32dc_grf512_hbm_traffic.md",
+                                             r"traffic_bytes_per_launch k_jac4c_lag: (\d+)")}
     if spec["name"] != "grf_512" or form not in known or precision != "f32dc":
         return None
     path = os.path.join(ROOT, "profiles", known[form][0])
