@@ -122,6 +122,7 @@ struct pm_ctx {
     bool sor_compact = true;    // FluidLists::compact (POREMAPS_SOR=split / natural: off)
     bool jac_compact = true;    // FluidLists::compact_jac (POREMAPS_JACOBI naming a dense form: off)
     float layout_ms[2] = {0.f, 0.f};  // FP32+DC Jacobi: ms per sweep, compact / dense layout (0: not timed)
+    bool jac_links_aos = false; // FluidLists::links_aos (POREMAPS_JAC_LINKS=aos)
     Tuner tuner;
     // the state holds (u^n, p^{n-1}) of a lagged Jacobi sequence
     bool lagged = false;

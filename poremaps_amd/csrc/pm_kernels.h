@@ -148,6 +148,9 @@ struct FluidLists {
     // image (no halo entries; the flags hold the codes of the cell and of its
     // recompute targets, the outlet and first-plane flags)
     bool compact_jac = false;
+    // compact Jacobi: the links as one record of six words per entry (+x, +y,
+    // +z, -x, -y, -z) instead of three arrays of (-, +) pairs
+    bool links_aos = false;
     uint2 *clink = nullptr;
     int n_centries = 0;
     int2 *chalo = nullptr;

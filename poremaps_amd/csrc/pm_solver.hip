@@ -227,6 +227,7 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     // FP32+DC Jacobi on the compact state: not with slip walls (their
     // mirrored halos are not copies of images)
     if (p->walls == 2) c->jac_compact = false;
+    if (const char *jl = std::getenv("POREMAPS_JAC_LINKS")) c->jac_links_aos = std::strcmp(jl, "aos") == 0;
     if (const char *zs = std::getenv("POREMAPS_STREAM_ZS")) c->stream_zs = std::max(1, std::atoi(zs));
     if (c->g.slip && (c->jac_impl == JI_STREAM || c->jac_impl == JI_MARCH)) c->jac_impl = JI_RECORDS;
     // z-march form (POREMAPS_JACOBI=march): tile rows 4 or 8, planes per z
@@ -427,7 +428,10 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
             }
     const int64_t zero = next, n_ent = next + 1;
     if (!pm::compact_supported(g, n_ent)) return PM_OK;
-    // links: (-x, +x), (-y, +y), (-z, +z) arrays of n_ent entries
+    // links: (-x, +x), (-y, +y), (-z, +z) arrays of n_ent entries, or (the
+    // compact Jacobi with POREMAPS_JAC_LINKS=aos) one record of six words per
+    // entry; 24 B per entry either way
+    const bool aos = torus && c->jac_links_aos;
     std::vector<uint2> lk((size_t)(3 * n_ent));
     std::vector<int32_t> outl;
     auto link = [&](int64_t off, int64_t e) {
@@ -444,15 +448,25 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
             const int32_t m = cidx[(size_t)g.at(ii, jj, kk)];
             return (uint32_t)(m < 0 ? zero : m);
         };
-        lk[(size_t)e] = make_uint2(at(i, j, k - 1), at(i, j, k + 1));
-        lk[(size_t)(n_ent + e)] = make_uint2(at(i, j - 1, k), at(i, j + 1, k));
-        lk[(size_t)(2 * n_ent + e)] = make_uint2(at(i - 1, j, k), at(i + 1, j, k));
+        if (aos) {
+            // one record per entry: +x, +y, +z, -x, -y, -z
+            uint32_t *w = reinterpret_cast<uint32_t *>(lk.data()) + (size_t)e * 6;
+            w[0] = at(i, j, k + 1); w[1] = at(i, j + 1, k); w[2] = at(i + 1, j, k);
+            w[3] = at(i, j, k - 1); w[4] = at(i, j - 1, k); w[5] = at(i - 1, j, k);
+        } else {
+            lk[(size_t)e] = make_uint2(at(i, j, k - 1), at(i, j, k + 1));
+            lk[(size_t)(n_ent + e)] = make_uint2(at(i, j - 1, k), at(i, j + 1, k));
+            lk[(size_t)(2 * n_ent + e)] = make_uint2(at(i - 1, j, k), at(i + 1, j, k));
+        }
         return i;
     };
     for (int64_t t = 0; t < n_all; t++)
         if (link(all[(size_t)t], t) == iout) outl.push_back((int32_t)t);
     for (size_t h = 0; h < hoff.size(); h++) link(hoff[h], n_all + (int64_t)h);
-    for (int a = 0; a < 3; a++) lk[(size_t)(a * n_ent + zero)] = make_uint2((uint32_t)zero, (uint32_t)zero);
+    if (aos)
+        for (int q = 0; q < 6; q++) reinterpret_cast<uint32_t *>(lk.data())[(size_t)zero * 6 + q] = (uint32_t)zero;
+    else
+        for (int a = 0; a < 3; a++) lk[(size_t)(a * n_ent + zero)] = make_uint2((uint32_t)zero, (uint32_t)zero);
     RC(dalloc(&c->lists.clink, 3 * n_ent));
     HIPCHK(hipMemcpy(c->lists.clink, lk.data(), sizeof(uint2) * lk.size(), hipMemcpyHostToDevice));
     RC(dalloc(&c->lists.chalo, std::max<int64_t>(1, (int64_t)halo.size())));
@@ -479,6 +493,7 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
     c->lists.n_cout = (int)outl.size();
     if (torus) c->lists.compact_jac = true;
     else c->lists.compact = true;
+    c->lists.links_aos = aos;
     return PM_OK;
 }
 
@@ -891,9 +906,10 @@ static void tuner_setup(pm_ctx *c) {
     if (c->lists.compact_jac) {
         // the compact state: its pass with every load and with each load only
         // where it is used (the dense forms read another layout)
+        // (8 waves per SIMD spills and measured 1.54 against 0.70 ms per sweep
+        // on the 512^3 field; it is not a candidate)
         const SweepVariant cmp[] = {{JI_COMPACT, true, 64}, {JI_COMPACT, true, 64, 4, 0, true, true},
-                                    {JI_COMPACT, true, 64, 4, 6, true, true}, {JI_COMPACT, true, 64, 4, 8, true, true},
-                                    {JI_COMPACT, true, 64}};
+                                    {JI_COMPACT, true, 64, 4, 6, true, true}, {JI_COMPACT, true, 64}};
         for (const auto &v : cmp) tn.cand[tn.n++] = v;
     } else if (c->dc()) {
         for (const auto &v : aos)

@@ -251,25 +251,51 @@ struct JLinks {
     uint64_t f;
 };
 
+// Where link word w (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z) of entry e lies,
+// in 32-bit words: three arrays of (-, +) pairs, or (AOS) one record of six
+// words per entry ordered +x, +y, +z, -x, -y, -z, so that the +1 links a
+// second hop needs from one neighbour are adjacent.
+template <bool AOS>
+__device__ __forceinline__ uint32_t link_word(int32_t e, int w, int nE) {
+    if (AOS) return (uint32_t)e * 6u + (uint32_t)((w & 1) ? (w >> 1) : 3 + (w >> 1));
+    return ((uint32_t)(w >> 1) * (uint32_t)nE + (uint32_t)e) * 2u + (uint32_t)(w & 1);
+}
+
+template <bool AOS>
+__device__ __forceinline__ int32_t buf_load_lw(__amdgpu_buffer_rsrc_t rs, int32_t e, int w, int nE, bool need) {
+    const uint32_t off = need ? link_word<AOS>(e, w, nE) * 4u : 0xFFFFFFF0u;
+    return (int32_t)((uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0) & kLinkMask);
+}
+
+template <bool AOS>
 __device__ __forceinline__ JLinks jac_links(const uint2 *__restrict__ L, int nE, int t) {
-    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
-    const v2u lx = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[t]));
-    const v2u ly = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[nE + t]));
-    const v2u lz = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[2 * nE + t]));
+    uint32_t w[6];
+    if (AOS) {
+        typedef unsigned int v3u __attribute__((ext_vector_type(3)));
+        const uint32_t *W = reinterpret_cast<const uint32_t *>(L) + (size_t)t * 6;
+        const v3u p = __builtin_nontemporal_load(reinterpret_cast<const v3u *>(W));
+        const v3u m = __builtin_nontemporal_load(reinterpret_cast<const v3u *>(W + 3));
+        w[0] = m.x; w[1] = p.x; w[2] = m.y; w[3] = p.y; w[4] = m.z; w[5] = p.z;
+    } else {
+        typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+        const v2u lx = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[t]));
+        const v2u ly = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[nE + t]));
+        const v2u lz = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[2 * nE + t]));
+        w[0] = lx.x; w[1] = lx.y; w[2] = ly.x; w[3] = ly.y; w[4] = lz.x; w[5] = lz.y;
+    }
     JLinks r;
-    r.xm = (int32_t)(lx.x & kLinkMask); r.xp = (int32_t)(lx.y & kLinkMask);
-    r.ym = (int32_t)(ly.x & kLinkMask); r.yp = (int32_t)(ly.y & kLinkMask);
-    r.zm = (int32_t)(lz.x & kLinkMask); r.zp = (int32_t)(lz.y & kLinkMask);
-    r.f = (uint64_t)(lx.x >> kLinkBits) | ((uint64_t)(lx.y >> kLinkBits) << 6) |
-          ((uint64_t)(ly.x >> kLinkBits) << 12) | ((uint64_t)(ly.y >> kLinkBits) << 18) |
-          ((uint64_t)(lz.x >> kLinkBits) << 24) | ((uint64_t)(lz.y >> kLinkBits) << 30);
+    r.xm = (int32_t)(w[0] & kLinkMask); r.xp = (int32_t)(w[1] & kLinkMask);
+    r.ym = (int32_t)(w[2] & kLinkMask); r.yp = (int32_t)(w[3] & kLinkMask);
+    r.zm = (int32_t)(w[4] & kLinkMask); r.zp = (int32_t)(w[5] & kLinkMask);
+    r.f = 0;
+    for (int q = 0; q < 6; q++) r.f |= (uint64_t)(w[q] >> kLinkBits) << (6 * q);
     return r;
 }
 
 // The cell's stencil: the +-1 neighbours (each only where its side is open,
 // with ALL1) and the +-2 neighbours as second hops, only where the case reads
 // them.
-template <bool ALL1>
+template <bool ALL1, bool AOS>
 __device__ __forceinline__ Star4 star_compact(const JLinks &l, const Cases &cc, const float4 *__restrict__ S,
                                               __amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS, int nE) {
     Star4 s;
@@ -284,9 +310,9 @@ __device__ __forceinline__ Star4 star_compact(const JLinks &l, const Cases &cc, 
     }
     const bool nxm = cc.cx == 5, nxp = cc.cx == 4, nym = cc.cy == 5, nyp = cc.cy == 4;
     const bool nzm = cc.cz == 5, nzp = cc.cz == 4;
-    const int32_t xm2 = buf_load_link(rsL, l.xm, 0, nE, nxm), xp2 = buf_load_link(rsL, l.xp, 1, nE, nxp);
-    const int32_t ym2 = buf_load_link(rsL, l.ym, 2, nE, nym), yp2 = buf_load_link(rsL, l.yp, 3, nE, nyp);
-    const int32_t zm2 = buf_load_link(rsL, l.zm, 4, nE, nzm), zp2 = buf_load_link(rsL, l.zp, 5, nE, nzp);
+    const int32_t xm2 = buf_load_lw<AOS>(rsL, l.xm, 0, nE, nxm), xp2 = buf_load_lw<AOS>(rsL, l.xp, 1, nE, nxp);
+    const int32_t ym2 = buf_load_lw<AOS>(rsL, l.ym, 2, nE, nym), yp2 = buf_load_lw<AOS>(rsL, l.yp, 3, nE, nyp);
+    const int32_t zm2 = buf_load_lw<AOS>(rsL, l.zm, 4, nE, nzm), zp2 = buf_load_lw<AOS>(rsL, l.zp, 5, nE, nzp);
     s.XM2 = buf_load4(rsS, xm2, nxm); s.XP2 = buf_load4(rsS, xp2, nxp);
     s.YM2 = buf_load4(rsS, ym2, nym); s.YP2 = buf_load4(rsS, yp2, nyp);
     s.ZM2 = buf_load4(rsS, zm2, nzm); s.ZP2 = buf_load4(rsS, zp2, nzp);
@@ -295,21 +321,50 @@ __device__ __forceinline__ Star4 star_compact(const JLinks &l, const Cases &cc, 
 
 // Component q of the state at the entry behind link word w of entry e (a
 // second hop), only where need is set.
+template <bool AOS>
 __device__ __forceinline__ float hop1(__amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS, int32_t e, int w,
                                       int q, int nE, bool need) {
-    const int32_t n = buf_load_link(rsL, e, w, nE, need);
+    const int32_t n = buf_load_lw<AOS>(rsL, e, w, nE, need);
     return buf_load1(rsS, n, q, need);
 }
 
+// The +1 links of the recompute targets a lagged pass hops through, with one
+// load per target in the record layout: (+y, +z) of the -x target, (+x, +z)
+// of the -y target, (+x, +y) of the -z target; each pair only where one of
+// its two values is used.
+template <bool AOS>
+__device__ __forceinline__ void target_links(__amdgpu_buffer_rsrc_t rsL, const JLinks &l, int nE, const bool need[6],
+                                             int32_t out[6]) {
+    if (AOS) {
+        typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+        typedef unsigned int v3u __attribute__((ext_vector_type(3)));
+        const bool nx = need[0] || need[1], ny = need[2] || need[3], nz = need[4] || need[5];
+        const v2u a = __builtin_bit_cast(v2u, __builtin_amdgcn_raw_buffer_load_b64(
+                                                   rsL, nx ? ((uint32_t)l.xm * 6u + 1u) * 4u : 0xFFFFFFF0u, 0, 0));
+        const v3u b = __builtin_bit_cast(v3u, __builtin_amdgcn_raw_buffer_load_b96(
+                                                   rsL, ny ? (uint32_t)l.ym * 24u : 0xFFFFFFF0u, 0, 0));
+        const v2u c = __builtin_bit_cast(v2u, __builtin_amdgcn_raw_buffer_load_b64(
+                                                   rsL, nz ? (uint32_t)l.zm * 24u : 0xFFFFFFF0u, 0, 0));
+        out[0] = (int32_t)(a.x & kLinkMask); out[1] = (int32_t)(a.y & kLinkMask);
+        out[2] = (int32_t)(b.x & kLinkMask); out[3] = (int32_t)(b.z & kLinkMask);
+        out[4] = (int32_t)(c.x & kLinkMask); out[5] = (int32_t)(c.y & kLinkMask);
+    } else {
+        out[0] = buf_load_lw<false>(rsL, l.xm, 3, nE, need[0]); out[1] = buf_load_lw<false>(rsL, l.xm, 5, nE, need[1]);
+        out[2] = buf_load_lw<false>(rsL, l.ym, 1, nE, need[2]); out[3] = buf_load_lw<false>(rsL, l.ym, 5, nE, need[3]);
+        out[4] = buf_load_lw<false>(rsL, l.zm, 1, nE, need[4]); out[5] = buf_load_lw<false>(rsL, l.zm, 3, nE, need[5]);
+    }
+}
+
 // lag_voxel4_rec on the compact state.  ALL: every load, else each +-1
-// neighbour and each recompute-target value only where it is used.
-template <bool ALL>
+// neighbour and each recompute-target value only where it is used.  AOS: the
+// links as one record per entry.
+template <bool ALL, bool AOS>
 __device__ __forceinline__ void lag_voxel4_compact(int t, const uint2 *__restrict__ L, int nE,
                                                    __amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS,
                                                    const float4 *__restrict__ S0, float4 *__restrict__ S1, Res4 R,
                                                    float cu, float cp, float pin_out) {
     typedef float v4f __attribute__((ext_vector_type(4)));
-    const JLinks l = jac_links(L, nE, t);
+    const JLinks l = jac_links<AOS>(L, nE, t);
     const uint8_t pcv = (uint8_t)(l.f & 0xff), pcx = (uint8_t)((l.f >> 8) & 0xff),
                   pcy = (uint8_t)((l.f >> 16) & 0xff), pcz = (uint8_t)((l.f >> 24) & 0xff);
     const bool out = (l.f >> 32) & 1u, first = (l.f >> 33) & 1u;
@@ -317,21 +372,23 @@ __device__ __forceinline__ void lag_voxel4_compact(int t, const uint2 *__restric
     const float rpx = R.pc[l.xm], rpy = R.pc[l.ym], rpz = R.pc[l.zm];
     const float4 C = S0[t];
     const Cases cc = decode(pcv);
-    const Star4 s = star_compact<ALL>(l, cc, S0, rsL, rsS, nE);
-    float vyx, wzx, uxy, wzy, uxz, vyz;
+    const Star4 s = star_compact<ALL, AOS>(l, cc, S0, rsL, rsS, nE);
+    // the recompute targets' +1 neighbours: v of (x-1, y+1), w of (x-1, z+1),
+    // u of (x+1, y-1), w of (y-1, z+1), u of (x+1, z-1), v of (y+1, z-1)
+    bool need[6];
     if (ALL) {
-        vyx = hop1(rsL, rsS, l.xm, 3, 2, nE, true); wzx = hop1(rsL, rsS, l.xm, 5, 3, nE, true);
-        uxy = hop1(rsL, rsS, l.ym, 1, 1, nE, true); wzy = hop1(rsL, rsS, l.ym, 5, 3, nE, true);
-        uxz = hop1(rsL, rsS, l.zm, 1, 1, nE, true); vyz = hop1(rsL, rsS, l.zm, 3, 2, nE, true);
+        for (int q = 0; q < 6; q++) need[q] = true;
     } else {
         const Cases tx = decode(pcx), ty = decode(pcy), tz = decode(pcz);
-        vyx = hop1(rsL, rsS, l.xm, 3, 2, nE, !cc.bx && !fwd_solid(tx.cy));
-        wzx = hop1(rsL, rsS, l.xm, 5, 3, nE, !cc.bx && !fwd_solid(tx.cz));
-        uxy = hop1(rsL, rsS, l.ym, 1, 1, nE, !cc.by && !fwd_solid(ty.cx));
-        wzy = hop1(rsL, rsS, l.ym, 5, 3, nE, !cc.by && !fwd_solid(ty.cz));
-        uxz = hop1(rsL, rsS, l.zm, 1, 1, nE, !cc.bz && !fwd_solid(tz.cx));
-        vyz = hop1(rsL, rsS, l.zm, 3, 2, nE, !cc.bz && !fwd_solid(tz.cy));
+        need[0] = !cc.bx && !fwd_solid(tx.cy); need[1] = !cc.bx && !fwd_solid(tx.cz);
+        need[2] = !cc.by && !fwd_solid(ty.cx); need[3] = !cc.by && !fwd_solid(ty.cz);
+        need[4] = !cc.bz && !fwd_solid(tz.cx); need[5] = !cc.bz && !fwd_solid(tz.cy);
     }
+    int32_t dl[6];
+    target_links<AOS>(rsL, l, nE, need, dl);
+    const float vyx = buf_load1(rsS, dl[0], 2, need[0]), wzx = buf_load1(rsS, dl[1], 3, need[1]);
+    const float uxy = buf_load1(rsS, dl[2], 1, need[2]), wzy = buf_load1(rsS, dl[3], 3, need[3]);
+    const float uxz = buf_load1(rsS, dl[4], 1, need[4]), vyz = buf_load1(rsS, dl[5], 2, need[5]);
     const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
     const float4 Rm = make_float4(r.x, r.y, r.z, r.w);
     const float rpc = r.w;  // the cell's continuity residual
@@ -363,28 +420,30 @@ __device__ __forceinline__ void lag_voxel4_compact(int t, const uint2 *__restric
 }
 
 // OCC: waves per SIMD the registers must allow (0: the compiler's choice)
-template <bool ALL, int OCC = 0>
-__global__ __launch_bounds__(kListBlock, OCC > 0 ? OCC : 1) void k_jac4c_lag(const uint2 *__restrict__ L, int nE, int n,
-                                                          const float4 *__restrict__ S0, float4 *__restrict__ S1,
-                                                          Res4 R, float cu, float cp) {
+template <bool ALL, bool AOS, int OCC = 0>
+__global__ __launch_bounds__(kListBlock, OCC > 0 ? OCC : 1) void k_jac4c_lag(const uint2 *__restrict__ L, int nE,
+                                                                             int n, const float4 *__restrict__ S0,
+                                                                             float4 *__restrict__ S1, Res4 R, float cu,
+                                                                             float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
     const __amdgpu_buffer_rsrc_t rsL =
         __builtin_amdgcn_make_buffer_rsrc((void *)L, (short)0, (int)((uint32_t)nE * 24u), 0x00020000);
     const __amdgpu_buffer_rsrc_t rsS =
         __builtin_amdgcn_make_buffer_rsrc((void *)S0, (short)0, (int)((uint32_t)nE * 16u), 0x00020000);
-    lag_voxel4_compact<ALL>(t, L, nE, rsL, rsS, S0, S1, R, cu, cp, 0.f);
+    lag_voxel4_compact<ALL, AOS>(t, L, nE, rsL, rsS, S0, S1, R, cu, cp, 0.f);
 }
 
 // first_voxel4_zero on the compact state (no halo copies to write).
+template <bool AOS>
 __global__ __launch_bounds__(kListBlock) void k_jac4c_first_zero(const uint2 *__restrict__ L, int nE, int n,
                                                                  float4 *__restrict__ S1, Res4 R, float cu) {
     typedef float v4f __attribute__((ext_vector_type(4)));
-    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
-    const v2u lx = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[t]));
-    const uint32_t code = (lx.x >> kLinkBits) | (((lx.y >> kLinkBits) & 3u) << 6);
+    const uint32_t *W = reinterpret_cast<const uint32_t *>(L);
+    const uint32_t wxm = W[link_word<AOS>(t, 0, nE)], wxp = W[link_word<AOS>(t, 1, nE)];
+    const uint32_t code = (wxm >> kLinkBits) | (((wxp >> kLinkBits) & 3u) << 6);
     const Cases cs = decode((uint8_t)code);
     const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
     const float z = 0.f;
@@ -398,6 +457,7 @@ __global__ __launch_bounds__(kListBlock) void k_jac4c_first_zero(const uint2 *__
 
 // vel_voxel4 (the first pass from a settled state, p carried) on the compact
 // state; the outlet buffer by entry.
+template <bool AOS>
 __global__ __launch_bounds__(kListBlock) void k_jac4c_first(const uint2 *__restrict__ L, int nE, int n,
                                                             const float4 *__restrict__ S0,
                                                             const float *__restrict__ q0, float4 *__restrict__ S1,
@@ -408,11 +468,11 @@ __global__ __launch_bounds__(kListBlock) void k_jac4c_first(const uint2 *__restr
         __builtin_amdgcn_make_buffer_rsrc((void *)L, (short)0, (int)((uint32_t)nE * 24u), 0x00020000);
     const __amdgpu_buffer_rsrc_t rsS =
         __builtin_amdgcn_make_buffer_rsrc((void *)S0, (short)0, (int)((uint32_t)nE * 16u), 0x00020000);
-    const JLinks l = jac_links(L, nE, t);
+    const JLinks l = jac_links<AOS>(L, nE, t);
     const Cases cs = decode((uint8_t)(l.f & 0xff));
     const bool out = (l.f >> 32) & 1u;
     const float4 C = S0[t];
-    const Star4 s = star_compact<false>(l, cs, S0, rsL, rsS, nE);
+    const Star4 s = star_compact<false, AOS>(l, cs, S0, rsL, rsS, nE);
     const float pxm = out ? q0[l.xm] : s.XM.x;
     const float pym = out ? q0[l.ym] : s.YM.x;
     const float pzm = s.ZM.x;
@@ -427,12 +487,13 @@ __global__ __launch_bounds__(kListBlock) void k_jac4c_first(const uint2 *__restr
 // pres_voxel4 in place (the settle pass) on the compact state, with the pin
 // the dense ghost fill then applies: an outlet cell's correction pressure is
 // 0 and its outlet buffer value pin_out - c2 dt div.  The continuity residual
-// of the cell is the first lane of its gathered residuals.
+// of the cell is the fourth lane of its momentum residual.
+template <bool AOS>
 __global__ __launch_bounds__(kListBlock) void k_jac4c_settle(const uint2 *__restrict__ L, int nE, int n, float4 *S,
                                                              float *__restrict__ q1, Res4 R, float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
-    const JLinks l = jac_links(L, nE, t);
+    const JLinks l = jac_links<AOS>(L, nE, t);
     const Cases cs = decode((uint8_t)(l.f & 0xff));
     const bool out = (l.f >> 32) & 1u;
     const float4 N = S[t];
@@ -452,6 +513,7 @@ __global__ __launch_bounds__(kListBlock) void k_jac4c_settle(const uint2 *__rest
 // The compact Jacobi flags into the links' top bits, from the lagged pass's
 // per-voxel records (natural offset; codes of the cell and of its three
 // recompute targets) and the cell's plane.
+template <bool AOS>
 __global__ __launch_bounds__(kListBlock) void k_compact_jac_flags(Grid g, const uint2 *__restrict__ rec, int n,
                                                                   int nE, uint2 *L) {
     const int t = blockIdx.x * kListBlock + threadIdx.x;
@@ -460,9 +522,11 @@ __global__ __launch_bounds__(kListBlock) void k_compact_jac_flags(Grid g, const 
     int i, j, k;
     cell_ijk(g, (int64_t)r.x, i, j, k);
     const uint64_t f = (uint64_t)r.y | ((uint64_t)(i == g.nz + 1) << 32) | ((uint64_t)(i == 2) << 33);
-    uint32_t *w[6] = {&L[t].x, &L[t].y, &L[nE + t].x, &L[nE + t].y, &L[2 * nE + t].x, &L[2 * nE + t].y};
-    for (int q = 0; q < 6; q++)
-        *w[q] = (*w[q] & kLinkMask) | ((uint32_t)((f >> (6 * q)) & 63u) << kLinkBits);
+    uint32_t *W = reinterpret_cast<uint32_t *>(L);
+    for (int q = 0; q < 6; q++) {
+        uint32_t &w = W[link_word<AOS>(t, q, nE)];
+        w = (w & kLinkMask) | ((uint32_t)((f >> (6 * q)) & 63u) << kLinkBits);
+    }
 }
 
 // The compact state's ghost fill (ghost_line4 on the entries): halo entries
@@ -1012,26 +1076,32 @@ bool jacobi4_fills_halos(int stage, const FluidLists &L) {
     return stage == JAC_LAG && L.select_loads && L.rec && L.use_rec && L.fill_halos;
 }
 
+template <bool AOS>
+static void launch_jacobi4_compact(int stage, const FluidLists &L, const float4 *S0, const float *q0, float4 *S1,
+                                   float *q1, Res4 R, float cu, float cp, hipStream_t s) {
+    const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
+    const int nE = L.n_centries;
+    if (stage == JAC_FIRST_ZERO)
+        hipLaunchKernelGGL(k_jac4c_first_zero<AOS>, grid, blk, 0, s, L.clink, nE, L.n_all, S1, R, cu);
+    else if (stage == JAC_FIRST)
+        hipLaunchKernelGGL(k_jac4c_first<AOS>, grid, blk, 0, s, L.clink, nE, L.n_all, S0, q0, S1, R, cu);
+    else if (stage == JAC_LAG && L.rec_skip1 && L.rec_occ == 6)
+        hipLaunchKernelGGL((k_jac4c_lag<false, AOS, 6>), grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
+    else if (stage == JAC_LAG && L.rec_skip1)
+        hipLaunchKernelGGL((k_jac4c_lag<false, AOS>), grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
+    else if (stage == JAC_LAG)
+        hipLaunchKernelGGL((k_jac4c_lag<true, AOS>), grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
+    else  // JAC_SETTLE (in place, S1 == S0); the pair has no compact form
+        hipLaunchKernelGGL(k_jac4c_settle<AOS>, grid, blk, 0, s, L.clink, nE, L.n_all, S1, q1, R, cp);
+}
+
 void launch_jacobi4(int stage, const Grid &g, const FluidLists &L, const uint8_t *pc, const float4 *S0,
                     const float *q0, float4 *S1, float *q1, Res4 R, float cu, float cp, hipStream_t s) {
     if (L.n_all == 0) return;
     const dim3 grid(lblocks(L.n_all)), blk(kListBlock);
     if (L.compact_jac) {
-        const int nE = L.n_centries;
-        if (stage == JAC_FIRST_ZERO)
-            hipLaunchKernelGGL(k_jac4c_first_zero, grid, blk, 0, s, L.clink, nE, L.n_all, S1, R, cu);
-        else if (stage == JAC_FIRST)
-            hipLaunchKernelGGL(k_jac4c_first, grid, blk, 0, s, L.clink, nE, L.n_all, S0, q0, S1, R, cu);
-        else if (stage == JAC_LAG && L.rec_skip1 && L.rec_occ == 8)
-            hipLaunchKernelGGL((k_jac4c_lag<false, 8>), grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
-        else if (stage == JAC_LAG && L.rec_skip1 && L.rec_occ == 6)
-            hipLaunchKernelGGL((k_jac4c_lag<false, 6>), grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
-        else if (stage == JAC_LAG && L.rec_skip1)
-            hipLaunchKernelGGL(k_jac4c_lag<false>, grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
-        else if (stage == JAC_LAG)
-            hipLaunchKernelGGL(k_jac4c_lag<true>, grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
-        else  // JAC_SETTLE (in place, S1 == S0); the pair has no compact form
-            hipLaunchKernelGGL(k_jac4c_settle, grid, blk, 0, s, L.clink, nE, L.n_all, S1, q1, R, cp);
+        if (L.links_aos) launch_jacobi4_compact<true>(stage, L, S0, q0, S1, q1, R, cu, cp, s);
+        else launch_jacobi4_compact<false>(stage, L, S0, q0, S1, q1, R, cu, cp, s);
         return;
     }
     if (stage == JAC_FIRST_ZERO && !jacobi4_fills_halos(stage, L)) stage = JAC_FIRST;  // the state is zero
@@ -1175,8 +1245,14 @@ void launch_lag_records(const Grid &g, const FluidLists &L, const uint8_t *pc, h
         hipLaunchKernelGGL(k_lag_records, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, pc,
                            L.rec);
         if (L.compact_jac)
-            hipLaunchKernelGGL(k_compact_jac_flags, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.rec,
-                               L.n_all, L.n_centries, L.clink);
+        {
+            if (L.links_aos)
+                hipLaunchKernelGGL(k_compact_jac_flags<true>, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.rec,
+                                   L.n_all, L.n_centries, L.clink);
+            else
+                hipLaunchKernelGGL(k_compact_jac_flags<false>, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g,
+                                   L.rec, L.n_all, L.n_centries, L.clink);
+        }
     }
 }
 

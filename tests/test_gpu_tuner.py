@@ -87,7 +87,7 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
         if form is None:
             # FP32+DC: the layout (compact or dense) timed first, then the
             # forms of that layout: the compact pass with every load and with
-            # each load only where used (also at 6 and 8 waves per SIMD), or
+            # each load only where used (also at 6 waves per SIMD), or
             # the dense interleaved forms; FP64: pair, lagged list, records
             # and the dense 2.5-D forms
             forms_tried = {k: v for k, v in tried.items() if not k.startswith("layout-")}
@@ -96,7 +96,7 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
                 assert set(tried) >= {"layout-compact", "layout-dense"}
                 compact = tried["layout-compact"] <= tried["layout-dense"]
                 assert name.startswith("interleaved-compact") == compact
-                assert len(forms_tried) == (4 if compact else 9)
+                assert len(forms_tried) == (3 if compact else 9)
             else:
                 assert len(forms_tried) == 7
 
