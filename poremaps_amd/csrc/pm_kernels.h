@@ -202,6 +202,10 @@ void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, bool sp
 // state's halo copies and outlet pins (and the outlet buffer's periodic
 // copies), else launch_ghost4 (natural or colour-split rows).
 void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, float4 *S, float *q, hipStream_t s);
+// The compact states' +-2 links folded into the slots of the solid sides
+// (cases 4 and 5), from L0 into L1 (both of L.n_centries entries), after the
+// records' flags are in the links.
+void launch_link_subst(const FluidLists &L, const uint2 *L0, uint2 *L1, hipStream_t s);
 // Whether the compact SOR state can hold this medium: no slip walls, and
 // every entry's byte offset below 2^32 (its masked loads are buffer loads).
 bool compact_supported(const Grid &g, int64_t n_entries);

@@ -744,6 +744,15 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
     HIPCHK(hipMemsetAsync(c->code, 0, sizeof(int32_t) * (size_t)g.total, c->stream));
     pm::launch_eval_geometry(g, c->G, c->code, c->pc, c->stream);
     pm::launch_lag_records(g, c->lists, c->pc, c->stream);
+    if (c->lists.compact || c->lists.compact_jac) {
+        // the +-2 links into the solid sides' slots (out of place)
+        uint2 *L1 = nullptr;
+        RC(dalloc(&L1, 3 * (int64_t)c->lists.n_centries));
+        pm::launch_link_subst(c->lists, c->lists.clink, L1, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->lists.clink);
+        c->lists.clink = L1;
+    }
     RC(init_state(c));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -183,14 +183,12 @@ __device__ __forceinline__ void colour_voxel4_compact(int t, const uint2 *__rest
     s.XM = S[xm]; s.XP = S[xp];
     s.YM = S[ym]; s.YP = S[yp];
     s.ZM = S[zm]; s.ZP = S[zp];
-    const bool nxm = c2.cx == 5, nxp = c2.cx == 4, nym = c2.cy == 5, nyp = c2.cy == 4;
-    const bool nzm = c2.cz == 5, nzp = c2.cz == 4;
-    const int32_t xm2 = buf_load_link(rsL, xm, 0, nE, nxm), xp2 = buf_load_link(rsL, xp, 1, nE, nxp);
-    const int32_t ym2 = buf_load_link(rsL, ym, 2, nE, nym), yp2 = buf_load_link(rsL, yp, 3, nE, nyp);
-    const int32_t zm2 = buf_load_link(rsL, zm, 4, nE, nzm), zp2 = buf_load_link(rsL, zp, 5, nE, nzp);
-    s.XM2 = buf_load4(rsS, xm2, nxm); s.XP2 = buf_load4(rsS, xp2, nxp);
-    s.YM2 = buf_load4(rsS, ym2, nym); s.YP2 = buf_load4(rsS, yp2, nyp);
-    s.ZM2 = buf_load4(rsS, zm2, nzm); s.ZP2 = buf_load4(rsS, zp2, nzp);
+    // the +-2 neighbour comes through the slot of the solid side (case 4:
+    // the -1 slot holds the +2 link, case 5: the +1 slot the -2 link)
+    s.XP2 = s.XM; s.XM2 = s.XP;
+    s.YP2 = s.YM; s.YM2 = s.YP;
+    s.ZP2 = s.ZM; s.ZM2 = s.ZP;
+    (void)rsL; (void)rsS;
     const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
     const float4 Rm = make_float4(r.x, r.y, r.z, 0.f);
     const float rp = r.w;
@@ -300,22 +298,20 @@ __device__ __forceinline__ Star4 star_compact(const JLinks &l, const Cases &cc, 
                                               __amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS, int nE) {
     Star4 s;
     if (ALL1) {
-        s.XM = buf_load4(rsS, l.xm, !cc.bx); s.XP = buf_load4(rsS, l.xp, !fwd_solid(cc.cx));
-        s.YM = buf_load4(rsS, l.ym, !cc.by); s.YP = buf_load4(rsS, l.yp, !fwd_solid(cc.cy));
-        s.ZM = buf_load4(rsS, l.zm, !cc.bz); s.ZP = buf_load4(rsS, l.zp, !fwd_solid(cc.cz));
+        // a slot is read where its side is open or where it holds the +-2 link
+        s.XM = buf_load4(rsS, l.xm, !cc.bx || cc.cx == 4); s.XP = buf_load4(rsS, l.xp, !fwd_solid(cc.cx) || cc.cx == 5);
+        s.YM = buf_load4(rsS, l.ym, !cc.by || cc.cy == 4); s.YP = buf_load4(rsS, l.yp, !fwd_solid(cc.cy) || cc.cy == 5);
+        s.ZM = buf_load4(rsS, l.zm, !cc.bz || cc.cz == 4); s.ZP = buf_load4(rsS, l.zp, !fwd_solid(cc.cz) || cc.cz == 5);
     } else {
         s.XM = S[l.xm]; s.XP = S[l.xp];
         s.YM = S[l.ym]; s.YP = S[l.yp];
         s.ZM = S[l.zm]; s.ZP = S[l.zp];
     }
-    const bool nxm = cc.cx == 5, nxp = cc.cx == 4, nym = cc.cy == 5, nyp = cc.cy == 4;
-    const bool nzm = cc.cz == 5, nzp = cc.cz == 4;
-    const int32_t xm2 = buf_load_lw<AOS>(rsL, l.xm, 0, nE, nxm), xp2 = buf_load_lw<AOS>(rsL, l.xp, 1, nE, nxp);
-    const int32_t ym2 = buf_load_lw<AOS>(rsL, l.ym, 2, nE, nym), yp2 = buf_load_lw<AOS>(rsL, l.yp, 3, nE, nyp);
-    const int32_t zm2 = buf_load_lw<AOS>(rsL, l.zm, 4, nE, nzm), zp2 = buf_load_lw<AOS>(rsL, l.zp, 5, nE, nzp);
-    s.XM2 = buf_load4(rsS, xm2, nxm); s.XP2 = buf_load4(rsS, xp2, nxp);
-    s.YM2 = buf_load4(rsS, ym2, nym); s.YP2 = buf_load4(rsS, yp2, nyp);
-    s.ZM2 = buf_load4(rsS, zm2, nzm); s.ZP2 = buf_load4(rsS, zp2, nzp);
+    // the +-2 neighbour comes through the slot of the solid side (case 4:
+    // the -1 slot holds the +2 link, case 5: the +1 slot the -2 link)
+    s.XP2 = s.XM; s.XM2 = s.XP;
+    s.YP2 = s.YM; s.YM2 = s.YP;
+    s.ZP2 = s.ZM; s.ZM2 = s.ZP;
     return s;
 }
 
@@ -527,6 +523,50 @@ __global__ __launch_bounds__(kListBlock) void k_compact_jac_flags(Grid g, const 
         uint32_t &w = W[link_word<AOS>(t, q, nE)];
         w = (w & kLinkMask) | ((uint32_t)((f >> (6 * q)) & 63u) << kLinkBits);
     }
+}
+
+// The +-2 links folded into the +-1 slots (both compact states).  A +-2
+// neighbour is read only by the one-sided cases 4 (+2: the -1 neighbour is
+// solid) and 5 (-2: the +1 neighbour is solid), where the slot of the solid
+// side links to the zero entry and its value is never used: that slot takes
+// the +-2 link instead (the +1 neighbour's own +1 link, the -1 neighbour's
+// own -1 link), so a pass reads the +-2 neighbour with the load it makes for
+// that slot anyway and needs no second hop.  Out of place (L0 read, L1
+// written), interior entries t < n; the flags in the top bits are kept.
+template <bool AOS>
+__global__ __launch_bounds__(kListBlock) void k_link_subst(const uint2 *__restrict__ L0, uint2 *__restrict__ L1, int n,
+                                                           int nE) {
+    const int64_t t = (int64_t)blockIdx.x * kListBlock + threadIdx.x;
+    const uint32_t *W0 = reinterpret_cast<const uint32_t *>(L0);
+    uint32_t *W1 = reinterpret_cast<uint32_t *>(L1);
+    if (t >= nE) return;
+    uint32_t w[6];
+    for (int q = 0; q < 6; q++) w[q] = W0[link_word<AOS>((int32_t)t, q, nE)];
+    if (t < n) {
+        const uint32_t code = (w[0] >> kLinkBits) | (((w[1] >> kLinkBits) & 3u) << 6);
+        const Cases cs = decode((uint8_t)code);
+        const int cc[3] = {cs.cx, cs.cy, cs.cz};
+        for (int a = 0; a < 3; a++) {
+            const int m = 2 * a, pl = 2 * a + 1;
+            if (cc[a] == 4) {
+                const int32_t e1 = (int32_t)(w[pl] & kLinkMask);
+                const uint32_t e2 = W0[link_word<AOS>(e1, pl, nE)] & kLinkMask;
+                w[m] = (w[m] & ~kLinkMask) | e2;
+            } else if (cc[a] == 5) {
+                const int32_t e1 = (int32_t)(w[m] & kLinkMask);
+                const uint32_t e2 = W0[link_word<AOS>(e1, m, nE)] & kLinkMask;
+                w[pl] = (w[pl] & ~kLinkMask) | e2;
+            }
+        }
+    }
+    for (int q = 0; q < 6; q++) W1[link_word<AOS>((int32_t)t, q, nE)] = w[q];
+}
+
+void launch_link_subst(const FluidLists &L, const uint2 *L0, uint2 *L1, hipStream_t s) {
+    const int nE = L.n_centries;
+    const dim3 grid((unsigned)((nE + kListBlock - 1) / kListBlock)), blk(kListBlock);
+    if (L.links_aos) hipLaunchKernelGGL(k_link_subst<true>, grid, blk, 0, s, L0, L1, L.n_all, nE);
+    else hipLaunchKernelGGL(k_link_subst<false>, grid, blk, 0, s, L0, L1, L.n_all, nE);
 }
 
 // The compact state's ghost fill (ghost_line4 on the entries): halo entries
