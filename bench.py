@@ -9,7 +9,10 @@
 A step is one it_eval interval of the solver (100 sweeps of the bulk sweep,
 with the defect-correction fold and residual of that interval) over the whole
 synthetic field, fields resident in HBM.  value = GLUP/s = nx*ny*nz*sweeps/s
-(dense count); fluid_glups counts the fluid voxels only.
+(dense count); fluid_glups counts the fluid voxels only.  The default solver
+is the FP32+FP64-refined 3-colour SOR (relaxation 1.8 / 6.0): the red-black
+SOR of the target in a race-free colouring, and the fastest refined solver to
+eps on the default field (time_to_eps reports the refined Jacobi beside it).
 
 Workloads (synthetic, generated from a seed):
   config 0: the reference's own CPU case, a tube 54x54x50 (r 20, bc 0, dx 2e-5);
@@ -437,13 +440,16 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--mode", default="jacobi")
+    ap.add_argument("--mode", default="sor3",
+                    help="the FP32+DC 3-colour SOR by default (the fastest refined solver to eps; jacobi, "
+                         "reference)")
     ap.add_argument("--precision", default="f32dc")
     ap.add_argument("--omega", type=float, default=None,
-                    help="velocity relaxation of the fast modes (default 2.5 for one field; batch and serving: "
-                         "0 = the pair chosen per geometry from its porosity)")
+                    help="velocity relaxation of the fast modes (default for one field: 1.8 SOR, 2.5 Jacobi; "
+                         "batch and serving: 0 = the pair chosen per geometry from its porosity)")
     ap.add_argument("--omega-p", type=float, default=None,
-                    help="pressure relaxation of the fast modes (default 7.0 for one field; 0 with omega 0)")
+                    help="pressure relaxation of the fast modes (default for one field: 6.0 SOR, 7.0 Jacobi; "
+                         "0 with omega 0)")
     ap.add_argument("--it-eval", type=int, default=100)
     ap.add_argument("--tte", action="store_true", help="time-to-eps of the selected mode beside the plain solver")
     ap.add_argument("--no-tte", action="store_true", help="skip the default time-to-eps set")
@@ -468,9 +474,9 @@ def main():
               f"multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
         sys.exit(2)
     if a.omega is None:
-        a.omega = 0.0 if a.config in (3, 4) else 2.5
+        a.omega = 0.0 if a.config in (3, 4) else 1.8 if a.mode == "sor3" else 2.5
     if a.omega_p is None:
-        a.omega_p = 0.0 if a.omega == 0.0 else 7.0
+        a.omega_p = 0.0 if a.omega == 0.0 else 6.0 if a.mode == "sor3" else 7.0
     if a.config == 3:
         a.count = a.count or 64
         return bench_batch(a)
@@ -532,6 +538,13 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic[0] if traffic else None,
                      "traffic_source": traffic[1] if traffic else None,
+                     # the measured HBM bytes per launch over this run's
+                     # launch time: the fraction of the peak the kernel really
+                     # moves (the dense algorithmic count above charges every
+                     # cell of the domain, while the passes touch the fluid
+                     # cells and their neighbours only, so it can exceed 1)
+                     "traffic_frac": (round(traffic[0] / (sw["avg_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+                                      if traffic else None),
                      "kernel": group, "avg_ms": round(sw["avg_ms"], 4),
                      "bytes_per_launch": algo, "bytes_per_voxel": 8 * S + 1,
                      "achieved_with_residual_GBps": round(sw["bytes"] / (sw["avg_ms"] * 1e-3) / 1e9, 1)},
