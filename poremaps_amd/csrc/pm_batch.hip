@@ -10,6 +10,7 @@
 // replayed while the set of running samples is unchanged; finished samples
 // drop out of the tables (eviction) and new ones join between intervals.
 #include "../../include/poremaps_amd_batch.h"
+#include "pm_compact.h"
 #include "pm_ctx.h"
 #include "pm_reduce.h"
 #include "pm_voxel.h"
@@ -52,6 +53,12 @@ struct BSample {
     // the lagged pass reads its +-2 stencil neighbours only where the case
     // uses them (rec_skip2_supported)
     int skip2;
+    // the correction state is the compact one (pm_compact.h; FluidLists::
+    // compact_jac): entry t = list position t, links clink of nE entries,
+    // outlet buffers and R4.pc by entry, no halo copies
+    int compact;
+    const uint2 *clink;
+    int nE;
 };
 
 struct BChunk {
@@ -106,24 +113,35 @@ __global__ __launch_bounds__(kB) void k_b_first0(const BSample *__restrict__ S, 
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    first_voxel4_zero(s.g, t, s.rec, parity ? s.A4 : s.B4, s.R4, (float)s.cu);
+    if (s.compact) first_zero_compact<false>(t, s.clink, s.nE, parity ? s.A4 : s.B4, s.R4, (float)s.cu);
+    else first_voxel4_zero(s.g, t, s.rec, parity ? s.A4 : s.B4, s.R4, (float)s.cu);
 }
 
 // Lagged single pass (the sweeps after the first of an interval; from the
 // per-voxel records, every load issued up front) and the settle pass that
 // completes the pressure at the interval's end, in place.
+// One lagged pass at list position t of sample s: the compact pass (each load
+// only where used) or the dense record pass.
+__device__ __forceinline__ void b_lag_cell(const BSample &s, int t, int parity) {
+    if (s.compact) {
+        const float4 *S0 = parity ? s.B4 : s.A4;
+        lag_voxel4_compact<false, false>(t, s.clink, s.nE, links_rsrc(s.clink, s.nE), state_rsrc(S0, s.nE), S0,
+                                         parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+    } else if (s.skip2)
+        lag_voxel4_rec<true, true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
+                                   parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+    else
+        lag_voxel4_rec<true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
+                             parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+}
+
 __global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, const BChunk *__restrict__ C,
                                                int parity) {
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    if (s.skip2)
-        lag_voxel4_rec<true, true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-                                   parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
-    else
-        lag_voxel4_rec<true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-                             parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+    b_lag_cell(s, t, parity);
 }
 
 // The same two passes as persistent kernels over tile queues: the samples'
@@ -167,7 +185,9 @@ __global__ __launch_bounds__(kB) void k_b_first0_q(const BSample *__restrict__ S
         const BChunk ch = C[b];
         const BSample &s = S[ch.s];
         const int t = ch.start + threadIdx.x;
-        if (t < s.n) first_voxel4_zero(s.g, t, s.rec, parity ? s.A4 : s.B4, s.R4, (float)s.cu);
+        if (t >= s.n) return;
+        if (s.compact) first_zero_compact<false>(t, s.clink, s.nE, parity ? s.A4 : s.B4, s.R4, (float)s.cu);
+        else first_voxel4_zero(s.g, t, s.rec, parity ? s.A4 : s.B4, s.R4, (float)s.cu);
     });
 }
 
@@ -178,12 +198,7 @@ __global__ __launch_bounds__(kB) void k_b_lag4_q(const BSample *__restrict__ S, 
         const BSample &s = S[ch.s];
         const int t = ch.start + threadIdx.x;
         if (t >= s.n) return;
-        if (s.skip2)
-            lag_voxel4_rec<true, true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-                                       parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
-        else
-            lag_voxel4_rec<true>(s.g, t, s.rec, parity ? s.B4 : s.A4, (const float *)(parity ? s.qB : s.qA),
-                                 parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
+        b_lag_cell(s, t, parity);
     });
 }
 
@@ -193,8 +208,12 @@ __global__ __launch_bounds__(kB) void k_b_settle4(const BSample *__restrict__ S,
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    const int64_t c = s.list[t];
     float4 *set = which ? s.B4 : s.A4;
+    if (s.compact) {
+        settle_compact<false>(t, s.clink, s.nE, set, (float *)(which ? s.qB : s.qA), s.R4, (float)s.cp);
+        return;
+    }
+    const int64_t c = s.list[t];
     pres_voxel4(s.g, c, s.pc[c], set, set, (float *)(which ? s.qB : s.qA), s.R4, (float)s.cp, 0.f);
 }
 
@@ -203,7 +222,8 @@ __global__ __launch_bounds__(kB) void k_b_ghost4(const BSample *__restrict__ S, 
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int64_t t = (int64_t)ch.start + threadIdx.x;
-    if (t >= ghost_lines(s.g)) return;
+    // the compact state has no halo copies; its settle pass pins the outlet
+    if (s.compact || t >= ghost_lines(s.g)) return;
     ghost_line4(s.g, t, s.G, which ? s.B4 : s.A4, (float *)(which ? s.qB : s.qA));
 }
 
@@ -213,7 +233,8 @@ __global__ __launch_bounds__(kB) void k_b_fold4(const BSample *__restrict__ S, c
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
-    fold_voxel4(s.list[t], s.X[0], s.X[1], s.X[2], s.X[3], which ? s.B4 : s.A4);
+    if (s.compact) fold_compact(t, s.list[t], s.X[0], s.X[1], s.X[2], s.X[3], which ? s.B4 : s.A4);
+    else fold_voxel4(s.list[t], s.X[0], s.X[1], s.X[2], s.X[3], which ? s.B4 : s.A4);
 }
 
 __global__ __launch_bounds__(kB) void k_b_residual4(const BSample *__restrict__ S, const BChunk *__restrict__ C) {
@@ -232,6 +253,11 @@ __global__ __launch_bounds__(kB) void k_b_gather_rq(const BSample *__restrict__ 
     const int t = ch.start + threadIdx.x;
     if (t >= s.n) return;
     const int64_t c = s.list[t];
+    if (s.compact) {
+        // the compact pass reads the continuity residual per entry
+        s.R4.pc[t] = s.R4.p[c];
+        return;
+    }
     int i, j, k;
     cell_ijk(s.g, c, i, j, k);
     int64_t nxn, nyn, nzn;
@@ -273,7 +299,9 @@ __global__ __launch_bounds__(kB) void k_b_zero_q(const BSample *__restrict__ S, 
     const BChunk ch = C[blockIdx.x];
     const BSample &s = S[ch.s];
     const int t = ch.start + threadIdx.x;
-    if (t >= s.g.NY * s.g.NX) return;
+    // the compact state's outlet buffers (by entry) are written by its settle
+    // pass and never read by the batch's passes
+    if (s.compact || t >= s.g.NY * s.g.NX) return;
     const int64_t e = s.g.qat(t / s.g.NX, t % s.g.NX);
     ((float *)s.qA)[e] = 0.f;
     ((float *)s.qB)[e] = 0.f;
@@ -474,7 +502,7 @@ extern "C" int64_t pm_batch_sample_bytes(const pm_batch_params *p, int nx, int n
 // kernels, so the context's autotuner is off and the sweep uses the pair.
 static int prologue(pm_ctx *c) {
     c->tuner = Tuner();
-    c->jac_impl = JI_PAIR;
+    c->jac_impl = c->lists.compact_jac ? JI_COMPACT : JI_PAIR;
     int rc = pm_one_sweep(c);
     if (rc == PM_OK) rc = pm_settle(c);
     if (rc == PM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = pm_fail(PM_E_HIP, "first sweep failed");
@@ -494,9 +522,8 @@ extern "C" int pm_batch_add(pm_batch *b, const uint8_t *raw, int nx, int ny, int
     pm_ctx *c = nullptr;
     int rc = pm_ctx_create(&q, &c);
     if (rc != PM_OK) return rc;
-    // the batch's kernels read the dense layout
-    c->jac_compact = false;
-    c->sor_compact = false;
+    // each sample's layout is timed at load like a solo solve's
+    // (choose_jacobi_layout); the batch's kernels read either
     rc = pm_load_geometry(c, raw);
     if (rc == PM_OK) rc = prologue(c);
     if (rc != PM_OK) {
@@ -573,6 +600,9 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
         d.qB = b->dc() ? (void *)c->qD[1] : (void *)c->qX[1];
         d.qX = c->qX[0];
         d.skip2 = b->skip2 && pm::rec_skip2_supported(d.g) ? 1 : 0;
+        d.compact = c->lists.compact_jac ? 1 : 0;
+        d.clink = c->lists.clink;
+        d.nE = c->lists.n_centries;
         const int s = (int)S.size();
         d.chunk_begin = (int)C.size();
         for (int t = 0; t < d.n; t += pm::kB) C.push_back({s, t});

@@ -381,9 +381,7 @@ int pm_restart_slower(pm_ctx *c) {
 // records' part of the interior entries is filled on the device from the
 // colour passes' records (launch_lag_records).  Falls back to the
 // colour-split rows where the medium does not fit the compact form.
-// torus (FP32+DC Jacobi): no halo entries; a link across a periodic face goes
-// to the image cell itself.
-static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_t> &all, bool torus) {
+static int build_compact_sor(pm_ctx *c, const uint8_t *raw, const std::vector<int32_t> &all) {
     const pm::Grid &g = c->g;
     const int64_t n_all = (int64_t)all.size();
     std::vector<int32_t> cidx((size_t)g.total, -1);
@@ -405,19 +403,19 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
         next++;
     };
     const int iout = g.nz + 1;
-    if (g.px && !torus)
+    if (g.px)
         for (int i = 2; i < g.nz + 2; i++)
             for (int j = 2; j < g.ny + 2; j++) {
                 const int src[4] = {g.nx, g.nx + 1, 2, 3}, dst[4] = {0, 1, g.nx + 2, g.nx + 3};
                 for (int e = 0; e < 4; e++) add(i, j, dst[e], i, j, src[e], i == iout, i == iout);
             }
-    if (g.py && !torus)
+    if (g.py)
         for (int i = 2; i < g.nz + 2; i++)
             for (int k = 2; k < g.nx + 2; k++) {
                 const int src[4] = {g.ny, g.ny + 1, 2, 3}, dst[4] = {0, 1, g.ny + 2, g.ny + 3};
                 for (int e = 0; e < 4; e++) add(i, dst[e], k, i, src[e], k, i == iout, i == iout);
             }
-    if (g.pz && !torus)
+    if (g.pz)
         for (int j = 2; j < g.ny + 2; j++)
             for (int k = 2; k < g.nx + 2; k++) {
                 // the copies of the outlet plane and the pinned planes 1 and
@@ -428,45 +426,26 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
             }
     const int64_t zero = next, n_ent = next + 1;
     if (!pm::compact_supported(g, n_ent)) return PM_OK;
-    // links: (-x, +x), (-y, +y), (-z, +z) arrays of n_ent entries, or (the
-    // compact Jacobi with POREMAPS_JAC_LINKS=aos) one record of six words per
-    // entry; 24 B per entry either way
-    const bool aos = torus && c->jac_links_aos;
+    // links: (-x, +x), (-y, +y), (-z, +z) arrays of n_ent entries (24 B per entry)
     std::vector<uint2> lk((size_t)(3 * n_ent));
     std::vector<int32_t> outl;
     auto link = [&](int64_t off, int64_t e) {
         const int64_t r = off - g.base;
         const int i = (int)(r / g.sy), j = (int)((r % g.sy) / g.sx), k = (int)(r % g.sx);
         auto at = [&](int ii, int jj, int kk) -> uint32_t {
-            if (torus) {
-                // one step beyond a periodic face: the image (interior) cell
-                if (g.px) kk = kk < 2 ? kk + g.nx : kk >= g.nx + 2 ? kk - g.nx : kk;
-                if (g.py) jj = jj < 2 ? jj + g.ny : jj >= g.ny + 2 ? jj - g.ny : jj;
-                if (g.pz) ii = ii < 2 ? ii + g.nz : ii >= g.nz + 2 ? ii - g.nz : ii;
-            }
             if (ii < 0 || ii >= g.NZ || jj < 0 || jj >= g.NY || kk < 0 || kk >= g.NX) return (uint32_t)zero;
             const int32_t m = cidx[(size_t)g.at(ii, jj, kk)];
             return (uint32_t)(m < 0 ? zero : m);
         };
-        if (aos) {
-            // one record per entry: +x, +y, +z, -x, -y, -z
-            uint32_t *w = reinterpret_cast<uint32_t *>(lk.data()) + (size_t)e * 6;
-            w[0] = at(i, j, k + 1); w[1] = at(i, j + 1, k); w[2] = at(i + 1, j, k);
-            w[3] = at(i, j, k - 1); w[4] = at(i, j - 1, k); w[5] = at(i - 1, j, k);
-        } else {
-            lk[(size_t)e] = make_uint2(at(i, j, k - 1), at(i, j, k + 1));
-            lk[(size_t)(n_ent + e)] = make_uint2(at(i, j - 1, k), at(i, j + 1, k));
-            lk[(size_t)(2 * n_ent + e)] = make_uint2(at(i - 1, j, k), at(i + 1, j, k));
-        }
+        lk[(size_t)e] = make_uint2(at(i, j, k - 1), at(i, j, k + 1));
+        lk[(size_t)(n_ent + e)] = make_uint2(at(i, j - 1, k), at(i, j + 1, k));
+        lk[(size_t)(2 * n_ent + e)] = make_uint2(at(i - 1, j, k), at(i + 1, j, k));
         return i;
     };
     for (int64_t t = 0; t < n_all; t++)
         if (link(all[(size_t)t], t) == iout) outl.push_back((int32_t)t);
     for (size_t h = 0; h < hoff.size(); h++) link(hoff[h], n_all + (int64_t)h);
-    if (aos)
-        for (int q = 0; q < 6; q++) reinterpret_cast<uint32_t *>(lk.data())[(size_t)zero * 6 + q] = (uint32_t)zero;
-    else
-        for (int a = 0; a < 3; a++) lk[(size_t)(a * n_ent + zero)] = make_uint2((uint32_t)zero, (uint32_t)zero);
+    for (int a = 0; a < 3; a++) lk[(size_t)(a * n_ent + zero)] = make_uint2((uint32_t)zero, (uint32_t)zero);
     RC(dalloc(&c->lists.clink, 3 * n_ent));
     HIPCHK(hipMemcpy(c->lists.clink, lk.data(), sizeof(uint2) * lk.size(), hipMemcpyHostToDevice));
     RC(dalloc(&c->lists.chalo, std::max<int64_t>(1, (int64_t)halo.size())));
@@ -475,24 +454,52 @@ static int build_compact(pm_ctx *c, const uint8_t *raw, const std::vector<int32_
     RC(dalloc(&c->lists.cout, std::max<int64_t>(1, (int64_t)outl.size())));
     if (!outl.empty())
         HIPCHK(hipMemcpy(c->lists.cout, outl.data(), sizeof(int32_t) * outl.size(), hipMemcpyHostToDevice));
-    // the outlet buffer(s) by entry
-    if (n_ent > g.qtotal())
-        for (int q = 0; q < (torus ? 2 : 1); q++) {
-            (void)hipFree(c->qD[q]);
-            c->qD[q] = nullptr;
-            RC(dalloc(&c->qD[q], n_ent));
-        }
-    if (torus) {
-        // the neighbours' continuity residuals come from the entries'
-        // residual array through the links, not gathered per cell
-        RC(dalloc(&c->R4.pc, n_ent));
-        RC(fill_f32(c, c->R4.pc, n_ent, 0.f));
+    // the outlet buffer by entry
+    if (n_ent > g.qtotal()) {
+        (void)hipFree(c->qD[0]);
+        c->qD[0] = nullptr;
+        RC(dalloc(&c->qD[0], n_ent));
     }
     c->lists.n_centries = (int)n_ent;
     c->lists.n_chalo = (int)halo.size();
     c->lists.n_cout = (int)outl.size();
-    if (torus) c->lists.compact_jac = true;
-    else c->lists.compact = true;
+    c->lists.compact = true;
+    c->lists.links_aos = false;
+    return PM_OK;
+}
+
+// The compact FP32+DC Jacobi state (FluidLists::compact_jac): entry t is list
+// position t, then the zero entry; no halo entries, a link across a periodic
+// face goes to the image cell itself.  Links built on the device from the
+// uploaded fluid list (launch_torus_links), in three arrays or
+// (POREMAPS_JAC_LINKS=aos) one record per entry; the records' flags follow
+// in launch_lag_records.
+static int build_compact_jacobi(pm_ctx *c) {
+    const pm::Grid &g = c->g;
+    const int64_t n_all = c->lists.n_all, n_ent = n_all + 1;
+    if (!pm::compact_supported(g, n_ent)) return PM_OK;
+    const bool aos = c->jac_links_aos;
+    RC(dalloc(&c->lists.clink, 3 * n_ent));
+    int32_t *cidx = nullptr;
+    RC(dalloc(&cidx, g.total));
+    pm::launch_torus_links(g, c->lists.all, (int)n_all, cidx, c->lists.clink, (int)n_ent, aos, c->stream);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(cidx);
+    if (e != hipSuccess) return fail(PM_E_HIP, "compact links: %s", hipGetErrorString(e));
+    HIPCHK(hipGetLastError());
+    // the outlet buffers by entry
+    if (n_ent > g.qtotal())
+        for (int q = 0; q < 2; q++) {
+            (void)hipFree(c->qD[q]);
+            c->qD[q] = nullptr;
+            RC(dalloc(&c->qD[q], n_ent));
+        }
+    // the neighbours' continuity residuals come from the entries' residual
+    // array through the links, not gathered per cell
+    RC(dalloc(&c->R4.pc, n_ent));
+    RC(fill_f32(c, c->R4.pc, n_ent, 0.f));
+    c->lists.n_centries = (int)n_ent;
+    c->lists.compact_jac = true;
     c->lists.links_aos = aos;
     return PM_OK;
 }
@@ -717,9 +724,9 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         if (sor && c->dc()) {
             c->lists.col_in_all = true;
             for (int q = 0; q < 3; q++) c->lists.col[q] = c->lists.all + c->lists.off_col[q];
-            if (c->lists.split && c->sor_compact && c->lists.sor_chunk == 0) RC(build_compact(c, raw, all, false));
+            if (c->lists.split && c->sor_compact && c->lists.sor_chunk == 0) RC(build_compact_sor(c, raw, all));
         } else if (c->dc() && c->jacobi() && c->jac_compact) {
-            RC(build_compact(c, raw, all, true));
+            RC(build_compact_jacobi(c));
             if (c->lists.compact_jac) c->jac_impl = JI_COMPACT;
         } else if (sor)
             for (int q = 0; q < 3; q++) {
