@@ -157,6 +157,9 @@ struct FluidLists {
     int n_chalo = 0;
     int32_t *cout = nullptr;
     int n_cout = 0;
+    // compact SOR: the fluid list in column order (natural offsets) and each
+    // cell's position in the colour lists, for the residual and the fold
+    int32_t *nat = nullptr, *natpos = nullptr;
 };
 
 // FP64 fast modes (per-field state p, u, v, w): the velocity + pressure

@@ -160,7 +160,7 @@ static void free_all(pm_ctx *c) {
     f(c->lists.all);
     f(c->lists.rec);
     f(c->lists.rowstart);
-    f(c->lists.clink); f(c->lists.chalo); f(c->lists.cout);
+    f(c->lists.clink); f(c->lists.chalo); f(c->lists.cout); f(c->lists.nat); f(c->lists.natpos);
     for (int q = 0; q < 4; q++) { f(c->ref_snap[q]); c->ref_snap[q] = nullptr; }
     f(c->ref_outlet); f(c->ref_tab);
     c->ref_outlet = nullptr; c->ref_tab = nullptr; c->ref_nblk = 0;
@@ -389,18 +389,18 @@ static int build_compact_sor(pm_ctx *c, const uint8_t *raw, const std::vector<in
     auto fluid_at = [&](int i, int j, int k) {  // halo coordinates of an interior cell
         return raw[((int64_t)(i - 2) * g.ny + (j - 2)) * g.nx + (k - 2)] == 0;
     };
-    std::vector<int64_t> hoff;   // natural offsets of the halo entries
-    std::vector<int2> halo;      // (entry, image entry | bit 31: p = 0)
-    int64_t next = n_all;
-    // zero_p: the copy's p is 0; with_q: its outlet buffer value is copied
+    // the halo cells that get entries: (image entry, natural offset, zero_p,
+    // with_q); zero_p: the copy's p is 0; with_q: its outlet buffer value is
+    // copied too
+    struct HaloCell {
+        int32_t img;
+        int64_t off;
+        bool zero_p, with_q;
+    };
+    std::vector<HaloCell> cand;
     auto add = [&](int i, int j, int k, int si, int sj, int sk, bool zero_p, bool with_q) {
         if (!fluid_at(si, sj, sk)) return;
-        const int32_t e = cidx[(size_t)g.at(si, sj, sk)];
-        cidx[(size_t)g.at(i, j, k)] = (int32_t)next;
-        hoff.push_back(g.at(i, j, k));
-        halo.push_back(make_int2(with_q ? (int)((uint32_t)next | 0x80000000u) : (int)next,
-                                 zero_p ? (int)((uint32_t)e | 0x80000000u) : (int)e));
-        next++;
+        cand.push_back({cidx[(size_t)g.at(si, sj, sk)], g.at(i, j, k), zero_p, with_q});
     };
     const int iout = g.nz + 1;
     if (g.px)
@@ -424,6 +424,19 @@ static int build_compact_sor(pm_ctx *c, const uint8_t *raw, const std::vector<in
                 for (int e = 0; e < 4; e++)
                     add(dst[e], j, k, src[e], j, k, dst[e] == 1 || dst[e] == g.nz + 2, false);
             }
+    // halo entries in the order of their images, so that the ghost fill reads
+    // and writes both in ascending order
+    std::stable_sort(cand.begin(), cand.end(), [](const HaloCell &a, const HaloCell &b) { return a.img < b.img; });
+    std::vector<int64_t> hoff;   // natural offsets of the halo entries
+    std::vector<int2> halo;      // (entry | bit 31: with_q, image entry | bit 31: p = 0)
+    int64_t next = n_all;
+    for (const HaloCell &h : cand) {
+        cidx[(size_t)h.off] = (int32_t)next;
+        hoff.push_back(h.off);
+        halo.push_back(make_int2(h.with_q ? (int)((uint32_t)next | 0x80000000u) : (int)next,
+                                 h.zero_p ? (int)((uint32_t)h.img | 0x80000000u) : (int)h.img));
+        next++;
+    }
     const int64_t zero = next, n_ent = next + 1;
     if (!pm::compact_supported(g, n_ent)) return PM_OK;
     // links: (-x, +x), (-y, +y), (-z, +z) arrays of n_ent entries (24 B per entry)
@@ -641,6 +654,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         const int tx = c->prm_tile_x > 0 ? std::min(c->prm_tile_x, g.nx) : g.nx;
         std::vector<int32_t> all;
         std::vector<int32_t> col[3];
+        std::vector<int32_t> nat, natpos;
         // FP32+DC Jacobi with columns 64 cells wide: the list position of
         // every row segment's first fluid cell (the z-march form's index)
         // (one column as wide as the domain is the same for nx <= 64)
@@ -688,6 +702,20 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
             // POREMAPS_SOR=natural.
             const char *so = std::getenv("POREMAPS_SOR");
             c->lists.split = !(so && std::strcmp(so, "natural") == 0);
+            // the list in column order and each cell's position in the
+            // colour lists: the compact state's residual and fold walk it
+            nat = all;
+            natpos.resize(all.size());
+            {
+                int64_t cnt[3] = {0, 0, 0};
+                const int64_t off[3] = {0, (int64_t)col[0].size(), (int64_t)(col[0].size() + col[1].size())};
+                for (size_t u = 0; u < all.size(); u++) {
+                    const int64_t r = all[u] - g.base;
+                    const int i = (int)(r / g.sy), j = (int)((r % g.sy) / g.sx), k = (int)(r % g.sx);
+                    const int q = (i + j + k + 6) % 3;
+                    natpos[u] = (int32_t)(off[q] + cnt[q]++);
+                }
+            }
             all.clear();
             for (int q = 0; q < 3; q++) {
                 c->lists.off_col[q] = (int)all.size();
@@ -725,6 +753,13 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
             c->lists.col_in_all = true;
             for (int q = 0; q < 3; q++) c->lists.col[q] = c->lists.all + c->lists.off_col[q];
             if (c->lists.split && c->sor_compact && c->lists.sor_chunk == 0) RC(build_compact_sor(c, raw, all));
+            if (c->lists.compact) {
+                RC(dalloc(&c->lists.nat, std::max<int64_t>(1, (int64_t)nat.size())));
+                RC(dalloc(&c->lists.natpos, std::max<int64_t>(1, (int64_t)natpos.size())));
+                HIPCHK(hipMemcpy(c->lists.nat, nat.data(), sizeof(int32_t) * nat.size(), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(c->lists.natpos, natpos.data(), sizeof(int32_t) * natpos.size(),
+                                 hipMemcpyHostToDevice));
+            }
         } else if (c->dc() && c->jacobi() && c->jac_compact) {
             RC(build_compact_jacobi(c));
             if (c->lists.compact_jac) c->jac_impl = JI_COMPACT;

@@ -1142,9 +1142,40 @@ void launch_gather_rq(const Grid &g, const FluidLists &L, Res4 R, hipStream_t s)
     hipLaunchKernelGGL(k_gather_rq, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.all, L.n_all, R);
 }
 
+// The residual and the fold of the compact SOR state walked in column order
+// (nat) rather than colour by colour: the FP64 state is read in space order,
+// the residual's momentum part and the correction written at the cells'
+// colour-list positions (natpos).
+__global__ __launch_bounds__(kListBlock) void k_residual4_nat(Grid g, const int32_t *__restrict__ nat,
+                                                              const int32_t *__restrict__ natpos, int n,
+                                                              const uint8_t *__restrict__ pc,
+                                                              const double *__restrict__ p,
+                                                              const double *__restrict__ u,
+                                                              const double *__restrict__ v,
+                                                              const double *__restrict__ w,
+                                                              const double *__restrict__ q, Res4 R) {
+    const int x = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (x >= n) return;
+    const int64_t c = nat[x];
+    residual_voxel4(g, c, natpos[x], pc[c], p, u, v, w, q, R);
+}
+
+__global__ __launch_bounds__(kListBlock) void k_fold4_nat(const int32_t *__restrict__ nat,
+                                                          const int32_t *__restrict__ natpos, int n, double *Xp,
+                                                          double *Xu, double *Xv, double *Xw, float4 *d) {
+    const int x = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (x >= n) return;
+    fold_compact(natpos[x], nat[x], Xp, Xu, Xv, Xw, d);
+}
+
 void launch_residual4(const Grid &g, const FluidLists &L, const uint8_t *pc, const double *p, const double *u,
                       const double *v, const double *w, const double *q, Res4 R, hipStream_t s) {
     if (L.n_all == 0) return;
+    if (L.compact && L.nat) {
+        hipLaunchKernelGGL(k_residual4_nat, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, g, L.nat, L.natpos,
+                           L.n_all, pc, p, u, v, w, q, R);
+        return;
+    }
     int threads = 0;
     const ListWalk W = list_walk(L, threads);
     hipLaunchKernelGGL(k_residual4, dim3(lblocks(threads)), dim3(kListBlock), 0, s, g, L.all, L.n_all, W, pc, p, u,
@@ -1171,6 +1202,12 @@ void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, f
 }
 
 void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s) {
+    if (L.compact && L.nat) {
+        if (L.n_all > 0)
+            hipLaunchKernelGGL(k_fold4_nat, dim3(lblocks(L.n_all)), dim3(kListBlock), 0, s, L.nat, L.natpos, L.n_all,
+                               Xp, Xu, Xv, Xw, d);
+        return;
+    }
     if (L.compact || L.compact_jac) {
         if (L.n_all == 0) return;
         int threads = 0;
