@@ -222,6 +222,8 @@ def measured_traffic(spec, form, precision):
                                                 r"traffic_bytes_per_sweep k_colour4_split_skip2 x3: (\d+)"),
              "interleaved-colour-split": ("sor3_f32dc_grf512_hbm_traffic.md",
                                           r"traffic_bytes_per_sweep k_colour4_split x3: (\d+)"),
+             "interleaved-colour-compact-packed": ("sor3_packed_f32dc_grf512_hbm_traffic.md",
+                                                   "traffic_bytes_per_sweep k_colour4_packed x3"),
              "interleaved-colour-compact": ("sor3_compact_f32dc_grf512_hbm_traffic.md",
                                             r"traffic_bytes_per_sweep k_colour4_compact x3: (\d+)"),
              "interleaved-compact-skipall": ("jacobi_compact_f32dc_grf512_hbm_traffic.md",
@@ -550,7 +552,9 @@ def main():
                      "achieved_with_residual_GBps": round(sw["bytes"] / (sw["avg_ms"] * 1e-3) / 1e9, 1)},
         "kernels": {k2: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"],
                          "GBps": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k2, v in kt.items()},
-        "sweep_kernel": {"form": choice, "autotuner_ms_per_sweep": tuned},
+        "sweep_kernel": {"form": choice, "autotuner_ms_per_sweep": {k: v for k, v in tuned.items()
+                                                                    if k != "packed-far"},
+                         **({"packed_far_entries": int(tuned["packed-far"])} if "packed-far" in tuned else {})},
     }
     if not (a.tte or a.accuracy or a.no_tte) and a.mode != "reference":
         line["time_to_eps"] = default_tte(g, kw)

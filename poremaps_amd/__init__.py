@@ -244,7 +244,10 @@ class Solver:
 
     def kernel_choice(self):
         """(sweep kernel form in use, {form: ms per sweep} measured by the
-        autotuner, empty until it has finished)."""
+        autotuner, empty until it has finished).  The dict also holds the
+        FP32+DC Jacobi layout timings ("layout-compact", "layout-dense", ms
+        per sweep) and, for the packed 3-colour SOR, "packed-far": the number
+        of entries whose links fit no packed record."""
         L = lib()
         L.pm_kernel_choice.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
         name, tried = C.create_string_buffer(64), C.create_string_buffer(1024)

@@ -6,6 +6,7 @@ CPU oracle in-tree.
 Outputs (git-ignored, shipped with the tree):
     poremaps_amd/libporemaps_amd.so   HIP kernels + host solver + C ABI
     poremaps_amd/bin/POREMAPS         the drop-in command line
+    poremaps_amd/bin/pm_stream_probe  HBM bandwidth at a read : write mix
     oracle/libpm_oracle.so            CPU checker (tests only)
 """
 from __future__ import annotations
@@ -21,6 +22,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libporemaps_amd.so")
 CLI = os.path.join(HERE, "bin", "POREMAPS")
+PROBE = os.path.join(HERE, "bin", "pm_stream_probe")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -74,6 +76,14 @@ def build_cli() -> str:
     return CLI
 
 
+def build_probe() -> str:
+    os.makedirs(os.path.dirname(PROBE), exist_ok=True)
+    tmp = PROBE + ".tmp"
+    _run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] + ["-o", tmp, os.path.join(CSRC, "pm_stream_probe.hip")])
+    os.replace(tmp, PROBE)
+    return PROBE
+
+
 def build_oracle() -> str:
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
     # The reference binary (CPU baseline and fixture source) is compiled from
@@ -88,6 +98,7 @@ def build_oracle() -> str:
 def build_all() -> None:
     build_library()
     build_cli()
+    build_probe()
     build_oracle()
 
 

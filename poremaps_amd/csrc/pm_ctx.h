@@ -120,6 +120,9 @@ struct pm_ctx {
     bool sor_skip2 = true;      // FluidLists::sor_skip2 (POREMAPS_SOR_SKIP2=0: off)
     bool sor_fused = true;      // FluidLists::sor_fused (POREMAPS_SOR_STAGE=seq: false)
     bool sor_compact = true;    // FluidLists::compact (POREMAPS_SOR=split / natural: off)
+    bool sor_packed = true;     // FluidLists::packed (POREMAPS_SOR_LINKS=full: off)
+    int pack_span = 4095;       // launch_pack_links' span (POREMAPS_SOR_PACK_SPAN)
+    int pack_far = 0;           // entries left on the full links
     bool jac_compact = true;    // FluidLists::compact_jac (POREMAPS_JACOBI naming a dense form: off)
     float layout_ms[2] = {0.f, 0.f};  // FP32+DC Jacobi: ms per sweep, compact / dense layout (0: not timed)
     bool jac_links_aos = false; // FluidLists::links_aos (POREMAPS_JAC_LINKS=aos)

@@ -160,6 +160,16 @@ struct FluidLists {
     // compact SOR: the fluid list in column order (natural offsets) and each
     // cell's position in the colour lists, for the residual and the fold
     int32_t *nat = nullptr, *natpos = nullptr;
+    // compact SOR, packed links (launch_pack_links): 12 B per interior entry
+    // instead of 24 -- per link 2 bits naming one of three bases of the
+    // entry's wave and slot (3: the zero entry) and a 12-bit offset, the
+    // record flags and a flag for entries whose links fit no base (read from
+    // clink); cbase: 18 bases (3 x the six slots) per wave of 64 entries of a
+    // colour, wave_off[col]: the colour's first wave
+    bool packed = false;
+    uint32_t *cpack = nullptr;
+    int32_t *cbase = nullptr;
+    int wave_off[3] = {0, 0, 0};
 };
 
 // FP64 fast modes (per-field state p, u, v, w): the velocity + pressure
@@ -213,6 +223,14 @@ void launch_torus_links(const Grid &g, const int32_t *list, int n, int32_t *cidx
 // The compact states' +-2 links folded into the slots of the solid sides
 // (cases 4 and 5), from L0 into L1 (both of L.n_centries entries), after the
 // records' flags are in the links.
+// Packed links of the compact SOR's interior entries from clink (final links
+// and flags) into L.cpack (3 n_all words) and L.cbase (18 per wave), both
+// allocated and L.wave_off set by the caller; adds the number of entries left
+// on the full links to the device counter *d_far, then decodes every record
+// and adds the entries whose links or flags differ from clink's to *d_bad.
+// span: the widest offset from a base (4095, the most the record holds;
+// smaller only in tests).
+void launch_pack_links(const FluidLists &L, int span, int *d_far, int *d_bad, hipStream_t s);
 void launch_link_subst(const FluidLists &L, const uint2 *L0, uint2 *L1, hipStream_t s);
 // Whether the compact SOR state can hold this medium: no slip walls, and
 // every entry's byte offset below 2^32 (its masked loads are buffer loads).

@@ -136,8 +136,9 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     // Jacobi and FP32+DC SOR: per-voxel records (8 B per fluid cell)
     if (fast && (jac || dc)) per += 8;
     // FP32+DC on the compact state: the links (24 B per entry) and the outlet
-    // buffer(s) by entry (4 B each; two for Jacobi)
-    if (dc) per += jac ? 32 : 28;
+    // buffer(s) by entry (4 B each; two for Jacobi); SOR: the packed links
+    // (12 B per entry, 72 B per 64)
+    if (dc) per += jac ? 32 : 28 + 14;
     // FP32+DC Jacobi: the z-march form's row-segment starts (4 B per 64 cells)
     if (dc && jac) per += 1;
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
@@ -161,6 +162,7 @@ static void free_all(pm_ctx *c) {
     f(c->lists.rec);
     f(c->lists.rowstart);
     f(c->lists.clink); f(c->lists.chalo); f(c->lists.cout); f(c->lists.nat); f(c->lists.natpos);
+    f(c->lists.cpack); f(c->lists.cbase);
     for (int q = 0; q < 4; q++) { f(c->ref_snap[q]); c->ref_snap[q] = nullptr; }
     f(c->ref_outlet); f(c->ref_tab);
     c->ref_outlet = nullptr; c->ref_tab = nullptr; c->ref_nblk = 0;
@@ -201,6 +203,8 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     if (const char *sc = std::getenv("POREMAPS_SOR_CHUNK")) c->sor_chunk = std::max(0, std::atoi(sc));
     if (const char *so = std::getenv("POREMAPS_SOR"))
         c->sor_compact = std::strcmp(so, "split") != 0 && std::strcmp(so, "natural") != 0;
+    if (const char *sl = std::getenv("POREMAPS_SOR_LINKS")) c->sor_packed = std::strcmp(sl, "full") != 0;
+    if (const char *sp = std::getenv("POREMAPS_SOR_PACK_SPAN")) c->pack_span = std::max(0, std::atoi(sp));
     // slip walls and the z-chunked passes: colour-split rows
     if (p->walls == 2 || c->sor_chunk > 0) c->sor_compact = false;
     if (p->mode == PM_MODE_SOR3 && p->precision == PM_PREC_F32DC && !c->sor_compact) c->prm_tile_x = 0;
@@ -478,6 +482,37 @@ static int build_compact_sor(pm_ctx *c, const uint8_t *raw, const std::vector<in
     c->lists.n_cout = (int)outl.size();
     c->lists.compact = true;
     c->lists.links_aos = false;
+    return PM_OK;
+}
+
+// The compact SOR's links packed to 12 B per entry (launch_pack_links), with
+// 18 bases per wave of 64 entries; the 24-byte links stay for the entries
+// whose links fit no base (none on the 512^3 Gaussian random field).
+static int pack_sor_links(pm_ctx *c) {
+    pm::FluidLists &L = c->lists;
+    int64_t waves = 0;
+    for (int col = 0; col < 3; col++) {
+        L.wave_off[col] = (int)waves;
+        waves += (L.n_col[col] + 63) / 64;
+    }
+    RC(dalloc(&L.cpack, 3 * std::max<int64_t>(1, L.n_all)));
+    RC(dalloc(&L.cbase, 18 * std::max<int64_t>(1, waves)));
+    int *d_cnt = nullptr;  // entries on the full links, records that do not decode
+    RC(dalloc(&d_cnt, 2));
+    int cnt[2] = {0, 0};
+    hipError_t e = hipMemsetAsync(d_cnt, 0, 2 * sizeof(int), c->stream);
+    if (e == hipSuccess) {
+        pm::launch_pack_links(L, c->pack_span, d_cnt, d_cnt + 1, c->stream);
+        e = hipMemcpyAsync(cnt, d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_cnt);
+    if (e != hipSuccess) return fail(PM_E_HIP, "packed links: %s", hipGetErrorString(e));
+    HIPCHK(hipGetLastError());
+    if (cnt[1] != 0) return fail(PM_E_HIP, "packed links: %d records do not decode to their links", cnt[1]);
+    const int far = cnt[0];
+    L.packed = true;
+    c->pack_far = far;
     return PM_OK;
 }
 
@@ -794,6 +829,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         HIPCHK(hipStreamSynchronize(c->stream));
         (void)hipFree(c->lists.clink);
         c->lists.clink = L1;
+        if (c->lists.compact && c->sor_packed) RC(pack_sor_links(c));
     }
     RC(init_state(c));
     HIPCHK(hipGetLastError());
@@ -1412,7 +1448,7 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
         std::snprintf(buf, sizeof buf, "interleaved-colour-split%s%s", z,
                       c->lists.sor_skip2 && pm::rec_skip2_supported(c->g) ? "-skip2" : "");
         sor = buf;
-        if (c->lists.compact) sor = "interleaved-colour-compact";
+        if (c->lists.compact) sor = c->lists.packed ? "interleaved-colour-compact-packed" : "interleaved-colour-compact";
     }
     std::snprintf(name, (size_t)len, "%s", c->jacobi() ? variant_name(c, cur, buf, sizeof buf) : sor);
     if (tried && tried_len > 0) {
@@ -1421,6 +1457,12 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
             // the FP32+DC Jacobi layout choice (choose_jacobi_layout)
             char e[96];
             std::snprintf(e, sizeof e, "layout-compact=%.4f;layout-dense=%.4f", c->layout_ms[0], c->layout_ms[1]);
+            t += e;
+        }
+        if (c->lists.packed) {
+            // the compact SOR's entries on the full links (no packed form)
+            char e[48];
+            std::snprintf(e, sizeof e, "%spacked-far=%d", t.empty() ? "" : ";", c->pack_far);
             t += e;
         }
         const Tuner &tn = c->tuner;

@@ -165,18 +165,12 @@ __device__ __forceinline__ int32_t buf_load_link(__amdgpu_buffer_rsrc_t rs, int3
     return (int32_t)((uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0) & kLinkMask);
 }
 
-__device__ __forceinline__ void colour_voxel4_compact(int t, const uint2 *__restrict__ L, int nE,
-                                                      __amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS,
-                                                      float4 *S, float *q, Res4 R, float cu, float cp,
-                                                      float pin_out) {
-    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+// The colour pass of entry t from its six links (0 -x, 1 +x, 2 -y, 3 +y, 4 -z,
+// 5 +z) and its record flags (packed code, block, outlet flag: bits 0-10).
+__device__ __forceinline__ void colour_body4(int t, const int32_t (&lk)[6], uint32_t f, float4 *S, float *q, Res4 R,
+                                             float cu, float cp, float pin_out) {
     typedef float v4f __attribute__((ext_vector_type(4)));
-    const v2u lx = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[t]));
-    const v2u ly = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[nE + t]));
-    const v2u lz = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[2 * nE + t]));
-    const uint32_t f = (lx.x >> kLinkBits) | ((lx.y >> kLinkBits) << 6);
-    const int32_t xm = (int32_t)(lx.x & kLinkMask), xp = (int32_t)(lx.y & kLinkMask);
-    const int32_t ym = (int32_t)ly.x, yp = (int32_t)ly.y, zm = (int32_t)lz.x, zp = (int32_t)lz.y;
+    const int32_t xm = lk[0], xp = lk[1], ym = lk[2], yp = lk[3], zm = lk[4], zp = lk[5];
     const bool out = (f >> 10) & 1u;
     const Cases c2 = decode((uint8_t)(f & 0xffu));
     const float4 C = S[t];
@@ -189,7 +183,6 @@ __device__ __forceinline__ void colour_voxel4_compact(int t, const uint2 *__rest
     s.XP2 = s.XM; s.XM2 = s.XP;
     s.YP2 = s.YM; s.YM2 = s.YP;
     s.ZP2 = s.ZM; s.ZM2 = s.ZP;
-    (void)rsL; (void)rsS;
     const v4f r = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&R.m[t]));
     const float4 Rm = make_float4(r.x, r.y, r.z, 0.f);
     const float rp = r.w;
@@ -215,17 +208,163 @@ __device__ __forceinline__ void colour_voxel4_compact(int t, const uint2 *__rest
     if (out) q[t] = pin_out - div0 * (float)(kC2 * kDt);
 }
 
+// The links and flags of entry t from the three link arrays.
+__device__ __forceinline__ uint32_t full_links(int t, const uint2 *__restrict__ L, int nE, int32_t (&lk)[6]) {
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const v2u lx = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[t]));
+    const v2u ly = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[nE + t]));
+    const v2u lz = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&L[2 * nE + t]));
+    lk[0] = (int32_t)(lx.x & kLinkMask); lk[1] = (int32_t)(lx.y & kLinkMask);
+    lk[2] = (int32_t)ly.x; lk[3] = (int32_t)ly.y; lk[4] = (int32_t)lz.x; lk[5] = (int32_t)lz.y;
+    return (lx.x >> kLinkBits) | ((lx.y >> kLinkBits) << 6);
+}
+
+__device__ __forceinline__ void colour_voxel4_compact(int t, const uint2 *__restrict__ L, int nE, float4 *S, float *q,
+                                                      Res4 R, float cu, float cp, float pin_out) {
+    int32_t lk[6];
+    const uint32_t f = full_links(t, L, nE, lk);
+    colour_body4(t, lk, f, S, q, R, cu, cp, pin_out);
+}
+
+// Packed links.  The interior entries of one colour go by waves of 64 (list
+// positions t0 + 64 k ...), as the colour pass runs them.  Within a wave the
+// links of one slot are nearly consecutive: the neighbours of consecutive
+// cells of one colour are consecutive cells of the neighbouring colour, so
+// link - lane is the same up to the fluid cells the rows skip, and takes a
+// few values apart where the wave's cells reach across a column edge (or a
+// +-2 link in a solid side's slot reaches across the other one).  Each slot
+// therefore has up to three bases per wave, and a link is
+// base[sel][slot] + lane + d with d < 4096, coded in 14 bits (sel 3: the
+// zero entry).  Record of an entry, three words: codes of slots 0-1, 2-3,
+// 4-5 in bits 0-27 of each, the 11 flag bits in bits 28-31 of the first two
+// and 28-30 of the third, bit 31 of the third set when a link fits no base
+// (then all six are read from the link arrays).
+constexpr int kPackSpan = 4095;
+constexpr uint32_t kPackZero = 0x3FFFu;
+
+// One thread per entry of colour list positions t0 .. t0 + n - 1, whole waves
+// (lanes past n take part in the wave's choices without an entry).  The
+// bases of a slot: three times the value of link - lane that starts the
+// window [base, base + span] holding most of the wave's links not yet placed
+// (ties to the lowest lane); span <= kPackSpan (smaller in tests, to send
+// entries to the full links).
+__global__ __launch_bounds__(256) void k_pack_links(const uint2 *__restrict__ L, int nE, int t0, int n, int wb,
+                                                    int span, uint32_t *__restrict__ P, int32_t *__restrict__ B,
+                                                    int *__restrict__ nfar) {
+    const int r = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    const int lane = (int)threadIdx.x & 63;
+    const bool ok = r < n;
+    const int t = t0 + r;
+    const int32_t zero = nE - 1;
+    int32_t lk[6] = {zero, zero, zero, zero, zero, zero};
+    uint32_t f = 0;
+    if (ok) f = full_links(t, L, nE, lk);
+    int32_t *bw = B + (size_t)(wb + (r >> 6)) * 18;
+    uint32_t code[6];
+    bool far = false;
+    for (int sl = 0; sl < 6; sl++) {
+        const bool live = ok && lk[sl] != zero;
+        const int32_t v = lk[sl] - lane;
+        bool placed = !live;
+        uint32_t c = kPackZero;
+        for (int k = 0; k < 3; k++) {
+            int cnt = 0;
+            for (int j = 0; j < 64; j++) {
+                const int32_t vj = __shfl(v, j, 64);
+                const int pj = __shfl((int)placed, j, 64);
+                cnt += (!pj && vj >= v && vj - v <= span) ? 1 : 0;
+            }
+            int key = placed ? -1 : ((cnt << 6) | (63 - lane));
+            for (int o = 32; o > 0; o >>= 1) key = max(key, __shfl_xor(key, o, 64));
+            const int32_t base = key < 0 ? 0 : __shfl(v, 63 - (key & 63), 64);
+            if (key >= 0 && !placed && v >= base && v - base <= span) {
+                placed = true;
+                c = ((uint32_t)k << 12) | (uint32_t)(v - base);
+            }
+            if (lane == 0 && r < n) bw[k * 6 + sl] = base;
+        }
+        if (!placed) far = true;
+        code[sl] = c;
+    }
+    if (!ok) return;
+    uint32_t *rec = P + (size_t)t * 3;
+    rec[0] = code[0] | (code[1] << 14) | ((f & 15u) << 28);
+    rec[1] = code[2] | (code[3] << 14) | (((f >> 4) & 15u) << 28);
+    rec[2] = code[4] | (code[5] << 14) | (((f >> 8) & 7u) << 28) | (far ? 0x80000000u : 0u);
+    if (far) atomicAdd(nfar, 1);
+}
+
+// The links and flags of entry t from its packed record (b: the 18 bases of
+// its wave; lane: its place in the wave).
+__device__ __forceinline__ uint32_t packed_links(int t, int lane, const int32_t (&b)[18],
+                                                 const uint32_t *__restrict__ P, const uint2 *__restrict__ L, int nE,
+                                                 int32_t (&lk)[6]) {
+    typedef unsigned int v3u __attribute__((ext_vector_type(3)));
+    const v3u w = __builtin_nontemporal_load(reinterpret_cast<const v3u *>(P + (size_t)t * 3));
+    if (w.z >> 31) return full_links(t, L, nE, lk);
+    const uint32_t cw[3] = {w.x, w.y, w.z};
+    const int32_t zero = nE - 1;
+#pragma unroll
+    for (int sl = 0; sl < 6; sl++) {
+        const uint32_t c = (cw[sl >> 1] >> (14 * (sl & 1))) & 0x3FFFu;
+        const uint32_t sel = c >> 12;
+        // the base by masks, not by choosing between array elements (which
+        // the compiler turns into a per-lane indexed read)
+        const int32_t m1 = -(int32_t)(sel == 1), m2 = -(int32_t)(sel == 2), m0 = ~(m1 | m2);
+        const int32_t base = (b[sl] & m0) | (b[6 + sl] & m1) | (b[12 + sl] & m2);
+        lk[sl] = c == kPackZero ? zero : base + lane + (int32_t)(c & 0xFFFu);
+    }
+    return (w.x >> 28) | ((w.y >> 28) << 4) | (((w.z >> 28) & 7u) << 8);
+}
+
+// The 18 bases of the wave of colour position r (wave-uniform).
+__device__ __forceinline__ void wave_bases(const int32_t *__restrict__ B, int wb, int r, int32_t (&b)[18]) {
+    const int wave = __builtin_amdgcn_readfirstlane(r >> 6);
+    const int32_t *bw = B + (size_t)(wb + wave) * 18;
+#pragma unroll
+    for (int i = 0; i < 18; i++) b[i] = bw[i];
+}
+
+// Load-time check of the packed records of colour positions t0 .. t0 + n - 1
+// against the link arrays: counts the entries whose decoded links or flags
+// differ (none, or the packed pass is not used).
+__global__ __launch_bounds__(kListBlock) void k_check_packed(const uint32_t *__restrict__ P,
+                                                             const int32_t *__restrict__ B, int wb,
+                                                             const uint2 *__restrict__ L, int nE, int n, int t0,
+                                                             int *__restrict__ bad) {
+    const int r = (int)blockIdx.x * kListBlock + (int)threadIdx.x;
+    if (r >= n) return;
+    int32_t b[18];
+    wave_bases(B, wb, r, b);
+    int32_t a[6], c[6];
+    const uint32_t fa = full_links(t0 + r, L, nE, a);
+    const uint32_t fc = packed_links(t0 + r, (int)threadIdx.x & 63, b, P, L, nE, c);
+    bool same = (fa & 0x7FFu) == (fc & 0x7FFu);
+    for (int i = 0; i < 6; i++) same = same && a[i] == c[i];
+    if (!same) atomicAdd(bad, 1);
+}
+
 // One colour's entries t0 .. t0 + n - 1 (its list positions).  nE: entries
 // of the state (the buffers' extents).
 __global__ __launch_bounds__(kListBlock) void k_colour4_compact(const uint2 *__restrict__ L, int nE, int n, int t0,
                                                                 float4 *S, float *q, Res4 R, float cu, float cp) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
     if (t >= n) return;
-    const __amdgpu_buffer_rsrc_t rsL =
-        __builtin_amdgcn_make_buffer_rsrc((void *)L, (short)0, (int)((uint32_t)nE * 24u), 0x00020000);
-    const __amdgpu_buffer_rsrc_t rsS =
-        __builtin_amdgcn_make_buffer_rsrc((void *)S, (short)0, (int)((uint32_t)nE * 16u), 0x00020000);
-    colour_voxel4_compact(t0 + t, L, nE, rsL, rsS, S, q, R, cu, cp, 0.f);
+    colour_voxel4_compact(t0 + t, L, nE, S, q, R, cu, cp, 0.f);
+}
+
+// The same from the packed links; wb: the colour's first wave in B.
+__global__ __launch_bounds__(kListBlock) void k_colour4_packed(const uint32_t *__restrict__ P,
+                                                               const int32_t *__restrict__ B, int wb,
+                                                               const uint2 *__restrict__ L, int nE, int n, int t0,
+                                                               float4 *S, float *q, Res4 R, float cu, float cp) {
+    const int r = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (r >= n) return;
+    int32_t b[18];
+    wave_bases(B, wb, r, b);
+    int32_t lk[6];
+    const uint32_t f = packed_links(t0 + r, (int)threadIdx.x & 63, b, P, L, nE, lk);
+    colour_body4(t0 + r, lk, f, S, q, R, cu, cp, 0.f);
 }
 
 // FP32+DC Jacobi on the compact state: pm_compact.h.
@@ -394,6 +533,22 @@ __global__ __launch_bounds__(kListBlock) void k_link_subst(const uint2 *__restri
         }
     }
     for (int q = 0; q < 6; q++) W1[link_word<AOS>((int32_t)t, q, nE)] = w[q];
+}
+
+void launch_pack_links(const FluidLists &L, int span, int *d_far, int *d_bad, hipStream_t s) {
+    span = std::max(0, std::min(span, kPackSpan));
+    for (int col = 0; col < 3; col++) {
+        const int n = L.n_col[col];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_pack_links, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L.clink, L.n_centries,
+                           L.off_col[col], n, L.wave_off[col], span, L.cpack, L.cbase, d_far);
+    }
+    for (int col = 0; col < 3; col++) {
+        const int n = L.n_col[col];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_check_packed, dim3((unsigned)((n + kListBlock - 1) / kListBlock)), dim3(kListBlock), 0, s,
+                           L.cpack, L.cbase, L.wave_off[col], L.clink, L.n_centries, n, L.off_col[col], d_bad);
+    }
 }
 
 void launch_link_subst(const FluidLists &L, const uint2 *L0, uint2 *L1, hipStream_t s) {
@@ -1076,8 +1231,12 @@ void launch_colour4(const Grid &g, const FluidLists &L, const uint8_t *pc, float
         for (int col = 0; col < 3; col++) {
             const int n = L.n_col[col];
             if (n == 0) continue;
-            hipLaunchKernelGGL(k_colour4_compact, dim3(lblocks(n)), dim3(kListBlock), 0, s, L.clink,
-                               L.n_centries, n, L.off_col[col], S, q, R, cu, cp);
+            if (L.packed)
+                hipLaunchKernelGGL(k_colour4_packed, dim3(lblocks(n)), dim3(kListBlock), 0, s, L.cpack, L.cbase,
+                                   L.wave_off[col], L.clink, L.n_centries, n, L.off_col[col], S, q, R, cu, cp);
+            else
+                hipLaunchKernelGGL(k_colour4_compact, dim3(lblocks(n)), dim3(kListBlock), 0, s, L.clink,
+                                   L.n_centries, n, L.off_col[col], S, q, R, cu, cp);
         }
         return;
     }

@@ -135,7 +135,7 @@ def test_workload_media_sor_compact_solve_matches_oracle_solve(medium, monkeypat
     monkeypatch.setenv("POREMAPS_SOR", "compact")
     with pm.Solver(raw, boundary_method=bc, voxel_size=1e-5, mode="sor3", precision="f32dc", omega=pair[0],
                    omega_p=pair[1], eps=1e-6) as s:
-        assert s.kernel_choice()[0] == "interleaved-colour-compact"
+        assert s.kernel_choice()[0] == "interleaved-colour-compact-packed"
         res, _ = s.solve()
         velz, press = s.field("velz"), s.field("press")
     assert res.sweeps == t

@@ -132,14 +132,15 @@ def test_reference_sweep_forms_bit_exact(form, solver, bc):
 
 
 @pytest.mark.parametrize("form", ["tiles", "planes"])
-def test_reference_sweeps_large_domain_bit_exact(form):
+@pytest.mark.parametrize("solver", [2, 3])
+def test_reference_sweeps_large_domain_bit_exact(form, solver):
     # many tiles per diagonal, partial tiles on every axis (60 x 70 x 98)
     raw = random_geometry((60, 70, 98), 0.35, 5)
-    say("reference large domain", form, raw.shape)
+    say("reference large domain", form, solver, raw.shape)
     G, code, st = oracle_state(raw, 0)
-    with ref_form(form), pm.Solver(raw, boundary_method=0, solving_algorithm=2) as s:
+    with ref_form(form), pm.Solver(raw, boundary_method=0, solving_algorithm=solver) as s:
         for _ in range(2):
-            O.sweep_ref(2, G, code, *st)
+            O.sweep_ref(solver, G, code, *st)
             for f in st:
                 O.exchange(f, 0)
             O.pins(G, st[0])
@@ -308,7 +309,8 @@ def test_fast_f32dc_sweeps_bit_exact(mode, bc, omega, omega_p):
             assert np.array_equal(a[m], b[m]), f"{name} differs"
 
 
-@pytest.mark.parametrize("layout", ["split", "split-all-loads", "natural", "compact"])
+@pytest.mark.parametrize("layout", ["split", "split-all-loads", "natural", "compact", "compact-full", "compact-span0",
+                                    "compact-span3"])
 @pytest.mark.parametrize("shape,bc", [((6, 5, 28), 0), ((5, 7, 27), 1), ((4, 6, 2), 2), ((7, 4, 60), 0),
                                       ((5, 6, 13), 3), ((9, 12, 130), 1), ((10, 9, 97), 0), ((6, 7, 20), 4)])
 def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
@@ -318,18 +320,28 @@ def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
     the natural ones; nx 2 gives blocks of two cells; nx 97 and 130 rows of
     several lines per block.  Split rows with the +-2 loads made only where
     the case reads them (default) and with every load; the compact state
-    (fluid cells and periodic halo copies only, linked entries)."""
+    (fluid cells and periodic halo copies only, linked entries) with its
+    links packed (default), in full, and packed with windows of 1 and 4
+    (span 0, 3) so that most entries fall back to the full links."""
     raw = random_geometry(shape, 0.3, sum(shape) + bc)
     say("sor f32dc layout", layout, shape, bc)
     ref = oracle_dc(raw, bc, "sor3", 1.8, 16, 5, 6.0)
     split = layout.startswith("split")
-    with env("POREMAPS_SOR", "split" if split else layout), \
+    span = layout[len("compact-span"):] if layout.startswith("compact-span") else "4095"
+    with env("POREMAPS_SOR", "split" if split else "natural" if layout == "natural" else "compact"), \
             env("POREMAPS_SOR_SKIP2", "0" if layout == "split-all-loads" else "1"), \
+            env("POREMAPS_SOR_LINKS", "full" if layout == "compact-full" else "packed"), \
+            env("POREMAPS_SOR_PACK_SPAN", span), \
             pm.Solver(raw, boundary_method=bc, mode="sor3", omega=1.8, omega_p=6.0, precision="f32dc",
                       it_eval=5) as s:
         want = {"split": "interleaved-colour-split-skip2", "split-all-loads": "interleaved-colour-split",
-                "natural": "interleaved-colour", "compact": "interleaved-colour-compact"}[layout]
-        assert s.kernel_choice()[0] == want
+                "natural": "interleaved-colour", "compact-full": "interleaved-colour-compact"}.get(
+                    layout, "interleaved-colour-compact-packed")
+        name, tried = s.kernel_choice()
+        assert name == want
+        if layout == "compact-span0" and raw.size >= 500:
+            # (the smallest boxes have at most three distinct offsets a slot)
+            assert tried["packed-far"] > 0
         s.step(16)
         m = nonc(ref[0])
         for name, a, b in zip("puvw", s.state_halo(), ref):
