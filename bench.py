@@ -223,7 +223,7 @@ def measured_traffic(spec, form, precision):
              "interleaved-colour-split": ("sor3_f32dc_grf512_hbm_traffic.md",
                                           r"traffic_bytes_per_sweep k_colour4_split x3: (\d+)"),
              "interleaved-colour-compact-packed": ("sor3_packed_f32dc_grf512_hbm_traffic.md",
-                                                   "traffic_bytes_per_sweep k_colour4_packed x3"),
+                                                   r"traffic_bytes_per_sweep k_colour4_packed x3: (\d+)"),
              "interleaved-colour-compact": ("sor3_compact_f32dc_grf512_hbm_traffic.md",
                                             r"traffic_bytes_per_sweep k_colour4_compact x3: (\d+)"),
              "interleaved-compact-skipall": ("jacobi_compact_f32dc_grf512_hbm_traffic.md",
