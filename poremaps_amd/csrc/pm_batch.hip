@@ -59,6 +59,11 @@ struct BSample {
     int compact;
     const uint2 *clink;
     int nE;
+    // the compact lagged pass from packed records (jac_packed_links; chunks
+    // start at multiples of 64, so a wave's entries share their bases)
+    int packed;
+    const uint32_t *cpack;
+    const int32_t *cbase;
 };
 
 struct BChunk {
@@ -123,7 +128,12 @@ __global__ __launch_bounds__(kB) void k_b_first0(const BSample *__restrict__ S, 
 // One lagged pass at list position t of sample s: the compact pass (each load
 // only where used) or the dense record pass.
 __device__ __forceinline__ void b_lag_cell(const BSample &s, int t, int parity) {
-    if (s.compact) {
+    if (s.compact && s.packed) {
+        const float4 *S0 = parity ? s.B4 : s.A4;
+        lag_voxel4_compact<false, false, true>(t, s.clink, s.nE, links_rsrc(s.clink, s.nE), state_rsrc(S0, s.nE), S0,
+                                               parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f, s.cpack,
+                                               jac_wave_bases(s.cbase, t), t & 63);
+    } else if (s.compact) {
         const float4 *S0 = parity ? s.B4 : s.A4;
         lag_voxel4_compact<false, false>(t, s.clink, s.nE, links_rsrc(s.clink, s.nE), state_rsrc(S0, s.nE), S0,
                                          parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f);
@@ -603,6 +613,9 @@ static void fill_desc(const pm_batch *b, std::vector<BSample> &S, std::vector<BC
         d.compact = c->lists.compact_jac ? 1 : 0;
         d.clink = c->lists.clink;
         d.nE = c->lists.n_centries;
+        d.packed = c->lists.compact_jac && c->lists.packed ? 1 : 0;
+        d.cpack = c->lists.cpack;
+        d.cbase = c->lists.cbase;
         const int s = (int)S.size();
         d.chunk_begin = (int)C.size();
         for (int t = 0; t < d.n; t += pm::kB) C.push_back({s, t});

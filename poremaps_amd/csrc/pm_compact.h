@@ -127,14 +127,73 @@ __device__ __forceinline__ void target_links(__amdgpu_buffer_rsrc_t rsL, const J
 
 // lag_voxel4_rec on the compact state.  ALL: every load, else each +-1
 // neighbour and each recompute-target value only where it is used.  AOS: the
+// Packed links of the compact Jacobi (FluidLists::packed with compact_jac),
+// 24 B per entry: twelve links -- slots 0-5 the +-1 neighbours as in the link
+// arrays (after the +-2 substitution), slots 6-11 the recompute targets'
+// links target_links reads (the -x neighbour's +y and +z links, the -y
+// neighbour's +x and +z, the -z neighbour's +x and +y) -- each in 12 bits: 2
+// bits naming one of three bases of the entry's wave of 64 and the slot (3:
+// the zero entry) and a 10-bit offset, link = base + lane + offset.  Word w
+// holds slots 2w, 2w + 1 in bits 0-23 and flag bits 6w .. 6w + 5 in bits
+// 24-29; bit 31 of word 5 marks an entry whose links fit no base (read from
+// the link arrays).  36 bases per wave, [base][slot].
+constexpr int kJPackSpan = 1023;
+constexpr uint32_t kJPackZero = 0xFFFu;
+
+__device__ __forceinline__ void jac_packed_links(int t, int lane, const int32_t *__restrict__ bw,
+                                                 const uint32_t *__restrict__ P, const uint2 *__restrict__ L, int nE,
+                                                 __amdgpu_buffer_rsrc_t rsL, JLinks &l, int32_t (&dl)[6]) {
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const uint32_t *rec = P + (size_t)t * 6;
+    const v2u w01 = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(rec));
+    const v2u w23 = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(rec + 2));
+    const v2u w45 = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(rec + 4));
+    const uint32_t W[6] = {w01.x, w01.y, w23.x, w23.y, w45.x, w45.y};
+    if (W[5] >> 31) {
+        l = jac_links<false>(L, nE, t);
+        const bool all[6] = {true, true, true, true, true, true};
+        target_links<false>(rsL, l, nE, all, dl);
+        return;
+    }
+    const int32_t zero = nE - 1;
+    int32_t lk[12];
+#pragma unroll
+    for (int sl = 0; sl < 12; sl++) {
+        const uint32_t c = (W[sl >> 1] >> (12 * (sl & 1))) & 0xFFFu;
+        const uint32_t sel = c >> 10;
+        // the base by masks, not by choosing between array elements (which
+        // the compiler turns into a per-lane indexed read)
+        const int32_t m1 = -(int32_t)(sel == 1), m2 = -(int32_t)(sel == 2), m0 = ~(m1 | m2);
+        const int32_t base = (bw[sl] & m0) | (bw[12 + sl] & m1) | (bw[24 + sl] & m2);
+        lk[sl] = c == kJPackZero ? zero : base + lane + (int32_t)(c & 0x3FFu);
+    }
+    l.xm = lk[0]; l.xp = lk[1]; l.ym = lk[2]; l.yp = lk[3]; l.zm = lk[4]; l.zp = lk[5];
+    l.f = 0;
+#pragma unroll
+    for (int w = 0; w < 6; w++) l.f |= (uint64_t)((W[w] >> 24) & 63u) << (6 * w);
+#pragma unroll
+    for (int q = 0; q < 6; q++) dl[q] = lk[6 + q];
+}
+
+// The 36 bases of the wave that holds list position r (a wave-uniform
+// pointer, read with scalar loads).
+__device__ __forceinline__ const int32_t *jac_wave_bases(const int32_t *__restrict__ B, int r) {
+    return B + (size_t)__builtin_amdgcn_readfirstlane(r >> 6) * 36;
+}
+
 // links as one record per entry.
-template <bool ALL, bool AOS>
+template <bool ALL, bool AOS, bool PACKED = false>
 __device__ __forceinline__ void lag_voxel4_compact(int t, const uint2 *__restrict__ L, int nE,
                                                    __amdgpu_buffer_rsrc_t rsL, __amdgpu_buffer_rsrc_t rsS,
                                                    const float4 *__restrict__ S0, float4 *__restrict__ S1, Res4 R,
-                                                   float cu, float cp, float pin_out) {
+                                                   float cu, float cp, float pin_out,
+                                                   const uint32_t *__restrict__ P = nullptr,
+                                                   const int32_t *__restrict__ bw = nullptr, int lane = 0) {
     typedef float v4f __attribute__((ext_vector_type(4)));
-    const JLinks l = jac_links<AOS>(L, nE, t);
+    JLinks l;
+    int32_t dlp[6];
+    if (PACKED) jac_packed_links(t, lane, bw, P, L, nE, rsL, l, dlp);
+    else l = jac_links<AOS>(L, nE, t);
     const uint8_t pcv = (uint8_t)(l.f & 0xff), pcx = (uint8_t)((l.f >> 8) & 0xff),
                   pcy = (uint8_t)((l.f >> 16) & 0xff), pcz = (uint8_t)((l.f >> 24) & 0xff);
     const bool out = (l.f >> 32) & 1u, first = (l.f >> 33) & 1u;
@@ -155,7 +214,11 @@ __device__ __forceinline__ void lag_voxel4_compact(int t, const uint2 *__restric
         need[4] = !cc.bz && !fwd_solid(tz.cx); need[5] = !cc.bz && !fwd_solid(tz.cy);
     }
     int32_t dl[6];
-    target_links<AOS>(rsL, l, nE, need, dl);
+    if (PACKED) {
+        for (int q = 0; q < 6; q++) dl[q] = dlp[q];
+    } else {
+        target_links<AOS>(rsL, l, nE, need, dl);
+    }
     const float vyx = buf_load1(rsS, dl[0], 2, need[0]), wzx = buf_load1(rsS, dl[1], 3, need[1]);
     const float uxy = buf_load1(rsS, dl[2], 1, need[2]), wzy = buf_load1(rsS, dl[3], 3, need[3]);
     const float uxz = buf_load1(rsS, dl[4], 1, need[4]), vyz = buf_load1(rsS, dl[5], 2, need[5]);

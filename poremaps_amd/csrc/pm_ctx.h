@@ -121,6 +121,7 @@ struct pm_ctx {
     bool sor_fused = true;      // FluidLists::sor_fused (POREMAPS_SOR_STAGE=seq: false)
     bool sor_compact = true;    // FluidLists::compact (POREMAPS_SOR=split / natural: off)
     bool sor_packed = true;     // FluidLists::packed (POREMAPS_SOR_LINKS=full: off)
+    bool jac_packed = true;     // FluidLists::packed for the compact Jacobi (POREMAPS_JAC_LINKS=full / aos: off)
     int pack_span = 4095;       // launch_pack_links' span (POREMAPS_SOR_PACK_SPAN)
     int pack_far = 0;           // entries left on the full links
     bool jac_compact = true;    // FluidLists::compact_jac (POREMAPS_JACOBI naming a dense form: off)

@@ -161,6 +161,8 @@ struct FluidLists {
     // cell's position in the colour lists, for the residual and the fold
     int32_t *nat = nullptr, *natpos = nullptr;
     // compact SOR, packed links (launch_pack_links): 12 B per interior entry
+    // (compact Jacobi: launch_pack_jac_links, 24 B per entry, 36 bases per
+    // wave, cpack / cbase / packed used the same way)
     // instead of 24 -- per link 2 bits naming one of three bases of the
     // entry's wave and slot (3: the zero entry) and a 12-bit offset, the
     // record flags and a flag for entries whose links fit no base (read from
@@ -231,6 +233,11 @@ void launch_torus_links(const Grid &g, const int32_t *list, int n, int32_t *cidx
 // span: the widest offset from a base (4095, the most the record holds;
 // smaller only in tests).
 void launch_pack_links(const FluidLists &L, int span, int *d_far, int *d_bad, hipStream_t s);
+// Packed records of the compact Jacobi's entries (24 B each, the twelve
+// links of the lagged pass and the flags) from clink into L.cpack (6 n_all
+// words) and L.cbase (36 per wave of 64), both allocated by the caller;
+// counters as launch_pack_links.  span <= 1023.
+void launch_pack_jac_links(const FluidLists &L, int span, int *d_far, int *d_bad, hipStream_t s);
 void launch_link_subst(const FluidLists &L, const uint2 *L0, uint2 *L1, hipStream_t s);
 // Whether the compact SOR state can hold this medium: no slip walls, and
 // every entry's byte offset below 2^32 (its masked loads are buffer loads).

@@ -136,9 +136,9 @@ extern "C" int64_t pm_device_bytes(const pm_params *p) {
     // Jacobi and FP32+DC SOR: per-voxel records (8 B per fluid cell)
     if (fast && (jac || dc)) per += 8;
     // FP32+DC on the compact state: the links (24 B per entry) and the outlet
-    // buffer(s) by entry (4 B each; two for Jacobi); SOR: the packed links
-    // (12 B per entry, 72 B per 64)
-    if (dc) per += jac ? 32 : 28 + 14;
+    // buffer(s) by entry (4 B each; two for Jacobi); the packed links (SOR:
+    // 12 B per entry and 72 B per 64; Jacobi: 24 B and 144 B per 64)
+    if (dc) per += jac ? 32 + 27 : 28 + 14;
     // FP32+DC Jacobi: the z-march form's row-segment starts (4 B per 64 cells)
     if (dc && jac) per += 1;
     // fluid lists (upper bound: every cell fluid); SOR on per-field arrays
@@ -204,7 +204,7 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     if (const char *so = std::getenv("POREMAPS_SOR"))
         c->sor_compact = std::strcmp(so, "split") != 0 && std::strcmp(so, "natural") != 0;
     if (const char *sl = std::getenv("POREMAPS_SOR_LINKS")) c->sor_packed = std::strcmp(sl, "full") != 0;
-    if (const char *sp = std::getenv("POREMAPS_SOR_PACK_SPAN")) c->pack_span = std::max(0, std::atoi(sp));
+    if (const char *sp = std::getenv("POREMAPS_PACK_SPAN")) c->pack_span = std::max(0, std::atoi(sp));
     // slip walls and the z-chunked passes: colour-split rows
     if (p->walls == 2 || c->sor_chunk > 0) c->sor_compact = false;
     if (p->mode == PM_MODE_SOR3 && p->precision == PM_PREC_F32DC && !c->sor_compact) c->prm_tile_x = 0;
@@ -231,7 +231,10 @@ extern "C" int pm_ctx_create(const pm_params *p, pm_ctx **out) {
     // FP32+DC Jacobi on the compact state: not with slip walls (their
     // mirrored halos are not copies of images)
     if (p->walls == 2) c->jac_compact = false;
-    if (const char *jl = std::getenv("POREMAPS_JAC_LINKS")) c->jac_links_aos = std::strcmp(jl, "aos") == 0;
+    if (const char *jl = std::getenv("POREMAPS_JAC_LINKS")) {
+        c->jac_links_aos = std::strcmp(jl, "aos") == 0;
+        c->jac_packed = std::strcmp(jl, "full") != 0 && !c->jac_links_aos;
+    }
     if (const char *zs = std::getenv("POREMAPS_STREAM_ZS")) c->stream_zs = std::max(1, std::atoi(zs));
     if (c->g.slip && (c->jac_impl == JI_STREAM || c->jac_impl == JI_MARCH)) c->jac_impl = JI_RECORDS;
     // z-march form (POREMAPS_JACOBI=march): tile rows 4 or 8, planes per z
@@ -516,6 +519,32 @@ static int pack_sor_links(pm_ctx *c) {
     return PM_OK;
 }
 
+// The compact Jacobi's packed records (launch_pack_jac_links): 24 B per entry
+// holding the lagged pass's twelve links, 36 bases per wave of 64 entries;
+// the link arrays stay for the other passes and the entries that fit no base.
+static int pack_jac_links(pm_ctx *c) {
+    pm::FluidLists &L = c->lists;
+    const int64_t waves = (L.n_all + 63) / 64;
+    RC(dalloc(&L.cpack, 6 * std::max<int64_t>(1, L.n_all)));
+    RC(dalloc(&L.cbase, 36 * std::max<int64_t>(1, waves)));
+    int *d_cnt = nullptr;  // entries on the full links, records that do not decode
+    RC(dalloc(&d_cnt, 2));
+    int cnt[2] = {0, 0};
+    hipError_t e = hipMemsetAsync(d_cnt, 0, 2 * sizeof(int), c->stream);
+    if (e == hipSuccess) {
+        pm::launch_pack_jac_links(L, std::min(c->pack_span, 1023), d_cnt, d_cnt + 1, c->stream);
+        e = hipMemcpyAsync(cnt, d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_cnt);
+    if (e != hipSuccess) return fail(PM_E_HIP, "packed links: %s", hipGetErrorString(e));
+    HIPCHK(hipGetLastError());
+    if (cnt[1] != 0) return fail(PM_E_HIP, "packed links: %d records do not decode to their links", cnt[1]);
+    L.packed = true;
+    c->pack_far = cnt[0];
+    return PM_OK;
+}
+
 // The compact FP32+DC Jacobi state (FluidLists::compact_jac): entry t is list
 // position t, then the zero entry; no halo entries, a link across a periodic
 // face goes to the image cell itself.  Links built on the device from the
@@ -617,8 +646,13 @@ static int choose_jacobi_layout(pm_ctx *c) {
     } else {
         (void)hipFree(L.clink);
         (void)hipFree(c->R4.pc);
+        (void)hipFree(L.cpack);
+        (void)hipFree(L.cbase);
         L.clink = nullptr;
         c->R4.pc = nullptr;
+        L.cpack = nullptr;
+        L.cbase = nullptr;
+        L.packed = false;
         L.n_centries = 0;
         c->jac_impl = JI_RECORDS;
     }
@@ -830,6 +864,7 @@ extern "C" int pm_load_geometry(pm_ctx *c, const uint8_t *raw) {
         (void)hipFree(c->lists.clink);
         c->lists.clink = L1;
         if (c->lists.compact && c->sor_packed) RC(pack_sor_links(c));
+        if (c->lists.compact_jac && c->jac_packed && !c->lists.links_aos) RC(pack_jac_links(c));
     }
     RC(init_state(c));
     HIPCHK(hipGetLastError());
@@ -940,8 +975,9 @@ static void apply_variant(pm_ctx *c, const SweepVariant &v) {
 
 static const char *variant_name(const pm_ctx *c, const SweepVariant &v, char *buf, size_t len) {
     if (v.impl == JI_COMPACT && v.skip1 && v.occ > 0)
-        std::snprintf(buf, len, "interleaved-compact-skipall-occ%d", v.occ);
-    else if (v.impl == JI_COMPACT) std::snprintf(buf, len, "interleaved-compact%s", v.skip1 ? "-skipall" : "");
+        std::snprintf(buf, len, "interleaved-compact%s-skipall-occ%d", c->lists.packed ? "-packed" : "", v.occ);
+    else if (v.impl == JI_COMPACT)
+        std::snprintf(buf, len, "interleaved-compact%s%s", c->lists.packed ? "-packed" : "", v.skip1 ? "-skipall" : "");
     else if (c->dc() && v.impl == JI_PAIR) std::snprintf(buf, len, "interleaved-pair");
     else if (c->dc() && v.impl == JI_MARCH) std::snprintf(buf, len, "interleaved-zmarch-ty%d-zs%d", v.ty, v.zs);
     else if (c->dc() && v.impl == JI_RECORDS && v.skip2)

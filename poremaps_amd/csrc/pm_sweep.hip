@@ -384,6 +384,117 @@ __global__ __launch_bounds__(kListBlock, OCC > 0 ? OCC : 1) void k_jac4c_lag(con
     lag_voxel4_compact<ALL, AOS>(t, L, nE, rsL, rsS, S0, S1, R, cu, cp, 0.f);
 }
 
+// The lagged pass from the packed records (jac_packed_links): the same cell
+// work, with the recompute targets' links in the entry's own record.
+template <bool ALL, int OCC = 0>
+__global__ __launch_bounds__(kListBlock, OCC > 0 ? OCC : 1) void k_jac4c_lagp(
+    const uint32_t *__restrict__ P, const int32_t *__restrict__ B, const uint2 *__restrict__ L, int nE, int n,
+    const float4 *__restrict__ S0, float4 *__restrict__ S1, Res4 R, float cu, float cp) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kListBlock + threadIdx.x;
+    if (t >= n) return;
+    const __amdgpu_buffer_rsrc_t rsL =
+        __builtin_amdgcn_make_buffer_rsrc((void *)L, (short)0, (int)((uint32_t)nE * 24u), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsS =
+        __builtin_amdgcn_make_buffer_rsrc((void *)S0, (short)0, (int)((uint32_t)nE * 16u), 0x00020000);
+    lag_voxel4_compact<ALL, false, true>(t, L, nE, rsL, rsS, S0, S1, R, cu, cp, 0.f, P, jac_wave_bases(B, t),
+                                         (int)threadIdx.x & 63);
+}
+
+// The twelve links a packed Jacobi record holds, from the link arrays.
+__device__ __forceinline__ uint64_t jac_twelve(int t, const uint2 *__restrict__ L, int nE, int32_t (&lk)[12]) {
+    const JLinks l = jac_links<false>(L, nE, t);
+    const uint32_t *W = reinterpret_cast<const uint32_t *>(L);
+    lk[0] = l.xm; lk[1] = l.xp; lk[2] = l.ym; lk[3] = l.yp; lk[4] = l.zm; lk[5] = l.zp;
+    lk[6] = (int32_t)(W[link_word<false>(l.xm, 3, nE)] & kLinkMask);
+    lk[7] = (int32_t)(W[link_word<false>(l.xm, 5, nE)] & kLinkMask);
+    lk[8] = (int32_t)(W[link_word<false>(l.ym, 1, nE)] & kLinkMask);
+    lk[9] = (int32_t)(W[link_word<false>(l.ym, 5, nE)] & kLinkMask);
+    lk[10] = (int32_t)(W[link_word<false>(l.zm, 1, nE)] & kLinkMask);
+    lk[11] = (int32_t)(W[link_word<false>(l.zm, 3, nE)] & kLinkMask);
+    return l.f;
+}
+
+// Packed Jacobi records of list positions 0 .. n - 1, one thread per entry,
+// whole waves (lanes past n take part in the choices without an entry); the
+// bases as k_pack_links chooses them, with windows [base, base + span],
+// span <= kJPackSpan.
+__global__ __launch_bounds__(256) void k_pack_jac(const uint2 *__restrict__ L, int nE, int n, int span,
+                                                  uint32_t *__restrict__ P, int32_t *__restrict__ B,
+                                                  int *__restrict__ nfar) {
+    const int r = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    const int lane = (int)threadIdx.x & 63;
+    const bool ok = r < n;
+    const int32_t zero = nE - 1;
+    int32_t lk[12];
+    for (int q = 0; q < 12; q++) lk[q] = zero;
+    uint64_t f = 0;
+    if (ok) f = jac_twelve(r, L, nE, lk);
+    int32_t *bw = B + (size_t)(r >> 6) * 36;
+    uint32_t code[12];
+    bool far = false;
+    for (int sl = 0; sl < 12; sl++) {
+        const bool live = ok && lk[sl] != zero;
+        const int32_t v = lk[sl] - lane;
+        bool placed = !live;
+        uint32_t c = kJPackZero;
+        for (int k = 0; k < 3; k++) {
+            int cnt = 0;
+            for (int j = 0; j < 64; j++) {
+                const int32_t vj = __shfl(v, j, 64);
+                const int pj = __shfl((int)placed, j, 64);
+                cnt += (!pj && vj >= v && vj - v <= span) ? 1 : 0;
+            }
+            int key = placed ? -1 : ((cnt << 6) | (63 - lane));
+            for (int o = 32; o > 0; o >>= 1) key = max(key, __shfl_xor(key, o, 64));
+            const int32_t base = key < 0 ? 0 : __shfl(v, 63 - (key & 63), 64);
+            if (key >= 0 && !placed && v >= base && v - base <= span) {
+                placed = true;
+                c = ((uint32_t)k << 10) | (uint32_t)(v - base);
+            }
+            if (lane == 0 && r < n) bw[k * 12 + sl] = base;
+        }
+        if (!placed) far = true;
+        code[sl] = c;
+    }
+    if (!ok) return;
+    uint32_t *rec = P + (size_t)r * 6;
+    for (int w = 0; w < 6; w++)
+        rec[w] = code[2 * w] | (code[2 * w + 1] << 12) | ((uint32_t)((f >> (6 * w)) & 63u) << 24) |
+                 (w == 5 && far ? 0x80000000u : 0u);
+    if (far) atomicAdd(nfar, 1);
+}
+
+// Load-time check of the packed Jacobi records against the link arrays:
+// counts the entries whose twelve decoded links or flags differ.
+__global__ __launch_bounds__(kListBlock) void k_check_jac(const uint32_t *__restrict__ P,
+                                                          const int32_t *__restrict__ B,
+                                                          const uint2 *__restrict__ L, int nE, int n,
+                                                          int *__restrict__ bad) {
+    const int r = (int)blockIdx.x * kListBlock + (int)threadIdx.x;
+    if (r >= n) return;
+    const __amdgpu_buffer_rsrc_t rsL =
+        __builtin_amdgcn_make_buffer_rsrc((void *)L, (short)0, (int)((uint32_t)nE * 24u), 0x00020000);
+    int32_t a[12];
+    const uint64_t fa = jac_twelve(r, L, nE, a);
+    JLinks l;
+    int32_t dl[6];
+    jac_packed_links(r, (int)threadIdx.x & 63, jac_wave_bases(B, r), P, L, nE, rsL, l, dl);
+    const int32_t c[12] = {l.xm, l.xp, l.ym, l.yp, l.zm, l.zp, dl[0], dl[1], dl[2], dl[3], dl[4], dl[5]};
+    bool same = (fa & 0xFFFFFFFFFull) == (l.f & 0xFFFFFFFFFull);
+    for (int q = 0; q < 12; q++) same = same && a[q] == c[q];
+    if (!same) atomicAdd(bad, 1);
+}
+
+void launch_pack_jac_links(const FluidLists &L, int span, int *d_far, int *d_bad, hipStream_t s) {
+    const int n = L.n_all;
+    if (n == 0) return;
+    span = std::max(0, std::min(span, kJPackSpan));
+    hipLaunchKernelGGL(k_pack_jac, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L.clink, L.n_centries, n,
+                       span, L.cpack, L.cbase, d_far);
+    hipLaunchKernelGGL(k_check_jac, dim3((unsigned)((n + kListBlock - 1) / kListBlock)), dim3(kListBlock), 0, s,
+                       L.cpack, L.cbase, L.clink, L.n_centries, n, d_bad);
+}
+
 // first_voxel4_zero on the compact state (no halo copies to write).
 template <bool AOS>
 __global__ __launch_bounds__(kListBlock) void k_jac4c_first_zero(const uint2 *__restrict__ L, int nE, int n,
@@ -1108,6 +1219,15 @@ static void launch_jacobi4_compact(int stage, const FluidLists &L, const float4 
         hipLaunchKernelGGL(k_jac4c_first_zero<AOS>, grid, blk, 0, s, L.clink, nE, L.n_all, S1, R, cu);
     else if (stage == JAC_FIRST)
         hipLaunchKernelGGL(k_jac4c_first<AOS>, grid, blk, 0, s, L.clink, nE, L.n_all, S0, q0, S1, R, cu);
+    else if (stage == JAC_LAG && L.packed && L.rec_skip1 && L.rec_occ == 6)
+        hipLaunchKernelGGL((k_jac4c_lagp<false, 6>), grid, blk, 0, s, L.cpack, L.cbase, L.clink, nE, L.n_all, S0,
+                           S1, R, cu, cp);
+    else if (stage == JAC_LAG && L.packed && L.rec_skip1)
+        hipLaunchKernelGGL((k_jac4c_lagp<false>), grid, blk, 0, s, L.cpack, L.cbase, L.clink, nE, L.n_all, S0, S1,
+                           R, cu, cp);
+    else if (stage == JAC_LAG && L.packed)
+        hipLaunchKernelGGL((k_jac4c_lagp<true>), grid, blk, 0, s, L.cpack, L.cbase, L.clink, nE, L.n_all, S0, S1, R,
+                           cu, cp);
     else if (stage == JAC_LAG && L.rec_skip1 && L.rec_occ == 6)
         hipLaunchKernelGGL((k_jac4c_lag<false, AOS, 6>), grid, blk, 0, s, L.clink, nE, L.n_all, S0, S1, R, cu, cp);
     else if (stage == JAC_LAG && L.rec_skip1)

@@ -331,7 +331,7 @@ def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
     with env("POREMAPS_SOR", "split" if split else "natural" if layout == "natural" else "compact"), \
             env("POREMAPS_SOR_SKIP2", "0" if layout == "split-all-loads" else "1"), \
             env("POREMAPS_SOR_LINKS", "full" if layout == "compact-full" else "packed"), \
-            env("POREMAPS_SOR_PACK_SPAN", span), \
+            env("POREMAPS_PACK_SPAN", span), \
             pm.Solver(raw, boundary_method=bc, mode="sor3", omega=1.8, omega_p=6.0, precision="f32dc",
                       it_eval=5) as s:
         want = {"split": "interleaved-colour-split-skip2", "split-all-loads": "interleaved-colour-split",
@@ -372,7 +372,7 @@ def test_sor_f32dc_z_chunks_bit_exact(stage, chunk, shape, bc):
 
 
 @pytest.mark.parametrize("halos", ["fused", "separate", "march4", "march8", "occ6", "occ8", "skip2", "skipall",
-                                   "compact", "compact-skipall", "compact-aos"])
+                                   "compact", "compact-skipall", "compact-aos", "compact-full", "compact-span0"])
 @pytest.mark.parametrize("shape,bc", [((2, 5, 6), 0), ((3, 3, 7), 1), ((6, 2, 3), 0), ((5, 6, 7), 2),
                                       ((4, 5, 6), 4), ((7, 6, 9), 3), ((9, 8, 40), 0), ((11, 13, 70), 0),
                                       ((6, 9, 130), 1)])
@@ -382,8 +382,10 @@ def test_jacobi_f32dc_record_pass_halo_fill_bit_exact(halos, shape, bc):
     equal the oracle bit for bit.  Sides of 2 and 3 give cells with two
     copies on one axis (planes 2 and nz both, or 3 and nz+1).  The compact
     state (no halo copies: links across a periodic face go to the image) with
-    every load and with each load only where used, its links in three arrays
-    or in one record per entry (aos)."""
+    every load and with each load only where used, the lagged pass reading
+    packed records of its twelve links (default; span0: windows of one value,
+    so that most entries fall back to the link arrays), the link arrays
+    (full) or one record per entry (aos)."""
     raw = random_geometry(shape, 0.3, 3 * sum(shape) + bc)
     say("record pass halos", halos, shape, bc)
     ref = oracle_dc(raw, bc, "jacobi", 2.5, 16, 5, 6.0)
@@ -391,17 +393,22 @@ def test_jacobi_f32dc_record_pass_halo_fill_bit_exact(halos, shape, bc):
     compact = halos.startswith("compact")
     occ = halos[3:] if halos.startswith("occ") else "0"
     form = "march" if march else "compact" if compact else "records"
-    skip = {"skip2": "1", "skipall": "2", "compact-skipall": "2", "compact-aos": "2"}.get(halos, "0")
+    skip = {"skip2": "1", "skipall": "2", "compact-skipall": "2", "compact-aos": "2", "compact-full": "2",
+            "compact-span0": "2"}.get(halos, "0")
+    links = {"compact-aos": "aos", "compact-full": "full"}.get(halos, "packed")
     with env("POREMAPS_HALOS", "separate" if halos == "separate" else "fused"), env("POREMAPS_JACOBI", form), \
-            env("POREMAPS_JAC_LINKS", "aos" if halos == "compact-aos" else "soa"), \
+            env("POREMAPS_JAC_LINKS", links), env("POREMAPS_PACK_SPAN", "0" if halos == "compact-span0" else "4095"), \
             env("POREMAPS_MARCH_TY", halos[5:] if march else "4"), env("POREMAPS_MARCH_ZSEG", "3"), \
             env("POREMAPS_REC_OCC", occ), env("POREMAPS_REC_SKIP2", skip), \
             pm.Solver(raw, boundary_method=bc, mode="jacobi", omega=2.5, omega_p=6.0, precision="f32dc",
                       it_eval=5) as s:
         s.step(16)
-        name = s.kernel_choice()[0]
+        name, tried = s.kernel_choice()
         if compact:
-            assert name == "interleaved-" + halos.replace("-aos", "-skipall"), name
+            want = "interleaved-compact" + ("-packed" if links == "packed" else "") + ("-skipall" if skip == "2" else "")
+            assert name == want, name
+            if halos == "compact-span0" and raw.size >= 500:
+                assert tried["packed-far"] > 0
         else:
             want = f"interleaved-zmarch-ty{halos[5:]}-zs3" if march else "interleaved-lagged-records"
             assert name == want + (f"-occ{occ}" if occ != "0" else "") + \

@@ -90,7 +90,7 @@ def test_kernel_forms_and_tuned_solve_are_bit_identical(precision, bc):
             # each load only where used (also at 6 waves per SIMD), or
             # the dense interleaved forms; FP64: pair, lagged list, records
             # and the dense 2.5-D forms
-            forms_tried = {k: v for k, v in tried.items() if not k.startswith("layout-")}
+            forms_tried = {k: v for k, v in tried.items() if not k.startswith("layout-") and k != "packed-far"}
             assert name in forms_tried
             if precision == "f32dc":
                 assert set(tried) >= {"layout-compact", "layout-dense"}
