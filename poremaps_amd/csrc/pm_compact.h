@@ -140,7 +140,14 @@ __device__ __forceinline__ void target_links(__amdgpu_buffer_rsrc_t rsL, const J
 constexpr int kJPackSpan = 1023;
 constexpr uint32_t kJPackZero = 0xFFFu;
 
-__device__ __forceinline__ void jac_packed_links(int t, int lane, const int32_t *__restrict__ bw,
+// The bases are read through the constant address space: nothing writes
+// them during a pass, and a wave-uniform address there is read with scalar
+// loads even where the pointer comes from memory the kernel also writes
+// through (the batch's sample table), which would otherwise make every lane
+// load all 36.
+typedef const int32_t __attribute__((address_space(4))) kconst_i32;
+
+__device__ __forceinline__ void jac_packed_links(int t, int lane, const kconst_i32 *bw,
                                                  const uint32_t *__restrict__ P, const uint2 *__restrict__ L, int nE,
                                                  __amdgpu_buffer_rsrc_t rsL, JLinks &l, int32_t (&dl)[6]) {
     typedef unsigned int v2u __attribute__((ext_vector_type(2)));
@@ -175,10 +182,13 @@ __device__ __forceinline__ void jac_packed_links(int t, int lane, const int32_t 
     for (int q = 0; q < 6; q++) dl[q] = lk[6 + q];
 }
 
-// The 36 bases of the wave that holds list position r (a wave-uniform
-// pointer, read with scalar loads).
-__device__ __forceinline__ const int32_t *jac_wave_bases(const int32_t *__restrict__ B, int r) {
-    return B + (size_t)__builtin_amdgcn_readfirstlane(r >> 6) * 36;
+// The 36 bases of the wave that holds list position r (B and r the same on
+// every lane of the wave).
+__device__ __forceinline__ const kconst_i32 *jac_wave_bases(const int32_t *B, int r) {
+    const uint64_t a = (uint64_t)(B + (size_t)(r >> 6) * 36);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return (const kconst_i32 *)((uint64_t)hi << 32 | lo);
 }
 
 // links as one record per entry.
@@ -188,7 +198,7 @@ __device__ __forceinline__ void lag_voxel4_compact(int t, const uint2 *__restric
                                                    const float4 *__restrict__ S0, float4 *__restrict__ S1, Res4 R,
                                                    float cu, float cp, float pin_out,
                                                    const uint32_t *__restrict__ P = nullptr,
-                                                   const int32_t *__restrict__ bw = nullptr, int lane = 0) {
+                                                   const kconst_i32 *bw = nullptr, int lane = 0) {
     typedef float v4f __attribute__((ext_vector_type(4)));
     JLinks l;
     int32_t dlp[6];
