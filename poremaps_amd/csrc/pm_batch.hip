@@ -154,6 +154,21 @@ __global__ __launch_bounds__(kB) void k_b_lag4(const BSample *__restrict__ S, co
     b_lag_cell(s, t, parity);
 }
 
+// The lagged pass of the samples on packed compact records (the run order
+// puts them first), without the dense branch, so that it runs at 6 waves per
+// SIMD as the single-solve pass does.
+__global__ __launch_bounds__(kB, 6) void k_b_lag4p(const BSample *__restrict__ S, const BChunk *__restrict__ C,
+                                                  int parity) {
+    const BChunk ch = C[blockIdx.x];
+    const BSample &s = S[ch.s];
+    const int t = ch.start + threadIdx.x;
+    if (t >= s.n) return;
+    const float4 *S0 = parity ? s.B4 : s.A4;
+    lag_voxel4_compact<false, false, true>(t, s.clink, s.nE, links_rsrc(s.clink, s.nE), state_rsrc(S0, s.nE), S0,
+                                           parity ? s.A4 : s.B4, s.R4, (float)s.cu, (float)s.cp, 0.f, s.cpack,
+                                           jac_wave_bases(s.cbase, t), t & 63);
+}
+
 // The same two passes as persistent kernels over tile queues: the samples'
 // (sample, chunk) tiles back to back, largest sample first (the table
 // order), cut into nq contiguous queues (one per XCD by default: workgroup b
@@ -400,6 +415,7 @@ struct pm_batch {
     int capS = 0, capC = 0, capG = 0, capP = 0;
     int capPart = 0, capOut = 0;     // reduction partials / outputs (grow-only)
     int nS = 0, nC = 0, nG = 0, nP = 0;
+    int nCp = 0;                     // the first nCp chunks: samples on packed compact records
     bool shape_dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -659,7 +675,9 @@ static void launch_interval(pm_batch *b) {
                 if (s == 0) {
                     hipLaunchKernelGGL(k_b_first0, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
                 } else {
-                    hipLaunchKernelGGL(k_b_lag4, dim3(b->nC), blk, 0, st, b->dS, b->dC, par);
+                    if (b->nCp > 0) hipLaunchKernelGGL(k_b_lag4p, dim3(b->nCp), blk, 0, st, b->dS, b->dC, par);
+                    if (b->nC > b->nCp)
+                        hipLaunchKernelGGL(k_b_lag4, dim3(b->nC - b->nCp), blk, 0, st, b->dS, b->dC + b->nCp, par);
                 }
             }
         }
@@ -733,9 +751,13 @@ extern "C" int pm_batch_launch(pm_batch *b) {
         for (size_t s = 0; s < b->ctx.size(); s++)
             if (b->ctx[s] && b->state[s] == PM_SAMPLE_RUNNING && b->ctx[s]->lists.n_all > 0)
                 b->run.push_back((int)s);
-        // largest sample first: the tile queue hands out its tiles first, the
-        // small samples' tiles fill the end of each pass
-        std::stable_sort(b->run.begin(), b->run.end(), [b](int x, int y) {
+        // the samples on packed compact records first (their chunks get the
+        // lagged kernel without the dense branch), then largest sample first:
+        // the tile queue hands out its tiles first, the small samples' tiles
+        // fill the end of each pass
+        const auto packed = [b](int x) { return b->ctx[x]->lists.compact_jac && b->ctx[x]->lists.packed; };
+        std::stable_sort(b->run.begin(), b->run.end(), [b, packed](int x, int y) {
+            if (packed(x) != packed(y)) return packed(x);
             return b->ctx[x]->lists.n_all > b->ctx[y]->lists.n_all;
         });
     }
@@ -784,6 +806,9 @@ extern "C" int pm_batch_launch(pm_batch *b) {
         BCHK(hipMemcpyAsync(b->dP, P.data(), sizeof(BChunk) * P.size(), hipMemcpyHostToDevice, b->stream));
         b->nS = (int)S.size();
         b->nC = (int)C.size();
+        b->nCp = 0;
+        for (const BSample &d : S)
+            if (d.compact && d.packed) b->nCp = d.chunk_end;
         b->nG = (int)G.size();
         b->nP = (int)P.size();
     }
