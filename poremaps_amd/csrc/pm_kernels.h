@@ -219,9 +219,9 @@ void launch_ghost4(const Grid &g, const uint8_t *G, float4 *S, float *q, bool sp
 void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, float4 *S, float *q, hipStream_t s);
 // The compact Jacobi state's links (no halo entries; links across periodic
 // faces to the image) built on the device from the fluid list; cidx is
-// scratch of Grid::total entries.
-void launch_torus_links(const Grid &g, const int32_t *list, int n, int32_t *cidx, uint2 *L, int nE, bool aos,
-                        hipStream_t s);
+// scratch of Grid::total entries.  Returns the error of clearing cidx.
+hipError_t launch_torus_links(const Grid &g, const int32_t *list, int n, int32_t *cidx, uint2 *L, int nE, bool aos,
+                              hipStream_t s);
 // The compact states' +-2 links folded into the slots of the solid sides
 // (cases 4 and 5), from L0 into L1 (both of L.n_centries entries), after the
 // records' flags are in the links.

@@ -175,7 +175,6 @@ __global__ __launch_bounds__(kMarchThreads) void k_jac4_march(Grid g, const uint
     __shared__ float4 L[NS][RN];
     __shared__ uint8_t Lc[NS][RN];
     __shared__ float Lr[NS][RN];
-    typedef float v4f __attribute__((ext_vector_type(4)));
     typedef float v3f __attribute__((ext_vector_type(3)));
 
     const int r = march_xcd_block(blockIdx.x, gridDim.x);
@@ -183,7 +182,7 @@ __global__ __launch_bounds__(kMarchThreads) void k_jac4_march(Grid g, const uint
     const int xi0 = txi * kMarchTX, yi0 = tyi * TY;
     const int z0 = 2 + seg * zseg, z1 = min(g.nz + 2, z0 + zseg);  // padded planes [z0, z1)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t sx = g.sx, sy = g.sy;
+    const int64_t sy = g.sy;
 
     // element offsets within a plane of this thread's rim-box cells (-1: none)
     int32_t off[LOADS];

@@ -559,8 +559,9 @@ static int build_compact_jacobi(pm_ctx *c) {
     RC(dalloc(&c->lists.clink, 3 * n_ent));
     int32_t *cidx = nullptr;
     RC(dalloc(&cidx, g.total));
-    pm::launch_torus_links(g, c->lists.all, (int)n_all, cidx, c->lists.clink, (int)n_ent, aos, c->stream);
-    const hipError_t e = hipStreamSynchronize(c->stream);
+    hipError_t e = pm::launch_torus_links(g, c->lists.all, (int)n_all, cidx, c->lists.clink, (int)n_ent, aos,
+                                          c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(cidx);
     if (e != hipSuccess) return fail(PM_E_HIP, "compact links: %s", hipGetErrorString(e));
     HIPCHK(hipGetLastError());

@@ -598,15 +598,17 @@ __global__ __launch_bounds__(kListBlock) void k_torus_links(Grid g, const int32_
     for (int q = 0; q < 6; q++) W[link_word<AOS>(t, q, nE)] = w[q];
 }
 
-void launch_torus_links(const Grid &g, const int32_t *list, int n, int32_t *cidx, uint2 *L, int nE, bool aos,
-                        hipStream_t s) {
-    hipMemsetAsync(cidx, 0xff, sizeof(int32_t) * (size_t)g.total, s);
+hipError_t launch_torus_links(const Grid &g, const int32_t *list, int n, int32_t *cidx, uint2 *L, int nE, bool aos,
+                              hipStream_t s) {
+    const hipError_t e = hipMemsetAsync(cidx, 0xff, sizeof(int32_t) * (size_t)g.total, s);
+    if (e != hipSuccess) return e;
     if (n > 0)
         hipLaunchKernelGGL(k_cidx_scatter, dim3((unsigned)((n + kListBlock - 1) / kListBlock)), dim3(kListBlock), 0, s,
                            list, n, cidx);
     const dim3 grid((unsigned)((nE + kListBlock - 1) / kListBlock)), blk(kListBlock);
     if (aos) hipLaunchKernelGGL(k_torus_links<true>, grid, blk, 0, s, g, list, n, cidx, nE, L);
     else hipLaunchKernelGGL(k_torus_links<false>, grid, blk, 0, s, g, list, n, cidx, nE, L);
+    return hipSuccess;
 }
 
 // The +-2 links folded into the +-1 slots (both compact states).  A +-2
