@@ -73,6 +73,33 @@ def test_batch_equals_single_solves(channel_raw, precision, graph, skip2, queue)
             assert np.array_equal(b.field(slot, "press"), press)
 
 
+@pytest.mark.parametrize("links", ["packed-span0", "full"])
+def test_batch_compact_link_forms_equal_single_solves(channel_raw, links, monkeypatch):
+    """Every sample on the compact state (POREMAPS_JACOBI=compact): its lagged
+    pass from packed records whose windows hold one value each (span 0, so
+    that most entries read the link arrays, in the batch's packed kernel) and
+    from the link arrays themselves (the batch's general kernel); each sample
+    ends as the same solve alone, bit for bit."""
+    say("batch compact links", links)
+    monkeypatch.setenv("POREMAPS_JACOBI", "compact")
+    if links == "full":
+        monkeypatch.setenv("POREMAPS_JAC_LINKS", "full")
+    else:
+        monkeypatch.setenv("POREMAPS_PACK_SPAN", "0")
+    cs = cases(channel_raw)
+    with pm.Batch(precision="f32dc", omega=2.0, omega_p=pm.OMEGA_P) as b:
+        slots = [b.add(raw, boundary_method=bc, voxel_size=dx) for raw, bc, dx in cs]
+        st = run_batch(b, slots)
+        for slot, (raw, bc, dx) in zip(slots, cs):
+            state, res = st[slot]
+            ref, velz, press = alone(raw, bc, dx, "f32dc", 2.0)
+            assert state == "converged"
+            assert res.sweeps == ref.sweeps
+            assert res.last.wk33 == ref.last.wk33
+            assert np.array_equal(b.field(slot, "velz"), velz)
+            assert np.array_equal(b.field(slot, "press"), press)
+
+
 def test_batch_join_and_evict(channel_raw):
     """Continuous batching: a sample admitted while others run, and samples
     evicted as they finish, still reproduce their solo results."""
