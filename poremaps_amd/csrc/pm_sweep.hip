@@ -678,20 +678,54 @@ void launch_link_subst(const FluidLists &L, const uint2 *L0, uint2 *L1, hipStrea
 // (bit 31 of the halo entry), then the outlet plane's p pinned to 0.  A copy
 // whose image is an outlet entry zeroes p itself, so it does not depend on
 // the order of the two writes.
-__global__ void k_ghost4_compact(const int2 *__restrict__ halo, int nh, const int32_t *__restrict__ outl, int no,
-                                 float4 *S, float *q) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nh) {
-        const int2 h = halo[t];
-        const int32_t dst = h.x & 0x7fffffff, src = h.y & 0x7fffffff;
-        float4 v = S[src];
-        if (h.y < 0) v.x = 0.f;
-        if (h.x < 0) q[dst] = q[src];
-        S[dst] = v;
-        return;
+// Four items per thread (item k at t + k * gridDim.x * blockDim.x), all
+// loads issued before any store: a few hundred thousand independent 16-byte
+// copies are latency-bound, so each thread keeps four in flight.  Reading
+// every item before writing any is the same as the one-at-a-time order: the
+// images are interior entries (no copy writes what another reads), and a
+// copy of an outlet entry zeroes its p itself.
+constexpr int kGhostItems = 4;
+__global__ __launch_bounds__(256) void k_ghost4_compact(const int2 *__restrict__ halo, int nh,
+                                                        const int32_t *__restrict__ outl, int no, float4 *S,
+                                                        float *q) {
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int kind[kGhostItems];  // 0: nothing, 1: halo copy, 2: outlet pin
+    int32_t dst[kGhostItems];
+    int2 h[kGhostItems];
+    float4 v[kGhostItems];
+    float qv[kGhostItems];
+#pragma unroll
+    for (int k = 0; k < kGhostItems; k++) {
+        const int64_t t = t0 + k * stride;
+        kind[k] = 0;
+        dst[k] = 0;
+        qv[k] = 0.f;
+        v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        h[k] = make_int2(0, 0);
+        if (t < nh) {
+            h[k] = halo[t];
+            const int32_t src = h[k].y & 0x7fffffff;
+            dst[k] = h[k].x & 0x7fffffff;
+            v[k] = S[src];
+            if (h[k].x < 0) qv[k] = q[src];
+            kind[k] = 1;
+        } else if (t - nh < no) {
+            dst[k] = outl[t - nh];
+            kind[k] = 2;
+        }
     }
-    const int64_t u = t - nh;
-    if (u < no) S[outl[u]].x = 0.f;
+#pragma unroll
+    for (int k = 0; k < kGhostItems; k++) {
+        if (kind[k] == 1) {
+            float4 w = v[k];
+            if (h[k].y < 0) w.x = 0.f;
+            if (h[k].x < 0) q[dst[k]] = qv[k];
+            S[dst[k]] = w;
+        } else if (kind[k] == 2) {
+            S[dst[k]].x = 0.f;
+        }
+    }
 }
 
 // The colour passes' record flags (packed code, block, outlet flag: bits
@@ -1478,8 +1512,8 @@ void launch_ghost4_lists(const Grid &g, const FluidLists &L, const uint8_t *G, f
     }
     const int64_t n = (int64_t)L.n_chalo + L.n_cout;
     if (n > 0)
-        hipLaunchKernelGGL(k_ghost4_compact, dim3(nblocks(n, 256)), dim3(256), 0, s, L.chalo, L.n_chalo, L.cout,
-                           L.n_cout, S, q);
+        hipLaunchKernelGGL(k_ghost4_compact, dim3(nblocks(n, 256 * kGhostItems)), dim3(256), 0, s, L.chalo,
+                           L.n_chalo, L.cout, L.n_cout, S, q);
 }
 
 void launch_fold4(const FluidLists &L, double *Xp, double *Xu, double *Xv, double *Xw, float4 *d, hipStream_t s) {
