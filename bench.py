@@ -36,6 +36,10 @@ time_to_eps (default on; --no-tte skips it): eps 1e-6 from fields resident in
 HBM to the stop rule, for the plain FP64 Jacobi solver, the FP32+FP64-refined
 Jacobi and 3-colour SOR, and the FP64 solves at the same relaxations (k_zz of
 each refined solve relative to its FP64 twin).
+batch / serving (default run): the batch of 64 media (config 3) and the
+serving stream of 24 tensor requests (config 4) with their defaults,
+summarised in the same line (solves/h; p50 / p99 latency); --no-batch skips
+them.
 cpu_baseline: the reference POREMAPS itself (oracle/_ref/POREMAPS, built from
 the reference's sources by `make -C oracle ref`) under mpiexec on this
 host's cores (at most 16), on the same field: the rate of the difference of a
@@ -361,7 +365,7 @@ def bench_batch(a):
                   file=sys.stderr, flush=True)
         line["k_rel_err_vs_fp64_same_relaxation"] = {"max": max(errs), "median": float(np.median(errs)),
                                                      "samples": len(errs)}
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_serve(a):
@@ -435,7 +439,32 @@ def bench_serve(a):
                   file=sys.stderr, flush=True)
         line["k_err_vs_fp64_same_relaxation"] = {"requests": n_acc, "diagonal_rel_max": max(diag),
                                                  "off_diagonal_rel_to_diag_max": max(off)}
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def batch_and_serving(a):
+    """The batch (64 media) and serving (24 requests at 4/s) workloads with
+    their defaults, summarised for the default bench line, which then carries
+    the whole metric: GLUP/s and time to eps of one field, solves/h of the
+    batch, solves/h and p50 / p99 latency of the serving stream."""
+    import argparse as _ap
+
+    out = {}
+    b = _ap.Namespace(**vars(a))
+    b.omega, b.omega_p, b.mode, b.compare, b.accuracy = 0.0, 0.0, "jacobi", False, False
+    b.count = 64
+    ln = bench_batch(b)
+    out["batch"] = {"solves_per_hour": ln["value"], "seconds": ln["seconds"], "load_seconds": ln["load_seconds"],
+                    "aggregate_glups": ln["aggregate_glups"], "sweeps": ln["sweeps"],
+                    "workload": ln["config"]["workload"], "relaxation": "pair by porosity"}
+    b.count = 24
+    ln = bench_serve(b)
+    out["serving"] = {"solves_per_hour": ln["value"], "latency_p50_s": ln["latency_p50_s"],
+                      "latency_p99_s": ln["latency_p99_s"], "latency_max_s": ln["latency_max_s"],
+                      "wall_s": ln["wall_s"], "completed": ln["completed"],
+                      "offered_solves_per_hour": ln["offered_solves_per_hour"],
+                      "peak_device_GB": ln["peak_device_GB"], "workload": ln["config"]["workload"]}
+    return out
 
 
 def main():
@@ -464,6 +493,8 @@ def main():
     ap.add_argument("--reference-eps", type=float, default=1e-9,
                     help="--accuracy: eps of the tightly converged reference solve (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batch and serving summaries of the default run")
     ap.add_argument("--count", type=int, default=None, help="media in the batch / serving workloads")
     ap.add_argument("--rate", type=float, default=4.0, help="serving arrival rate, requests/s")
     ap.add_argument("--compare", action="store_true", help="also time the one-at-a-time baselines")
@@ -483,10 +514,12 @@ def main():
         a.omega_p = 0.0 if a.omega == 0.0 else 6.0 if a.mode == "sor3" else 7.0
     if a.config == 3:
         a.count = a.count or 64
-        return bench_batch(a)
+        print(json.dumps(bench_batch(a)), flush=True)
+        return None
     if a.config == 4:
         a.count = a.count or 24
-        return bench_serve(a)
+        print(json.dumps(bench_serve(a)), flush=True)
+        return None
 
     import poremaps_amd as pm
 
@@ -560,6 +593,8 @@ def main():
     }
     if not (a.tte or a.accuracy or a.no_tte) and a.mode != "reference":
         line["time_to_eps"] = default_tte(g, kw)
+    if not (a.tte or a.accuracy or a.no_tte or a.no_batch) and a.mode != "reference" and a.config == 2:
+        line.update(batch_and_serving(a))
     if a.tte or a.accuracy:
         tte = {}
         runs = [("fp64_plain_jacobi", dict(mode="jacobi", precision="f64", omega=1.0, omega_p=0.0)),
