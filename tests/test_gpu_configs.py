@@ -18,6 +18,8 @@
 Reductions are compensated parallel sums on the GPU and serial sums in the
 oracle, so k is compared to 1e-12 relative; fields and sweep counts exactly.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -251,8 +253,48 @@ def test_sphere_pack_256_refined_k_matches_fp64(mode, pair):
     assert abs(r["f32dc"].last.wk33 / r["f64"].last.wk33 - 1) < 1e-5
 
 
+@functools.lru_cache(maxsize=1)
+def _grf512():
+    return geo.gaussian_field(512, 8.0, 0.20, seed=1)
+
+
+@pytest.mark.parametrize("mode", ["sor3", "jacobi"])
+def test_gaussian_field_512_packed_links_equal_link_arrays(mode, monkeypatch):
+    """The Gaussian-field workload at full size (512^3, bc 1): 300 sweeps (three
+    intervals, folds and residuals included) with the passes reading packed
+    link records (the default; for the SOR also with windows of one value, so
+    that most entries read the link arrays instead) leave the same velocity
+    and pressure fields, bit for bit, as with the passes reading the link
+    arrays."""
+    raw = _grf512()
+    pair = (1.8, 6.0) if mode == "sor3" else (2.5, 7.0)
+    if mode == "jacobi":
+        monkeypatch.setenv("POREMAPS_JACOBI", "compact")
+    full = {"POREMAPS_SOR_LINKS" if mode == "sor3" else "POREMAPS_JAC_LINKS": "full"}
+    forms = [("packed", {}), ("full", full)]
+    if mode == "sor3":
+        forms.append(("span0", {"POREMAPS_PACK_SPAN": "0"}))
+    out = {}
+    for name, envs in forms:
+        with monkeypatch.context() as m:
+            for k, v in envs.items():
+                m.setenv(k, v)
+            with pm.Solver(raw, boundary_method=1, voxel_size=5e-6, mode=mode, precision="f32dc", omega=pair[0],
+                           omega_p=pair[1], it_eval=100) as s:
+                kname, tried = s.kernel_choice()
+                s.step(300)
+                out[name] = (kname, tried.get("packed-far"), s.field("velz"), s.field("press"))
+        say("gaussian field 512^3", mode, name, out[name][0], "packed-far", out[name][1])
+    assert "-packed" in out["packed"][0] and "-packed" not in out["full"][0]
+    for name in out:
+        assert np.array_equal(out[name][2], out["packed"][2]), name
+        assert np.array_equal(out[name][3], out["packed"][3]), name
+    if mode == "sor3":
+        assert out["span0"][1] > 1_000_000
+
+
 def test_gaussian_field_512_refined_k_matches_fp64():
-    raw = geo.gaussian_field(512, 8.0, 0.20, seed=1)
+    raw = _grf512()
     say("gaussian field 512^3 bc 1")
     r = _refined_vs_fp64(raw, 1, 5e-6, "jacobi", (2.5, 7.0))
     assert r["f32dc"].sweeps == r["f64"].sweeps
