@@ -123,7 +123,7 @@ struct pm_ctx {
     bool sor_packed = true;     // FluidLists::packed (POREMAPS_SOR_LINKS=full: off)
     bool jac_packed = true;     // FluidLists::packed for the compact Jacobi (POREMAPS_JAC_LINKS=full / aos: off)
     int pack_span = 4095;       // launch_pack_links' span (POREMAPS_SOR_PACK_SPAN)
-    int pack_far = 0;           // entries left on the full links
+    int pack_far = -1;          // entries left on the full links (-1: links not packed)
     bool jac_compact = true;    // FluidLists::compact_jac (POREMAPS_JACOBI naming a dense form: off)
     float layout_ms[2] = {0.f, 0.f};  // FP32+DC Jacobi: ms per sweep, compact / dense layout (0: not timed)
     bool jac_links_aos = false; // FluidLists::links_aos (POREMAPS_JAC_LINKS=aos)

@@ -169,6 +169,7 @@ static void free_all(pm_ctx *c) {
     if (!c->lists.col_in_all)
         for (int q = 0; q < 3; q++) f(c->lists.col[q]);
     c->lists = pm::FluidLists();
+    c->pack_far = -1;
     if (c->red_host) (void)hipHostFree(c->red_host);
     c->G = c->pc = nullptr; c->code = nullptr;
     for (int q = 0; q < 4; q++) c->X[q] = c->X2[q] = nullptr;
@@ -488,6 +489,28 @@ static int build_compact_sor(pm_ctx *c, const uint8_t *raw, const std::vector<in
     return PM_OK;
 }
 
+// Use the packed records unless more than a quarter of the entries fit no
+// base: such an entry reads its record and then the link arrays, so a layout
+// where many do (a medium whose fluid cells of one colour are far apart in the
+// neighbouring colours' lists) runs faster on the link arrays alone.  With the
+// test setting POREMAPS_PACK_KEEP=1 the records are kept whatever the count.
+static int keep_packed(pm_ctx *c, int far) {
+    pm::FluidLists &L = c->lists;
+    c->pack_far = far;
+    const char *keep = std::getenv("POREMAPS_PACK_KEEP");
+    const bool always = keep && std::strcmp(keep, "0") != 0;
+    if (!always && (int64_t)far * 4 > L.n_all) {
+        (void)hipFree(L.cpack);
+        (void)hipFree(L.cbase);
+        L.cpack = nullptr;
+        L.cbase = nullptr;
+        L.packed = false;
+        return PM_OK;
+    }
+    L.packed = true;
+    return PM_OK;
+}
+
 // The compact SOR's links packed to 12 B per entry (launch_pack_links), with
 // 18 bases per wave of 64 entries; the 24-byte links stay for the entries
 // whose links fit no base (none on the 512^3 Gaussian random field).
@@ -513,10 +536,7 @@ static int pack_sor_links(pm_ctx *c) {
     if (e != hipSuccess) return fail(PM_E_HIP, "packed links: %s", hipGetErrorString(e));
     HIPCHK(hipGetLastError());
     if (cnt[1] != 0) return fail(PM_E_HIP, "packed links: %d records do not decode to their links", cnt[1]);
-    const int far = cnt[0];
-    L.packed = true;
-    c->pack_far = far;
-    return PM_OK;
+    return keep_packed(c, cnt[0]);
 }
 
 // The compact Jacobi's packed records (launch_pack_jac_links): 24 B per entry
@@ -540,9 +560,7 @@ static int pack_jac_links(pm_ctx *c) {
     if (e != hipSuccess) return fail(PM_E_HIP, "packed links: %s", hipGetErrorString(e));
     HIPCHK(hipGetLastError());
     if (cnt[1] != 0) return fail(PM_E_HIP, "packed links: %d records do not decode to their links", cnt[1]);
-    L.packed = true;
-    c->pack_far = cnt[0];
-    return PM_OK;
+    return keep_packed(c, cnt[0]);
 }
 
 // The compact FP32+DC Jacobi state (FluidLists::compact_jac): entry t is list
@@ -1496,8 +1514,9 @@ extern "C" int pm_kernel_choice(pm_ctx *c, char *name, int len, char *tried, int
             std::snprintf(e, sizeof e, "layout-compact=%.4f;layout-dense=%.4f", c->layout_ms[0], c->layout_ms[1]);
             t += e;
         }
-        if (c->lists.packed) {
-            // the compact SOR's entries on the full links (no packed form)
+        if (c->pack_far >= 0) {
+            // the entries whose links fit no packed record (the records are
+            // kept unless more than a quarter of the entries are such)
             char e[48];
             std::snprintf(e, sizeof e, "%spacked-far=%d", t.empty() ? "" : ";", c->pack_far);
             t += e;

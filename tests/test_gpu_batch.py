@@ -86,6 +86,7 @@ def test_batch_compact_link_forms_equal_single_solves(channel_raw, links, monkey
         monkeypatch.setenv("POREMAPS_JAC_LINKS", "full")
     else:
         monkeypatch.setenv("POREMAPS_PACK_SPAN", "0")
+        monkeypatch.setenv("POREMAPS_PACK_KEEP", "1")
     cs = cases(channel_raw)
     with pm.Batch(precision="f32dc", omega=2.0, omega_p=pm.OMEGA_P) as b:
         slots = [b.add(raw, boundary_method=bc, voxel_size=dx) for raw, bc, dx in cs]

@@ -263,9 +263,10 @@ def test_gaussian_field_512_packed_links_equal_link_arrays(mode, monkeypatch):
     """The Gaussian-field workload at full size (512^3, bc 1): 300 sweeps (three
     intervals, folds and residuals included) with the passes reading packed
     link records (the default; for the SOR also with windows of one value, so
-    that most entries read the link arrays instead) leave the same velocity
-    and pressure fields, bit for bit, as with the passes reading the link
-    arrays."""
+    that most entries read the link arrays instead, and with those windows
+    without the keep setting, where the records are given up for the link
+    arrays) leave the same velocity and pressure fields, bit for bit, as with
+    the passes reading the link arrays."""
     raw = _grf512()
     pair = (1.8, 6.0) if mode == "sor3" else (2.5, 7.0)
     if mode == "jacobi":
@@ -273,7 +274,10 @@ def test_gaussian_field_512_packed_links_equal_link_arrays(mode, monkeypatch):
     full = {"POREMAPS_SOR_LINKS" if mode == "sor3" else "POREMAPS_JAC_LINKS": "full"}
     forms = [("packed", {}), ("full", full)]
     if mode == "sor3":
-        forms.append(("span0", {"POREMAPS_PACK_SPAN": "0"}))
+        forms.append(("span0", {"POREMAPS_PACK_SPAN": "0", "POREMAPS_PACK_KEEP": "1"}))
+        # the same windows without the keep setting: 71 % of the entries fall
+        # back, so the records are given up for the link arrays
+        forms.append(("span0-fallback", {"POREMAPS_PACK_SPAN": "0"}))
     out = {}
     for name, envs in forms:
         with monkeypatch.context() as m:
@@ -291,6 +295,7 @@ def test_gaussian_field_512_packed_links_equal_link_arrays(mode, monkeypatch):
         assert np.array_equal(out[name][3], out["packed"][3]), name
     if mode == "sor3":
         assert out["span0"][1] > 1_000_000
+        assert "-packed" not in out["span0-fallback"][0] and out["span0-fallback"][1] > 1_000_000
 
 
 def test_gaussian_field_512_refined_k_matches_fp64():

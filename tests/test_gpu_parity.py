@@ -310,7 +310,7 @@ def test_fast_f32dc_sweeps_bit_exact(mode, bc, omega, omega_p):
 
 
 @pytest.mark.parametrize("layout", ["split", "split-all-loads", "natural", "compact", "compact-full", "compact-span0",
-                                    "compact-span3"])
+                                    "compact-span3", "compact-span0-fallback"])
 @pytest.mark.parametrize("shape,bc", [((6, 5, 28), 0), ((5, 7, 27), 1), ((4, 6, 2), 2), ((7, 4, 60), 0),
                                       ((5, 6, 13), 3), ((9, 12, 130), 1), ((10, 9, 97), 0), ((6, 7, 20), 4)])
 def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
@@ -322,22 +322,28 @@ def test_sor_f32dc_state_layouts_bit_exact(layout, shape, bc):
     the case reads them (default) and with every load; the compact state
     (fluid cells and periodic halo copies only, linked entries) with its
     links packed (default), in full, and packed with windows of 1 and 4
-    (span 0, 3) so that most entries fall back to the full links."""
+    (span 0, 3) so that most entries fall back to the full links, the records
+    kept whatever the count (POREMAPS_PACK_KEEP=1) or, by default, given up
+    for the link arrays when more than a quarter of the entries fall back."""
     raw = random_geometry(shape, 0.3, sum(shape) + bc)
     say("sor f32dc layout", layout, shape, bc)
     ref = oracle_dc(raw, bc, "sor3", 1.8, 16, 5, 6.0)
     split = layout.startswith("split")
-    span = layout[len("compact-span"):] if layout.startswith("compact-span") else "4095"
+    span = layout[len("compact-span"):].split("-")[0] if layout.startswith("compact-span") else "4095"
+    keep = "0" if layout.endswith("fallback") else "1"
     with env("POREMAPS_SOR", "split" if split else "natural" if layout == "natural" else "compact"), \
             env("POREMAPS_SOR_SKIP2", "0" if layout == "split-all-loads" else "1"), \
             env("POREMAPS_SOR_LINKS", "full" if layout == "compact-full" else "packed"), \
-            env("POREMAPS_PACK_SPAN", span), \
+            env("POREMAPS_PACK_SPAN", span), env("POREMAPS_PACK_KEEP", keep), \
             pm.Solver(raw, boundary_method=bc, mode="sor3", omega=1.8, omega_p=6.0, precision="f32dc",
                       it_eval=5) as s:
         want = {"split": "interleaved-colour-split-skip2", "split-all-loads": "interleaved-colour-split",
                 "natural": "interleaved-colour", "compact-full": "interleaved-colour-compact"}.get(
                     layout, "interleaved-colour-compact-packed")
         name, tried = s.kernel_choice()
+        if layout == "compact-span0-fallback":
+            gave_up = tried["packed-far"] * 4 > int((raw == 0).sum())
+            want = "interleaved-colour-compact" if gave_up else want
         assert name == want
         if layout == "compact-span0" and raw.size >= 500:
             # (the smallest boxes have at most three distinct offsets a slot)
@@ -398,6 +404,7 @@ def test_jacobi_f32dc_record_pass_halo_fill_bit_exact(halos, shape, bc):
     links = {"compact-aos": "aos", "compact-full": "full"}.get(halos, "packed")
     with env("POREMAPS_HALOS", "separate" if halos == "separate" else "fused"), env("POREMAPS_JACOBI", form), \
             env("POREMAPS_JAC_LINKS", links), env("POREMAPS_PACK_SPAN", "0" if halos == "compact-span0" else "4095"), \
+            env("POREMAPS_PACK_KEEP", "1"), \
             env("POREMAPS_MARCH_TY", halos[5:] if march else "4"), env("POREMAPS_MARCH_ZSEG", "3"), \
             env("POREMAPS_REC_OCC", occ), env("POREMAPS_REC_SKIP2", skip), \
             pm.Solver(raw, boundary_method=bc, mode="jacobi", omega=2.5, omega_p=6.0, precision="f32dc",
