@@ -140,6 +140,12 @@ __device__ __forceinline__ void target_links(__amdgpu_buffer_rsrc_t rsL, const J
 constexpr int kJPackSpan = 1023;
 constexpr uint32_t kJPackZero = 0xFFFu;
 
+// One float of a buffer of floats (element e), only for lanes with need set.
+__device__ __forceinline__ float buf_loadf(__amdgpu_buffer_rsrc_t rs, int32_t e, bool need) {
+    const uint32_t off = need ? (uint32_t)e * 4u : 0xFFFFFFF0u;
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
+
 // The bases are read through the constant address space: nothing writes
 // them during a pass, and a wave-uniform address there is read with scalar
 // loads even where the pointer comes from memory the kernel also writes
@@ -208,9 +214,21 @@ __device__ __forceinline__ void lag_voxel4_compact(int t, const uint2 *__restric
                   pcy = (uint8_t)((l.f >> 16) & 0xff), pcz = (uint8_t)((l.f >> 24) & 0xff);
     const bool out = (l.f >> 32) & 1u, first = (l.f >> 33) & 1u;
     const int32_t zero = nE - 1;
-    const float rpx = R.pc[l.xm], rpy = R.pc[l.ym], rpz = R.pc[l.zm];
-    const float4 C = S0[t];
     const Cases cc = decode(pcv);
+    // the -x, -y, -z neighbours' continuity residuals (their recomputed
+    // pressures); without ALL only where used: an open side, off the outlet
+    // plane (x, y) and off the first plane (z)
+    float rpx, rpy, rpz;
+    if (ALL) {
+        rpx = R.pc[l.xm]; rpy = R.pc[l.ym]; rpz = R.pc[l.zm];
+    } else {
+        const __amdgpu_buffer_rsrc_t rsP =
+            __builtin_amdgcn_make_buffer_rsrc((void *)R.pc, (short)0, (int)((uint32_t)nE * 4u), 0x00020000);
+        rpx = buf_loadf(rsP, l.xm, !cc.bx && !out);
+        rpy = buf_loadf(rsP, l.ym, !cc.by && !out);
+        rpz = buf_loadf(rsP, l.zm, !cc.bz && !first);
+    }
+    const float4 C = S0[t];
     const Star4 s = star_compact<ALL, AOS>(l, cc, S0, rsL, rsS, nE);
     // the recompute targets' +1 neighbours: v of (x-1, y+1), w of (x-1, z+1),
     // u of (x+1, y-1), w of (y-1, z+1), u of (x+1, z-1), v of (y+1, z-1)
